@@ -1,0 +1,275 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is a C++ restatement of Delta2's contact-detection path
+(collision_detection/*.h, common/*.h, model/* of the reference).  Only the
+test-suite, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` load it, and only as the checker.  The product never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "libd2oracle.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+
+_lib = None
+
+
+def build():
+    """Compile the oracle library (g++, -O2 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for prec in ("f64", "f32"):
+            for name, args in [
+                ("transformation_matrix_euler", [_dp, _dp, _dp]),
+                ("transform_point", [_dp, _dp, _dp]),
+                ("comparison_dist", [_dp, _dp, _dp]),
+                ("edge_closest", [_dp, _dp, _dp]),
+                ("barycentric", [_dp, _dp, _dp]),
+                ("intersect_segment", [_dp, _dp, _dp]),
+                ("project_to_plane", [_dp, _dp, _dp]),
+                ("point_tri_ccd", [_dp] * 7),
+                ("edge_edge_ccd", [_dp] * 5),
+                ("tri_tri_ccd", [_dp] * 7),
+            ]:
+                f = getattr(L, f"d2o_{name}_{prec}")
+                f.argtypes = args
+                f.restype = None
+            f = getattr(L, f"d2o_dist_to_point_{prec}")
+            f.argtypes = [_dp, _dp]
+            f.restype = ctypes.c_double
+            f = getattr(L, f"d2o_sphere_sphere_{prec}")
+            f.argtypes = [_dp, _dp, ctypes.c_double, _dp, _dp, ctypes.c_double]
+            f.restype = ctypes.c_double
+        L.d2o_extrapolate.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        L.d2o_state_transformation.argtypes = [_dp, _dp]
+        L.d2o_pcg_draws.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, _u32p]
+        L.d2o_pcg_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u64p]
+        L.d2o_tree_build.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, _u64p]
+        L.d2o_tree_build.restype = _vp
+        L.d2o_tree_num_nodes.argtypes = [_vp]
+        L.d2o_tree_num_nodes.restype = ctypes.c_int
+        L.d2o_tree_export.argtypes = [_vp, _ip, _dp]
+        L.d2o_tree_import.argtypes = [_ip, _dp, ctypes.c_int]
+        L.d2o_tree_import.restype = _vp
+        L.d2o_tree_free.argtypes = [_vp]
+        L.d2o_scene_create.argtypes = [ctypes.c_int, ctypes.POINTER(_vp), ctypes.c_int, _ip, _dp]
+        L.d2o_scene_create.restype = _vp
+        L.d2o_scene_free.argtypes = [_vp]
+        L.d2o_scene_get_bodies.argtypes = [_vp, _dp]
+        L.d2o_narrow.argtypes = [_vp, ctypes.c_int, _ip, ctypes.c_int]
+        L.d2o_narrow.restype = ctypes.c_int
+        L.d2o_ccd.argtypes = [_vp, ctypes.c_int, _ip, _dp]
+        L.d2o_ccd.restype = ctypes.c_int
+        L.d2o_scene_result.argtypes = [_vp, _ip, _ip, _dp, _dp]
+        L.d2o_cluster_step.argtypes = [_vp, ctypes.c_int, _ip, ctypes.c_float]
+        L.d2o_cluster_step.restype = ctypes.c_double
+        L.d2o_time_pass.argtypes = [_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp]
+        L.d2o_time_pass.restype = ctypes.c_double
+        L.d2o_scene_filter_result.argtypes = [_vp, ctypes.c_int]
+        L.d2o_scene_filter_result.restype = ctypes.c_int
+        L.d2o_step_scale.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double]
+        L.d2o_step_scale.restype = ctypes.c_double
+        _lib = L
+    return _lib
+
+
+def _d(a, n=None):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel())
+    if n is not None:
+        assert a.size == n, (a.size, n)
+    return a
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _call(name, prec, *arrays, out=0):
+    o = np.zeros(out)
+    arrs = [_d(a) for a in arrays]
+    getattr(lib(), f"d2o_{name}_{prec}")(*[_p(a) for a in arrs], _p(o))
+    return o
+
+
+# --- primitives (prec is "f64" or "f32") ------------------------------------
+def transformation_matrix_euler(euler, pos, prec="f64"):
+    return _call("transformation_matrix_euler", prec, euler, pos, out=16).reshape(4, 4)
+
+
+def transform_point(p, M, prec="f64"):
+    return _call("transform_point", prec, p, M, out=3)
+
+
+def comparison_dist(tri_a, tri_b, prec="f64"):
+    o = _call("comparison_dist", prec, tri_a, tri_b, out=7)
+    return o[0], o[1:4], o[4:7]
+
+
+def edge_closest(e, f, prec="f64"):
+    o = _call("edge_closest", prec, e, f, out=7)
+    return o[0], o[1:4], o[4:7]
+
+
+def barycentric(tri, p, prec="f64"):
+    return _call("barycentric", prec, tri, p, out=3)
+
+
+def intersect_segment(tri, e, prec="f64"):
+    o = _call("intersect_segment", prec, tri, e, out=6)
+    return o[0], bool(o[1]), bool(o[2]), o[3:6]
+
+
+def project_to_plane(tri, p, prec="f64"):
+    o = _call("project_to_plane", prec, tri, p, out=4)
+    return bool(o[0]), o[1:4]
+
+
+def dist_to_point(tri, p, prec="f64"):
+    return getattr(lib(), f"d2o_dist_to_point_{prec}")(_p(_d(tri)), _p(_d(p)))
+
+
+def sphere_sphere(ap, av, ar, bp, bv, br, prec="f64"):
+    return getattr(lib(), f"d2o_sphere_sphere_{prec}")(_p(_d(ap)), _p(_d(av)), ar, _p(_d(bp)), _p(_d(bv)), br)
+
+
+def point_tri_ccd(pt, pTs, pTe, tri, tTs, tTe, prec="f64"):
+    o = _call("point_tri_ccd", prec, pt, pTs, pTe, tri, tTs, tTe, out=8)
+    return o[0], o[1:4], o[4:7], o[7]
+
+
+def edge_edge_ccd(a_s, a_e, b_s, b_e, prec="f64"):
+    o = _call("edge_edge_ccd", prec, a_s, a_e, b_s, b_e, out=8)
+    return o[0], o[1:4], o[4:7], o[7]
+
+
+def tri_tri_ccd(a, aTs, aTe, b, bTs, bTe, prec="f64"):
+    o = _call("tri_tri_ccd", prec, a, aTs, aTe, b, bTs, bTe, out=8)
+    return o[0], o[1:4], o[4:7], o[7]
+
+
+def extrapolate(state14, inv_inertia9, t):
+    o = np.zeros(14)
+    lib().d2o_extrapolate(_p(_d(state14, 14)), _p(_d(inv_inertia9, 9)), float(t), _p(o))
+    return o
+
+
+def state_transformation(state14):
+    o = np.zeros(16)
+    lib().d2o_state_transformation(_p(_d(state14, 14)), _p(o))
+    return o.reshape(4, 4)
+
+
+def pcg_draws(s0, s1, bound, n):
+    o = np.zeros(n, dtype=np.uint32)
+    lib().d2o_pcg_draws(s0, s1, bound, n, o.ctypes.data_as(_u32p))
+    return o
+
+
+def pcg_seed(s0=42, s1=54):
+    st = np.zeros(2, dtype=np.uint64)
+    lib().d2o_pcg_seed(s0, s1, st.ctypes.data_as(_u64p))
+    return st
+
+
+# --- trees ------------------------------------------------------------------
+def build_tree(V, F, rng_state):
+    """Oracle surrogate tree of mesh (V, F); rng_state (uint64[2]) advances in place.
+
+    Returns (nodes int32[n,8], buckets float64[n,88])."""
+    V = _d(V)
+    F = np.ascontiguousarray(np.asarray(F, dtype=np.int32).ravel())
+    assert rng_state.dtype == np.uint64
+    h = lib().d2o_tree_build(_p(V), V.size // 3, F.ctypes.data_as(_ip), F.size // 3, rng_state.ctypes.data_as(_u64p))
+    n = lib().d2o_tree_num_nodes(h)
+    nodes = np.zeros((n, 8), dtype=np.int32)
+    buckets = np.zeros((n, 88))
+    lib().d2o_tree_export(h, nodes.ctypes.data_as(_ip), _p(buckets))
+    lib().d2o_tree_free(h)
+    return nodes, buckets
+
+
+class Scene:
+    """Oracle scene: trees (flat arrays) + body records (40 doubles each)."""
+
+    def __init__(self, trees, tree_of_body, bodies):
+        L = lib()
+        handles = []
+        for nodes, buckets in trees:
+            nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+            buckets = np.ascontiguousarray(buckets, dtype=np.float64)
+            handles.append(L.d2o_tree_import(nodes.ctypes.data_as(_ip), _p(buckets), nodes.shape[0]))
+        arr = (_vp * len(handles))(*handles)
+        self._tob = np.ascontiguousarray(tree_of_body, dtype=np.int32)
+        self._bodies = _d(bodies)
+        self.n_bodies = self._tob.size
+        self.h = L.d2o_scene_create(len(handles), arr, self.n_bodies, self._tob.ctypes.data_as(_ip), _p(self._bodies))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().d2o_scene_free(self.h)
+            self.h = None
+
+    def bodies(self):
+        o = np.zeros((self.n_bodies, 40))
+        lib().d2o_scene_get_bodies(self.h, _p(o))
+        return o
+
+    def _result(self, n_pairs, n, per_pair):
+        off = np.zeros(n_pairs + 1, dtype=np.int32)
+        ints = np.zeros((n, 8), dtype=np.int32)
+        reals = np.zeros((n, 11))
+        pp = np.zeros(per_pair)
+        lib().d2o_scene_result(self.h, off.ctypes.data_as(_ip), ints.ctypes.data_as(_ip), _p(reals), _p(pp))
+        return off, ints, reals, pp
+
+    def narrow(self, pairs, filter=True):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        n = lib().d2o_narrow(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), int(bool(filter)))
+        off, ints, reals, _ = self._result(pairs.shape[0], n, pairs.shape[0])
+        return off, ints, reals
+
+    def ccd(self, pairs, max_time):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        mt = _d(max_time)
+        n = lib().d2o_ccd(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), _p(mt))
+        off, ints, reals, pp = self._result(pairs.shape[0], n, 2 * pairs.shape[0])
+        return off, ints, reals, pp.reshape(-1, 2)
+
+    def filter_last(self, continuous):
+        """filterContacts (discrete) / filterContinuousContacts on the last result."""
+        n = lib().d2o_scene_filter_result(self.h, 1 if continuous else 0)
+        off, ints, reals, _ = self._result(1, n, 0)
+        return ints, reals
+
+    def cluster_step(self, pairs, dt):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        return lib().d2o_cluster_step(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), float(dt))
+
+    def time_pass(self, pairs, max_time, threads=1, repeats=5):
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        mt = _d(max_time)
+        cc = np.zeros(1)
+        t = lib().d2o_time_pass(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), _p(mt), threads, repeats, _p(cc))
+        return t, int(cc[0])
+
+
+def step_scale(reals, body_a, body_b, max_time):
+    """TOI -> step rule on explicit contact reals (n x 11)."""
+    r = _d(reals)
+    return lib().d2o_step_scale(_p(r), r.size // 11, _p(_d(body_a, 40)), _p(_d(body_b, 40)), float(max_time))
