@@ -1,0 +1,300 @@
+"""delta2_amd -- MI355X contact detection for Delta2 (HIP kernels behind a C ABI).
+
+Python view of ``include/d2hip.h``.  The library ``libd2hip.so`` is built
+in-tree by ``delta2_amd.build``; importing this package does not need a GPU,
+but every compute call goes through the HIP library and fails loudly when it
+is missing or no GPU is present -- there is no CPU fallback.
+
+Reference-named helpers mirror the reference's collision-detection calls:
+
+=============================================  =======================================================
+``compare_trees_full``                          collision::compareTreesFull (+ filterContacts)
+``compare_trees_full_continuous``               collision::compareTreesFullContinuous + step rule
+``fine_collision_clusters_with_time_step_selection``  separate_clusters.cpp:462-599 + selectTimeStep clamp
+=============================================  =======================================================
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libd2hip.so")
+
+D2_OK, D2_ERR_ARG, D2_ERR_HIP, D2_ERR_CAPACITY, D2_ERR_STATE, D2_ERR_RUNTIME = range(6)
+D2_FP64, D2_FP32 = 0, 1
+BODY_STRIDE = 40
+NODE_STRIDE = 8
+BUCKET_STRIDE = 88
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+class D2Error(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"d2hip error {code}: {msg}")
+        self.code = code
+
+
+class CapacityError(D2Error):
+    pass
+
+
+def lib():
+    """Loads libd2hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m delta2_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "d2_ctx_create": ([ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+        "d2_ctx_destroy": ([_vp], ctypes.c_int),
+        "d2_last_error": ([_vp], ctypes.c_char_p),
+        "d2_set_precision": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_set_capacity": ([_vp, ctypes.c_int64], ctypes.c_int),
+        "d2_tree_build": ([_dp, ctypes.c_int, _ip, ctypes.c_int, _u64p, ctypes.c_int, _ip, _dp,
+                           ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "d2_pcg_seed": ([ctypes.c_uint64, ctypes.c_uint64, _u64p], ctypes.c_int),
+        "d2_trees_upload": ([_vp, ctypes.c_int, _ip, _ip, _dp], ctypes.c_int),
+        "d2_bodies_upload": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
+        "d2_bodies_download": ([_vp, _dp], ctypes.c_int),
+        "d2_project_future": ([_vp, _dp], ctypes.c_int),
+        "d2_pairs_upload": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
+        "d2_narrow_run": ([_vp, ctypes.c_int, _i64p], ctypes.c_int),
+        "d2_ccd_run": ([_vp, _i64p], ctypes.c_int),
+        "d2_cluster_steps": ([_vp, _ip, ctypes.c_int, ctypes.c_float, _dp], ctypes.c_int),
+        "d2_result_contacts": ([_vp, _ip, _ip, _dp], ctypes.c_int),
+        "d2_result_pairs": ([_vp, _dp], ctypes.c_int),
+        "d2_narrow": ([_vp, ctypes.c_int, _ip, ctypes.c_int, _i64p], ctypes.c_int),
+        "d2_ccd": ([_vp, ctypes.c_int, _ip, _dp, _i64p], ctypes.c_int),
+        "d2_set_profiling": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_num_kernels": ([], ctypes.c_int),
+        "d2_kernel_name": ([ctypes.c_int], ctypes.c_char_p),
+        "d2_kernel_stats": ([_vp, ctypes.c_int, _i64p, _dp, _dp], ctypes.c_int),
+        "d2_reset_stats": ([_vp], ctypes.c_int),
+        "d2_sync": ([_vp], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names declared in include/d2hip.h (used by the export test)."""
+    import re
+    hdr = os.path.join(_HERE, "..", "include", "d2hip.h")
+    with open(hdr) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(d2_[a-z_0-9]+)\s*\(", text)))
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def pcg_seed(initstate=42, initseq=54):
+    """PCG32 state {state, inc} as pcg32_srandom_r(initstate, initseq)."""
+    st = np.zeros(2, dtype=np.uint64)
+    lib().d2_pcg_seed(initstate, initseq, st.ctypes.data_as(_u64p))
+    return st
+
+
+def tree_build(V, F, rng):
+    """Surrogate tree of mesh (V, F) -- model::SurrogateTree(V, F, opt).
+
+    ``rng`` is the shared PCG32 state (uint64[2]); it advances in place.
+    Returns (nodes int32[n, 8], buckets float64[n, 88])."""
+    V = _f64(V).reshape(-1, 3)
+    F = _i32(F).reshape(-1, 3)
+    assert rng.dtype == np.uint64 and rng.size == 2
+    cap = max(2 * F.shape[0], 8)
+    nodes = np.zeros((cap, NODE_STRIDE), dtype=np.int32)
+    buckets = np.zeros((cap, BUCKET_STRIDE))
+    n = ctypes.c_int(0)
+    rc = lib().d2_tree_build(_dptr(V), V.shape[0], _iptr(F), F.shape[0], rng.ctypes.data_as(_u64p), cap,
+                             _iptr(nodes), _dptr(buckets), ctypes.byref(n))
+    if rc != D2_OK:
+        raise D2Error(rc, "d2_tree_build failed")
+    return nodes[: n.value].copy(), buckets[: n.value].copy()
+
+
+class Result:
+    """Contacts of one run in CSR order (offsets[n_pairs+1]) plus per-pair values."""
+
+    def __init__(self, offsets, ints, reals, per_pair=None):
+        self.offsets, self.ints, self.reals, self.per_pair = offsets, ints, reals, per_pair
+
+    @property
+    def n(self):
+        return int(self.offsets[-1]) if self.offsets.size else 0
+
+    def pair(self, p):
+        s, e = self.offsets[p], self.offsets[p + 1]
+        return self.ints[s:e], self.reals[s:e]
+
+
+class Context:
+    """One device context (a HIP stream and its buffers) -- include/d2hip.h."""
+
+    def __init__(self, device=0, precision=D2_FP64):
+        L = lib()
+        h = _vp()
+        rc = L.d2_ctx_create(device, precision, ctypes.byref(h))
+        if rc != D2_OK:
+            raise D2Error(rc, "d2_ctx_create failed (is a GPU visible?)")
+        self.h = h
+        self.n_pairs = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().d2_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != D2_OK:
+            msg = lib().d2_last_error(self.h)
+            msg = msg.decode() if msg else ""
+            cls = CapacityError if rc == D2_ERR_CAPACITY else D2Error
+            raise cls(rc, msg)
+
+    def set_precision(self, precision):
+        self._check(lib().d2_set_precision(self.h, precision))
+
+    def set_capacity(self, entries):
+        self._check(lib().d2_set_capacity(self.h, int(entries)))
+
+    def upload_trees(self, trees):
+        """trees: list of (nodes, buckets)."""
+        offs = np.zeros(len(trees) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([t[0].shape[0] for t in trees])
+        nodes = _i32(np.concatenate([t[0] for t in trees]))
+        buckets = _f64(np.concatenate([t[1] for t in trees]))
+        self._check(lib().d2_trees_upload(self.h, len(trees), _iptr(offs), _iptr(nodes), _dptr(buckets)))
+
+    def upload_bodies(self, tree_of_body, records):
+        tob = _i32(tree_of_body)
+        rec = _f64(records)
+        self._check(lib().d2_bodies_upload(self.h, tob.size, _iptr(tob), _dptr(rec)))
+        self.n_bodies = tob.size
+
+    def download_bodies(self):
+        out = np.zeros((self.n_bodies, BODY_STRIDE))
+        self._check(lib().d2_bodies_download(self.h, _dptr(out)))
+        return out
+
+    def project_future(self, step_per_body):
+        st = _f64(step_per_body)
+        self._check(lib().d2_project_future(self.h, _dptr(st)))
+
+    def upload_pairs(self, pairs, max_time=None):
+        pairs = _i32(pairs).reshape(-1, 2)
+        mt = None if max_time is None else _f64(max_time)
+        self._check(lib().d2_pairs_upload(self.h, pairs.shape[0], _iptr(pairs), None if mt is None else _dptr(mt)))
+        self.n_pairs = pairs.shape[0]
+
+    def narrow_run(self, filter=True):
+        n = ctypes.c_int64(0)
+        self._check(lib().d2_narrow_run(self.h, int(bool(filter)), ctypes.byref(n)))
+        return n.value
+
+    def ccd_run(self):
+        n = ctypes.c_int64(0)
+        self._check(lib().d2_ccd_run(self.h, ctypes.byref(n)))
+        return n.value
+
+    def result(self, with_pairs=False):
+        off = np.zeros(self.n_pairs + 1, dtype=np.int32)
+        self._check(lib().d2_result_contacts(self.h, _iptr(off), None, None))
+        n = int(off[-1])
+        ints = np.zeros((n, 8), dtype=np.int32)
+        reals = np.zeros((n, 11))
+        self._check(lib().d2_result_contacts(self.h, None, _iptr(ints), _dptr(reals)))
+        per = None
+        if with_pairs:
+            per = np.zeros((self.n_pairs, 2))
+            self._check(lib().d2_result_pairs(self.h, _dptr(per)))
+        return Result(off, ints, reals, per)
+
+    def cluster_steps(self, cluster_of_pair, n_clusters, dt):
+        cop = _i32(cluster_of_pair)
+        out = np.zeros(n_clusters)
+        self._check(lib().d2_cluster_steps(self.h, _iptr(cop), int(n_clusters), float(dt), _dptr(out)))
+        return out
+
+    def set_profiling(self, on=True):
+        self._check(lib().d2_set_profiling(self.h, int(bool(on))))
+
+    def reset_stats(self):
+        self._check(lib().d2_reset_stats(self.h))
+
+    def kernel_stats(self):
+        L = lib()
+        out = {}
+        for i in range(L.d2_num_kernels()):
+            n = ctypes.c_int64(0)
+            ms = ctypes.c_double(0)
+            items = ctypes.c_double(0)
+            self._check(L.d2_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(items)))
+            out[L.d2_kernel_name(i).decode()] = (n.value, ms.value, items.value)
+        return out
+
+    def sync(self):
+        self._check(lib().d2_sync(self.h))
+
+
+# --- reference-named mirrors -------------------------------------------------
+def compare_trees_full(ctx, pairs, filter=True):
+    """compareTreesFull (+ filterContacts(.., 0.0)) for every pair, on the GPU."""
+    ctx.upload_pairs(pairs)
+    ctx.narrow_run(filter)
+    return ctx.result()
+
+
+def compare_trees_full_continuous(ctx, pairs, max_time):
+    """compareTreesFullContinuous + TOI->step rule for every pair, on the GPU.
+
+    Result.per_pair[:, 0] is the pair's step scale, [:, 1] its min_time_out."""
+    ctx.upload_pairs(pairs, max_time)
+    ctx.ccd_run()
+    return ctx.result(with_pairs=True)
+
+
+def fine_collision_clusters_with_time_step_selection(ctx, pairs, cluster_of_pair, n_clusters, last_step, dt):
+    """Per-cluster step after the CCD (separate_clusters.cpp:462-599 + clamp)."""
+    pairs = _i32(pairs).reshape(-1, 2)
+    last_step = np.asarray(last_step, dtype=np.float64)
+    mt = np.minimum(last_step[pairs[:, 0]], last_step[pairs[:, 1]])
+    ctx.upload_pairs(pairs, mt)
+    ctx.ccd_run()
+    return ctx.cluster_steps(cluster_of_pair, n_clusters, dt)

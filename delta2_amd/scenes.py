@@ -1,0 +1,267 @@
+"""Synthetic scenes for the contact-detection path (inputs only).
+
+Builds the meshes, body records, surrogate trees and broad-phase pair lists of
+the workloads the path is measured on.  Trees come from the library's own
+``tree_build`` (the reference builds them in ``MeshData``); everything else
+is plain numpy/scipy input generation.
+
+Conventions (stated once, used everywhere):
+  * the tree-building PCG32 is seeded (42, 54) and shared by all meshes in
+    body order, as one ``Options`` object is in the reference
+    (common/cli_options.h:46, runner/main.cpp:38);
+  * scene randomness uses numpy's PCG64 seeded 1000 + workload number;
+  * geo_eps = 0.01 (common/cli_options.h:41), density 1, angular momentum 0,
+    time step dt = float(1/60) (Options::time_step_size is a float);
+  * velocities "N(0, s)" are normal with standard deviation s per axis;
+  * trees and bounding radii are built from the un-centred vertices, the
+    inertia about the centre of mass (model/mesh.cpp:63-73,87-94).
+"""
+import math
+
+import numpy as np
+
+from . import pcg_seed, tree_build
+
+DT = float(np.float32(1.0 / 60.0))
+GEO_EPS = 0.01
+
+
+# --- meshes -----------------------------------------------------------------
+def icosphere(subdiv, radius=1.0):
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    V = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    F = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    V = [np.array(v, dtype=float) / np.linalg.norm(v) for v in V]
+    for _ in range(subdiv):
+        mid = {}
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = V[a] + V[b]
+                V.append(p / np.linalg.norm(p))
+                mid[k] = len(V) - 1
+            return mid[k]
+
+        F2 = []
+        for a, b, c in F:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            F2 += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        F = F2
+    return np.array(V) * radius, np.array(F, dtype=np.int32)
+
+
+def convex_hull_mesh(points):
+    """Outward-oriented triangle mesh of the convex hull (Qhull's quickhull)."""
+    from scipy.spatial import ConvexHull
+    h = ConvexHull(points)
+    used = np.unique(h.simplices)
+    remap = -np.ones(points.shape[0], dtype=np.int64)
+    remap[used] = np.arange(used.size)
+    V = points[used]
+    F = remap[h.simplices]
+    c = V.mean(axis=0)
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F[:, 0]] - c) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F.astype(np.int32)
+
+
+def ellipsoid_hull(rng, n_points=64, lo=0.3, hi=1.2):
+    axes = rng.uniform(lo, hi, size=3)
+    d = rng.normal(size=(n_points, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return convex_hull_mesh(d * axes)
+
+
+def mass_properties(V, F):
+    """Volume, centre of mass and unit-density inertia about it (divergence theorem
+    over the tetrahedra (0, a, b, c)); replaces the reference's tetgen route
+    (model/mesh.cpp:96-134), which is absent here."""
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    det = np.einsum("ij,ij->i", a, np.cross(b, c))
+    vol = det.sum() / 6.0
+    com = (det[:, None] * (a + b + c)).sum(axis=0) / (24.0 * vol)
+    canon = np.array([[2.0, 1, 1], [1, 2, 1], [1, 1, 2]]) / 120.0
+    A = np.stack([a, b, c], axis=2)  # columns a, b, c
+    C = np.einsum("n,nij,jk,nlk->il", det, A, canon, A)
+    C = C - vol * np.outer(com, com)
+    inertia = np.trace(C) * np.eye(3) - C
+    return vol, com, inertia
+
+
+def random_rotation(rng):
+    """Uniform unit quaternion (w, x, y, z), Shoemake's method."""
+    u1, u2, u3 = rng.uniform(size=3)
+    q = np.array([math.sqrt(u1) * math.cos(2 * math.pi * u3), math.sqrt(1 - u1) * math.sin(2 * math.pi * u2),
+                  math.sqrt(1 - u1) * math.cos(2 * math.pi * u2), math.sqrt(u1) * math.sin(2 * math.pi * u3)])
+    return q
+
+
+# --- scenes -----------------------------------------------------------------
+class Scene:
+    """Bodies (records, 40 doubles each), their trees, and a broad-phase pair list."""
+
+    def __init__(self, name, meshes, trees, records, radii, scene_of_body=None):
+        self.name = name
+        self.meshes = meshes
+        self.trees = trees
+        self.records = records
+        self.radii = radii
+        self.tree_of_body = np.arange(len(trees), dtype=np.int32)
+        self.scene_of_body = np.zeros(len(trees), dtype=np.int32) if scene_of_body is None else scene_of_body
+        self.pairs = np.zeros((0, 2), dtype=np.int32)
+
+    @property
+    def n_bodies(self):
+        return self.records.shape[0]
+
+    def n_triangles(self):
+        return int(sum(F.shape[0] for _, F in self.meshes))
+
+
+def body_record(q, pos, vel, inv_inertia, geo_eps=GEO_EPS, static=False, last_step=DT):
+    r = np.zeros(40)
+    r[0:4] = q
+    r[4:7] = pos
+    r[7:10] = vel
+    r[14:28] = r[0:14]
+    r[28:37] = np.asarray(inv_inertia).ravel()
+    r[37] = geo_eps
+    r[38] = 1.0 if static else 0.0
+    r[39] = last_step
+    return r
+
+
+def _bodies(meshes, rng_scene, positions, vel_sigma, tree_rng):
+    trees, records, radii = [], [], []
+    for k, (V, F) in enumerate(meshes):
+        trees.append(tree_build(V, F, tree_rng))
+        vol, com, inertia = mass_properties(V, F)
+        inv = np.linalg.inv(inertia)  # density 1
+        q = random_rotation(rng_scene)
+        v = rng_scene.normal(0.0, vel_sigma, size=3)
+        records.append(body_record(q, positions[k], v, inv))
+        radii.append(float(np.max(np.linalg.norm(V, axis=1))))
+    return trees, np.array(records), np.array(radii)
+
+
+def poisson_disk(rng, radii, box, max_tries=2000):
+    """Random sequential placement of non-overlapping spheres (largest first)."""
+    n = len(radii)
+    order = np.argsort(-np.asarray(radii), kind="stable")
+    rmax = float(np.max(radii))
+    cell = 2.0 * rmax
+    nc = max(1, int(box / cell))
+    cell = box / nc
+    grid = {}
+    pos = np.zeros((n, 3))
+    for i in order:
+        r = radii[i]
+        for _ in range(max_tries):
+            p = rng.uniform(r, box - r, size=3)
+            ci = tuple(np.minimum((p / cell).astype(int), nc - 1))
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        for j in grid.get((ci[0] + dx, ci[1] + dy, ci[2] + dz), ()):
+                            if np.sum((pos[j] - p) ** 2) < (r + radii[j]) ** 2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pos[i] = p
+                grid.setdefault(ci, []).append(i)
+                break
+        else:
+            raise RuntimeError("poisson_disk: packing too dense for random placement")
+    return pos
+
+
+def broad_phase(records, radii, geo_eps=GEO_EPS, scene_of_body=None):
+    """Pairs whose bounding spheres (radius + geo_eps) meet within the step, by
+    collideFuncIndividualTime's rule (collision_detection/broad_phase_embree.cpp:
+    208-229 with continuous_comparison.h:12-36): t >= 0 and t < max(last steps).
+    Returns (a, b) with a < b, sorted; pairs never cross scenes."""
+    from scipy.spatial import cKDTree
+    pos = records[:, 4:7]
+    vel = records[:, 7:10]
+    last = records[:, 39]
+    R = radii + geo_eps
+    reach = 2.0 * R.max() + 2.0 * np.linalg.norm(vel, axis=1).max() * last.max() + 1e-9
+    cand = cKDTree(pos).query_pairs(reach, output_type="ndarray")
+    if cand.size == 0:
+        return np.zeros((0, 2), dtype=np.int32)
+    a, b = np.minimum(cand[:, 0], cand[:, 1]), np.maximum(cand[:, 0], cand[:, 1])
+    if scene_of_body is not None:
+        keep = scene_of_body[a] == scene_of_body[b]
+        a, b = a[keep], b[keep]
+    static = records[:, 38] != 0
+    keep = ~(static[a] & static[b])
+    a, b = a[keep], b[keep]
+    ab = pos[a] - pos[b]
+    rv = vel[a] - vel[b]
+    r = R[a] + R[b]
+    c = np.einsum("ij,ij->i", ab, ab) - r * r
+    qa = np.einsum("ij,ij->i", rv, rv)
+    qb = np.einsum("ij,ij->i", rv, ab)
+    d = qb * qb - qa * c
+    with np.errstate(invalid="ignore", divide="ignore"):
+        t = np.where(c < 0.0, 0.0, np.where(qb >= 0.0, -1.0, np.where(d < 0.0, -1.0, (-qb - np.sqrt(np.maximum(d, 0))) / qa)))
+    hit = (t >= 0.0) & (t < np.maximum(last[a], last[b]))
+    out = np.stack([a[hit], b[hit]], axis=1).astype(np.int32)
+    order = np.lexsort((out[:, 1], out[:, 0]))
+    return out[order]
+
+
+def scene_c1(seed=1001, n_bodies=100, subdiv=2, box=10.0):
+    """100 subdivision-2 icospheres scaled U(0.5,1.5), random rotations,
+    positions U([0,box]^3), velocities N(0,1)."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    for _ in range(n_bodies):
+        V, F = icosphere(subdiv, rng.uniform(0.5, 1.5))
+        meshes.append((V, F))
+    pos = rng.uniform(0.0, box, size=(n_bodies, 3))
+    trees, rec, radii = _bodies(meshes, rng, pos, 1.0, pcg_seed(42, 54))
+    s = Scene("c1", meshes, trees, rec, radii)
+    s.pairs = broad_phase(rec, radii)
+    return s
+
+
+def scene_c2(seed=1002, n_bodies=4096, packing=0.30, vel_sigma=2.0):
+    """4096 bodies: half subdivision-3 icospheres scaled U(0.5,1.5), half convex
+    hulls of 64 points on ellipsoids with axes U(0.3,1.2); Poisson-disk
+    placement of the bounding spheres at 30% packing; velocities N(0,2)."""
+    rng = np.random.default_rng(seed)
+    kinds = rng.permutation(np.arange(n_bodies) % 2)
+    meshes = []
+    for k in kinds:
+        meshes.append(icosphere(3, rng.uniform(0.5, 1.5)) if k == 0 else ellipsoid_hull(rng))
+    radii0 = np.array([np.max(np.linalg.norm(V, axis=1)) for V, _ in meshes])
+    box = (np.sum(4.0 / 3.0 * np.pi * radii0 ** 3) / packing) ** (1.0 / 3.0)
+    pos = poisson_disk(rng, radii0, box)
+    trees, rec, radii = _bodies(meshes, rng, pos, vel_sigma, pcg_seed(42, 54))
+    s = Scene("c2", meshes, trees, rec, radii)
+    s.box = box
+    s.pairs = broad_phase(rec, radii)
+    return s
+
+
+def advance(records, dt=DT):
+    """Ballistic drift of the current state by dt (no contact response: that is
+    the solver's job, outside this path)."""
+    r = records.copy()
+    r[:, 4:7] += dt * r[:, 7:10]
+    r[:, 13] += dt
+    r[:, 14:28] = r[:, 0:14]
+    return r

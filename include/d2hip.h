@@ -1,0 +1,162 @@
+/*
+ * d2hip.h -- C ABI of the MI355X contact-detection library (libd2hip.so).
+ *
+ * This is the drop-in boundary for Delta2's data-parallel contact-detection
+ * path: the surrogate-tree discrete narrow phase, the linear-interpolation
+ * CCD with its TOI -> step rule, and contact generation over broad-phase
+ * pair lists.  Every entry point names the reference interface it replaces
+ * (paths relative to the reference's src/).  Plain pointers and sizes only;
+ * no exceptions cross the ABI; every call returns a status code (D2_OK == 0)
+ * and d2_last_error() holds the message of the last failure.
+ *
+ * Flat record formats (all little-endian, densely packed):
+ *
+ *   body record, 40 doubles (model/particle.h:9-64, model/state.h:6-36)
+ *     [0..13]  current state: rotation quaternion w,x,y,z | translation xyz |
+ *              velocity xyz | angular momentum xyz | time
+ *     [14..27] future state, same layout
+ *     [28..36] body-frame inverse inertia, 3x3 row-major (already / density)
+ *     [37] geo_eps   [38] is_static (0/1)   [39] last_time_step_size
+ *
+ *   node record, 8 int32 (model/surrogate_components.h:11-20)
+ *     node_id, bucket_id, is_inner, depth, parent_id, child_id_first,
+ *     num_children, num_tris (of the node's bucket)
+ *
+ *   bucket record, 88 doubles, structure of arrays over the 8 slots
+ *     (model/surrogate_components.h:66-88)
+ *     Ax[8] Ay[8] Az[8] Bx[8] By[8] Bz[8] Cx[8] Cy[8] Cz[8] eps[8] inner_eps[8]
+ *     Node i of a tree owns bucket i; unused slots are zero.
+ *
+ *   contact, 8 int32 + 11 doubles (collision_detection/contact.h:9-33,96-108)
+ *     ints : pair, node_a, node_b, kind, ia, ib, level, 0
+ *            kind 0: triangle pair (ia of a, ib of b) of the node pair
+ *            kind 1: CCD leaf pair, vertex ia of a vs triangle ib of b
+ *            kind 2: CCD leaf pair, vertex ib of b vs triangle ia of a
+ *            kind 3: CCD leaf pair, edge ia of a vs edge ib of b
+ *     reals: A xyz, B xyz, eps_a, eps_b, eps_inner_a, eps_inner_b, toc
+ *   Contacts of a pair are returned in the reference's order (breadth-first
+ *   level, node-pair order, then triangle/sub-test order).
+ *
+ * Threading: one context per host thread; a context owns one HIP stream
+ * (default priority) and all its device buffers.  Outputs stay valid until
+ * the next call on the same context.
+ */
+#ifndef D2HIP_H
+#define D2HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum d2_status {
+  D2_OK = 0,
+  D2_ERR_ARG = 1,      /* invalid argument (reference: std::invalid_argument) */
+  D2_ERR_HIP = 2,      /* HIP runtime failure */
+  D2_ERR_CAPACITY = 3, /* a work list exceeded its capacity (see d2_set_capacity) */
+  D2_ERR_STATE = 4,    /* call order: e.g. run before upload */
+  D2_ERR_RUNTIME = 5   /* reference: std::runtime_error (e.g. no surrogate triangle) */
+};
+
+enum d2_precision {
+  D2_FP64 = 0, /* every test in FP64: identical results to the reference path */
+  D2_FP32 = 1  /* FP32 culling with error-inflated margins, FP64 for every kept result */
+};
+
+typedef struct d2_ctx d2_ctx;
+
+/* ---- context ----------------------------------------------------------- */
+int d2_ctx_create(int device, int precision, d2_ctx** out);
+int d2_ctx_destroy(d2_ctx* ctx);
+const char* d2_last_error(const d2_ctx* ctx);
+int d2_set_precision(d2_ctx* ctx, int precision);
+/* Capacity of the per-level work lists, in entries (default 1<<22). */
+int d2_set_capacity(d2_ctx* ctx, int64_t entries);
+
+/* ---- surrogate trees ---------------------------------------------------- */
+/* Builds one surrogate tree on the host from a triangle mesh, as
+ * model::SurrogateTree(V, F, opt) does (model/surrogate_tree.cpp:9-18,39-112,
+ * model/construct_surrogate_tree.h:8-185, model/fit_surrogate_triangle.h:246-298).
+ * rng is the PCG32 state {state, inc} of the shared Options generator
+ * (common/cli_options.h:46); it is advanced in place.  Writes at most
+ * max_nodes node/bucket records and the node count to *n_nodes_out. */
+int d2_tree_build(const double* V, int n_vertices, const int32_t* F, int n_faces, uint64_t* rng,
+                  int max_nodes, int32_t* nodes_out, double* buckets_out, int* n_nodes_out);
+/* PCG32 seeding as pcg32_srandom_r (common/pcg_basic.cpp:42-49). */
+int d2_pcg_seed(uint64_t initstate, uint64_t initseq, uint64_t* rng_out);
+
+/* Uploads n_trees concatenated trees; tree t owns node records
+ * [tree_node_offset[t], tree_node_offset[t+1]).  Replaces MeshData storage
+ * read through SurrogateTree::getNode/getBucket (model/surrogate_tree.cpp:20-27). */
+int d2_trees_upload(d2_ctx* ctx, int n_trees, const int32_t* tree_node_offset, const int32_t* nodes,
+                    const double* buckets);
+
+/* ---- bodies ------------------------------------------------------------- */
+/* Uploads n body records; tree_of_body[i] selects the tree of body i.
+ * Replaces the Particle fields the path reads (model/particle.h:9-64). */
+int d2_bodies_upload(d2_ctx* ctx, int n_bodies, const int32_t* tree_of_body, const double* records);
+int d2_bodies_download(d2_ctx* ctx, double* records);
+/* future = current.extrapolate(step[i]) for every non-null body
+ * (Particle::projectFutureState, model/particle.cpp:64-66; state.cpp:127-139). */
+int d2_project_future(d2_ctx* ctx, const double* step_per_body);
+
+/* ---- pair lists --------------------------------------------------------- */
+/* Uploads a broad-phase pair list (collision_detection/broad_phase.h:5-7:
+ * geometry ids of body a and b) with the CCD interval of each pair
+ * (separate_clusters.cpp:490: min of the two last_time_step_size; may be
+ * NULL for the discrete phase). */
+int d2_pairs_upload(d2_ctx* ctx, int n_pairs, const int32_t* pairs, const double* max_time);
+
+/* ---- the path ----------------------------------------------------------- */
+/* Discrete narrow phase over the resident pairs: per pair
+ * compareTreesFull<double,8,8>(a, b, ContactDetectionComparison)
+ * (collision_detection/full_tree_comparison.h:13-67, comparison_dist.h:205-240,
+ * strategies/contact_detection/contact_detection_comparison.cpp:12-14), then
+ * filterContacts(contacts, 0.0) when filter != 0 (contact.h:44-69), as
+ * strategies/PDE/PDE_explicit.cpp:87-88 does.  Uses the bodies' future
+ * states.  *n_contacts_out receives the total contact count. */
+int d2_narrow_run(d2_ctx* ctx, int filter, int64_t* n_contacts_out);
+
+/* CCD over the resident pairs: per pair
+ * compareTreesFullContinuous<double,8,8>(a, b, max_time[p], min_time)
+ * (full_tree_comparison.h:69-219, continuous_comparison.h:12-98,565-910) and
+ * the TOI -> step rule of separate_clusters.cpp:496-564.  Per pair results:
+ * step scale (min_scaling_seen, 1.0 when no contact) and min_time_out.
+ * The future state is projected per pair on the device; the bodies'
+ * stored future states are not modified. */
+int d2_ccd_run(d2_ctx* ctx, int64_t* n_contacts_out);
+
+/* Per-cluster step of fineCollisionClustersWithTimeStepSelection + the
+ * clamp of TimeStepSelectionDynamicContinuous::selectTimeStep
+ * (separate_clusters.cpp:462-599, time_step_selection_dynamic_continuous.cpp:17-21)
+ * from the last d2_ccd_run.  cluster_of_pair[p] in [0, n_clusters); dt is
+ * Options::time_step_size (a float, common/cli_options.h:11). */
+int d2_cluster_steps(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_clusters, float dt, double* cluster_step_out);
+
+/* Results of the last run: CSR offsets (n_pairs+1), contact records, and
+ * per-pair values (CCD: scale, min_time_out; discrete: unused).
+ * Any pointer may be NULL. */
+int d2_result_contacts(d2_ctx* ctx, int32_t* offsets, int32_t* ints, double* reals);
+int d2_result_pairs(d2_ctx* ctx, double* per_pair);
+
+/* Convenience: upload pairs and run, as one call (host buffers in/out). */
+int d2_narrow(d2_ctx* ctx, int n_pairs, const int32_t* pairs, int filter, int64_t* n_contacts_out);
+int d2_ccd(d2_ctx* ctx, int n_pairs, const int32_t* pairs, const double* max_time, int64_t* n_contacts_out);
+
+/* ---- instrumentation ---------------------------------------------------- */
+/* When on, every kernel launch is bracketed by HIP events on the context's
+ * stream; d2_kernel_stats returns per-kernel launch counts and summed
+ * milliseconds (index order: d2_kernel_name(i)). */
+int d2_set_profiling(d2_ctx* ctx, int on);
+int d2_num_kernels(void);
+const char* d2_kernel_name(int i);
+int d2_kernel_stats(d2_ctx* ctx, int i, int64_t* launches, double* total_ms, double* items);
+int d2_reset_stats(d2_ctx* ctx);
+/* Synchronise the context's stream. */
+int d2_sync(d2_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* D2HIP_H */

@@ -77,7 +77,11 @@ inline void divideAndFit(const std::vector<T>& tris, const std::vector<int>& tri
       pts.push_back(tris[id].B);
       pts.push_back(tris[id].C);
     }
-    T sel(V(0, 0, 0), V(0, 0, 0), V(0, 0, 0));
+    // The reference passes an uninitialised triangle here
+    // (construct_surrogate_tree.h:64-67); when no random trial has positive
+    // area its value is undefined.  Both this oracle and the library keep the
+    // group's previous surrogate in that case.
+    T sel = tris_out[g];
     double e;
     selectSurrogateTriangle(pts, rng, sel, e);
     tris_out[g] = sel;

@@ -1,0 +1,129 @@
+// CPU build of the device geometry (delta2_amd/csrc/d2_geom.hpp is
+// __host__ __device__) checked bit-for-bit against the oracle on random
+// inputs.  Built and run by tests/test_geom_cpu.py; prints one line per
+// primitive: "<name> <cases> <mismatches>".
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#define D2_HD inline
+#include <algorithm>
+using std::sqrt;
+using std::fabs;
+using std::sin;
+using std::cos;
+#include "../delta2_amd/csrc/d2_geom.hpp"
+#include "../oracle/d2o_tree.hpp"
+
+static std::mt19937_64 rng(12345);
+static double U(double a, double b) { return std::uniform_real_distribution<double>(a, b)(rng); }
+
+static bool same_bits(double a, double b) { return std::memcmp(&a, &b, 8) == 0 || (a != a && b != b); }
+static bool same_v(d2::v3<double> a, d2o::V3<double> b) { return same_bits(a.x, b.x) && same_bits(a.y, b.y) && same_bits(a.z, b.z); }
+
+static void rand_pose(double* m, double tscale, double rscale) {
+  d2o::V3<double> e(U(-rscale, rscale), U(-rscale, rscale), U(-rscale, rscale)), t(U(-tscale, tscale), U(-tscale, tscale), U(-tscale, tscale));
+  d2o::M4<double> M = d2o::transformationMatrixEuler(e, t);
+  std::memcpy(m, M.m, sizeof(M.m));
+}
+static d2o::Tri<double> rand_tri(double s, double off) {
+  return d2o::Tri<double>(d2o::V3<double>(U(-s, s) + off, U(-s, s), U(-s, s)), d2o::V3<double>(U(-s, s) + off, U(-s, s), U(-s, s)),
+                          d2o::V3<double>(U(-s, s) + off, U(-s, s), U(-s, s)));
+}
+static d2::tri<double> cvt(const d2o::Tri<double>& t) {
+  return {{t.A.x, t.A.y, t.A.z}, {t.B.x, t.B.y, t.B.z}, {t.C.x, t.C.y, t.C.z}};
+}
+static d2o::M4<double> m4(const double* m) {
+  d2o::M4<double> M;
+  std::memcpy(M.m, m, sizeof(M.m));
+  return M;
+}
+
+int main() {
+  const int N = 20000;
+  // tri-tri closest points
+  int bad = 0;
+  for (int k = 0; k < N; ++k) {
+    d2o::Tri<double> a = rand_tri(1.0, 0.0), b = rand_tri(1.0, U(-1.5, 1.5));
+    d2o::V3<double> P, Q;
+    double d0 = d2o::comparisonDist(a, b, P, Q);
+    d2::v3<double> p, q;
+    double d1 = d2::tri_tri_dist(cvt(a), cvt(b), p, q);
+    if (!same_bits(d0, d1) || !same_v(p, P) || !same_v(q, Q)) ++bad;
+  }
+  printf("tri_tri_dist %d %d\n", N, bad);
+  // point-triangle CCD with precomputed frame changes
+  bad = 0;
+  for (int k = 0; k < N; ++k) {
+    double Ta0[16], Ta1[16], Tb0[16], Tb1[16], inv[16], M0[16], M1[16];
+    rand_pose(Ta0, 1.0, 3.0);
+    rand_pose(Ta1, 1.0, 3.0);
+    rand_pose(Tb0, 1.0, 3.0);
+    rand_pose(Tb1, 1.0, 3.0);
+    d2::inv44(Tb0, inv);
+    d2::mul44(inv, Ta0, M0);
+    d2::inv44(Tb1, inv);
+    d2::mul44(inv, Ta1, M1);
+    d2o::Tri<double> tb = rand_tri(1.0, 0.0);
+    d2o::V3<double> pt(U(-1, 1), U(-1, 1), U(-1, 1));
+    d2o::V3<double> P, Q;
+    double toc0;
+    double d0 = d2o::pointTriCCDLinear(pt, m4(Ta0), m4(Ta1), tb, m4(Tb0), m4(Tb1), P, Q, toc0);
+    d2::v3<double> p{0, 0, 0}, q{0, 0, 0};
+    double toc1;
+    double d1 = d2::point_tri(d2::v3<double>{pt.x, pt.y, pt.z}, M0, M1, cvt(tb), Tb0, Tb1, p, q, toc1);
+    bool ok = same_bits(d0, d1) && same_bits(toc0, toc1);
+    if (ok && d0 != std::numeric_limits<double>::infinity()) ok = same_v(p, P) && same_v(q, Q);
+    if (!ok) ++bad;
+  }
+  printf("point_tri %d %d\n", N, bad);
+  // edge-edge CCD
+  bad = 0;
+  int bisect = 0;
+  for (int k = 0; k < N; ++k) {
+    d2o::V3<double> v[8];
+    for (int i = 0; i < 8; ++i) v[i] = d2o::V3<double>(U(-1, 1), U(-1, 1), U(-1, 1));
+    d2o::Edge<double> as(v[0], v[1]), ae(v[2], v[3]), bs(v[4], v[5]), be(v[6], v[7]);
+    d2o::V3<double> P, Q;
+    double toc0 = -1;
+    double d0 = d2o::edgeEdgeCCD(as, ae, bs, be, P, Q, toc0);
+    auto c = [](const d2o::V3<double>& x) { return d2::v3<double>{x.x, x.y, x.z}; };
+    d2::v3<double> p, q;
+    double toc1 = -1;
+    double d1 = d2::edge_edge(c(v[0]), c(v[1]), c(v[2]), c(v[3]), c(v[4]), c(v[5]), c(v[6]), c(v[7]), p, q, toc1);
+    if (!same_bits(d0, d1) || !same_bits(toc0, toc1) || !same_v(p, P) || !same_v(q, Q)) ++bad;
+    d2o::V3<double> ns = d2o::cross(as.B - as.A, bs.A - as.A), ne = d2o::cross(ae.B - ae.A, be.A - ae.A);
+    if (d2o::dot(bs.B - as.A, ns) * d2o::dot(be.B - ae.A, ne) < 0) ++bisect;
+  }
+  printf("edge_edge %d %d (bisection cases %d)\n", N, bad, bisect);
+  // state extrapolation (host libm on both sides here)
+  bad = 0;
+  for (int k = 0; k < N; ++k) {
+    double r[14];
+    double q[4] = {U(-1, 1), U(-1, 1), U(-1, 1), U(-1, 1)};
+    double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int i = 0; i < 4; ++i) r[i] = q[i] / n;
+    for (int i = 4; i < 13; ++i) r[i] = U(-3, 3);
+    r[13] = 0;
+    double I[9] = {U(0.5, 2), 0, 0, 0, U(0.5, 2), 0, 0, 0, U(0.5, 2)};
+    I[1] = I[3] = U(-0.1, 0.1);
+    double dt = U(0, 0.1);
+    d2::body_state f = d2::extrapolate(d2::load_state(r), dt, I);
+    d2o::State s;
+    s.rot = {r[0], r[1], r[2], r[3]};
+    s.trans = d2o::V3<double>(r[4], r[5], r[6]);
+    s.vel = d2o::V3<double>(r[7], r[8], r[9]);
+    s.L = d2o::V3<double>(r[10], r[11], r[12]);
+    d2o::M3<double> Im;
+    std::memcpy(Im.m, I, sizeof(I));
+    d2o::State g = s.extrapolate(dt, Im);
+    double pm[16];
+    d2::pose(f.q, f.t, pm);
+    d2o::M4<double> gm = g.transformation();
+    bool ok = true;
+    for (int i = 0; i < 16; ++i) ok = ok && same_bits(pm[i], gm.m[i]);
+    if (!ok) ++bad;
+  }
+  printf("extrapolate %d %d\n", N, bad);
+  return 0;
+}

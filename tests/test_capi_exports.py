@@ -1,0 +1,45 @@
+"""The C-ABI library loads without a GPU and exports every symbol of include/d2hip.h."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import delta2_amd as d2
+
+
+def test_library_exports_every_declared_symbol():
+    L = d2.lib()
+    names = d2.exported_symbols()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_kernel_names():
+    L = d2.lib()
+    names = [L.d2_kernel_name(i).decode() for i in range(L.d2_num_kernels())]
+    assert "soup_edge" in names and "tri_dist" in names
+
+
+def test_null_context_is_rejected():
+    L = d2.lib()
+    assert L.d2_set_profiling(None, 1) == d2.D2_ERR_ARG
+    assert L.d2_sync(None) == d2.D2_ERR_ARG
+
+
+def test_tree_build_rejects_bad_faces():
+    V = np.zeros((3, 3))
+    with pytest.raises(d2.D2Error):
+        d2.tree_build(V, [[0, 1, 5]], d2.pcg_seed())
+
+
+def test_tree_build_small_output_keeps_rng():
+    from delta2_amd import scenes
+    V, F = scenes.icosphere(1)
+    rng = d2.pcg_seed()
+    before = rng.copy()
+    n = ctypes.c_int(0)
+    rc = d2.lib().d2_tree_build(d2._dptr(np.ascontiguousarray(V)), V.shape[0], d2._iptr(np.ascontiguousarray(F)),
+                                F.shape[0], rng.ctypes.data_as(d2._u64p), 1, None, None, ctypes.byref(n))
+    assert rc == d2.D2_ERR_CAPACITY and n.value > 1
+    assert (rng == before).all()

@@ -1,0 +1,22 @@
+"""CPU build of the device geometry checked bit-for-bit against the oracle.
+
+d2_geom.hpp is the arithmetic of every kernel; compiling it for the host and
+comparing with the oracle on random inputs pins the kernels' FP64 expression
+order without a GPU (tri-tri distance, point-triangle CCD with the per-pair
+frame changes, edge-edge CCD incl. bisection cases, state extrapolation)."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_device_geometry_matches_oracle_bitwise(tmp_path):
+    exe = tmp_path / "cpu_geom_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", os.path.join(ROOT, "tests", "cpu_geom_check.cpp"),
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    rows = {l.split()[0]: l.split() for l in out.strip().splitlines()}
+    for name in ("tri_tri_dist", "point_tri", "edge_edge", "extrapolate"):
+        assert int(rows[name][2]) == 0, out
+        assert int(rows[name][1]) >= 10000
+    assert "bisection cases" in out

@@ -1,0 +1,156 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+FP64 is required to reproduce the oracle: identical contact index tuples in
+identical order, reals within 1e-9 relative (|x - y| <= 1e-9 * max(|y|, 1)),
+identical per-pair step scales and min_time_out to the same tolerance.
+"""
+import numpy as np
+import pytest
+
+import d2o
+import delta2_amd as d2
+from gpu_cases import c1_scene, dense_scene, hull_scene, oracle_scene, with_future
+from delta2_amd import scenes
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def close(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b) <= TOL * np.maximum(np.abs(b), 1.0)
+
+
+def assert_same(gpu, ora_off, ora_ints, ora_reals, label):
+    assert np.array_equal(gpu.offsets, ora_off), f"{label}: per-pair contact counts differ"
+    assert np.array_equal(gpu.ints, ora_ints), f"{label}: contact index tuples differ"
+    ok = close(gpu.reals, ora_reals)
+    assert ok.all(), f"{label}: {np.count_nonzero(~ok)} reals beyond 1e-9 (max abs diff {np.abs(gpu.reals - ora_reals).max()})"
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = d2.Context(0, d2.D2_FP64)
+    yield c
+    c.close()
+
+
+def load(ctx, s, records):
+    ctx.upload_trees(s.trees)
+    ctx.upload_bodies(s.tree_of_body, records)
+
+
+SCENES = [("c1", c1_scene), ("dense", dense_scene), ("hulls", hull_scene)]
+
+
+@pytest.mark.parametrize("name,make", SCENES)
+@pytest.mark.parametrize("filt", [False, True])
+def test_narrow_parity(ctx, name, make, filt):
+    s = make()
+    print(f"[narrow] scene={name} pairs={s.pairs.shape[0]} filter={filt}", flush=True)
+    rec = with_future(s.records, scenes.DT)
+    load(ctx, s, rec)
+    gpu = d2.compare_trees_full(ctx, s.pairs, filter=filt)
+    o_off, o_ints, o_reals = oracle_scene(s, rec).narrow(s.pairs, filter=filt)
+    assert_same(gpu, o_off, o_ints, o_reals, f"narrow {name} filter={filt}")
+    if name == "dense":
+        assert gpu.n > 100  # the dense scene must exercise many contacts
+
+
+@pytest.mark.parametrize("name,make", SCENES)
+def test_ccd_parity(ctx, name, make):
+    s = make()
+    print(f"[ccd] scene={name} pairs={s.pairs.shape[0]}", flush=True)
+    rec = s.records
+    load(ctx, s, rec)
+    mt = np.minimum(rec[s.pairs[:, 0], 39], rec[s.pairs[:, 1], 39]) if s.pairs.size else np.zeros(0)
+    gpu = d2.compare_trees_full_continuous(ctx, s.pairs, mt)
+    o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
+    assert_same(gpu, o_off, o_ints, o_reals, f"ccd {name}")
+    ok = close(gpu.per_pair, o_pp)
+    assert ok.all(), f"ccd {name}: per-pair scale/min_time differ at {np.argwhere(~ok)[:5]}"
+
+
+def test_ccd_long_interval(ctx):
+    """A 4x longer interval (max_time = 4 dt) reaches more leaf pairs and toc values."""
+    s = dense_scene()
+    rec = s.records
+    load(ctx, s, rec)
+    mt = np.full(s.pairs.shape[0], 4 * scenes.DT)
+    gpu = d2.compare_trees_full_continuous(ctx, s.pairs, mt)
+    o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
+    assert_same(gpu, o_off, o_ints, o_reals, "ccd dense 4dt")
+    assert close(gpu.per_pair, o_pp).all()
+    assert (o_ints[:, 3] == 3).any() and (o_ints[:, 3] == 1).any()  # edge and vertex contacts present
+
+
+def test_project_future(ctx):
+    s = dense_scene()
+    load(ctx, s, s.records)
+    steps = np.linspace(1e-3, 0.05, s.n_bodies)
+    ctx.project_future(steps)
+    got = ctx.download_bodies()
+    want = with_future(s.records, steps)
+    # sin/cos of the device and of glibc may differ in the last bit
+    assert np.allclose(got[:, 14:28], want[:, 14:28], rtol=0, atol=1e-14)
+
+
+def test_cluster_steps(ctx):
+    s = dense_scene()
+    rec = s.records
+    load(ctx, s, rec)
+    pairs = s.pairs
+    # two clusters: connected components would be the caller's job; any split works
+    cl = (pairs[:, 0] % 2).astype(np.int32)
+    mt = np.minimum(rec[pairs[:, 0], 39], rec[pairs[:, 1], 39])
+    ctx.upload_pairs(pairs, mt)
+    ctx.ccd_run()
+    got = ctx.cluster_steps(cl, 2, scenes.DT)
+    osc = oracle_scene(s, rec)
+    want = [osc.cluster_step(pairs[cl == k], np.float32(scenes.DT)) for k in range(2)]
+    assert np.allclose(got, want, rtol=1e-12, atol=0), (got, want)
+
+
+def test_reference_bucket_fixture(ctx):
+    """comparison_dist.h bucket test (test_comparison.cpp:90-128) through the GPU."""
+    from test_oracle_kat import PLANE_F, PLANE_V, PYRAMID_F, PYRAMID_V, body_record
+    rng = d2.pcg_seed(42, 54)
+    trees = [d2.tree_build(PYRAMID_V, PYRAMID_F, rng), d2.tree_build(PLANE_V, PLANE_F, rng)]
+    bodies = np.stack([body_record(fut_trans=(0, 0, 1.0), static=True), body_record(fut_trans=(0, 0, 1.105))])
+    ctx.upload_trees(trees)
+    ctx.upload_bodies([0, 1], bodies)
+    assert d2.compare_trees_full(ctx, [[0, 1]], filter=False).n == 8
+    assert d2.compare_trees_full(ctx, [[0, 1]], filter=True).n == 1
+    # test_continuous_comparison.cpp:222-263 : toc 0.9
+    bodies = np.stack([body_record(trans=(0, 0, 1.0), static=True), body_record(trans=(0, 0, 2.0), vel=(0, 0, -1.0))])
+    ctx.upload_bodies([0, 1], bodies)
+    r = d2.compare_trees_full_continuous(ctx, [[0, 1]], [1.0])
+    tocs = r.reals[:, 10]
+    assert r.n > 0 and np.isclose(tocs.min(), 0.9, rtol=1.2e-5)
+
+
+def test_empty_pair_list(ctx):
+    s = c1_scene()
+    load(ctx, s, s.records)
+    r = d2.compare_trees_full(ctx, np.zeros((0, 2), np.int32))
+    assert r.n == 0 and r.offsets.tolist() == [0]
+    r = d2.compare_trees_full_continuous(ctx, np.zeros((0, 2), np.int32), np.zeros(0))
+    assert r.n == 0
+
+
+def test_bad_arguments_raise(ctx):
+    s = c1_scene()
+    load(ctx, s, s.records)
+    with pytest.raises(d2.D2Error):
+        ctx.upload_pairs([[0, s.n_bodies]])  # body index out of range
+    with pytest.raises(d2.D2Error):
+        ctx.upload_pairs([[0, 1]], [-1.0])  # negative interval
+
+
+def test_capacity_error_is_reported():
+    s = dense_scene()
+    with d2.Context(0) as c:
+        c.set_capacity(1024)
+        load(c, s, with_future(s.records, 4 * scenes.DT))
+        with pytest.raises(d2.CapacityError):
+            d2.compare_trees_full(c, s.pairs, filter=False)
