@@ -1,0 +1,151 @@
+"""Benchmark: narrow+CCD pair queries/s on one MI355X.
+
+A step is one pass of the contact-detection path over the workload's
+broad-phase pair list with inputs resident in HBM:
+  CCD (compareTreesFullContinuous over [0, max_time]) + the TOI -> step rule,
+  then the discrete narrow phase (compareTreesFull) + filterContacts,
+for every pair.  value = pairs * steps / wall time of the timed steps.
+
+The workload is the 4096-body scene (2048 subdivision-3 icospheres scaled
+U(0.5,1.5) and 2048 convex hulls of 64 ellipsoid points, Poisson-disk placed
+at 30% packing of the bounding spheres, velocities N(0,2), dt = 1/60 s).
+
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "narrow+CCD pair queries/s, 1 GPU; learned vs plain speedup at equal TOI error"
+
+# Algorithmic HBM bytes per unit of work of each kernel (unit = the item the
+# kernel's launch counts): the geometry a unit must read plus what it writes.
+#   soup_edge / leaf_edge : 4 edge endpoints (96 B) + one 8-double result (64 B)
+#   soup_point/leaf_point : a point (24 B) + a triangle (72 B) + a result (64 B)
+#   tri_dist              : two triangles (144 B) + their two eps (16 B)
+#   soup_cull / tri_enum  : two 8-slot buckets (2 x 576 B) + two node records (64 B)
+#   soup_reduce           : 15 results (960 B) read
+ALG_BYTES = {"soup_edge": 160.0, "leaf_edge": 160.0, "soup_point": 160.0, "leaf_point": 160.0,
+             "tri_dist": 160.0, "soup_cull": 1216.0, "tri_enum": 1216.0, "soup_reduce": 960.0}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(scene, max_time, budget_s=12.0):
+    """The oracle (C++ restatement of the reference path) on the host cores,
+    on a bounded random sample of the same pairs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import d2o
+    d2o.lib()
+    osc = d2o.Scene(scene.trees, scene.tree_of_body, scene.records)
+    pairs = scene.pairs
+    rng = np.random.default_rng(0)
+    order = rng.permutation(pairs.shape[0])
+    # calibrate on a small sample, then size the sample to the budget
+    n = min(16, pairs.shape[0])
+    t1, _ = osc.time_pass(pairs[order[:n]], max_time[order[:n]], threads=1, repeats=1)
+    per_pair = max(t1 / n, 1e-6)
+    n = int(min(pairs.shape[0], max(n, budget_s / 3.0 / per_pair)))
+    sel = order[:n]
+    t1, _ = osc.time_pass(pairs[sel], max_time[sel], threads=1, repeats=3)
+    cores = len(os.sched_getaffinity(0))
+    tn, _ = osc.time_pass(pairs[sel], max_time[sel], threads=cores, repeats=3)
+    return {"value": n / tn, "unit": "pair queries/s", "cores": cores, "kind": "port",
+            "value_1_thread": n / t1,
+            "sample": f"{n} of {pairs.shape[0]} pairs drawn uniformly (seed 0), CCD+step rule+discrete+filter, "
+                      f"median of 3 passes; oracle built with g++ -O2 -ffp-contract=off"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is not sharded in this build",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    d2.lib()
+    t0 = time.time()
+    scene = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
+    log(f"[bench] scene {scene.name}: {scene.n_bodies} bodies, {scene.n_triangles()} triangles, "
+        f"{scene.pairs.shape[0]} pairs, built in {time.time() - t0:.1f}s")
+    pairs = scene.pairs
+    max_time = np.minimum(scene.records[pairs[:, 0], 39], scene.records[pairs[:, 1], 39])
+    prec = d2.D2_FP64 if args.precision == "fp64" else d2.D2_FP32
+    ctx = d2.Context(0, prec)
+    ctx.upload_trees(scene.trees)
+    ctx.upload_bodies(scene.tree_of_body, scene.records)
+    ctx.project_future(np.full(scene.n_bodies, scenes.DT))
+    ctx.upload_pairs(pairs, max_time)
+
+    def step():
+        ctx.ccd_run()
+        ctx.narrow_run(True)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    ts = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    te = time.perf_counter()
+    ctx.set_profiling(False)
+    stats = ctx.kernel_stats()
+    wall = te - ts
+    ms_per_step = 1e3 * wall / args.steps
+    value = pairs.shape[0] * args.steps / wall
+    log(f"[bench] {args.steps} steps in {wall:.3f}s = {ms_per_step:.3f} ms/step")
+    for k, (n, ms, items) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        if n:
+            log(f"[bench]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")
+
+    dom = max((k for k in stats if k in ALG_BYTES and stats[k][0]), key=lambda k: stats[k][1])
+    n, ms, items = stats[dom]
+    avg_s = ms / n / 1e3
+    bytes_per_launch = ALG_BYTES[dom] * items / n
+    achieved = bytes_per_launch / avg_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch}
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(scene, max_time)
+        log(f"[bench] cpu baseline in {time.time() - t0:.1f}s: {cpu['value']:.1f} pairs/s on {cpu['cores']} cores, "
+            f"{cpu['value_1_thread']:.1f} on 1")
+    out = {
+        "metric": METRIC, "value": value, "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == d2.D2_FP64 else "f32+f64", "data": "synthetic",
+        "config": {"workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
+                               "Poisson-disk 30% packing; v~N(0,2); dt=1/60; CCD+step rule then discrete+filter"
+                   if scene.name == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60",
+                   "bodies": scene.n_bodies, "triangles": scene.n_triangles(), "pairs": int(pairs.shape[0]),
+                   "precision": args.precision},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -166,6 +166,7 @@ class Context:
             raise D2Error(rc, "d2_ctx_create failed (is a GPU visible?)")
         self.h = h
         self.n_pairs = 0
+        self.n_out = 0
 
     def close(self):
         if getattr(self, "h", None):
@@ -226,17 +227,21 @@ class Context:
     def narrow_run(self, filter=True):
         n = ctypes.c_int64(0)
         self._check(lib().d2_narrow_run(self.h, int(bool(filter)), ctypes.byref(n)))
+        self.n_out = n.value
         return n.value
 
     def ccd_run(self):
         n = ctypes.c_int64(0)
         self._check(lib().d2_ccd_run(self.h, ctypes.byref(n)))
+        self.n_out = n.value
         return n.value
 
     def result(self, with_pairs=False):
         off = np.zeros(self.n_pairs + 1, dtype=np.int32)
         self._check(lib().d2_result_contacts(self.h, _iptr(off), None, None))
         n = int(off[-1])
+        if n != self.n_out:
+            raise D2Error(D2_ERR_STATE, f"CSR total {n} != run count {self.n_out}")
         ints = np.zeros((n, 8), dtype=np.int32)
         reals = np.zeros((n, 11))
         self._check(lib().d2_result_contacts(self.h, None, _iptr(ints), _dptr(reals)))

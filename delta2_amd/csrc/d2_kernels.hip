@@ -205,19 +205,6 @@ __global__ void k_roots(int n_pairs, NP* soup, unsigned long long* cnt, unsigned
   }
 }
 
-// End of a level: next lists become current, per-level counters reset.
-__global__ void k_advance(unsigned long long* cnt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    cnt[C_SOUP] = cnt[C_SOUP_NEXT];
-    cnt[C_CONN] = cnt[C_CONN_NEXT];
-    cnt[C_SOUP_NEXT] = 0;
-    cnt[C_CONN_NEXT] = 0;
-    cnt[C_ITEMS] = 0;
-    cnt[C_PT] = 0;
-    cnt[C_EE] = 0;
-  }
-}
-
 // Per pass: fold "min toc of every hit of this pass" into min_time when a
 // leaf-leaf node pair of the pair had hits (full_tree_comparison.h:106-108,167-169).
 __global__ void k_fold_min(int n_pairs, unsigned long long* pass_min, int* pass_flag, double* min_time,
@@ -376,41 +363,6 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D, long long first, lon
   }
 }
 
-// Children of hit node pairs (full_tree_comparison.h:26-62, 99-150):
-// advance the inner side, the shallower one when both are inner (b on ties).
-// CCD routes children to the soup or leaf-leaf list; the discrete phase has
-// one list.
-__global__ void k_expand(Dev D, int from_soup) {
-  const long long n = (long long)D.cnt[from_soup ? C_SOUP : C_CONN];
-  const NP* list = from_soup ? D.soup : D.conn;
-  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (long long)gridDim.x * blockDim.x) {
-    if (!D.np_hit[w]) continue;
-    NP np = list[w];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    if (!na.inner && !nb.inner) continue;
-    bool adv_a = na.inner && (!nb.inner || na.depth < nb.depth);
-    const NodeRec& par = adv_a ? na : nb;
-    for (int c = 0; c < par.nchild; ++c) {
-      uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
-      uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
-      NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-      bool to_soup = true;
-      if (!D.discrete) {
-        const NodeRec xa = D.nodes[pi[2] + ca];
-        const NodeRec xb = D.nodes[pi[3] + cb];
-        to_soup = xa.inner || xb.inner;
-      }
-      unsigned long long* ctr = &D.cnt[to_soup ? C_SOUP_NEXT : C_CONN_NEXT];
-      NP* dst = to_soup ? D.soup_next : D.conn_next;
-      unsigned long long slot = atomicAdd(ctr, 1ull);
-      if ((long long)slot < D.cap_np) dst[slot] = child;
-      else D.cnt[C_OVERFLOW] = 1;
-    }
-  }
-}
-
 // ---------------------------------------------------- CCD leaf-leaf level
 // Sub-test numbering of a leaf pair with n (a) and m (b) triangles, in the
 // reference's order (continuous_comparison.h:855-907):
@@ -522,30 +474,45 @@ __global__ void __launch_bounds__(256) k_leaf_edge(Dev D, long long first, long 
   }
 }
 
-// ------------------------------------------------------- discrete level
-__global__ void __launch_bounds__(256) k_tri_enum(Dev D) {
-  const long long n = (long long)D.cnt[C_SOUP];
+// ------------------------------------------- ordered enumeration/expansion
+// Items of a node pair are its triangle pairs (i < na, j < nb).  counts[w] =
+// na*nb, counts[n] = 0; an exclusive scan gives each node pair its slice.
+__global__ void k_np_items_count(Dev D, long long n, int* counts) {
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w <= n; w += (long long)gridDim.x * blockDim.x) {
+    if (w == n) {
+      counts[w] = 0;
+      continue;
+    }
+    NP np = D.soup[w];
+    const int32_t* pi = D.pair_info + 4 * np.pair;
+    counts[w] = D.nodes[pi[2] + np.na].ntris * D.nodes[pi[3] + np.nb].ntris;
+  }
+}
+
+// One wavefront per node pair; lane (i, j) writes item i*nb + j of the slice.
+__global__ void __launch_bounds__(256) k_tri_enum_ordered(Dev D, long long n, const int* offs) {
   const int lane = lane_id();
   const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
   for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
     NP np = D.soup[w];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     int na = D.nodes[pi[2] + np.na].ntris, nb = D.nodes[pi[3] + np.nb].ntris;
-    bool ok = (lane >> 3) < na && (lane & 7) < nb;
-    long long slot = wave_alloc(ok, &D.cnt[C_ITEMS]);
-    if (ok) {
-      if (slot < D.cap_items) D.items[slot] = Item{(uint32_t)w, (uint32_t)lane};
-      else D.cnt[C_OVERFLOW] = 1;
+    int i = lane >> 3, j = lane & 7;
+    long long base = offs[w];
+    if (i < na && j < nb && base + i * nb + j < D.cap_items) D.items[base + i * nb + j] = Item{(uint32_t)w, (uint32_t)lane};
+    if (lane == 0) {
+      D.np_hit[w] = 0;
+      if (base + na * nb > D.cap_items) D.cnt[C_OVERFLOW] = 1;
     }
-    if (lane == 0) D.np_hit[w] = 0;
   }
 }
 
-// _comparisonDist of one triangle pair in the future poses, hit test
-// (comparison_dist.h:207,225-235), contact emission for leaf-leaf node pairs.
-__global__ void __launch_bounds__(256) k_tri_dist(Dev D, long long first, long long count) {
+// Grid-stride over a device-resident item count (no host round trip).
+__global__ void __launch_bounds__(256) k_tri_dist_dev(Dev D, const int* n_items_dev) {
+  long long count = (long long)*n_items_dev;
+  if (count > D.cap_items) count = D.cap_items;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
-    Item item = D.items[first + t];
+    Item item = D.items[t];
     NP np = D.soup[item.np];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
@@ -574,6 +541,76 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D, long long first, long l
     c.r[0] = P.x, c.r[1] = P.y, c.r[2] = P.z, c.r[3] = Q.x, c.r[4] = Q.y, c.r[5] = Q.z;
     c.r[6] = pr[PR_GEOA] + ea, c.r[7] = pr[PR_GEOB] + eb, c.r[8] = ba[80 + i], c.r[9] = bb[80 + j], c.r[10] = 0.0;
     D.contacts[slot] = c;
+  }
+}
+
+// Children of hit node pairs (full_tree_comparison.h:26-62, 99-150), in two
+// passes: counts per destination list, then writes at scanned offsets, which
+// keeps every list in the reference's breadth-first order.
+__device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
+  return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
+}
+__global__ void k_expand_count(Dev D, long long n, int from_soup, int* cs, int* cc) {
+  const NP* list = from_soup ? D.soup : D.conn;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w <= n; w += (long long)gridDim.x * blockDim.x) {
+    int ns = 0, nc = 0;
+    if (w < n && D.np_hit[w]) {
+      NP np = list[w];
+      const int32_t* pi = D.pair_info + 4 * np.pair;
+      const NodeRec na = D.nodes[pi[2] + np.na];
+      const NodeRec nb = D.nodes[pi[3] + np.nb];
+      if (na.inner || nb.inner) {
+        bool adv_a = expand_side(na, nb);
+        const NodeRec& par = adv_a ? na : nb;
+        for (int c = 0; c < par.nchild; ++c) {
+          bool to_soup = true;
+          if (!D.discrete) {
+            const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
+            const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
+            to_soup = x.inner || y.inner;
+          }
+          ns += to_soup, nc += !to_soup;
+        }
+      }
+    }
+    cs[w] = ns;
+    cc[w] = nc;
+  }
+}
+__global__ void k_expand_write(Dev D, long long n, int from_soup, const int* os, const int* oc) {
+  const NP* list = from_soup ? D.soup : D.conn;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (long long)gridDim.x * blockDim.x) {
+    if (!D.np_hit[w]) continue;
+    NP np = list[w];
+    const int32_t* pi = D.pair_info + 4 * np.pair;
+    const NodeRec na = D.nodes[pi[2] + np.na];
+    const NodeRec nb = D.nodes[pi[3] + np.nb];
+    if (!na.inner && !nb.inner) continue;
+    bool adv_a = expand_side(na, nb);
+    const NodeRec& par = adv_a ? na : nb;
+    long long ps = os[w], pc = oc[w];
+    for (int c = 0; c < par.nchild; ++c) {
+      uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
+      uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
+      NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
+      bool to_soup = true;
+      if (!D.discrete) to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
+      long long slot = to_soup ? ps++ : pc++;
+      if (slot < D.cap_np) (to_soup ? D.soup_next : D.conn_next)[slot] = child;
+      else D.cnt[C_OVERFLOW] = 1;
+    }
+  }
+}
+// Next-level sizes from the scanned totals.
+__global__ void k_advance_scanned(unsigned long long* cnt, const int* os_total, const int* oc_total, int have_conn) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    cnt[C_SOUP] = os_total ? (unsigned long long)*os_total : 0ull;
+    cnt[C_CONN] = (have_conn && oc_total) ? (unsigned long long)*oc_total : 0ull;
+    cnt[C_SOUP_NEXT] = 0;
+    cnt[C_CONN_NEXT] = 0;
+    cnt[C_ITEMS] = 0;
+    cnt[C_PT] = 0;
+    cnt[C_EE] = 0;
   }
 }
 
@@ -726,9 +763,9 @@ struct Sorter {
 
 // Per pair: restore the breadth-first order (insertion sort on the key), then
 // either the contact filter (discrete) or the step rule (CCD).
-__global__ void k_pair_finish(int n_pairs, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
-                              int* kept_cnt, const double* pair_rec, double* scale) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+__device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
+                                int* kept_cnt, const double* pair_rec, double* scale) {
+  {
     int b = pair_off[p], e = pair_off[p + 1];
     for (int i = b + 1; i < e; ++i) {
       int v = perm[i];
@@ -801,21 +838,323 @@ __global__ void k_pair_finish(int n_pairs, const ContactRec* c, const int* pair_
     }
   }
 }
+__global__ void k_pair_finish_big(const int* big, const int* n_big, const ContactRec* c, const int* pair_off, int* perm,
+                                  int mode, int* keep, int* kept_cnt, const double* pair_rec, double* scale) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < *n_big; t += gridDim.x * blockDim.x)
+    pair_finish_one(big[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
+}
 
-// Gathers the contacts of every pair in order into the output arrays.
-__global__ void k_gather(int n_pairs, const ContactRec* c, const int* pair_off, const int* perm, const int* keep,
-                         const int* out_off, int filtered, int32_t* out_i, double* out_r) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+// One wavefront per pair, contacts staged in LDS (pairs with more than
+// POST_CAP contacts go to k_pair_finish, one lane per pair).
+//   1. bitonic sort by emission key -> the reference's contact order
+//   2a. discrete + filter: the order std::sort(compareContacts) produces.  When
+//       no two contacts have equal (eps_a+eps_b, |A-B|) every correct sort
+//       gives that order, so a parallel sort is used; with exact ties lane 0
+//       replays libstdc++'s introsort (Sorter) on the LDS copy.  Then the
+//       greedy pass of filterContacts, each candidate tested against the kept
+//       set across the lanes.
+//   2b. CCD: the step rule, lanes over contacts, min-reduced.
+constexpr int POST_CAP = 512;
+struct LdsKey {
+  double eps, dist;
+};
+__device__ __forceinline__ bool lds_less(const LdsKey& x, const LdsKey& y) {
+  if (x.eps == y.eps) return x.dist < y.dist;
+  return x.eps < y.eps;
+}
+struct LdsSorter {  // Sorter over LDS keys (same algorithm, LDS operands)
+  int* a;
+  const LdsKey* k;
+  __device__ bool lt(int x, int y) const { return lds_less(k[x], k[y]); }
+  __device__ void swp(int i, int j) {
+    int t = a[i];
+    a[i] = a[j];
+    a[j] = t;
+  }
+  __device__ void median_to_first(int res, int x, int y, int z) {
+    if (lt(a[x], a[y])) {
+      if (lt(a[y], a[z])) swp(res, y);
+      else if (lt(a[x], a[z])) swp(res, z);
+      else swp(res, x);
+    } else if (lt(a[x], a[z])) swp(res, x);
+    else if (lt(a[y], a[z])) swp(res, z);
+    else swp(res, y);
+  }
+  __device__ int partition(int first, int last, int pivot) {
+    while (true) {
+      while (lt(a[first], a[pivot])) ++first;
+      --last;
+      while (lt(a[pivot], a[last])) --last;
+      if (!(first < last)) return first;
+      swp(first, last);
+      ++first;
+    }
+  }
+  __device__ void push_heap(int first, int hole, int top, int val) {
+    int parent = (hole - 1) / 2;
+    while (hole > top && lt(a[first + parent], val)) {
+      a[first + hole] = a[first + parent];
+      hole = parent;
+      parent = (hole - 1) / 2;
+    }
+    a[first + hole] = val;
+  }
+  __device__ void adjust_heap(int first, int hole, int len, int val) {
+    int top = hole, child = hole;
+    while (child < (len - 1) / 2) {
+      child = 2 * (child + 1);
+      if (lt(a[first + child], a[first + child - 1])) child--;
+      a[first + hole] = a[first + child];
+      hole = child;
+    }
+    if ((len & 1) == 0 && child == (len - 2) / 2) {
+      child = 2 * (child + 1);
+      a[first + hole] = a[first + child - 1];
+      hole = child - 1;
+    }
+    push_heap(first, hole, top, val);
+  }
+  __device__ void heap_sort(int first, int last) {
+    int len = last - first;
+    if (len >= 2)
+      for (int parent = (len - 2) / 2;; --parent) {
+        adjust_heap(first, parent, len, a[first + parent]);
+        if (parent == 0) break;
+      }
+    while (last - first > 1) {
+      --last;
+      int v = a[last];
+      a[last] = a[first];
+      adjust_heap(first, 0, last - first, v);
+    }
+  }
+  __device__ void linear_insert(int last) {
+    int v = a[last], next = last - 1;
+    while (lt(v, a[next])) {
+      a[last] = a[next];
+      last = next;
+      --next;
+    }
+    a[last] = v;
+  }
+  __device__ void insertion(int first, int last) {
+    if (first == last) return;
+    for (int i = first + 1; i != last; ++i) {
+      if (lt(a[i], a[first])) {
+        int v = a[i];
+        for (int q = i; q > first; --q) a[q] = a[q - 1];
+        a[first] = v;
+      } else {
+        linear_insert(i);
+      }
+    }
+  }
+  __device__ void sort(int first, int last) {
+    if (first == last) return;
+    int n = last - first;
+    int depth = 2 * (63 - __clzll((long long)n));
+    int st_f[64], st_l[64], st_d[64], sp = 0;
+    st_f[sp] = first, st_l[sp] = last, st_d[sp] = depth, ++sp;
+    while (sp) {
+      --sp;
+      int f = st_f[sp], l = st_l[sp], d = st_d[sp];
+      while (l - f > 16) {
+        if (d == 0) {
+          heap_sort(f, l);
+          break;
+        }
+        --d;
+        int mid = f + (l - f) / 2;
+        median_to_first(f, f + 1, mid, l - 1);
+        int cut = partition(f + 1, l, f);
+        st_f[sp] = cut, st_l[sp] = l, st_d[sp] = d, ++sp;
+        l = cut;
+      }
+    }
+    if (n > 16) {
+      insertion(first, first + 16);
+      for (int i = first + 16; i != last; ++i) linear_insert(i);
+    } else {
+      insertion(first, last);
+    }
+  }
+};
+
+__global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const ContactRec* c, const int* pair_off, int* perm,
+                                                         int mode, int* keep, int* kept_cnt, const double* pair_rec,
+                                                         double* scale, int* big, int* n_big) {
+  __shared__ unsigned long long skey[POST_CAP];
+  __shared__ int sidx[POST_CAP];
+  __shared__ LdsKey sk[POST_CAP];
+  __shared__ int ord[POST_CAP];
+  __shared__ double smid[POST_CAP * 3];
+  __shared__ int kept_list[POST_CAP];
+  const int lane = threadIdx.x;
+  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+    const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
+    if (n > POST_CAP) {
+      if (lane == 0) big[atomicAdd(n_big, 1)] = p;
+      continue;
+    }
+    int n2 = 1;
+    while (n2 < n) n2 <<= 1;
+    for (int i = lane; i < n2; i += 64) {
+      int idx = i < n ? perm[b + i] : -1;
+      skey[i] = i < n ? c[idx].key : ~0ull;
+      sidx[i] = idx;
+    }
+    __syncthreads();
+    // bitonic sort of (skey, sidx) ascending (keys are unique)
+    for (int kk = 2; kk <= n2; kk <<= 1)
+      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+        for (int i = lane; i < n2; i += 64) {
+          int l = i ^ jj;
+          if (l > i) {
+            bool up = (i & kk) == 0;
+            unsigned long long ki = skey[i], kl = skey[l];
+            if ((ki > kl) == up) {
+              skey[i] = kl, skey[l] = ki;
+              int t = sidx[i];
+              sidx[i] = sidx[l], sidx[l] = t;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    if (mode == 2) {  // step rule, separate_clusters.cpp:496-564
+      const double* pr = pair_rec + (size_t)PR_STRIDE * p;
+      double mt = pr[PR_MAXT], ms = 1.0;
+      v3<double> va{pr[PR_VA], pr[PR_VA + 1], pr[PR_VA + 2]}, wa{pr[PR_WA], pr[PR_WA + 1], pr[PR_WA + 2]};
+      v3<double> xa{pr[PR_XA], pr[PR_XA + 1], pr[PR_XA + 2]};
+      v3<double> vb{pr[PR_VB], pr[PR_VB + 1], pr[PR_VB + 2]}, wb{pr[PR_WB], pr[PR_WB + 1], pr[PR_WB + 2]};
+      v3<double> xb{pr[PR_XB], pr[PR_XB + 1], pr[PR_XB + 2]};
+      for (int i = lane; i < n; i += 64) {
+        const ContactRec& ct = c[sidx[i]];
+        v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
+        v3<double> nrm = sub(A, B);
+        double inter = ct.r[6] + ct.r[7], toc = ct.r[10];
+        v3<double> rel = sub(add(va, cross(wa, sub(A, xa))), add(vb, cross(wb, sub(B, xb))));
+        double len = vlen(nrm);
+        if (!(dot(unit(nrm), unit(rel)) <= 0.0 || len < 1e-4)) continue;
+        double s;
+        if (len < 1e-4) {
+          if (toc == 0.0) continue;
+          double st = toc - (vlen(rel) * mt) / inter;
+          st = st > 0.0 ? st : 0.0;
+          s = lerp(st, toc, 0.75);
+        } else {
+          if (!(toc < 1.0)) continue;
+          double pv = -(dot(unit(nrm), rel));
+          double depth = inter - len;
+          double pd = (pv * toc) * mt;
+          if (!(pd > 1e-6)) continue;
+          if (depth < pd) {
+            s = lerp(toc * (1.0 - depth / pd), toc, 0.5);
+          } else {
+            double fp = toc + ((1.0 - toc) * inter) / (pv * mt);
+            s = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);
+          }
+        }
+        if (s < ms) ms = s;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(ms, off);
+        if (o < ms) ms = o;
+      }
+      if (lane == 0) scale[p] = ms;
+      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[i];
+    } else if (mode == 1) {  // filterContacts(.., 0.0)
+      for (int i = lane; i < n; i += 64) {
+        const ContactRec& ct = c[sidx[i]];
+        sk[i].eps = ct.r[6] + ct.r[7];
+        sk[i].dist = vlen(sub(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}));
+        v3<double> m = dvs(add(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}), 2.0);
+        smid[3 * i] = m.x, smid[3 * i + 1] = m.y, smid[3 * i + 2] = m.z;
+      }
+      for (int i = lane; i < n2; i += 64) ord[i] = i;
+      __syncthreads();
+      // parallel sort by (eps, dist, position); padding sorts last
+      for (int kk = 2; kk <= n2; kk <<= 1)
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+          for (int i = lane; i < n2; i += 64) {
+            int l = i ^ jj;
+            if (l > i) {
+              bool up = (i & kk) == 0;
+              int oi = ord[i], ol = ord[l];
+              bool gt;  // ord[i] sorts after ord[l]
+              if (oi >= n || ol >= n) gt = oi > ol;
+              else if (lds_less(sk[ol], sk[oi])) gt = true;
+              else if (lds_less(sk[oi], sk[ol])) gt = false;
+              else gt = oi > ol;
+              if (gt == up) ord[i] = ol, ord[l] = oi;
+            }
+          }
+          __syncthreads();
+        }
+      bool tie = false;
+      for (int i = lane; i + 1 < n; i += 64) {
+        const LdsKey x = sk[ord[i]], y = sk[ord[i + 1]];
+        tie = tie || (x.eps == y.eps && x.dist == y.dist);
+      }
+      if (__ballot(tie)) {
+        __syncthreads();
+        for (int i = lane; i < n; i += 64) ord[i] = i;
+        __syncthreads();
+        if (lane == 0) {
+          LdsSorter srt{ord, sk};
+          srt.sort(0, n);
+        }
+        __syncthreads();
+      }
+      // greedy pass in sorted order
+      int nk = 0;
+      for (int t = 0; t < n; ++t) {
+        int ci = ord[t];
+        double mx = smid[3 * ci], my = smid[3 * ci + 1], mz = smid[3 * ci + 2];
+        double search = sk[ci].eps + 0.0;
+        bool near = false;
+        for (int q = lane; q < nk; q += 64) {
+          int h = kept_list[q];
+          v3<double> dd{mx - smid[3 * h], my - smid[3 * h + 1], mz - smid[3 * h + 2]};
+          near = near || (vlen(dd) < search);
+        }
+        bool fresh = __ballot(near) == 0ull;
+        if (fresh && lane == 0) kept_list[nk] = ci;
+        if (lane == 0) keep[b + t] = fresh;
+        nk += fresh;
+        __syncthreads();
+      }
+      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[ord[i]];
+      if (lane == 0) kept_cnt[p] = nk;
+    } else {
+      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Gathers one pair per wavefront into the output arrays (CSR order).
+__global__ void __launch_bounds__(64) k_gather_wave(int n_pairs, const ContactRec* c, const int* pair_off, const int* perm,
+                                                    const int* keep, const int* out_off, int filtered, int32_t* out_i,
+                                                    double* out_r) {
+  const int lane = threadIdx.x;
+  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+    int b = pair_off[p], e = pair_off[p + 1];
     int o = out_off[p];
-    for (int i = pair_off[p]; i < pair_off[p + 1]; ++i) {
-      if (filtered && !keep[i]) continue;
-      const ContactRec& ct = c[perm[i]];
-      int32_t* oi = out_i + 8 * (size_t)o;
-      oi[0] = ct.pair, oi[1] = ct.na, oi[2] = ct.nb, oi[3] = ct.kind, oi[4] = ct.ia, oi[5] = ct.ib, oi[6] = ct.level,
-      oi[7] = 0;
-      double* orr = out_r + 11 * (size_t)o;
-      for (int k = 0; k < 11; ++k) orr[k] = ct.r[k];
-      ++o;
+    for (int base = b; base < e; base += 64) {
+      int i = base + lane;
+      bool take = i < e && (!filtered || keep[i]);
+      unsigned long long m = __ballot(take);
+      if (take) {
+        int dst = o + __popcll(m & lanes_below());
+        const ContactRec& ct = c[perm[i]];
+        int32_t* oi = out_i + 8 * (size_t)dst;
+        oi[0] = ct.pair, oi[1] = ct.na, oi[2] = ct.nb, oi[3] = ct.kind, oi[4] = ct.ia, oi[5] = ct.ib, oi[6] = ct.level,
+        oi[7] = 0;
+        double* orr = out_r + 11 * (size_t)dst;
+        for (int k = 0; k < 11; ++k) orr[k] = ct.r[k];
+      }
+      o += __popcll(m);
     }
   }
 }
@@ -878,7 +1217,8 @@ struct d2_ctx {
   DBuf<int> pass_flag;
   DBuf<double> min_time, scale;
   // post
-  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off;
+  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, n_big;
+  DBuf<int> cnt_a, off_a, cnt_s, off_s, cnt_c, off_c;  // per-level scans
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
   DBuf<unsigned char> scan_tmp;
@@ -1008,7 +1348,10 @@ int alloc_work(d2_ctx* c) {
   if ((rc = ensure(c, c->soup, c->cap)) || (rc = ensure(c, c->conn, c->cap)) || (rc = ensure(c, c->soup_next, c->cap)) ||
       (rc = ensure(c, c->conn_next, c->cap)) || (rc = ensure(c, c->items, c->cap)) || (rc = ensure(c, c->pt_items, c->cap)) ||
       (rc = ensure(c, c->ee_items, c->cap)) || (rc = ensure(c, c->np_hit, c->cap)) ||
-      (rc = ensure(c, c->contacts, c->cap_contacts)) || (rc = ensure(c, c->cnt, C_COUNT)))
+      (rc = ensure(c, c->contacts, c->cap_contacts)) || (rc = ensure(c, c->cnt, C_COUNT)) ||
+      (rc = ensure(c, c->cnt_a, c->cap + 1)) || (rc = ensure(c, c->off_a, c->cap + 1)) ||
+      (rc = ensure(c, c->cnt_s, c->cap + 1)) || (rc = ensure(c, c->off_s, c->cap + 1)) ||
+      (rc = ensure(c, c->cnt_c, c->cap + 1)) || (rc = ensure(c, c->off_c, c->cap + 1)))
     return rc;
   const long long res_items = std::min<long long>(c->cap, 1ll << 20);  // results chunk
   if ((rc = ensure(c, c->res, (size_t)res_items * RES_SLOTS * 8))) return rc;
@@ -1022,38 +1365,48 @@ void swap_next(d2_ctx* c) {
   std::swap(c->conn, c->conn_next);
 }
 
+// Exclusive scan of n ints on the context stream.
+int scan_ints(d2_ctx* c, const int* in, int* out, long long n) {
+  size_t tmp = 0;
+  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, c->stream));
+  int rc = ensure(c, c->scan_tmp, tmp);
+  if (rc) return rc;
+  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, in, out, (int)n, c->stream));
+  return D2_OK;
+}
+
 // Runs the post-processing of the contacts gathered by a traversal.
 int finish(d2_ctx* c, int mode, long long n_contacts) {
   int rc;
   const int np = c->n_pairs;
   if ((rc = ensure(c, c->pair_cnt, np + 1)) || (rc = ensure(c, c->pair_off, np + 1)) || (rc = ensure(c, c->pair_fill, np)) ||
       (rc = ensure(c, c->perm, n_contacts)) || (rc = ensure(c, c->keep, n_contacts)) || (rc = ensure(c, c->kept_cnt, np + 1)) ||
-      (rc = ensure(c, c->out_off, np + 1)))
+      (rc = ensure(c, c->out_off, np + 1)) || (rc = ensure(c, c->big, np)) || (rc = ensure(c, c->n_big, 1)))
     return rc;
   Timer tm(c, K_POST, (double)n_contacts);
   HIPCHK(c, hipMemsetAsync(c->pair_cnt.p, 0, sizeof(int) * (np + 1), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->pair_fill.p, 0, sizeof(int) * np, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->keep.p, 0, sizeof(int) * std::max<long long>(n_contacts, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->pair_fill.p, 0, sizeof(int) * std::max(np, 1), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->n_big.p, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->kept_cnt.p, 0, sizeof(int) * (np + 1), c->stream));
   if (n_contacts > 0)
     hipLaunchKernelGGL(k_count, dim3(grid_for(c, n_contacts)), dim3(256), 0, c->stream, n_contacts, c->contacts.p,
                        c->pair_cnt.p);
-  size_t tmp = 0;
-  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, c->pair_cnt.p, c->pair_off.p, np + 1, c->stream));
-  if ((rc = ensure(c, c->scan_tmp, tmp))) return rc;
-  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->pair_cnt.p, c->pair_off.p, np + 1, c->stream));
+  if ((rc = scan_ints(c, c->pair_cnt.p, c->pair_off.p, np + 1))) return rc;
   if (n_contacts > 0)
     hipLaunchKernelGGL(k_scatter, dim3(grid_for(c, n_contacts)), dim3(256), 0, c->stream, n_contacts, c->contacts.p,
                        c->pair_off.p, c->pair_fill.p, c->perm.p);
-  int fmode = mode == 0 ? 1 : 2;
-  if (mode == 0 && !c->last_filter) fmode = 0;
-  hipLaunchKernelGGL(k_pair_finish, dim3(grid_for(c, np, 64)), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                     c->perm.p, fmode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p);
+  int fmode = mode == 0 ? (c->last_filter ? 1 : 0) : 2;
+  const int wblocks = std::max(1, std::min(np, c->cus * 16));
+  hipLaunchKernelGGL(k_pair_finish_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
+                     c->perm.p, fmode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
+  hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->big.p,
+                     c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, fmode, c->keep.p, c->kept_cnt.p, c->pair_rec.p,
+                     c->scale.p);
   const int* outoff = c->pair_off.p;
   long long n_out = n_contacts;
   int filtered = fmode == 1;
   if (filtered) {
-    HIPCHK(c, hipMemsetAsync(c->kept_cnt.p + np, 0, sizeof(int), c->stream));
-    HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, c->kept_cnt.p, c->out_off.p, np + 1, c->stream));
+    if ((rc = scan_ints(c, c->kept_cnt.p, c->out_off.p, np + 1))) return rc;
     int total = 0;
     HIPCHK(c, hipMemcpyAsync(&total, c->out_off.p + np, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1065,8 +1418,8 @@ int finish(d2_ctx* c, int mode, long long n_contacts) {
   if ((rc = ensure(c, c->out_i, (size_t)std::max<long long>(n_out, 1) * 8)) ||
       (rc = ensure(c, c->out_r, (size_t)std::max<long long>(n_out, 1) * 11)))
     return rc;
-  hipLaunchKernelGGL(k_gather, dim3(grid_for(c, np, 64)), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                     c->perm.p, c->keep.p, outoff, filtered, c->out_i.p, c->out_r.p);
+  hipLaunchKernelGGL(k_gather_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p, c->perm.p,
+                     c->keep.p, outoff, filtered, c->out_i.p, c->out_r.p);
   HIPCHK(c, hipGetLastError());
   c->n_out = n_out;
   return D2_OK;
@@ -1091,28 +1444,35 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
   hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, np, c->soup.p, c->cnt.p,
                      c->pass_min.p, c->pass_flag.p, c->min_time.p, c->pair_rec.p, c->scale.p);
   HIPCHK(c, hipGetLastError());
+  // The host copy of the counters must describe this run even when no level
+  // executes (an empty pair list).
+  if ((rc = read_counters(c))) return rc;
   long long n_soup = np, n_conn = 0;
   const int wave_block = 256;
+  const int stream_grid = c->cus * 8;  // persistent-style grid for device-sized launches
   for (int level = 0; n_soup + n_conn > 0; ++level) {
     if (level > 15) return fail(c, D2_ERR_RUNTIME, "traversal deeper than 15 levels (contact keys hold 16)");
     Dev D = make_dev(c, level, discrete);
     if (n_soup > 0) {
-      int gw = grid_for(c, n_soup * 64, wave_block);
       if (discrete) {
-        Timer tm(c, K_TRI_ENUM, (double)n_soup);
-        hipLaunchKernelGGL(k_tri_enum, dim3(gw), dim3(wave_block), 0, c->stream, D);
-      } else {
-        Timer tm(c, K_SOUP_CULL, (double)n_soup);
-        hipLaunchKernelGGL(k_soup_cull, dim3(gw), dim3(wave_block), 0, c->stream, D);
-      }
-      if ((rc = read_counters(c))) return rc;
-      long long n_items = (long long)c->host_cnt[C_ITEMS];
-      if (discrete) {
-        if (n_items > 0) {
-          Timer tm(c, K_TRI_DIST, (double)n_items);
-          hipLaunchKernelGGL(k_tri_dist, dim3(grid_for(c, n_items)), dim3(256), 0, c->stream, D, 0ll, n_items);
+        hipLaunchKernelGGL(k_np_items_count, dim3(grid_for(c, n_soup + 1)), dim3(256), 0, c->stream, D, n_soup, c->cnt_a.p);
+        if ((rc = scan_ints(c, c->cnt_a.p, c->off_a.p, n_soup + 1))) return rc;
+        {
+          Timer tm(c, K_TRI_ENUM, (double)n_soup);
+          hipLaunchKernelGGL(k_tri_enum_ordered, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0,
+                             c->stream, D, n_soup, c->off_a.p);
+        }
+        {
+          Timer tm(c, K_TRI_DIST, 0.0);
+          hipLaunchKernelGGL(k_tri_dist_dev, dim3(stream_grid), dim3(256), 0, c->stream, D, c->off_a.p + n_soup);
         }
       } else {
+        {
+          Timer tm(c, K_SOUP_CULL, (double)n_soup);
+          hipLaunchKernelGGL(k_soup_cull, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0, c->stream, D);
+        }
+        if ((rc = read_counters(c))) return rc;
+        long long n_items = (long long)c->host_cnt[C_ITEMS];
         const long long chunk = res_chunk(c);
         for (long long f = 0; f < n_items; f += chunk) {
           long long cntc = std::min(chunk, n_items - f);
@@ -1132,7 +1492,12 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
       }
       {
         Timer tm(c, K_EXPAND, (double)n_soup);
-        hipLaunchKernelGGL(k_expand, dim3(grid_for(c, n_soup)), dim3(256), 0, c->stream, D, 1);
+        hipLaunchKernelGGL(k_expand_count, dim3(grid_for(c, n_soup + 1)), dim3(256), 0, c->stream, D, n_soup, 1,
+                           c->cnt_s.p, c->cnt_c.p);
+        if ((rc = scan_ints(c, c->cnt_s.p, c->off_s.p, n_soup + 1))) return rc;
+        if (!discrete && (rc = scan_ints(c, c->cnt_c.p, c->off_c.p, n_soup + 1))) return rc;
+        hipLaunchKernelGGL(k_expand_write, dim3(grid_for(c, n_soup)), dim3(256), 0, c->stream, D, n_soup, 1, c->off_s.p,
+                           c->off_c.p);
       }
       if (!discrete)
         hipLaunchKernelGGL(k_fold_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pass_min.p, c->pass_flag.p,
@@ -1158,7 +1523,9 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
                          c->min_time.p, c->pair_rec.p);
       HIPCHK(c, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, c->stream, c->cnt.p);
+    hipLaunchKernelGGL(k_advance_scanned, dim3(1), dim3(64), 0, c->stream, c->cnt.p,
+                       n_soup > 0 ? c->off_s.p + n_soup : nullptr, n_soup > 0 ? c->off_c.p + n_soup : nullptr,
+                       discrete ? 0 : 1);
     if ((rc = read_counters(c))) return rc;
     swap_next(c);
     n_soup = (long long)c->host_cnt[C_SOUP];
@@ -1172,7 +1539,6 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
   if (n_contacts_out) *n_contacts_out = c->n_out;
   return D2_OK;
 }
-
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -1216,6 +1582,8 @@ int d2_ctx_destroy(d2_ctx* c) {
       release(c->scale);
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_tmp);
+  release(c->big), release(c->n_big), release(c->cnt_a), release(c->off_a), release(c->cnt_s), release(c->off_s),
+      release(c->cnt_c), release(c->off_c);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1359,6 +1727,12 @@ int d2_ccd(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* max_time,
 int d2_result_contacts(d2_ctx* c, int32_t* offsets, int32_t* ints, double* reals) {
   if (!c) return D2_ERR_ARG;
   if (c->last_mode < 0) return fail(c, D2_ERR_STATE, "no result: run the path first");
+  if ((ints || reals) && c->n_out > 0) {  // the CSR total and the record count must agree
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, c->out_off.p + c->n_pairs, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((long long)total != c->n_out) return fail(c, D2_ERR_STATE, "result offsets and contact count disagree");
+  }
   if (offsets)
     HIPCHK(c, hipMemcpyAsync(offsets, c->out_off.p, sizeof(int) * (c->n_pairs + 1), hipMemcpyDeviceToHost, c->stream));
   if (ints && c->n_out)
