@@ -64,6 +64,24 @@ def cpu_baseline(scene, max_time, budget_s=12.0):
                       f"median of 3 passes; oracle built with g++ -O2 -ffp-contract=off"}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of known size through the HIP runtime, so
+    FETCH_SIZE / WRITE_SIZE counters can be scaled against exact byte counts."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +90,12 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path is not sharded in this build",
               file=sys.stderr)
@@ -82,7 +105,7 @@ def main():
     from delta2_amd import scenes
     d2.lib()
     t0 = time.time()
-    scene = scenes.scene_c2() if args.workload == "c2" else scenes.scene_c1()
+    scene = scenes.cached(args.workload, scenes.scene_c2 if args.workload == "c2" else scenes.scene_c1)
     log(f"[bench] scene {scene.name}: {scene.n_bodies} bodies, {scene.n_triangles()} triangles, "
         f"{scene.pairs.shape[0]} pairs, built in {time.time() - t0:.1f}s")
     pairs = scene.pairs

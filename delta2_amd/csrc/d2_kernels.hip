@@ -72,7 +72,7 @@ constexpr int BUCKET = 88;
 constexpr int RES_SLOTS = 15;  // sub-results per soup triangle pair
 enum Counter : int {
   C_SOUP = 0, C_CONN = 1, C_SOUP_NEXT = 2, C_CONN_NEXT = 3, C_ITEMS = 4, C_PT = 5, C_EE = 6,
-  C_CONTACTS = 7, C_OVERFLOW = 8, C_COUNT = 16
+  C_CONTACTS = 7, C_OVERFLOW = 8, C_LVL_ITEMS = 9, C_COUNT = 16
 };
 // Key layout: level(5) | list(1) | path(48) | sub(10)
 __device__ __forceinline__ unsigned long long make_key(int level, int list, unsigned long long path, int sub) {
@@ -503,6 +503,7 @@ __global__ void __launch_bounds__(256) k_tri_enum_ordered(Dev D, long long n, co
     if (lane == 0) {
       D.np_hit[w] = 0;
       if (base + na * nb > D.cap_items) D.cnt[C_OVERFLOW] = 1;
+      if (w == n - 1) D.cnt[C_LVL_ITEMS] = (unsigned long long)offs[n];  // for the profiler's item count
     }
   }
 }
@@ -1527,6 +1528,7 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
                        n_soup > 0 ? c->off_s.p + n_soup : nullptr, n_soup > 0 ? c->off_c.p + n_soup : nullptr,
                        discrete ? 0 : 1);
     if ((rc = read_counters(c))) return rc;
+    if (discrete && n_soup > 0 && c->profiling) c->k_items[K_TRI_DIST] += (double)c->host_cnt[C_LVL_ITEMS];
     swap_next(c);
     n_soup = (long long)c->host_cnt[C_SOUP];
     n_conn = (long long)c->host_cnt[C_CONN];

@@ -114,13 +114,50 @@ class Scene:
         self.tree_of_body = np.arange(len(trees), dtype=np.int32)
         self.scene_of_body = np.zeros(len(trees), dtype=np.int32) if scene_of_body is None else scene_of_body
         self.pairs = np.zeros((0, 2), dtype=np.int32)
+        self._n_tris = None
 
     @property
     def n_bodies(self):
         return self.records.shape[0]
 
     def n_triangles(self):
-        return int(sum(F.shape[0] for _, F in self.meshes))
+        if self._n_tris is None:
+            self._n_tris = int(sum(F.shape[0] for _, F in self.meshes))
+        return self._n_tris
+
+    def save(self, path):
+        offs = np.zeros(len(self.trees) + 1, dtype=np.int64)
+        offs[1:] = np.cumsum([t[0].shape[0] for t in self.trees])
+        np.savez(path, name=self.name, offs=offs, nodes=np.concatenate([t[0] for t in self.trees]),
+                 buckets=np.concatenate([t[1] for t in self.trees]), records=self.records, radii=self.radii,
+                 scene_of_body=self.scene_of_body, pairs=self.pairs, n_tris=self.n_triangles(),
+                 box=getattr(self, "box", 0.0))
+
+    @staticmethod
+    def load(path):
+        z = np.load(path)
+        offs, nodes, buckets = z["offs"], z["nodes"], z["buckets"]
+        trees = [(nodes[offs[i]:offs[i + 1]], buckets[offs[i]:offs[i + 1]]) for i in range(offs.size - 1)]
+        s = Scene(str(z["name"]), [], trees, z["records"], z["radii"], z["scene_of_body"])
+        s.pairs = z["pairs"]
+        s._n_tris = int(z["n_tris"])
+        s.box = float(z["box"])
+        return s
+
+
+def cached(name, build):
+    """build() unless $D2_SCENE_CACHE holds <name>.npz (then load it; save after building)."""
+    import os
+    d = os.environ.get("D2_SCENE_CACHE")
+    if not d:
+        return build()
+    path = os.path.join(d, name + ".npz")
+    if os.path.exists(path):
+        return Scene.load(path)
+    s = build()
+    os.makedirs(d, exist_ok=True)
+    s.save(path)
+    return s
 
 
 def body_record(q, pos, vel, inv_inertia, geo_eps=GEO_EPS, static=False, last_step=DT):
