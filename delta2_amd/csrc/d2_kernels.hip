@@ -475,36 +475,149 @@ __global__ void __launch_bounds__(256) k_leaf_edge(Dev D, long long first, long 
 }
 
 // ------------------------------------------- ordered enumeration/expansion
-// Items of a node pair are its triangle pairs (i < na, j < nb).  counts[w] =
-// na*nb, counts[n] = 0; an exclusive scan gives each node pair its slice.
-__global__ void k_np_items_count(Dev D, long long n, int* counts) {
-  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w <= n; w += (long long)gridDim.x * blockDim.x) {
-    if (w == n) {
-      counts[w] = 0;
-      continue;
-    }
-    NP np = D.soup[w];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    counts[w] = D.nodes[pi[2] + np.na].ntris * D.nodes[pi[3] + np.nb].ntris;
+// ------------------------------------------------ FP32 conservative cull
+// Classifies one triangle pair of the discrete phase in FP32:
+//   0 = certain miss, 1 = needs the FP64 test, 2 = certain hit.
+// Lower bound: the reference's distance is the length of a pair of points on
+// the two triangles, except that a point projected onto a plane is accepted
+// as "inside" by float barycentrics (triangle.h:92-100), which can accept a
+// point up to eps_out ~ 16*eps_f*kappa*L outside (kappa = d00*d11/denom of the
+// barycentric system, L the longest edge).  So dist_ref >= true distance -
+// eps_out, and the true distance is >= the projection gap of the two vertex
+// sets on any axis (here both normals and the centroid direction).
+// Upper bound: every edge-edge candidate (comparison_dist.h:80-140) returns
+// |P - Q| with Q the closest point of one segment to a point X of the other
+// and P the closest point to Q, so it is <= max over the four endpoint pairs;
+// dist_ref <= the smallest such bound over the 9 edge pairs.
+// margin = 64*eps_f*kappa*L (4x the barycentric bound) + 16*eps_f*|coords|
+// (FP32 rounding of the relative coordinates and projections) + 1e-6 m.
+__device__ __forceinline__ float f3dot(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+__device__ int cull_class(const double* pa, const double* pb, const tri<double>& la, const tri<double>& lt, double thr_d) {
+  const float EPSF = 5.9604645e-08f;
+  float ra[9], rb[9], ta[3];
+  for (int r = 0; r < 3; ++r) {
+    for (int k = 0; k < 3; ++k) ra[3 * r + k] = (float)pa[4 * r + k], rb[3 * r + k] = (float)pb[4 * r + k];
+    ta[r] = (float)(pa[4 * r + 3] - pb[4 * r + 3]);
   }
+  float A[3][3], B[3][3];
+  const v3<double> av[3] = {la.a, la.b, la.c}, bv[3] = {lt.a, lt.b, lt.c};
+  float coord = 0.f;
+  for (int v = 0; v < 3; ++v) {
+    float x = (float)av[v].x, y = (float)av[v].y, z = (float)av[v].z;
+    float u = (float)bv[v].x, w = (float)bv[v].y, q = (float)bv[v].z;
+    for (int r = 0; r < 3; ++r) {
+      A[v][r] = ((ra[3 * r] * x + ra[3 * r + 1] * y) + ra[3 * r + 2] * z) + ta[r];
+      B[v][r] = (rb[3 * r] * u + rb[3 * r + 1] * w) + rb[3 * r + 2] * q;
+      coord = fmaxf(coord, fmaxf(fabsf(A[v][r]), fabsf(B[v][r])));
+    }
+  }
+  float ea0[3], ea1[3], eb0[3], eb1[3], nA[3], nB[3], cd[3];
+  for (int r = 0; r < 3; ++r) {
+    ea0[r] = A[1][r] - A[0][r], ea1[r] = A[2][r] - A[0][r];
+    eb0[r] = B[1][r] - B[0][r], eb1[r] = B[2][r] - B[0][r];
+    cd[r] = (B[0][r] + B[1][r] + B[2][r]) - (A[0][r] + A[1][r] + A[2][r]);
+  }
+  auto crs = [](const float* a, const float* b, float* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1], o[1] = a[2] * b[0] - a[0] * b[2], o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  crs(ea0, ea1, nA);
+  crs(eb0, eb1, nB);
+  float d00 = f3dot(ea0, ea0), d11 = f3dot(ea1, ea1), d01 = f3dot(ea0, ea1);
+  float e00 = f3dot(eb0, eb0), e11 = f3dot(eb1, eb1), e01 = f3dot(eb0, eb1);
+  float dena = d00 * d11 - d01 * d01, denb = e00 * e11 - e01 * e01;
+  // ill-conditioned barycentric systems get no cull
+  if (!(dena > 1e-4f * d00 * d11) || !(denb > 1e-4f * e00 * e11)) return 1;
+  float kappa = fmaxf(d00 * d11 / dena, e00 * e11 / denb);
+  float L2 = fmaxf(fmaxf(d00, d11), fmaxf(e00, e11));
+  float e12[3], f12[3];
+  for (int r = 0; r < 3; ++r) e12[r] = A[2][r] - A[1][r], f12[r] = B[2][r] - B[1][r];
+  L2 = fmaxf(L2, fmaxf(f3dot(e12, e12), f3dot(f12, f12)));
+  float L = sqrtf(L2);
+  float thr = (float)thr_d;
+  float margin = 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * coord + EPSF * thr;
+  // lower bound
+  float lb = 0.f;
+  const float* axes[3] = {nA, nB, cd};
+  for (int k = 0; k < 3; ++k) {
+    const float* n = axes[k];
+    float nn = f3dot(n, n);
+    if (!(nn > 0.f)) continue;
+    float a0 = f3dot(A[0], n), a1 = f3dot(A[1], n), a2 = f3dot(A[2], n);
+    float b0 = f3dot(B[0], n), b1 = f3dot(B[1], n), b2 = f3dot(B[2], n);
+    float amin = fminf(a0, fminf(a1, a2)), amax = fmaxf(a0, fmaxf(a1, a2));
+    float bmin = fminf(b0, fminf(b1, b2)), bmax = fmaxf(b0, fmaxf(b1, b2));
+    float g = fmaxf(bmin - amax, amin - bmax);
+    if (g > 0.f) lb = fmaxf(lb, g / sqrtf(nn));
+  }
+  if (lb > thr + margin) return 0;
+  // upper bound
+  float dd[3][3];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      float t[3] = {A[x][0] - B[y][0], A[x][1] - B[y][1], A[x][2] - B[y][2]};
+      dd[x][y] = f3dot(t, t);
+    }
+  float ub2 = 3.4e38f;
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      int x1 = x == 2 ? 0 : x + 1, y1 = y == 2 ? 0 : y + 1;
+      ub2 = fminf(ub2, fmaxf(fmaxf(dd[x][y], dd[x][y1]), fmaxf(dd[x1][y], dd[x1][y1])));
+    }
+  float lim = thr - margin;
+  if (lim > 0.f && ub2 < lim * lim) return 2;
+  return 1;
 }
 
-// One wavefront per node pair; lane (i, j) writes item i*nb + j of the slice.
-__global__ void __launch_bounds__(256) k_tri_enum_ordered(Dev D, long long n, const int* offs) {
+// One wavefront per node pair: which triangle pairs need the FP64 test.
+// FP64 mode keeps every valid pair; FP32 mode drops certain misses, and for
+// node pairs with an inner node (they emit no contacts, only "any hit"
+// matters) a certain hit settles the node pair without any FP64 work.
+__global__ void __launch_bounds__(256) k_tri_mask(Dev D, long long n, int fp32, unsigned long long* masks, int* counts) {
   const int lane = lane_id();
+  if (blockIdx.x == 0 && threadIdx.x == 0) counts[n] = 0;
   const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
   for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
     NP np = D.soup[w];
     const int32_t* pi = D.pair_info + 4 * np.pair;
-    int na = D.nodes[pi[2] + np.na].ntris, nb = D.nodes[pi[3] + np.nb].ntris;
+    const NodeRec na = D.nodes[pi[2] + np.na];
+    const NodeRec nb = D.nodes[pi[3] + np.nb];
     int i = lane >> 3, j = lane & 7;
-    long long base = offs[w];
-    if (i < na && j < nb && base + i * nb + j < D.cap_items) D.items[base + i * nb + j] = Item{(uint32_t)w, (uint32_t)lane};
-    if (lane == 0) {
-      D.np_hit[w] = 0;
-      if (base + na * nb > D.cap_items) D.cnt[C_OVERFLOW] = 1;
-      if (w == n - 1) D.cnt[C_LVL_ITEMS] = (unsigned long long)offs[n];  // for the profiler's item count
+    bool valid = i < na.ntris && j < nb.ntris;
+    int cls = valid ? 1 : 0;
+    if (fp32 && valid) {
+      const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+      const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+      const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+      double thr = (pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j];
+      cls = cull_class(pr + PR_TA1, pr + PR_TB1, load_tri<double>(ba, i), load_tri<double>(bb, j), thr);
     }
+    bool inner = na.inner || nb.inner;
+    unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
+    bool settled = inner && __ballot(cls == 2) != 0ull;
+    if (settled) keep = 0ull;
+    if (lane == 0) {
+      masks[w] = keep;
+      counts[w] = __popcll(keep);
+      D.np_hit[w] = settled ? 1 : 0;
+    }
+  }
+}
+
+// One wavefront per node pair: kept triangle pairs to the item list, at the
+// node pair's scanned offset, in triangle-pair order.
+__global__ void __launch_bounds__(256) k_tri_enum_masked(Dev D, long long n, const unsigned long long* masks,
+                                                          const int* offs) {
+  const int lane = lane_id();
+  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
+  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
+    unsigned long long m = masks[w];
+    long long base = offs[w];
+    if ((m >> lane) & 1ull) {
+      long long slot = base + __popcll(m & lanes_below());
+      if (slot < D.cap_items) D.items[slot] = Item{(uint32_t)w, (uint32_t)lane};
+      else D.cnt[C_OVERFLOW] = 1;
+    }
+    if (lane == 0 && w == n - 1) D.cnt[C_LVL_ITEMS] = (unsigned long long)offs[n];
   }
 }
 
@@ -1220,6 +1333,7 @@ struct d2_ctx {
   // post
   DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, n_big;
   DBuf<int> cnt_a, off_a, cnt_s, off_s, cnt_c, off_c;  // per-level scans
+  DBuf<unsigned long long> masks;
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
   DBuf<unsigned char> scan_tmp;
@@ -1352,7 +1466,7 @@ int alloc_work(d2_ctx* c) {
       (rc = ensure(c, c->contacts, c->cap_contacts)) || (rc = ensure(c, c->cnt, C_COUNT)) ||
       (rc = ensure(c, c->cnt_a, c->cap + 1)) || (rc = ensure(c, c->off_a, c->cap + 1)) ||
       (rc = ensure(c, c->cnt_s, c->cap + 1)) || (rc = ensure(c, c->off_s, c->cap + 1)) ||
-      (rc = ensure(c, c->cnt_c, c->cap + 1)) || (rc = ensure(c, c->off_c, c->cap + 1)))
+      (rc = ensure(c, c->cnt_c, c->cap + 1)) || (rc = ensure(c, c->off_c, c->cap + 1)) || (rc = ensure(c, c->masks, c->cap)))
     return rc;
   const long long res_items = std::min<long long>(c->cap, 1ll << 20);  // results chunk
   if ((rc = ensure(c, c->res, (size_t)res_items * RES_SLOTS * 8))) return rc;
@@ -1456,12 +1570,13 @@ int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
     Dev D = make_dev(c, level, discrete);
     if (n_soup > 0) {
       if (discrete) {
-        hipLaunchKernelGGL(k_np_items_count, dim3(grid_for(c, n_soup + 1)), dim3(256), 0, c->stream, D, n_soup, c->cnt_a.p);
-        if ((rc = scan_ints(c, c->cnt_a.p, c->off_a.p, n_soup + 1))) return rc;
         {
           Timer tm(c, K_TRI_ENUM, (double)n_soup);
-          hipLaunchKernelGGL(k_tri_enum_ordered, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0,
-                             c->stream, D, n_soup, c->off_a.p);
+          hipLaunchKernelGGL(k_tri_mask, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0, c->stream, D,
+                             n_soup, c->precision == D2_FP32 ? 1 : 0, c->masks.p, c->cnt_a.p);
+          if ((rc = scan_ints(c, c->cnt_a.p, c->off_a.p, n_soup + 1))) return rc;
+          hipLaunchKernelGGL(k_tri_enum_masked, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0,
+                             c->stream, D, n_soup, c->masks.p, c->off_a.p);
         }
         {
           Timer tm(c, K_TRI_DIST, 0.0);
@@ -1585,7 +1700,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_tmp);
   release(c->big), release(c->n_big), release(c->cnt_a), release(c->off_a), release(c->cnt_s), release(c->off_s),
-      release(c->cnt_c), release(c->off_c);
+      release(c->cnt_c), release(c->off_c), release(c->masks);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
   (void)hipStreamDestroy(c->stream);
   delete c;

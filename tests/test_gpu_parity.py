@@ -187,3 +187,32 @@ def test_empty_run_after_contacts_reports_zero(ctx):
         assert run() == 0
         r = ctx.result()
         assert r.n == 0 and r.ints.shape == (0, 8) and r.offsets.tolist() == [0]
+
+
+@pytest.fixture(scope="module")
+def ctx32():
+    c = d2.Context(0, d2.D2_FP32)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name,make", SCENES)
+@pytest.mark.parametrize("filt", [False, True])
+def test_narrow_parity_fp32_cull(ctx32, name, make, filt):
+    """FP32 mode: conservative FP32 culling, FP64 for every kept test -- the
+    contacts must be the oracle's exactly (same bar as FP64)."""
+    s = make()
+    print(f"[narrow fp32] scene={name} pairs={s.pairs.shape[0]} filter={filt}", flush=True)
+    rec = with_future(s.records, scenes.DT)
+    load(ctx32, s, rec)
+    ctx32.reset_stats()
+    ctx32.set_profiling(True)
+    gpu = d2.compare_trees_full(ctx32, s.pairs, filter=filt)
+    ctx32.set_profiling(False)
+    o_off, o_ints, o_reals = oracle_scene(s, rec).narrow(s.pairs, filter=filt)
+    assert_same(gpu, o_off, o_ints, o_reals, f"narrow fp32 {name} filter={filt}")
+    st = ctx32.kernel_stats()
+    tested = st["tri_dist"][2]
+    # node pairs seen by the mask kernel x up to 64 triangle pairs: the FP64
+    # work after the cull must be a small part of the full cross product
+    assert tested < 0.5 * 64 * st["tri_enum"][2], (tested, st["tri_enum"])
