@@ -1,10 +1,14 @@
 """Benchmark: narrow+CCD pair queries/s on one MI355X.
 
 A step is one pass of the contact-detection path over the workload's
-broad-phase pair list with inputs resident in HBM:
-  CCD (compareTreesFullContinuous over [0, max_time]) + the TOI -> step rule,
-  then the discrete narrow phase (compareTreesFull) + filterContacts,
-for every pair.  value = pairs * steps / wall time of the timed steps.
+broad-phase pair list with inputs resident in HBM (d2_step_run):
+  CCD (compareTreesFullContinuous over [0, min(last steps)]) + the TOI -> step
+  rule, the clamped cluster steps, the future states at those steps, then the
+  discrete narrow phase (compareTreesFull) + filterContacts,
+for every pair.  value = pairs * steps / wall time of the timed steps.  The
+timed steps replay one captured HIP graph per step (--graph off: eager
+launches); a separate eager pass with HIP events brackets every kernel for the
+roofline figures.
 
 The workload is the 4096-body scene (2048 subdivision-3 icospheres scaled
 U(0.5,1.5) and 2048 convex hulls of 64 ellipsoid points, Poisson-disk placed
@@ -89,6 +93,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--graph", default="on", choices=["on", "off"])
+    ap.add_argument("--profile-steps", type=int, default=5, help="eager steps timed per kernel for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for rocprofv3 --pmc)")
@@ -114,29 +120,38 @@ def main():
     ctx = d2.Context(0, prec)
     ctx.upload_trees(scene.trees)
     ctx.upload_bodies(scene.tree_of_body, scene.records)
-    ctx.project_future(np.full(scene.n_bodies, scenes.DT))
-    ctx.upload_pairs(pairs, max_time)
+    ctx.upload_pairs(pairs)
+    cl, n_cl = scenes.clusters(pairs, scene.n_bodies, scene.records)
+    ctx.upload_clusters(cl, n_cl)
+    ctx.set_graph(args.graph == "on")
+    n_contacts = 0
 
     def step():
-        ctx.ccd_run()
-        ctx.narrow_run(True)
+        nonlocal n_contacts
+        n_contacts = ctx.step_run(scenes.DT, True)
 
     for _ in range(args.warmup):
         step()
     ctx.sync()
-    ctx.reset_stats()
-    ctx.set_profiling(True)
     ts = time.perf_counter()
     for _ in range(args.steps):
         step()
     ctx.sync()
     te = time.perf_counter()
-    ctx.set_profiling(False)
-    stats = ctx.kernel_stats()
     wall = te - ts
     ms_per_step = 1e3 * wall / args.steps
     value = pairs.shape[0] * args.steps / wall
-    log(f"[bench] {args.steps} steps in {wall:.3f}s = {ms_per_step:.3f} ms/step")
+    log(f"[bench] {args.steps} steps in {wall:.3f}s = {ms_per_step:.3f} ms/step ({n_cl} clusters, "
+        f"{n_contacts} filtered contacts, graph {args.graph})")
+    # per-kernel times: eager steps, every kernel bracketed by HIP events on the context's stream
+    ctx.set_graph(False)
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    for _ in range(args.profile_steps):
+        step()
+    ctx.sync()
+    ctx.set_profiling(False)
+    stats = ctx.kernel_stats()
     for k, (n, ms, items) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
         if n:
             log(f"[bench]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")

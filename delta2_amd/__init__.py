@@ -45,6 +45,15 @@ class CapacityError(D2Error):
     pass
 
 
+class Tune(ctypes.Structure):
+    """Launch shapes (d2_tune): blocks of the chunked list kernels, of the
+    grid-stride kernels and of the per-pair post-processing."""
+    _fields_ = [("chunk_blocks", ctypes.c_int), ("stream_blocks", ctypes.c_int), ("post_blocks", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.chunk_blocks, self.stream_blocks, self.post_blocks)
+
+
 def lib():
     """Loads libd2hip.so (raises if it has not been built)."""
     global _lib
@@ -80,6 +89,11 @@ def lib():
         "d2_kernel_stats": ([_vp, ctypes.c_int, _i64p, _dp, _dp], ctypes.c_int),
         "d2_reset_stats": ([_vp], ctypes.c_int),
         "d2_sync": ([_vp], ctypes.c_int),
+        "d2_set_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
+        "d2_get_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
+        "d2_set_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_clusters_upload": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
+        "d2_step_run": ([_vp, ctypes.c_float, ctypes.c_int, _dp, _i64p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -166,6 +180,7 @@ class Context:
             raise D2Error(rc, "d2_ctx_create failed (is a GPU visible?)")
         self.h = h
         self.n_pairs = 0
+        self.n_clusters = 0
         self.n_out = 0
 
     def close(self):
@@ -250,6 +265,34 @@ class Context:
             per = np.zeros((self.n_pairs, 2))
             self._check(lib().d2_result_pairs(self.h, _dptr(per)))
         return Result(off, ints, reals, per)
+
+    def upload_clusters(self, cluster_of_pair, n_clusters):
+        cop = _i32(cluster_of_pair)
+        self._check(lib().d2_clusters_upload(self.h, _iptr(cop), int(n_clusters)))
+        self.n_clusters = int(n_clusters)
+
+    def step_run(self, dt, filter=True, want_steps=False):
+        """CCD + step rule + cluster steps + projection + discrete narrow phase
+        (+ filter) over the resident pairs; returns the contact count, and the
+        cluster steps when asked."""
+        n = ctypes.c_int64(0)
+        steps = np.zeros(self.n_clusters) if want_steps else None
+        self._check(lib().d2_step_run(self.h, float(dt), int(bool(filter)), None if steps is None else _dptr(steps),
+                                      ctypes.byref(n)))
+        self.n_out = n.value
+        return (n.value, steps) if want_steps else n.value
+
+    def set_graph(self, on=True):
+        self._check(lib().d2_set_graph(self.h, int(bool(on))))
+
+    def set_tune(self, chunk_blocks, stream_blocks, post_blocks):
+        t = Tune(int(chunk_blocks), int(stream_blocks), int(post_blocks))
+        self._check(lib().d2_set_tune(self.h, ctypes.byref(t)))
+
+    def get_tune(self):
+        t = Tune()
+        self._check(lib().d2_get_tune(self.h, ctypes.byref(t)))
+        return t.as_tuple()
 
     def cluster_steps(self, cluster_of_pair, n_clusters, dt):
         cop = _i32(cluster_of_pair)

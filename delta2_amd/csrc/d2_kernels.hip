@@ -32,6 +32,7 @@
 
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -70,10 +71,12 @@ enum : int {
 };
 constexpr int BUCKET = 88;
 constexpr int RES_SLOTS = 15;  // sub-results per soup triangle pair
-enum Counter : int {
-  C_SOUP = 0, C_CONN = 1, C_SOUP_NEXT = 2, C_CONN_NEXT = 3, C_ITEMS = 4, C_PT = 5, C_EE = 6,
-  C_CONTACTS = 7, C_OVERFLOW = 8, C_LVL_ITEMS = 9, C_COUNT = 16
-};
+// Global counters (atomics) and per-level sizes.  Level L reads its list sizes
+// from lvl[L*LV_STRIDE + ...] and the level's write kernel stores the sizes of
+// level L+1; no size ever travels through the host.
+enum Counter : int { C_CONTACTS = 0, C_OVERFLOW = 1, C_LVL_ITEMS = 2, C_OUT = 3, C_COUNT = 8 };
+enum LevelSlot : int { LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_STRIDE = 8 };
+constexpr int MAX_LEVELS = 16;  // contact keys hold 16 base-8 path digits
 // Key layout: level(5) | list(1) | path(48) | sub(10)
 __device__ __forceinline__ unsigned long long make_key(int level, int list, unsigned long long path, int sub) {
   return ((unsigned long long)level << 59) | ((unsigned long long)list << 58) | (path << 10) | (unsigned long long)sub;
@@ -126,25 +129,94 @@ struct Dev {
   int* np_hit;
   ContactRec* contacts;
   unsigned long long* cnt;
-  unsigned long long* pass_min;  // per pair, min toc of this pass (bits)
-  int* pass_flag;                // per pair, a leaf pair had hits this pass
-  long long cap_np, cap_items, cap_contacts;
+  unsigned long long* lvl;
+  // per node pair counts, masks, and per-block partial sums of the counts
+  unsigned long long* masks;
+  int *cnt_a, *cnt_s, *cnt_c, *cnt_pt, *cnt_ee;
+  int *part_a, *part_s, *part_c, *part_pt, *part_ee;
+  // per pair, min toc of the soup / connected pass (bits) and "a leaf pair
+  // had hits in this pass"
+  unsigned long long *pass_min_s, *pass_min_c;
+  int *pass_flag_s, *pass_flag_c;
+  double* min_time;
+  int n_pairs;
+  long long cap_np, cap_items, cap_res, cap_contacts;  // cap_res: CCD soup items (result slots)
   int level;
   int discrete;
+  __device__ long long lv(int level_, int slot) const { return (long long)lvl[level_ * LV_STRIDE + slot]; }
+  __device__ long long n_soup() const { long long n = lv(level, LV_SOUP); return n < cap_np ? n : cap_np; }
+  __device__ long long n_conn() const { long long n = lv(level, LV_CONN); return n < cap_np ? n : cap_np; }
+  __device__ long long n_lv(int slot) const { long long n = lv(level, slot); return n < cap_items ? n : cap_items; }
+  __device__ long long n_res() const { long long n = lv(level, LV_ITEMS); return n < cap_res ? n : cap_res; }
 };
+
+// ------------------------------------------------- block-level primitives
+// Chunked kernels split a list of n entries into gridDim.x contiguous ranges,
+// one per block; a producer stores each block's total in part[block], and its
+// consumer (launched with the same grid) takes its offset as the sum of the
+// partials before it.  This replaces a separate scan and keeps the lists in
+// order.
+__device__ __forceinline__ void chunk_of(long long n, long long& b, long long& e) {
+  const long long ch = (n + gridDim.x - 1) / gridDim.x;
+  b = (long long)blockIdx.x * ch;
+  e = b + ch;
+  if (b > n) b = n;
+  if (e > n) e = n;
+}
+__device__ long long block_sum(long long v) {
+  __shared__ long long red[16];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  long long s = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;
+}
+// Exclusive scan over the block; total = sum over the block.
+__device__ int block_scan(int v, int& total) {
+  __shared__ int ws[16];
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) ws[wid] = x;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+    if (i < wid) before += ws[i];
+    tot += ws[i];
+  }
+  total = tot;
+  return before + x - v;
+}
+__device__ void partial_prefix(const int* part, long long& before, long long& total) {
+  long long b = 0, t = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+    long long v = part[i];
+    t += v;
+    if (i < (int)blockIdx.x) b += v;
+  }
+  before = block_sum(b);
+  total = block_sum(t);
+}
 
 // ----------------------------------------------------------------- set-up
 // Per pair: poses, frame changes, kinematics (CCD: future = extrapolate(max_time)).
 __global__ void k_setup(int n_pairs, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
-                        int ccd) {
+                        int ccd, int mt_from_bodies) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     int a = pairs[2 * p], b = pairs[2 * p + 1];
     const double* ra = bodies + 40 * (size_t)a;
     const double* rb = bodies + 40 * (size_t)b;
     body_state ca = load_state(ra), cb = load_state(rb);
     body_state fa, fb;
-    double mt = ccd ? max_time[p] : 0.0;
+    // max_time: given per pair, or min(last steps of a, b) (separate_clusters.cpp:490)
+    double mt = !ccd ? 0.0 : mt_from_bodies ? ((rb[39] < ra[39]) ? rb[39] : ra[39]) : max_time[p];
     if (ccd) {
       fa = extrapolate(ca, mt, ra + 28);
       fb = extrapolate(cb, mt, rb + 28);
@@ -190,41 +262,59 @@ __global__ void k_setup(int n_pairs, const int32_t* pairs, const double* max_tim
   }
 }
 
-__global__ void k_roots(int n_pairs, NP* soup, unsigned long long* cnt, unsigned long long* pass_min, int* pass_flag,
-                        double* min_time, const double* pair_rec, double* scale) {
+// Root node pairs, pass state, counters.  keep_ccd leaves min_time / scale
+// (the CCD results of the same step) untouched.
+__global__ void k_roots(Dev D, const double* pair_rec, double* scale, int keep_ccd) {
+  const int n_pairs = D.n_pairs;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
-    soup[p] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
-    pass_min[p] = ~0ull;
-    pass_flag[p] = 0;
-    min_time[p] = pair_rec[(size_t)PR_STRIDE * p + PR_MAXT];
-    scale[p] = 1.0;
+    D.soup[p] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
+    D.pass_min_s[p] = ~0ull;
+    D.pass_min_c[p] = ~0ull;
+    D.pass_flag_s[p] = 0;
+    D.pass_flag_c[p] = 0;
+    if (!keep_ccd) {
+      D.min_time[p] = pair_rec[(size_t)PR_STRIDE * p + PR_MAXT];
+      scale[p] = 1.0;
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int i = 0; i < C_COUNT; ++i) cnt[i] = 0;
-    cnt[C_SOUP] = (unsigned long long)n_pairs;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)
+      if (!(keep_ccd && i == C_OVERFLOW)) D.cnt[i] = 0;  // a step reports an overflow of its CCD phase too
+    for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)
+      D.lvl[i] = i == LV_SOUP ? (unsigned long long)n_pairs : 0ull;
   }
 }
 
-// Per pass: fold "min toc of every hit of this pass" into min_time when a
+// Folds one pass: "min toc of every hit of this pass" enters min_time when a
 // leaf-leaf node pair of the pair had hits (full_tree_comparison.h:106-108,167-169).
-__global__ void k_fold_min(int n_pairs, unsigned long long* pass_min, int* pass_flag, double* min_time,
-                           const double* pair_rec) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void fold_pass(const Dev& D, const double* pair_rec, unsigned long long* pass_min,
+                                          int* pass_flag) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < D.n_pairs; p += gridDim.x * blockDim.x) {
     if (pass_flag[p] && pass_min[p] != ~0ull) {
       double t = __longlong_as_double((long long)pass_min[p]) * pair_rec[(size_t)PR_STRIDE * p + PR_MAXT];
-      if (t < min_time[p]) min_time[p] = t;
+      if (t < D.min_time[p]) D.min_time[p] = t;
     }
     pass_min[p] = ~0ull;
     pass_flag[p] = 0;
   }
 }
+__global__ void k_fold_final(Dev D) {
+  fold_pass(D, D.pair_rec, D.pass_min_s, D.pass_flag_s);
+  fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
+}
 
 // --------------------------------------------------------- CCD soup level
+// One wavefront per soup node pair (chunked ranges), lane = triangle pair:
+// swept-AABB overlap mask, its popcount and the block partial; also folds the
+// connected pass of the previous level.
 __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
-  const long long n = (long long)D.cnt[C_SOUP];
+  fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
+  const long long n = D.n_soup();
   const int lane = lane_id();
-  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
-  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
+  long long rb, re;
+  chunk_of(n, rb, re);
+  long long acc = 0;
+  for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {
     NP np = D.soup[w];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
@@ -259,13 +349,16 @@ __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
         hit = hit && o;
       }
     }
-    long long slot = wave_alloc(hit, &D.cnt[C_ITEMS]);
-    if (hit) {
-      if (slot < D.cap_items) D.items[slot] = Item{(uint32_t)w, (uint32_t)lane};
-      else D.cnt[C_OVERFLOW] = 1;
+    const unsigned long long m = __ballot(hit);
+    if (lane == 0) {
+      D.masks[w] = m;
+      D.cnt_a[w] = __popcll(m);
+      D.np_hit[w] = 0;
+      acc += __popcll(m);
     }
-    if (lane == 0) D.np_hit[w] = 0;
   }
+  const long long tot = block_sum(acc);
+  if (threadIdx.x == 0) D.part_a[blockIdx.x] = (int)tot;
 }
 
 __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<double> Q, double toc) {
@@ -274,10 +367,10 @@ __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<
 
 // Point-triangle sub-tests of soup items; slot 5k+dir (k: vertex index,
 // dir 0: vertex of a vs triangle of b, dir 1: vertex of b vs triangle of a).
-__global__ void __launch_bounds__(256) k_soup_point(Dev D, long long first, long long count) {
-  const long long total = count * 6;
+__global__ void __launch_bounds__(256) k_soup_point(Dev D) {
+  const long long total = D.n_res() * 6;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
-    long long it = first + t / 6;
+    long long it = t / 6;
     int s = (int)(t % 6), k = s >> 1, dir = s & 1;
     Item item = D.items[it];
     NP np = D.soup[item.np];
@@ -292,15 +385,15 @@ __global__ void __launch_bounds__(256) k_soup_point(Dev D, long long first, long
     double toc = 1.0, d;
     if (dir == 0) d = point_tri(corner(ta, k), pr + PR_MAB0, pr + PR_MAB1, tb, pr + PR_TB0, pr + PR_TB1, P, Q, toc);
     else d = point_tri(corner(tb, k), pr + PR_MBA0, pr + PR_MBA1, ta, pr + PR_TA0, pr + PR_TA1, P, Q, toc);
-    store_res(D.res + ((size_t)(it - first) * RES_SLOTS + 5 * k + dir) * 8, d, P, Q, toc);
+    store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + dir) * 8, d, P, Q, toc);
   }
 }
 
 // Edge-edge sub-tests of soup items; slot 5k+2+l (edge k of a, edge l of b).
-__global__ void __launch_bounds__(256) k_soup_edge(Dev D, long long first, long long count) {
-  const long long total = count * 9;
+__global__ void __launch_bounds__(256) k_soup_edge(Dev D) {
+  const long long total = D.n_res() * 9;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
-    long long it = first + t / 9;
+    long long it = t / 9;
     int s = (int)(t % 9), k = s / 3, l = s % 3;
     Item item = D.items[it];
     NP np = D.soup[item.np];
@@ -318,17 +411,17 @@ __global__ void __launch_bounds__(256) k_soup_edge(Dev D, long long first, long 
     double d = edge_edge(xform(pr + PR_TA0, a0), xform(pr + PR_TA0, a1), xform(pr + PR_TA1, a0), xform(pr + PR_TA1, a1),
                          xform(pr + PR_TB0, b0), xform(pr + PR_TB0, b1), xform(pr + PR_TB1, b0), xform(pr + PR_TB1, b1),
                          P, Q, toc);
-    store_res(D.res + ((size_t)(it - first) * RES_SLOTS + 5 * k + 2 + l) * 8, d, P, Q, toc);
+    store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8, d, P, Q, toc);
   }
 }
 
 // triTriCCDLinear's selection over its 15 sub-results in the reference order
 // (continuous_comparison.h:680-779), hit test (continuous_comparison.h:817),
 // contact emission for leaf-leaf node pairs.
-__global__ void __launch_bounds__(256) k_soup_reduce(Dev D, long long first, long long count) {
+__global__ void __launch_bounds__(256) k_soup_reduce(Dev D) {
+  const long long count = D.n_res();
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
-    long long it = first + t;
-    Item item = D.items[it];
+    Item item = D.items[t];
     const double* r = D.res + (size_t)t * RES_SLOTS * 8;
     double best = 1.0 / 0.0, toc = 1.0;
     int win = -1;
@@ -341,12 +434,12 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D, long long first, lon
     bool hit = best < pr[PR_SEARCH];
     if (!hit) continue;
     atomicOr(&D.np_hit[item.np], 1);
-    atomic_min_pos(&D.pass_min[np.pair], toc);
+    atomic_min_pos(&D.pass_min_s[np.pair], toc);
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const NodeRec na = D.nodes[pi[2] + np.na];
     const NodeRec nb = D.nodes[pi[3] + np.nb];
     if (na.inner || nb.inner) continue;
-    D.pass_flag[np.pair] = 1;
+    D.pass_flag_s[np.pair] = 1;
     unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
     if ((long long)slot >= D.cap_contacts) {
       D.cnt[C_OVERFLOW] = 1;
@@ -369,32 +462,7 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D, long long first, lon
 //   [0, 3nm)        vertex v = s/m of a   vs triangle s%m of b
 //   [3nm, 6nm)      vertex v = s'/n of b  vs triangle s'%n of a
 //   edge list       edge s/(3m) of a      vs edge s%(3m) of b   (offset 6nm)
-__global__ void __launch_bounds__(256) k_leaf_enum(Dev D) {
-  const long long n = (long long)D.cnt[C_CONN];
-  const int lane = lane_id();
-  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
-  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
-    NP np = D.conn[w];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    int na = D.nodes[pi[2] + np.na].ntris, nb = D.nodes[pi[3] + np.nb].ntris;
-    int npt = 6 * na * nb, nee = 9 * na * nb;
-    long long bpt = 0, bee = 0;
-    if (lane == 0) {
-      bpt = (long long)atomicAdd(&D.cnt[C_PT], (unsigned long long)npt);
-      bee = (long long)atomicAdd(&D.cnt[C_EE], (unsigned long long)nee);
-    }
-    bpt = __shfl(bpt, 0);
-    bee = __shfl(bee, 0);
-    if (bpt + npt > D.cap_items || bee + nee > D.cap_items) {
-      if (lane == 0) D.cnt[C_OVERFLOW] = 1;
-      continue;
-    }
-    for (int s = lane; s < npt; s += 64) D.pt_items[bpt + s] = Item{(uint32_t)w, (uint32_t)s};
-    for (int s = lane; s < nee; s += 64) D.ee_items[bee + s] = Item{(uint32_t)w, (uint32_t)s};
-    if (lane == 0) D.np_hit[w] = 0;
-  }
-}
-
+// The sub-test items are written by k_level_write.
 __device__ __forceinline__ void emit_leaf(Dev& D, const NP& np, const double* pr, int kind, int ia, int ib, int sub,
                                           v3<double> A, v3<double> B, double toc) {
   unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
@@ -411,9 +479,10 @@ __device__ __forceinline__ void emit_leaf(Dev& D, const NP& np, const double* pr
   D.contacts[slot] = c;
 }
 
-__global__ void __launch_bounds__(256) k_leaf_point(Dev D, long long first, long long count) {
+__global__ void __launch_bounds__(256) k_leaf_point(Dev D) {
+  const long long count = D.n_lv(LV_PT);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
-    Item item = D.pt_items[first + t];
+    Item item = D.pt_items[t];
     NP np = D.conn[item.np];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
@@ -437,18 +506,18 @@ __global__ void __launch_bounds__(256) k_leaf_point(Dev D, long long first, long
       kind = 2, ia = i, ib = v;
     }
     if (d < pr[PR_SEARCH]) {
-      D.np_hit[item.np] = 1;
-      D.pass_flag[np.pair] = 1;
-      atomic_min_pos(&D.pass_min[np.pair], toc);
+      D.pass_flag_c[np.pair] = 1;
+      atomic_min_pos(&D.pass_min_c[np.pair], toc);
       if (kind == 1) emit_leaf(D, np, pr, kind, ia, ib, s, P, Q, toc);
       else emit_leaf(D, np, pr, kind, ia, ib, s, Q, P, toc);  // continuous_comparison.h:876
     }
   }
 }
 
-__global__ void __launch_bounds__(256) k_leaf_edge(Dev D, long long first, long long count) {
+__global__ void __launch_bounds__(256) k_leaf_edge(Dev D) {
+  const long long count = D.n_lv(LV_EE);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
-    Item item = D.ee_items[first + t];
+    Item item = D.ee_items[t];
     NP np = D.conn[item.np];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
@@ -466,9 +535,8 @@ __global__ void __launch_bounds__(256) k_leaf_edge(Dev D, long long first, long 
                          xform(pr + PR_TB0, b0), xform(pr + PR_TB0, b1), xform(pr + PR_TB1, b0), xform(pr + PR_TB1, b1),
                          P, Q, toc);
     if (d < pr[PR_SEARCH]) {
-      D.np_hit[item.np] = 1;
-      D.pass_flag[np.pair] = 1;
-      atomic_min_pos(&D.pass_min[np.pair], toc);
+      D.pass_flag_c[np.pair] = 1;
+      atomic_min_pos(&D.pass_min_c[np.pair], toc);
       emit_leaf(D, np, pr, 3, ea, eb, 6 * n * m + s, Q, P, toc);  // continuous_comparison.h:903
     }
   }
@@ -568,15 +636,17 @@ __device__ int cull_class(const double* pa, const double* pb, const tri<double>&
   return 1;
 }
 
-// One wavefront per node pair: which triangle pairs need the FP64 test.
-// FP64 mode keeps every valid pair; FP32 mode drops certain misses, and for
-// node pairs with an inner node (they emit no contacts, only "any hit"
-// matters) a certain hit settles the node pair without any FP64 work.
-__global__ void __launch_bounds__(256) k_tri_mask(Dev D, long long n, int fp32, unsigned long long* masks, int* counts) {
+// One wavefront per node pair (chunked ranges): which triangle pairs need the
+// FP64 test.  FP64 mode keeps every valid pair; FP32 mode drops certain
+// misses, and for node pairs with an inner node (they emit no contacts, only
+// "any hit" matters) a certain hit settles the node pair without FP64 work.
+__global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
   const int lane = lane_id();
-  if (blockIdx.x == 0 && threadIdx.x == 0) counts[n] = 0;
-  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
-  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
+  const long long n = D.n_soup();
+  long long rb, re;
+  chunk_of(n, rb, re);
+  long long acc = 0;
+  for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {
     NP np = D.soup[w];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const NodeRec na = D.nodes[pi[2] + np.na];
@@ -596,35 +666,55 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, long long n, int fp32, 
     bool settled = inner && __ballot(cls == 2) != 0ull;
     if (settled) keep = 0ull;
     if (lane == 0) {
-      masks[w] = keep;
-      counts[w] = __popcll(keep);
+      D.masks[w] = keep;
+      D.cnt_a[w] = __popcll(keep);
       D.np_hit[w] = settled ? 1 : 0;
+      acc += __popcll(keep);
     }
+  }
+  const long long tot = block_sum(acc);
+  if (threadIdx.x == 0) D.part_a[blockIdx.x] = (int)tot;
+}
+
+// Kept triangle pairs of every node pair to the item list at scanned offsets
+// (node-pair order, then triangle-pair order); the level's item count goes to
+// its slot.  Same grid as the mask producer.
+__global__ void __launch_bounds__(256) k_mask_enum(Dev D, long long cap) {
+  __shared__ int toff[256];
+  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const long long n = D.n_soup();
+  long long rb, re, before, total;
+  chunk_of(n, rb, re);
+  partial_prefix(D.part_a, before, total);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    D.lvl[D.level * LV_STRIDE + LV_ITEMS] = (unsigned long long)total;
+    if (total > cap) D.cnt[C_OVERFLOW] = 1;
+    atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)total);
+  }
+  long long carry = before;
+  for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
+    const long long w = t0 + threadIdx.x;
+    int tile_total;
+    const int ex = block_scan(w < re ? D.cnt_a[w] : 0, tile_total);
+    toff[threadIdx.x] = ex;
+    __syncthreads();
+    const int in_tile = (int)((re - t0) < (long long)blockDim.x ? (re - t0) : (long long)blockDim.x);
+    for (int j = wid; j < in_tile; j += nw) {
+      const long long ww = t0 + j;
+      const unsigned long long m = D.masks[ww];
+      if ((m >> lane) & 1ull) {
+        const long long slot = carry + toff[j] + __popcll(m & lanes_below());
+        if (slot < cap) D.items[slot] = Item{(uint32_t)ww, (uint32_t)lane};
+      }
+    }
+    carry += tile_total;
+    __syncthreads();
   }
 }
 
-// One wavefront per node pair: kept triangle pairs to the item list, at the
-// node pair's scanned offset, in triangle-pair order.
-__global__ void __launch_bounds__(256) k_tri_enum_masked(Dev D, long long n, const unsigned long long* masks,
-                                                          const int* offs) {
-  const int lane = lane_id();
-  const long long wstride = (long long)gridDim.x * (blockDim.x >> 6);
-  for (long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); w < n; w += wstride) {
-    unsigned long long m = masks[w];
-    long long base = offs[w];
-    if ((m >> lane) & 1ull) {
-      long long slot = base + __popcll(m & lanes_below());
-      if (slot < D.cap_items) D.items[slot] = Item{(uint32_t)w, (uint32_t)lane};
-      else D.cnt[C_OVERFLOW] = 1;
-    }
-    if (lane == 0 && w == n - 1) D.cnt[C_LVL_ITEMS] = (unsigned long long)offs[n];
-  }
-}
-
-// Grid-stride over a device-resident item count (no host round trip).
-__global__ void __launch_bounds__(256) k_tri_dist_dev(Dev D, const int* n_items_dev) {
-  long long count = (long long)*n_items_dev;
-  if (count > D.cap_items) count = D.cap_items;
+// FP64 triangle-pair distances of the level's items (grid-stride).
+__global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
+  const long long count = D.n_lv(LV_ITEMS);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
     Item item = D.items[t];
     NP np = D.soup[item.np];
@@ -658,82 +748,166 @@ __global__ void __launch_bounds__(256) k_tri_dist_dev(Dev D, const int* n_items_
   }
 }
 
-// Children of hit node pairs (full_tree_comparison.h:26-62, 99-150), in two
-// passes: counts per destination list, then writes at scanned offsets, which
+// Children of hit node pairs (full_tree_comparison.h:26-62, 99-150) and, in
+// CCD, the sub-tests of the connected (leaf-leaf) list, in two chunked
+// passes: counts + block partials, then writes at scanned offsets, which
 // keeps every list in the reference's breadth-first order.
 __device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
   return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
 }
-__global__ void k_expand_count(Dev D, long long n, int from_soup, int* cs, int* cc) {
-  const NP* list = from_soup ? D.soup : D.conn;
-  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w <= n; w += (long long)gridDim.x * blockDim.x) {
-    int ns = 0, nc = 0;
-    if (w < n && D.np_hit[w]) {
-      NP np = list[w];
-      const int32_t* pi = D.pair_info + 4 * np.pair;
-      const NodeRec na = D.nodes[pi[2] + np.na];
-      const NodeRec nb = D.nodes[pi[3] + np.nb];
-      if (na.inner || nb.inner) {
-        bool adv_a = expand_side(na, nb);
-        const NodeRec& par = adv_a ? na : nb;
-        for (int c = 0; c < par.nchild; ++c) {
-          bool to_soup = true;
-          if (!D.discrete) {
-            const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
-            const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
-            to_soup = x.inner || y.inner;
+__global__ void __launch_bounds__(256) k_level_count(Dev D) {
+  {  // soup: children per destination list
+    const long long n = D.n_soup();
+    long long rb, re;
+    chunk_of(n, rb, re);
+    long long as = 0, ac = 0;
+    for (long long w = rb + threadIdx.x; w < re; w += blockDim.x) {
+      int ns = 0, nc = 0;
+      if (D.np_hit[w]) {
+        NP np = D.soup[w];
+        const int32_t* pi = D.pair_info + 4 * np.pair;
+        const NodeRec na = D.nodes[pi[2] + np.na];
+        const NodeRec nb = D.nodes[pi[3] + np.nb];
+        if (na.inner || nb.inner) {
+          bool adv_a = expand_side(na, nb);
+          const NodeRec& par = adv_a ? na : nb;
+          for (int c = 0; c < par.nchild; ++c) {
+            bool to_soup = true;
+            if (!D.discrete) {
+              const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
+              const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
+              to_soup = x.inner || y.inner;
+            }
+            ns += to_soup, nc += !to_soup;
           }
-          ns += to_soup, nc += !to_soup;
         }
       }
+      D.cnt_s[w] = ns;
+      D.cnt_c[w] = nc;
+      as += ns, ac += nc;
     }
-    cs[w] = ns;
-    cc[w] = nc;
+    const long long ts = block_sum(as), tc = block_sum(ac);
+    if (threadIdx.x == 0) D.part_s[blockIdx.x] = (int)ts, D.part_c[blockIdx.x] = (int)tc;
+  }
+  if (!D.discrete) {
+    {  // connected list: sub-test counts
+      const long long n = D.n_conn();
+      long long rb, re;
+      chunk_of(n, rb, re);
+      long long ap = 0, ae = 0;
+      for (long long w = rb + threadIdx.x; w < re; w += blockDim.x) {
+        NP np = D.conn[w];
+        const int32_t* pi = D.pair_info + 4 * np.pair;
+        int na = D.nodes[pi[2] + np.na].ntris, nb = D.nodes[pi[3] + np.nb].ntris;
+        D.cnt_pt[w] = 6 * na * nb;
+        D.cnt_ee[w] = 9 * na * nb;
+        ap += 6 * na * nb, ae += 9 * na * nb;
+      }
+      const long long tp = block_sum(ap), te = block_sum(ae);
+      if (threadIdx.x == 0) D.part_pt[blockIdx.x] = (int)tp, D.part_ee[blockIdx.x] = (int)te;
+    }
+    fold_pass(D, D.pair_rec, D.pass_min_s, D.pass_flag_s);  // the soup pass of this level is complete
   }
 }
-__global__ void k_expand_write(Dev D, long long n, int from_soup, const int* os, const int* oc) {
-  const NP* list = from_soup ? D.soup : D.conn;
-  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n; w += (long long)gridDim.x * blockDim.x) {
-    if (!D.np_hit[w]) continue;
-    NP np = list[w];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    if (!na.inner && !nb.inner) continue;
-    bool adv_a = expand_side(na, nb);
-    const NodeRec& par = adv_a ? na : nb;
-    long long ps = os[w], pc = oc[w];
-    for (int c = 0; c < par.nchild; ++c) {
-      uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
-      uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
-      NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-      bool to_soup = true;
-      if (!D.discrete) to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
-      long long slot = to_soup ? ps++ : pc++;
-      if (slot < D.cap_np) (to_soup ? D.soup_next : D.conn_next)[slot] = child;
-      else D.cnt[C_OVERFLOW] = 1;
+
+__global__ void __launch_bounds__(256) k_level_write(Dev D) {
+  __shared__ int toff_s[256], toff_c[256];
+  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  {
+    const long long n = D.n_soup();
+    long long rb, re, bs, ts, bc, tc;
+    chunk_of(n, rb, re);
+    partial_prefix(D.part_s, bs, ts);
+    partial_prefix(D.part_c, bc, tc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP] = (unsigned long long)ts;
+      D.lvl[(D.level + 1) * LV_STRIDE + LV_CONN] = D.discrete ? 0ull : (unsigned long long)tc;
+      if (ts > D.cap_np || tc > D.cap_np) D.cnt[C_OVERFLOW] = 1;
+    }
+    for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
+      const long long w = t0 + threadIdx.x;
+      int tts, ttc;
+      const int es = block_scan(w < re ? D.cnt_s[w] : 0, tts);
+      const int ec = block_scan(w < re ? D.cnt_c[w] : 0, ttc);
+      if (w < re && D.cnt_s[w] + D.cnt_c[w] > 0) {
+        NP np = D.soup[w];
+        const int32_t* pi = D.pair_info + 4 * np.pair;
+        const NodeRec na = D.nodes[pi[2] + np.na];
+        const NodeRec nb = D.nodes[pi[3] + np.nb];
+        bool adv_a = expand_side(na, nb);
+        const NodeRec& par = adv_a ? na : nb;
+        long long ps = bs + es, pc = bc + ec;
+        for (int c = 0; c < par.nchild; ++c) {
+          uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
+          uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
+          NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
+          bool to_soup = true;
+          if (!D.discrete) to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
+          long long slot = to_soup ? ps++ : pc++;
+          if (slot < D.cap_np) (to_soup ? D.soup_next : D.conn_next)[slot] = child;
+        }
+      }
+      bs += tts, bc += ttc;
     }
   }
-}
-// Next-level sizes from the scanned totals.
-__global__ void k_advance_scanned(unsigned long long* cnt, const int* os_total, const int* oc_total, int have_conn) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    cnt[C_SOUP] = os_total ? (unsigned long long)*os_total : 0ull;
-    cnt[C_CONN] = (have_conn && oc_total) ? (unsigned long long)*oc_total : 0ull;
-    cnt[C_SOUP_NEXT] = 0;
-    cnt[C_CONN_NEXT] = 0;
-    cnt[C_ITEMS] = 0;
-    cnt[C_PT] = 0;
-    cnt[C_EE] = 0;
+  if (!D.discrete) {
+    const long long n = D.n_conn();
+    long long rb, re, bp, tp, be, te;
+    chunk_of(n, rb, re);
+    partial_prefix(D.part_pt, bp, tp);
+    partial_prefix(D.part_ee, be, te);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      D.lvl[D.level * LV_STRIDE + LV_PT] = (unsigned long long)tp;
+      D.lvl[D.level * LV_STRIDE + LV_EE] = (unsigned long long)te;
+      if (tp > D.cap_items || te > D.cap_items) D.cnt[C_OVERFLOW] = 1;
+    }
+    for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
+      const long long w = t0 + threadIdx.x;
+      int ttp, tte;
+      const int ep = block_scan(w < re ? D.cnt_pt[w] : 0, ttp);
+      const int ee = block_scan(w < re ? D.cnt_ee[w] : 0, tte);
+      toff_s[threadIdx.x] = ep;
+      toff_c[threadIdx.x] = ee;
+      __syncthreads();
+      const int in_tile = (int)((re - t0) < (long long)blockDim.x ? (re - t0) : (long long)blockDim.x);
+      for (int j = wid; j < in_tile; j += nw) {  // one wavefront per connected node pair
+        const long long ww = t0 + j;
+        const int npt = D.cnt_pt[ww], nee = D.cnt_ee[ww];
+        const long long op = bp + toff_s[j], oe = be + toff_c[j];
+        for (int q = lane; q < npt; q += 64)
+          if (op + q < D.cap_items) D.pt_items[op + q] = Item{(uint32_t)ww, (uint32_t)q};
+        for (int q = lane; q < nee; q += 64)
+          if (oe + q < D.cap_items) D.ee_items[oe + q] = Item{(uint32_t)ww, (uint32_t)q};
+      }
+      bp += ttp, be += tte;
+      __syncthreads();
+    }
   }
 }
 
 // -------------------------------------------------------- post-processing
-__global__ void k_count(long long n, const ContactRec* c, int* pair_cnt) {
+__device__ __forceinline__ long long n_contacts(const unsigned long long* cnt, long long cap) {
+  long long n = (long long)cnt[C_CONTACTS];
+  return n < cap ? n : cap;
+}
+// Zeroes the per-pair post-processing counters (a kernel, so that the step
+// can be captured in a graph).
+__global__ void k_post_init(int n_pairs, int* pair_cnt, int* pair_fill, int* kept_cnt, int* n_big) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n_pairs; p += gridDim.x * blockDim.x) {
+    pair_cnt[p] = 0;
+    kept_cnt[p] = 0;
+    if (p < n_pairs) pair_fill[p] = 0;
+    if (p == 0) *n_big = 0;
+  }
+}
+__global__ void k_count(const unsigned long long* cnt, long long cap, const ContactRec* c, int* pair_cnt) {
+  const long long n = n_contacts(cnt, cap);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
     atomicAdd(&pair_cnt[c[t].pair], 1);
 }
-__global__ void k_scatter(long long n, const ContactRec* c, const int* pair_off, int* pair_fill, int* perm) {
+__global__ void k_scatter(const unsigned long long* cnt, long long cap, const ContactRec* c, const int* pair_off,
+                          int* pair_fill, int* perm) {
+  const long long n = n_contacts(cnt, cap);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
     int p = c[t].pair;
     int k = atomicAdd(&pair_fill[p], 1);
@@ -1094,6 +1268,123 @@ struct LdsSorter {  // Sorter over LDS keys (same algorithm, LDS operands)
   }
 };
 
+// The same permutation computed by the whole wavefront.  Each Hoare partition
+// of [lo, hi) around the pivot at lo-1 is derived from ranks: with A_k the k-th
+// position from the left holding an element not less than the pivot and B_k
+// the k-th from the right holding one not greater (the pivot's own slot ends
+// that list), the scalar loop swaps (A_k, B_k) for every k < K,
+// K = #{k : A_k < B_k}, and returns min(A_K, B_{K-1}).  The closing insertion
+// sort never moves an element across a leaf segment's border, so it equals a
+// stable sort of every leaf segment; heapsort (depth exhausted) stays on lane 0.
+__device__ int wave_partition(int lo, int hi, int* ord, const LdsKey* sk, int* Apos, int* Bpos) {
+  const int lane = lane_id();
+  const LdsKey P = sk[ord[lo - 1]];
+  int nA = 0, nB = 0;
+  for (int base = lo; base < hi; base += 64) {
+    const int i = base + lane;
+    const bool ge = i < hi && !lds_less(sk[ord[i]], P);
+    const unsigned long long m = __ballot(ge);
+    if (ge) Apos[nA + __popcll(m & lanes_below())] = i;
+    nA += __popcll(m);
+  }
+  for (int top = hi - 1; top >= lo - 1; top -= 64) {
+    const int i = top - lane;
+    const bool le = i >= lo - 1 && !lds_less(P, sk[ord[i]]);
+    const unsigned long long m = __ballot(le);
+    if (le) Bpos[nB + __popcll(m & lanes_below())] = i;
+    nB += __popcll(m);
+  }
+  __syncthreads();
+  const int lim = nA < nB ? nA : nB;
+  int K = 0;
+  for (int base = 0; base < lim; base += 64) {  // A_k < B_k holds for a prefix of k
+    const int q = base + lane;
+    const unsigned long long m = __ballot(q < lim && Apos[q] < Bpos[q]);
+    K += __popcll(m);
+    if (m != ~0ull) break;
+  }
+  const int ak = K < nA ? Apos[K] : hi;
+  const int bk1 = K > 0 ? Bpos[K - 1] : hi;
+  for (int q = lane; q < K; q += 64) {  // the swapped positions are pairwise distinct
+    const int x = ord[Apos[q]], y = ord[Bpos[q]];
+    ord[Apos[q]] = y;
+    ord[Bpos[q]] = x;
+  }
+  __syncthreads();
+  return ak < bk1 ? ak : bk1;
+}
+
+// ord[0, n) := the order std::sort gives.  mark, Apos, Bpos hold n ints each,
+// stk 3 * 64 ints (the segment stack is at most 2*log2(n)+1 deep).
+__device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int* Bpos, int* mark, int* stk) {
+  const int lane = lane_id();
+  for (int i = lane; i < n; i += 64) mark[i] = 0;
+  if (lane == 0) stk[0] = 0, stk[1] = n, stk[2] = 2 * (63 - __clzll((long long)(n > 0 ? n : 1)));
+  __syncthreads();
+  if (n <= 1) return;
+  int sp = 1;
+  while (sp > 0) {
+    --sp;
+    int f = stk[3 * sp], l = stk[3 * sp + 1], d = stk[3 * sp + 2];
+    __syncthreads();
+    bool heaped = false;
+    while (l - f > 16) {
+      if (d == 0) {
+        if (lane == 0) {
+          LdsSorter srt{ord, sk};
+          srt.heap_sort(f, l);
+        }
+        heaped = true;
+        __syncthreads();
+        break;
+      }
+      --d;
+      {  // median of three to the front (uniform decision, lane 0 swaps)
+        const int x = f + 1, y = f + (l - f) / 2, z = l - 1;
+        const LdsKey kx = sk[ord[x]], ky = sk[ord[y]], kz = sk[ord[z]];
+        int s;
+        if (lds_less(kx, ky)) s = lds_less(ky, kz) ? y : (lds_less(kx, kz) ? z : x);
+        else s = lds_less(kx, kz) ? x : (lds_less(ky, kz) ? z : y);
+        __syncthreads();
+        if (lane == 0) {
+          const int t = ord[f];
+          ord[f] = ord[s];
+          ord[s] = t;
+        }
+        __syncthreads();
+      }
+      const int cut = wave_partition(f + 1, l, ord, sk, Apos, Bpos);
+      if (lane == 0) stk[3 * sp] = cut, stk[3 * sp + 1] = l, stk[3 * sp + 2] = d;
+      ++sp;
+      l = cut;
+      __syncthreads();
+    }
+    if (lane == 0 && l > f) mark[f] = heaped ? 2 : 1;
+    __syncthreads();
+  }
+  // final segments tile [0, n); stable-sort the leaves (<= 16 elements each)
+  for (int j = lane; j < n; j += 64) {
+    int f = j;
+    while (mark[f] == 0) --f;
+    int dst = j;
+    if (mark[f] == 1) {
+      int l = j + 1;
+      while (l < n && mark[l] == 0) ++l;
+      const LdsKey kj = sk[ord[j]];
+      int r = 0;
+      for (int i = f; i < l; ++i) {
+        const LdsKey ki = sk[ord[i]];
+        r += lds_less(ki, kj) || (i < j && !lds_less(kj, ki));
+      }
+      dst = f + r;
+    }
+    Bpos[dst] = ord[j];
+  }
+  __syncthreads();
+  for (int j = lane; j < n; j += 64) ord[j] = Bpos[j];
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const ContactRec* c, const int* pair_off, int* perm,
                                                          int mode, int* keep, int* kept_cnt, const double* pair_rec,
                                                          double* scale, int* big, int* n_big) {
@@ -1103,6 +1394,7 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const Cont
   __shared__ int ord[POST_CAP];
   __shared__ double smid[POST_CAP * 3];
   __shared__ int kept_list[POST_CAP];
+  __shared__ int stk[3 * 64];
   const int lane = threadIdx.x;
   for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
@@ -1214,11 +1506,9 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const Cont
         __syncthreads();
         for (int i = lane; i < n; i += 64) ord[i] = i;
         __syncthreads();
-        if (lane == 0) {
-          LdsSorter srt{ord, sk};
-          srt.sort(0, n);
-        }
-        __syncthreads();
+        // skey (no longer needed) holds the two rank lists
+        wave_std_sort(n, ord, sk, reinterpret_cast<int*>(skey), reinterpret_cast<int*>(skey) + POST_CAP, kept_list,
+                      stk);
       }
       // greedy pass in sorted order
       int nk = 0;
@@ -1273,10 +1563,105 @@ __global__ void __launch_bounds__(64) k_gather_wave(int n_pairs, const ContactRe
   }
 }
 
+// Exclusive scan of in[0, n) into out[0, n], out[n] = total (also stored to
+// *total_dst when given), in two chunked kernels of the same grid.
+__global__ void __launch_bounds__(256) k_scan_part(const int* in, long long n, int* part) {
+  long long rb, re;
+  chunk_of(n, rb, re);
+  long long acc = 0;
+  for (long long i = rb + threadIdx.x; i < re; i += blockDim.x) acc += in[i];
+  const long long t = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = (int)t;
+}
+__global__ void __launch_bounds__(256) k_scan_apply(const int* in, int* out, long long n, const int* part,
+                                                    unsigned long long* total_dst) {
+  long long rb, re, before, total;
+  chunk_of(n, rb, re);
+  partial_prefix(part, before, total);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    out[n] = (int)total;
+    if (total_dst) *total_dst = (unsigned long long)total;
+  }
+  for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
+    const long long i = t0 + threadIdx.x;
+    int tt;
+    const int ex = block_scan(i < re ? in[i] : 0, tt);
+    if (i < re) out[i] = (int)(before + ex);
+    before += tt;
+  }
+}
+
+// ------------------------------------------------------ cluster step sizes
+// The cluster step (separate_clusters.cpp:541-598,
+// time_step_selection_dynamic_continuous.cpp:17-21) is a minimum over its
+// pairs' max_time*scale (pairs with CCD contacts) and bodies' last steps, the
+// first pair reading body a (:589-590); a minimum does not depend on the
+// order of the sequential loop.  Steps are non-negative, so they order like
+// their bit patterns.
+__global__ void k_cluster_init(int n_clusters, unsigned long long* step_bits, int* first_any, int* first_ns) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_clusters; k += gridDim.x * blockDim.x)
+    step_bits[k] = ~0ull, first_any[k] = 0x7fffffff, first_ns[k] = 0x7fffffff;
+}
+__global__ void k_cluster_first(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
+                                int* first_any, int* first_ns) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+    const int k = cluster_of_pair[p];
+    atomicMin(&first_any[k], p);
+    const bool sa = bodies[40 * (size_t)pairs[2 * p] + 38] != 0.0, sb = bodies[40 * (size_t)pairs[2 * p + 1] + 38] != 0.0;
+    if (!(sa && sb)) atomicMin(&first_ns[k], p);
+  }
+}
+__global__ void k_cluster_min(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
+                              const int* pair_cnt, const double* scale, const int* first_ns, unsigned long long* step_bits) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+    const int k = cluster_of_pair[p], a = pairs[2 * p], b = pairs[2 * p + 1];
+    const double* ra = bodies + 40 * (size_t)a;
+    const double* rb = bodies + 40 * (size_t)b;
+    const bool sa = ra[38] != 0.0, sb = rb[38] != 0.0;
+    if (sa && sb) continue;
+    const double mt = (rb[39] < ra[39]) ? rb[39] : ra[39];
+    double v = 1e300;
+    if (pair_cnt[p] > 0) {
+      const double s = mt * scale[p];
+      v = (s < v) ? s : v;
+    }
+    const int id = !sa ? a : b;
+    const double last = p == first_ns[k] ? ra[39] : bodies[40 * (size_t)id + 39];
+    v = (last < v) ? last : v;
+    atomicMin(&step_bits[k], (unsigned long long)__double_as_longlong(v));
+  }
+}
+__global__ void k_cluster_final(int n_clusters, float dt, const int32_t* pairs, const double* bodies,
+                                const unsigned long long* step_bits, const int* first_any, double* out) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_clusters; k += gridDim.x * blockDim.x) {
+    double s;
+    if (step_bits[k] != ~0ull) s = __longlong_as_double((long long)step_bits[k]);
+    else if (first_any[k] != 0x7fffffff) s = bodies[40 * (size_t)pairs[2 * first_any[k]] + 39];
+    else s = (double)dt;
+    const double hi = (double)dt;
+    s = (hi < s) ? hi : s;        // std::min(s, dt)
+    out[k] = (s < 1e-3) ? 1e-3 : s;  // std::max(1e-3, .)
+  }
+}
+// Per-body step: its cluster's step, dt for bodies in no pair.
+__global__ void k_body_step_fill(int n_bodies, float dt, double* body_step) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_bodies; i += gridDim.x * blockDim.x) body_step[i] = (double)dt;
+}
+__global__ void k_body_step_pairs(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair,
+                                  const double* cluster_step, double* body_step) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+    const double s = cluster_step[cluster_of_pair[p]];
+    body_step[pairs[2 * p]] = s;
+    body_step[pairs[2 * p + 1]] = s;
+  }
+}
+
 // Body future states: future = current.extrapolate(step).
-__global__ void k_project(int n, double* bodies, const double* step) {
+// skip_static: static bodies keep their future state (PDE_explicit.cpp:49-56).
+__global__ void k_project(int n, double* bodies, const double* step, int skip_static) {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     double* r = bodies + 40 * (size_t)i;
+    if (skip_static && r[38] != 0.0) continue;
     body_state f = extrapolate(load_state(r), step[i], r + 28);
     store_state(f, r + 14);
   }
@@ -1292,13 +1677,37 @@ using namespace d2;
 namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
-                              "leaf_enum", "leaf_point", "leaf_edge", "tri_enum", "tri_dist", "post"};
-enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_ENUM, K_LEAF_POINT,
-           K_LEAF_EDGE, K_TRI_ENUM, K_TRI_DIST, K_POST, K_NUM };
+                              "leaf_point", "leaf_edge", "tri_enum", "tri_dist", "post", "post_pairs", "cluster"};
+enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
+           K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
   size_t n = 0;
+};
+
+enum RunKind { RUN_NARROW = 0, RUN_CCD = 1, RUN_STEP = 2, RUN_KINDS = 3 };
+constexpr int MAX_SEGMENTS = 3;
+
+// Everything a captured run bakes in besides buffer addresses (a
+// reallocation drops every graph).
+struct Sig {
+  int n_pairs = -1, n_bodies = -1, n_clusters = -1, max_depth = -1, precision = -1, filter = -1;
+  float dt = -1.f;
+  long long cap = -1;
+  d2_tune tune{0, 0, 0};
+  int levels[2] = {-1, -1};  // graph levels of each phase
+  bool operator==(const Sig& o) const {
+    return levels[0] == o.levels[0] && levels[1] == o.levels[1] && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
+           precision == o.precision && filter == o.filter && dt == o.dt && cap == o.cap &&
+           tune.chunk_blocks == o.tune.chunk_blocks && tune.stream_blocks == o.tune.stream_blocks &&
+           tune.post_blocks == o.tune.post_blocks;
+  }
+};
+struct Captured {  // one instantiated graph per run kind
+  hipGraphExec_t exec = nullptr;
+  bool valid = false;
+  Sig sig;
 };
 
 }  // namespace
@@ -1308,39 +1717,54 @@ struct d2_ctx {
   int precision = D2_FP64;
   hipStream_t stream = nullptr;
   std::string err;
-  long long cap = 1ll << 22;          // node pairs / items per list
+  long long cap = 1ll << 22;  // node pairs / items per list
   long long cap_contacts = 1ll << 22;
+  long long cap_res = 1ll << 20;  // CCD soup items with result slots
   int cus = 256;
+  d2_tune tune{1024, 2048, 4096};  // 4, 8, 16 per CU of a 256-CU device; reset at creation
   // trees / bodies
   DBuf<NodeRec> nodes;
   DBuf<double> buckets;
   DBuf<int32_t> tree_off, body_tree;
   DBuf<double> bodies;
-  int n_trees = 0, n_bodies = 0;
-  // pairs
-  DBuf<int32_t> pairs, pair_info;
+  int n_trees = 0, n_bodies = 0, max_depth = 0;
+  // pairs and clusters
+  DBuf<int32_t> pairs, pair_info, cluster_of_pair;
   DBuf<double> max_time, pair_rec;
-  int n_pairs = 0;
+  int n_pairs = 0, n_clusters = 0;
   // work
-  DBuf<NP> soup, conn, soup_next, conn_next;
+  DBuf<NP> soup_list[2], conn_list[2];  // level l reads [l & 1] and writes [(l + 1) & 1]
   DBuf<Item> items, pt_items, ee_items;
   DBuf<double> res;
   DBuf<int> np_hit;
   DBuf<ContactRec> contacts;
-  DBuf<unsigned long long> cnt, pass_min;
-  DBuf<int> pass_flag;
+  DBuf<unsigned long long> cnt, lvl, masks;
+  DBuf<int> cnt_a, cnt_s, cnt_c, cnt_pt, cnt_ee, parts;
+  DBuf<unsigned long long> pass_min_s, pass_min_c;
+  DBuf<int> pass_flag_s, pass_flag_c;
   DBuf<double> min_time, scale;
   // post
-  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, n_big;
-  DBuf<int> cnt_a, off_a, cnt_s, off_s, cnt_c, off_c;  // per-level scans
-  DBuf<unsigned long long> masks;
+  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, n_big, scan_part;
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
-  DBuf<unsigned char> scan_tmp;
-  unsigned long long* host_cnt = nullptr;  // pinned
+  // clusters
+  DBuf<unsigned long long> step_bits;
+  DBuf<int> first_any, first_ns;
+  DBuf<double> cluster_step, body_step;
+  DBuf<int32_t> tmp_cluster;
+  // host mirrors (pinned)
+  unsigned long long* host_cnt = nullptr;
+  unsigned long long* host_lvl = nullptr;
   long long n_out = 0;
-  int last_mode = -1;  // 0 narrow, 1 ccd
+  int last_mode = -1;  // 0 narrow, 1 ccd (per-pair CCD values valid in both after a step)
+  bool ccd_valid = false;
   int last_filter = 1;
+  bool out_filtered = false;
+  // graphs
+  bool graph_on = false;
+  bool capturing = false;
+  Captured graphs[RUN_KINDS][MAX_SEGMENTS];
+  int phase_levels[4] = {-1, -1, -1, -1};  // deepest level count seen per phase (narrow, ccd, step ccd, step narrow)
   // profiling
   bool profiling = false;
   long long k_launch[K_NUM] = {};
@@ -1361,9 +1785,25 @@ int fail(d2_ctx* c, int code, const std::string& msg) {
     hipError_t e_ = (expr);                                                                     \
     if (e_ != hipSuccess) return fail(ctx, D2_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
+#define RC(expr)            \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_) return rc_;    \
+  } while (0)
 
+void drop_graphs(d2_ctx* c) {
+  for (auto& row : c->graphs)
+    for (auto& g : row) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g = Captured{};
+    }
+}
+
+// (Re)allocation invalidates captured graphs: they hold buffer addresses.
 template <class T> int ensure(d2_ctx* c, DBuf<T>& b, size_t n) {
   if (b.n >= n && b.p) return D2_OK;
+  if (c->capturing) return fail(c, D2_ERR_STATE, "internal: allocation during graph capture");
+  drop_graphs(c);
   if (b.p) HIPCHK(c, hipFree(b.p));
   b.p = nullptr;
   b.n = 0;
@@ -1385,21 +1825,24 @@ int grid_for(d2_ctx* c, long long threads, int block = 256) {
   if (g < 1) g = 1;
   return (int)g;
 }
+int chunk_grid(const d2_ctx* c) { return c->tune.chunk_blocks; }
+int stream_grid(const d2_ctx* c) { return c->tune.stream_blocks; }
 
 struct Timer {
   d2_ctx* c;
   int id;
   double items;
   hipEvent_t a = nullptr, b = nullptr;
-  Timer(d2_ctx* ctx, int k, double n) : c(ctx), id(k), items(n) {
-    if (c->profiling) {
+  bool on;
+  Timer(d2_ctx* ctx, int k, double n) : c(ctx), id(k), items(n), on(ctx->profiling && !ctx->capturing) {
+    if (on) {
       (void)hipEventCreate(&a);
       (void)hipEventCreate(&b);
       (void)hipEventRecord(a, c->stream);
     }
   }
   ~Timer() {
-    if (c->profiling) {
+    if (on) {
       (void)hipEventRecord(b, c->stream);
       c->pending.push_back({id, {a, b}});
       c->pending_items.push_back(items);
@@ -1429,10 +1872,10 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.buckets = c->buckets.p;
   D.pair_info = c->pair_info.p;
   D.pair_rec = c->pair_rec.p;
-  D.soup = c->soup.p;
-  D.conn = c->conn.p;
-  D.soup_next = c->soup_next.p;
-  D.conn_next = c->conn_next.p;
+  D.soup = c->soup_list[level & 1].p;
+  D.conn = c->conn_list[level & 1].p;
+  D.soup_next = c->soup_list[(level + 1) & 1].p;
+  D.conn_next = c->conn_list[(level + 1) & 1].p;
   D.items = c->items.p;
   D.pt_items = c->pt_items.p;
   D.ee_items = c->ee_items.p;
@@ -1440,222 +1883,386 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.np_hit = c->np_hit.p;
   D.contacts = c->contacts.p;
   D.cnt = c->cnt.p;
-  D.pass_min = c->pass_min.p;
-  D.pass_flag = c->pass_flag.p;
+  D.lvl = c->lvl.p;
+  D.masks = c->masks.p;
+  D.cnt_a = c->cnt_a.p, D.cnt_s = c->cnt_s.p, D.cnt_c = c->cnt_c.p, D.cnt_pt = c->cnt_pt.p, D.cnt_ee = c->cnt_ee.p;
+  const int g = chunk_grid(c);
+  D.part_a = c->parts.p, D.part_s = c->parts.p + g, D.part_c = c->parts.p + 2 * g, D.part_pt = c->parts.p + 3 * g,
+  D.part_ee = c->parts.p + 4 * g;
+  D.pass_min_s = c->pass_min_s.p, D.pass_min_c = c->pass_min_c.p;
+  D.pass_flag_s = c->pass_flag_s.p, D.pass_flag_c = c->pass_flag_c.p;
+  D.min_time = c->min_time.p;
+  D.n_pairs = c->n_pairs;
   D.cap_np = c->cap;
   D.cap_items = c->cap;
+  D.cap_res = c->cap_res;
   D.cap_contacts = c->cap_contacts;
   D.level = level;
   D.discrete = discrete;
   return D;
 }
 
-int read_counters(d2_ctx* c) {
-  HIPCHK(c, hipMemcpyAsync(c->host_cnt, c->cnt.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->host_cnt[C_OVERFLOW])
-    return fail(c, D2_ERR_CAPACITY, "work list capacity exceeded; raise it with d2_set_capacity");
+int level_budget(const d2_ctx* c) { return std::min(MAX_LEVELS, 2 * c->max_depth + 1); }
+
+// Every buffer a run touches, sized before any launch (and before capture).
+int prepare(d2_ctx* c) {
+  const size_t np = (size_t)std::max(c->n_pairs, 1);
+  const size_t g = (size_t)chunk_grid(c);
+  for (int k = 0; k < 2; ++k) {
+    RC(ensure(c, c->soup_list[k], c->cap));
+    RC(ensure(c, c->conn_list[k], c->cap));
+  }
+  RC(ensure(c, c->items, c->cap)); RC(ensure(c, c->pt_items, c->cap));
+  RC(ensure(c, c->ee_items, c->cap)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
+  RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
+  RC(ensure(c, c->contacts, c->cap_contacts)); RC(ensure(c, c->cnt, C_COUNT));
+  RC(ensure(c, c->lvl, (size_t)(MAX_LEVELS + 1) * LV_STRIDE));
+  RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
+  RC(ensure(c, c->pass_min_s, np)); RC(ensure(c, c->pass_min_c, np)); RC(ensure(c, c->pass_flag_s, np));
+  RC(ensure(c, c->pass_flag_c, np)); RC(ensure(c, c->min_time, np)); RC(ensure(c, c->scale, np));
+  RC(ensure(c, c->pair_cnt, np + 1)); RC(ensure(c, c->pair_off, np + 1)); RC(ensure(c, c->pair_fill, np));
+  RC(ensure(c, c->kept_cnt, np + 1)); RC(ensure(c, c->out_off, np + 1)); RC(ensure(c, c->big, np));
+  RC(ensure(c, c->n_big, 1)); RC(ensure(c, c->perm, c->cap_contacts)); RC(ensure(c, c->keep, c->cap_contacts));
+  RC(ensure(c, c->scan_part, g));
   return D2_OK;
 }
 
-int alloc_work(d2_ctx* c) {
-  int rc;
-  if ((rc = ensure(c, c->soup, c->cap)) || (rc = ensure(c, c->conn, c->cap)) || (rc = ensure(c, c->soup_next, c->cap)) ||
-      (rc = ensure(c, c->conn_next, c->cap)) || (rc = ensure(c, c->items, c->cap)) || (rc = ensure(c, c->pt_items, c->cap)) ||
-      (rc = ensure(c, c->ee_items, c->cap)) || (rc = ensure(c, c->np_hit, c->cap)) ||
-      (rc = ensure(c, c->contacts, c->cap_contacts)) || (rc = ensure(c, c->cnt, C_COUNT)) ||
-      (rc = ensure(c, c->cnt_a, c->cap + 1)) || (rc = ensure(c, c->off_a, c->cap + 1)) ||
-      (rc = ensure(c, c->cnt_s, c->cap + 1)) || (rc = ensure(c, c->off_s, c->cap + 1)) ||
-      (rc = ensure(c, c->cnt_c, c->cap + 1)) || (rc = ensure(c, c->off_c, c->cap + 1)) || (rc = ensure(c, c->masks, c->cap)))
-    return rc;
-  const long long res_items = std::min<long long>(c->cap, 1ll << 20);  // results chunk
-  if ((rc = ensure(c, c->res, (size_t)res_items * RES_SLOTS * 8))) return rc;
-  return D2_OK;
+// Exclusive scan of n (host-known) ints on the context stream.
+void scan_host_n(d2_ctx* c, const int* in, int* out, long long n, unsigned long long* total_dst) {
+  const int g = chunk_grid(c);
+  hipLaunchKernelGGL(k_scan_part, dim3(g), dim3(256), 0, c->stream, in, n, c->scan_part.p);
+  hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(256), 0, c->stream, in, out, n, c->scan_part.p, total_dst);
 }
 
-long long res_chunk(d2_ctx* c) { return (long long)(c->res.n / (RES_SLOTS * 8)); }
-
-void swap_next(d2_ctx* c) {
-  std::swap(c->soup, c->soup_next);
-  std::swap(c->conn, c->conn_next);
-}
-
-// Exclusive scan of n ints on the context stream.
-int scan_ints(d2_ctx* c, const int* in, int* out, long long n) {
-  size_t tmp = 0;
-  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, (int)n, c->stream));
-  int rc = ensure(c, c->scan_tmp, tmp);
-  if (rc) return rc;
-  HIPCHK(c, hipcub::DeviceScan::ExclusiveSum(c->scan_tmp.p, tmp, in, out, (int)n, c->stream));
-  return D2_OK;
-}
-
-// Runs the post-processing of the contacts gathered by a traversal.
-int finish(d2_ctx* c, int mode, long long n_contacts) {
-  int rc;
-  const int np = c->n_pairs;
-  if ((rc = ensure(c, c->pair_cnt, np + 1)) || (rc = ensure(c, c->pair_off, np + 1)) || (rc = ensure(c, c->pair_fill, np)) ||
-      (rc = ensure(c, c->perm, n_contacts)) || (rc = ensure(c, c->keep, n_contacts)) || (rc = ensure(c, c->kept_cnt, np + 1)) ||
-      (rc = ensure(c, c->out_off, np + 1)) || (rc = ensure(c, c->big, np)) || (rc = ensure(c, c->n_big, 1)))
-    return rc;
-  Timer tm(c, K_POST, (double)n_contacts);
-  HIPCHK(c, hipMemsetAsync(c->pair_cnt.p, 0, sizeof(int) * (np + 1), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->pair_fill.p, 0, sizeof(int) * std::max(np, 1), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->n_big.p, 0, sizeof(int), c->stream));
-  HIPCHK(c, hipMemsetAsync(c->kept_cnt.p, 0, sizeof(int) * (np + 1), c->stream));
-  if (n_contacts > 0)
-    hipLaunchKernelGGL(k_count, dim3(grid_for(c, n_contacts)), dim3(256), 0, c->stream, n_contacts, c->contacts.p,
-                       c->pair_cnt.p);
-  if ((rc = scan_ints(c, c->pair_cnt.p, c->pair_off.p, np + 1))) return rc;
-  if (n_contacts > 0)
-    hipLaunchKernelGGL(k_scatter, dim3(grid_for(c, n_contacts)), dim3(256), 0, c->stream, n_contacts, c->contacts.p,
-                       c->pair_off.p, c->pair_fill.p, c->perm.p);
-  int fmode = mode == 0 ? (c->last_filter ? 1 : 0) : 2;
-  const int wblocks = std::max(1, std::min(np, c->cus * 16));
-  hipLaunchKernelGGL(k_pair_finish_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                     c->perm.p, fmode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
-  hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->big.p,
-                     c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, fmode, c->keep.p, c->kept_cnt.p, c->pair_rec.p,
-                     c->scale.p);
-  const int* outoff = c->pair_off.p;
-  long long n_out = n_contacts;
-  int filtered = fmode == 1;
-  if (filtered) {
-    if ((rc = scan_ints(c, c->kept_cnt.p, c->out_off.p, np + 1))) return rc;
-    int total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, c->out_off.p + np, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    n_out = total;
-    outoff = c->out_off.p;
+// One breadth-first level of a phase.
+void enqueue_level(d2_ctx* c, int level, int discrete) {
+  Dev D = make_dev(c, level, discrete);
+  const int cg = chunk_grid(c), sg = stream_grid(c);
+  if (discrete) {
+    {
+      Timer tm(c, K_TRI_ENUM, 0.0);
+      hipLaunchKernelGGL(k_tri_mask, dim3(cg), dim3(256), 0, c->stream, D, c->precision == D2_FP32 ? 1 : 0);
+      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap);
+    }
+    {
+      Timer tm(c, K_TRI_DIST, 0.0);
+      hipLaunchKernelGGL(k_tri_dist, dim3(sg), dim3(256), 0, c->stream, D);
+    }
   } else {
-    HIPCHK(c, hipMemcpyAsync(c->out_off.p, c->pair_off.p, sizeof(int) * (np + 1), hipMemcpyDeviceToDevice, c->stream));
+    {
+      Timer tm(c, K_SOUP_CULL, 0.0);
+      hipLaunchKernelGGL(k_soup_cull, dim3(cg), dim3(256), 0, c->stream, D);
+      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap_res);
+    }
+    {
+      Timer tm(c, K_SOUP_POINT, 0.0);
+      hipLaunchKernelGGL(k_soup_point, dim3(sg), dim3(256), 0, c->stream, D);
+    }
+    {
+      Timer tm(c, K_SOUP_EDGE, 0.0);
+      hipLaunchKernelGGL(k_soup_edge, dim3(sg), dim3(256), 0, c->stream, D);
+    }
+    {
+      Timer tm(c, K_SOUP_REDUCE, 0.0);
+      hipLaunchKernelGGL(k_soup_reduce, dim3(sg), dim3(256), 0, c->stream, D);
+    }
   }
-  if ((rc = ensure(c, c->out_i, (size_t)std::max<long long>(n_out, 1) * 8)) ||
-      (rc = ensure(c, c->out_r, (size_t)std::max<long long>(n_out, 1) * 11)))
-    return rc;
-  hipLaunchKernelGGL(k_gather_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p, c->perm.p,
-                     c->keep.p, outoff, filtered, c->out_i.p, c->out_r.p);
+  {
+    Timer tm(c, K_EXPAND, 0.0);
+    hipLaunchKernelGGL(k_level_count, dim3(cg), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_level_write, dim3(cg), dim3(256), 0, c->stream, D);
+  }
+  if (!discrete) {
+    {
+      Timer tm(c, K_LEAF_POINT, 0.0);
+      hipLaunchKernelGGL(k_leaf_point, dim3(sg), dim3(256), 0, c->stream, D);
+    }
+    {
+      Timer tm(c, K_LEAF_EDGE, 0.0);
+      hipLaunchKernelGGL(k_leaf_edge, dim3(sg), dim3(256), 0, c->stream, D);
+    }
+  }
+}
+
+// Post-processing of the contacts of the last phase: per-pair order, then the
+// contact filter (mode 1), the step rule (mode 2) or nothing (mode 0).
+int enqueue_post(d2_ctx* c, int mode) {
+  const int np = c->n_pairs;
+  const int sg = stream_grid(c);
+  const bool filtered = mode == 1;
+  {
+    Timer tm(c, K_POST, 0.0);
+    hipLaunchKernelGGL(k_post_init, dim3(grid_for(c, np + 1)), dim3(256), 0, c->stream, np, c->pair_cnt.p,
+                       c->pair_fill.p, c->kept_cnt.p, c->n_big.p);
+    hipLaunchKernelGGL(k_count, dim3(sg), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts, c->contacts.p,
+                       c->pair_cnt.p);
+    scan_host_n(c, c->pair_cnt.p, c->pair_off.p, np, filtered ? nullptr : c->cnt.p + C_OUT);
+    hipLaunchKernelGGL(k_scatter, dim3(sg), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts, c->contacts.p,
+                       c->pair_off.p, c->pair_fill.p, c->perm.p);
+  }
+  {
+    Timer tm(c, K_POST_PAIRS, (double)np);
+    const int wblocks = std::max(1, std::min(np, c->tune.post_blocks));
+    hipLaunchKernelGGL(k_pair_finish_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
+                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
+    hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->big.p,
+                       c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
+                       c->pair_rec.p, c->scale.p);
+  }
+  if (filtered) {
+    Timer tm(c, K_POST, 0.0);
+    scan_host_n(c, c->kept_cnt.p, c->out_off.p, np, c->cnt.p + C_OUT);
+  }
   HIPCHK(c, hipGetLastError());
-  c->n_out = n_out;
   return D2_OK;
 }
 
-}  // namespace
-
-namespace {
-int traverse(d2_ctx* c, int discrete, int64_t* n_contacts_out) {
-  if (c->n_pairs < 0 || c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "upload trees, bodies and pairs first");
-  (void)hipSetDevice(c->device);
-  int rc = alloc_work(c);
-  if (rc) return rc;
+int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, bool project) {
   const int np = c->n_pairs;
-  if (np > c->cap) return fail(c, D2_ERR_CAPACITY, "more pairs than the work-list capacity");
-  {
-    Timer tm(c, K_SETUP, np);
-    hipLaunchKernelGGL(k_setup, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, np, c->pairs.p,
-                       c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
-                       discrete ? 0 : 1);
+  Timer tm(c, K_CLUSTER, np);
+  hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters,
+                     c->step_bits.p, c->first_any.p, c->first_ns.p);
+  if (np > 0) {
+    hipLaunchKernelGGL(k_cluster_first, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
+                       c->bodies.p, c->first_any.p, c->first_ns.p);
+    hipLaunchKernelGGL(k_cluster_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
+                       c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->step_bits.p);
   }
-  hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, np, c->soup.p, c->cnt.p,
-                     c->pass_min.p, c->pass_flag.p, c->min_time.p, c->pair_rec.p, c->scale.p);
+  hipLaunchKernelGGL(k_cluster_final, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters, dt,
+                     c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p, c->cluster_step.p);
+  if (project) {
+    hipLaunchKernelGGL(k_body_step_fill, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, dt,
+                       c->body_step.p);
+    if (np > 0)
+      hipLaunchKernelGGL(k_body_step_pairs, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p,
+                         cluster_of_pair, c->cluster_step.p, c->body_step.p);
+    hipLaunchKernelGGL(k_project, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p,
+                       c->body_step.p, 1);
+  }
   HIPCHK(c, hipGetLastError());
-  // The host copy of the counters must describe this run even when no level
-  // executes (an empty pair list).
-  if ((rc = read_counters(c))) return rc;
-  long long n_soup = np, n_conn = 0;
-  const int wave_block = 256;
-  const int stream_grid = c->cus * 8;  // persistent-style grid for device-sized launches
-  for (int level = 0; n_soup + n_conn > 0; ++level) {
-    if (level > 15) return fail(c, D2_ERR_RUNTIME, "traversal deeper than 15 levels (contact keys hold 16)");
-    Dev D = make_dev(c, level, discrete);
-    if (n_soup > 0) {
-      if (discrete) {
-        {
-          Timer tm(c, K_TRI_ENUM, (double)n_soup);
-          hipLaunchKernelGGL(k_tri_mask, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0, c->stream, D,
-                             n_soup, c->precision == D2_FP32 ? 1 : 0, c->masks.p, c->cnt_a.p);
-          if ((rc = scan_ints(c, c->cnt_a.p, c->off_a.p, n_soup + 1))) return rc;
-          hipLaunchKernelGGL(k_tri_enum_masked, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0,
-                             c->stream, D, n_soup, c->masks.p, c->off_a.p);
-        }
-        {
-          Timer tm(c, K_TRI_DIST, 0.0);
-          hipLaunchKernelGGL(k_tri_dist_dev, dim3(stream_grid), dim3(256), 0, c->stream, D, c->off_a.p + n_soup);
-        }
-      } else {
-        {
-          Timer tm(c, K_SOUP_CULL, (double)n_soup);
-          hipLaunchKernelGGL(k_soup_cull, dim3(grid_for(c, n_soup * 64, wave_block)), dim3(wave_block), 0, c->stream, D);
-        }
-        if ((rc = read_counters(c))) return rc;
-        long long n_items = (long long)c->host_cnt[C_ITEMS];
-        const long long chunk = res_chunk(c);
-        for (long long f = 0; f < n_items; f += chunk) {
-          long long cntc = std::min(chunk, n_items - f);
-          {
-            Timer tm(c, K_SOUP_POINT, (double)cntc * 6);
-            hipLaunchKernelGGL(k_soup_point, dim3(grid_for(c, cntc * 6)), dim3(256), 0, c->stream, D, f, cntc);
-          }
-          {
-            Timer tm(c, K_SOUP_EDGE, (double)cntc * 9);
-            hipLaunchKernelGGL(k_soup_edge, dim3(grid_for(c, cntc * 9)), dim3(256), 0, c->stream, D, f, cntc);
-          }
-          {
-            Timer tm(c, K_SOUP_REDUCE, (double)cntc);
-            hipLaunchKernelGGL(k_soup_reduce, dim3(grid_for(c, cntc)), dim3(256), 0, c->stream, D, f, cntc);
-          }
-        }
-      }
-      {
-        Timer tm(c, K_EXPAND, (double)n_soup);
-        hipLaunchKernelGGL(k_expand_count, dim3(grid_for(c, n_soup + 1)), dim3(256), 0, c->stream, D, n_soup, 1,
-                           c->cnt_s.p, c->cnt_c.p);
-        if ((rc = scan_ints(c, c->cnt_s.p, c->off_s.p, n_soup + 1))) return rc;
-        if (!discrete && (rc = scan_ints(c, c->cnt_c.p, c->off_c.p, n_soup + 1))) return rc;
-        hipLaunchKernelGGL(k_expand_write, dim3(grid_for(c, n_soup)), dim3(256), 0, c->stream, D, n_soup, 1, c->off_s.p,
-                           c->off_c.p);
-      }
-      if (!discrete)
-        hipLaunchKernelGGL(k_fold_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pass_min.p, c->pass_flag.p,
-                           c->min_time.p, c->pair_rec.p);
-      HIPCHK(c, hipGetLastError());
-    }
-    if (!discrete && n_conn > 0) {
-      {
-        Timer tm(c, K_LEAF_ENUM, (double)n_conn);
-        hipLaunchKernelGGL(k_leaf_enum, dim3(grid_for(c, n_conn * 64, wave_block)), dim3(wave_block), 0, c->stream, D);
-      }
-      if ((rc = read_counters(c))) return rc;
-      long long npt = (long long)c->host_cnt[C_PT], nee = (long long)c->host_cnt[C_EE];
-      if (npt > 0) {
-        Timer tm(c, K_LEAF_POINT, (double)npt);
-        hipLaunchKernelGGL(k_leaf_point, dim3(grid_for(c, npt)), dim3(256), 0, c->stream, D, 0ll, npt);
-      }
-      if (nee > 0) {
-        Timer tm(c, K_LEAF_EDGE, (double)nee);
-        hipLaunchKernelGGL(k_leaf_edge, dim3(grid_for(c, nee)), dim3(256), 0, c->stream, D, 0ll, nee);
-      }
-      hipLaunchKernelGGL(k_fold_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pass_min.p, c->pass_flag.p,
-                         c->min_time.p, c->pair_rec.p);
-      HIPCHK(c, hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_advance_scanned, dim3(1), dim3(64), 0, c->stream, c->cnt.p,
-                       n_soup > 0 ? c->off_s.p + n_soup : nullptr, n_soup > 0 ? c->off_c.p + n_soup : nullptr,
-                       discrete ? 0 : 1);
-    if ((rc = read_counters(c))) return rc;
-    if (discrete && n_soup > 0 && c->profiling) c->k_items[K_TRI_DIST] += (double)c->host_cnt[C_LVL_ITEMS];
-    swap_next(c);
-    n_soup = (long long)c->host_cnt[C_SOUP];
-    n_conn = (long long)c->host_cnt[C_CONN];
+  return D2_OK;
+}
+
+// ---------------------------------------------------------------- phases
+// A run is a list of traversal phases, each: set-up + roots, breadth-first
+// levels, closing fold + post-processing (a step adds the cluster steps and
+// projections between its CCD and discrete phases).
+struct PhaseSpec {
+  int discrete, keep_ccd, mt_from_bodies, post_mode, slot;
+};
+int phases_of(int kind, int filter, PhaseSpec* out) {
+  const int fmode = filter ? 1 : 0;
+  if (kind == RUN_NARROW) {
+    out[0] = {1, 0, 0, fmode, 0};
+    return 1;
   }
-  long long n_contacts = (long long)c->host_cnt[C_CONTACTS];
-  if ((rc = finish(c, discrete ? 0 : 1, n_contacts))) return rc;
+  if (kind == RUN_CCD) {
+    out[0] = {0, 0, 0, 2, 1};
+    return 1;
+  }
+  out[0] = {0, 0, 1, 2, 2};
+  out[1] = {1, 1, 0, fmode, 3};
+  return 2;
+}
+
+void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
+  const int np = c->n_pairs;
+  Timer tm(c, K_SETUP, np);
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, np, c->pairs.p,
+                     c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
+                     ph.discrete ? 0 : 1, ph.mt_from_bodies);
+  Dev D = make_dev(c, 0, ph.discrete);
+  hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, D, c->pair_rec.p,
+                     c->scale.p, ph.keep_ccd);
+}
+
+// Items of level l per kernel, from its sizes (profiling).
+void add_level_items(d2_ctx* c, int level, int discrete) {
+  const unsigned long long* s = c->host_lvl + level * LV_STRIDE;
+  if (discrete) {
+    c->k_items[K_TRI_ENUM] += (double)s[LV_SOUP];
+    c->k_items[K_TRI_DIST] += (double)s[LV_ITEMS];
+  } else {
+    c->k_items[K_SOUP_CULL] += (double)s[LV_SOUP];
+    c->k_items[K_SOUP_POINT] += 6.0 * (double)s[LV_ITEMS];
+    c->k_items[K_SOUP_EDGE] += 9.0 * (double)s[LV_ITEMS];
+    c->k_items[K_SOUP_REDUCE] += (double)s[LV_ITEMS];
+    c->k_items[K_LEAF_POINT] += (double)s[LV_PT];
+    c->k_items[K_LEAF_EDGE] += (double)s[LV_EE];
+  }
+  c->k_items[K_EXPAND] += (double)s[LV_SOUP];
+}
+
+int read_slots(d2_ctx* c, int first, int count) {
+  HIPCHK(c, hipMemcpyAsync(c->host_lvl + first * LV_STRIDE, c->lvl.p + first * LV_STRIDE,
+                           sizeof(unsigned long long) * LV_STRIDE * count, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+// Eager levels from `level` on while the lists are not empty (sizes read back
+// after every level).  *used receives the index one past the last level run.
+int eager_levels(d2_ctx* c, const PhaseSpec& ph, int level, long long n_soup, long long n_conn, int* used) {
+  const int budget = level_budget(c);
+  for (; level < budget && n_soup + n_conn > 0; ++level) {
+    enqueue_level(c, level, ph.discrete);
+    HIPCHK(c, hipGetLastError());
+    RC(read_slots(c, level, 2));
+    if (c->profiling) add_level_items(c, level, ph.discrete);
+    n_soup = (long long)c->host_lvl[(level + 1) * LV_STRIDE + LV_SOUP];
+    n_conn = (long long)c->host_lvl[(level + 1) * LV_STRIDE + LV_CONN];
+  }
+  if (n_soup + n_conn > 0) return fail(c, D2_ERR_RUNTIME, "traversal deeper than the level budget of the uploaded trees");
+  *used = level;
+  return D2_OK;
+}
+
+void enqueue_levels(d2_ctx* c, const PhaseSpec& ph, int from, int to) {
+  for (int level = from; level < to; ++level) enqueue_level(c, level, ph.discrete);
+}
+
+// Closing fold (CCD) and post-processing, then (a step's CCD phase) the
+// cluster steps and future states.
+int enqueue_end(d2_ctx* c, int kind, const PhaseSpec& ph, int levels, float dt) {
+  if (!ph.discrete) {
+    Dev D = make_dev(c, levels, 0);
+    hipLaunchKernelGGL(k_fold_final, dim3(grid_for(c, std::max(c->n_pairs, 1))), dim3(256), 0, c->stream, D);
+  }
+  RC(enqueue_post(c, ph.post_mode));
+  if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true));
+  HIPCHK(c, hipGetLastError());
+  return D2_OK;
+}
+
+int run_eager(d2_ctx* c, int kind, int filter, float dt) {
+  PhaseSpec ph[2];
+  const int n = phases_of(kind, filter, ph);
+  for (int i = 0; i < n; ++i) {
+    enqueue_begin(c, ph[i]);
+    HIPCHK(c, hipGetLastError());
+    int used = 0;
+    RC(eager_levels(c, ph[i], 0, c->n_pairs, 0, &used));
+    c->phase_levels[ph[i].slot] = std::max(c->phase_levels[ph[i].slot], used);
+    RC(enqueue_end(c, kind, ph[i], used, dt));
+  }
+  return D2_OK;
+}
+
+// Graph runs: each phase runs its observed level count inside the graph; when
+// that is below the level budget, the graph ends there, the host reads the
+// next level's sizes and continues eagerly if a deeper level is needed (and
+// the phase's level count grows for the next capture).
+int launch_segment(d2_ctx* c, int kind, int seg, const Sig& sig, const std::vector<std::function<int()>>& ops) {
+  if (seg >= MAX_SEGMENTS) return fail(c, D2_ERR_RUNTIME, "internal: too many graph segments");
+  Captured& g = c->graphs[kind][seg];
+  if (!g.valid || !(g.sig == sig)) {
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    g = Captured{};
+    hipGraph_t graph = nullptr;
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    c->capturing = true;
+    int rc = D2_OK;
+    for (auto& op : ops)
+      if ((rc = op())) break;
+    c->capturing = false;
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, D2_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(c, D2_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    g.valid = true, g.sig = sig;
+  }
+  HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+  return D2_OK;
+}
+
+int run_graph(d2_ctx* c, int kind, int filter, float dt) {
+  PhaseSpec ph[2];
+  const int n = phases_of(kind, filter, ph);
+  for (int i = 0; i < n; ++i)
+    if (c->phase_levels[ph[i].slot] < 0) return run_eager(c, kind, filter, dt);  // observe first
+  const int budget = level_budget(c);
+  Sig sig;
+  sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
+  sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt, sig.cap = c->cap;
+  sig.tune = c->tune;
+  for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
+  std::vector<std::function<int()>> ops;
+  int seg = 0;
+  for (int i = 0; i < n; ++i) {
+    const PhaseSpec p = ph[i];
+    const int L = sig.levels[i];
+    ops.push_back([c, p] { enqueue_begin(c, p); return D2_OK; });
+    ops.push_back([c, p, L] { enqueue_levels(c, p, 0, L); return D2_OK; });
+    int used = L;
+    if (L < budget) {  // end of a segment: check the next level on the host
+      RC(launch_segment(c, kind, seg++, sig, ops));
+      ops.clear();
+      RC(read_slots(c, L, 1));
+      const long long ns = (long long)c->host_lvl[L * LV_STRIDE + LV_SOUP];
+      const long long nc = (long long)c->host_lvl[L * LV_STRIDE + LV_CONN];
+      if (ns + nc > 0) {
+        RC(eager_levels(c, p, L, ns, nc, &used));
+        c->phase_levels[p.slot] = std::max(c->phase_levels[p.slot], used);
+      }
+    }
+    ops.push_back([c, kind, p, used, dt] { return enqueue_end(c, kind, p, used, dt); });
+  }
+  RC(launch_segment(c, kind, seg, sig, ops));
+  return D2_OK;
+}
+
+int ensure_clusters(d2_ctx* c, int n_clusters) {
+  RC(ensure(c, c->step_bits, n_clusters)); RC(ensure(c, c->first_any, n_clusters)); RC(ensure(c, c->first_ns, n_clusters));
+  RC(ensure(c, c->cluster_step, n_clusters)); RC(ensure(c, c->body_step, std::max(c->n_bodies, 1)));
+  return D2_OK;
+}
+
+// Waits for the run, reads the counters, checks capacity.
+int finish_run(d2_ctx* c, int kind, int filter) {
+  HIPCHK(c, hipMemcpyAsync(c->host_cnt, c->cnt.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->host_lvl, c->lvl.p, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE,
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipGetLastError());
   if (c->profiling) collect_timers(c);
-  c->last_mode = discrete ? 0 : 1;
+  const int budget = level_budget(c);
+  const unsigned long long* last = c->host_lvl + budget * LV_STRIDE;
+  if (c->host_cnt[C_OVERFLOW] || (long long)c->host_cnt[C_CONTACTS] > c->cap_contacts)
+    return fail(c, D2_ERR_CAPACITY, "work list capacity exceeded; raise it with d2_set_capacity");
+  if (last[LV_SOUP] || last[LV_CONN])
+    return fail(c, D2_ERR_RUNTIME, "traversal deeper than the level budget of the uploaded trees");
+  c->n_out = (long long)c->host_cnt[C_OUT];
+  c->last_mode = kind == RUN_CCD ? 1 : 0;
+  c->ccd_valid = kind != RUN_NARROW;
+  c->out_filtered = kind != RUN_CCD && filter != 0;
+  c->last_filter = filter ? 1 : 0;
+  return D2_OK;
+}
+
+int run(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
+  if (c->n_pairs < 0 || c->n_bodies <= 0 || c->n_trees <= 0)
+    return fail(c, D2_ERR_STATE, "upload trees, bodies and pairs first");
+  if (2 * c->max_depth + 1 > MAX_LEVELS) return fail(c, D2_ERR_ARG, "trees deeper than 7 levels are not supported");
+  if (kind == RUN_STEP && c->n_clusters <= 0) return fail(c, D2_ERR_STATE, "upload the pairs' clusters first");
+  (void)hipSetDevice(c->device);
+  if (c->n_pairs > c->cap) return fail(c, D2_ERR_CAPACITY, "more pairs than the work-list capacity");
+  RC(prepare(c));
+  if (kind == RUN_STEP) RC(ensure_clusters(c, c->n_clusters));
+  c->ccd_valid = false;
+  c->last_mode = -1;
+  if (c->graph_on) RC(run_graph(c, kind, filter, dt));
+  else RC(run_eager(c, kind, filter, dt));
+  RC(finish_run(c, kind, filter));
   if (n_contacts_out) *n_contacts_out = c->n_out;
   return D2_OK;
 }
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -1677,11 +2284,16 @@ int d2_ctx_create(int device, int precision, d2_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
-  if (hipHostMalloc((void**)&c->host_cnt, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess) {
+  c->tune = d2_tune{4 * c->cus, 8 * c->cus, 16 * c->cus};
+  if (hipHostMalloc((void**)&c->host_cnt, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_lvl, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE,
+                    hipHostMallocDefault) != hipSuccess) {
+    if (c->host_cnt) (void)hipHostFree(c->host_cnt);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return D2_ERR_HIP;
   }
+  std::memset(c->host_lvl, 0, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE);
   *out = c;
   return D2_OK;
 }
@@ -1691,17 +2303,22 @@ int d2_ctx_destroy(d2_ctx* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   collect_timers(c);
+  drop_graphs(c);
   release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
-  release(c->pairs), release(c->pair_info), release(c->max_time), release(c->pair_rec);
-  release(c->soup), release(c->conn), release(c->soup_next), release(c->conn_next);
+  release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
+  for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
   release(c->items), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit);
-  release(c->contacts), release(c->cnt), release(c->pass_min), release(c->pass_flag), release(c->min_time),
-      release(c->scale);
+  release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks);
+  release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
+  release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
+  release(c->min_time), release(c->scale);
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
-      release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_tmp);
-  release(c->big), release(c->n_big), release(c->cnt_a), release(c->off_a), release(c->cnt_s), release(c->off_s),
-      release(c->cnt_c), release(c->off_c), release(c->masks);
+      release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
+  release(c->big), release(c->n_big);
+  release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
+      release(c->tmp_cluster);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
+  if (c->host_lvl) (void)hipHostFree(c->host_lvl);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return D2_OK;
@@ -1719,8 +2336,31 @@ int d2_set_precision(d2_ctx* c, int precision) {
 int d2_set_capacity(d2_ctx* c, int64_t entries) {
   if (!c) return D2_ERR_ARG;
   if (entries < 1024) return fail(c, D2_ERR_ARG, "capacity must be >= 1024");
+  drop_graphs(c);
   c->cap = entries;
   c->cap_contacts = entries;
+  c->cap_res = std::min<long long>(entries, 1ll << 20);
+  return D2_OK;
+}
+
+int d2_set_tune(d2_ctx* c, const d2_tune* t) {
+  if (!c || !t) return D2_ERR_ARG;
+  if (t->chunk_blocks < 1 || t->chunk_blocks > 4096 || t->stream_blocks < 1 || t->stream_blocks > 16384 ||
+      t->post_blocks < 1 || t->post_blocks > 16384)
+    return fail(c, D2_ERR_ARG, "d2_set_tune: value out of range");
+  c->tune = *t;
+  return D2_OK;
+}
+int d2_get_tune(const d2_ctx* c, d2_tune* t) {
+  if (!c || !t) return D2_ERR_ARG;
+  *t = c->tune;
+  return D2_OK;
+}
+
+int d2_set_graph(d2_ctx* c, int on) {
+  if (!c) return D2_ERR_ARG;
+  if (!on) drop_graphs(c);
+  c->graph_on = on != 0;
   return D2_OK;
 }
 
@@ -1729,26 +2369,27 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   if (n_trees <= 0 || !tree_node_offset || !nodes || !buckets) return fail(c, D2_ERR_ARG, "d2_trees_upload: bad arguments");
   int total = tree_node_offset[n_trees];
   if (total <= 0 || tree_node_offset[0] != 0) return fail(c, D2_ERR_ARG, "d2_trees_upload: bad offsets");
+  int max_depth = 0;
   for (int t = 0; t < n_trees; ++t) {
     int b = tree_node_offset[t], e = tree_node_offset[t + 1];
     if (e <= b) return fail(c, D2_ERR_ARG, "d2_trees_upload: empty tree");
     for (int i = b; i < e; ++i) {
       const int32_t* r = nodes + 8 * (size_t)i;
       int cnt = e - b;
-      if (r[1] < 0 || r[1] >= cnt || r[7] < 0 || r[7] > 8 || (r[2] && (r[5] < 0 || r[5] + r[6] > cnt || r[6] < 1 || r[6] > 8)))
+      if (r[1] < 0 || r[1] >= cnt || r[7] < 0 || r[7] > 8 || r[3] < 0 ||
+          (r[2] && (r[5] < 0 || r[5] + r[6] > cnt || r[6] < 1 || r[6] > 8)))
         return fail(c, D2_ERR_ARG, "d2_trees_upload: inconsistent node record");
+      max_depth = std::max(max_depth, (int)r[3]);
     }
   }
   (void)hipSetDevice(c->device);
-  int rc;
-  if ((rc = ensure(c, c->nodes, total)) || (rc = ensure(c, c->buckets, (size_t)total * BUCKET)) ||
-      (rc = ensure(c, c->tree_off, n_trees + 1)))
-    return rc;
+  RC(ensure(c, c->nodes, total)); RC(ensure(c, c->buckets, (size_t)total * BUCKET)); RC(ensure(c, c->tree_off, n_trees + 1));
   HIPCHK(c, hipMemcpyAsync(c->nodes.p, nodes, sizeof(NodeRec) * total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->buckets.p, buckets, sizeof(double) * BUCKET * (size_t)total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->tree_off.p, tree_node_offset, sizeof(int32_t) * (n_trees + 1), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_trees = n_trees;
+  c->max_depth = max_depth;
   return D2_OK;
 }
 
@@ -1759,8 +2400,7 @@ int d2_bodies_upload(d2_ctx* c, int n_bodies, const int32_t* tree_of_body, const
   for (int i = 0; i < n_bodies; ++i)
     if (tree_of_body[i] < 0 || tree_of_body[i] >= c->n_trees) return fail(c, D2_ERR_ARG, "d2_bodies_upload: bad tree index");
   (void)hipSetDevice(c->device);
-  int rc;
-  if ((rc = ensure(c, c->bodies, (size_t)n_bodies * 40)) || (rc = ensure(c, c->body_tree, n_bodies))) return rc;
+  RC(ensure(c, c->bodies, (size_t)n_bodies * 40)); RC(ensure(c, c->body_tree, n_bodies));
   HIPCHK(c, hipMemcpyAsync(c->bodies.p, records, sizeof(double) * 40 * (size_t)n_bodies, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->body_tree.p, tree_of_body, sizeof(int32_t) * n_bodies, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1779,14 +2419,12 @@ int d2_bodies_download(d2_ctx* c, double* records) {
 int d2_project_future(d2_ctx* c, const double* step_per_body) {
   if (!c || !step_per_body) return D2_ERR_ARG;
   if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "no bodies uploaded");
-  DBuf<double> st;
-  int rc = ensure(c, st, c->n_bodies);
-  if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(st.p, step_per_body, sizeof(double) * c->n_bodies, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_project, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p, st.p);
+  RC(ensure(c, c->body_step, c->n_bodies));
+  HIPCHK(c, hipMemcpyAsync(c->body_step.p, step_per_body, sizeof(double) * c->n_bodies, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_project, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p,
+                     c->body_step.p, 0);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  release(st);
   return D2_OK;
 }
 
@@ -1800,12 +2438,9 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
     if (max_time && !(max_time[p] >= 0.0)) return fail(c, D2_ERR_ARG, "d2_pairs_upload: max_time must be >= 0");
   }
   (void)hipSetDevice(c->device);
-  int rc;
   size_t np = std::max(n_pairs, 1);
-  if ((rc = ensure(c, c->pairs, 2 * np)) || (rc = ensure(c, c->max_time, np)) || (rc = ensure(c, c->pair_rec, np * PR_STRIDE)) ||
-      (rc = ensure(c, c->pair_info, 4 * np)) || (rc = ensure(c, c->pass_min, np)) || (rc = ensure(c, c->pass_flag, np)) ||
-      (rc = ensure(c, c->min_time, np)) || (rc = ensure(c, c->scale, np)))
-    return rc;
+  RC(ensure(c, c->pairs, 2 * np)); RC(ensure(c, c->max_time, np)); RC(ensure(c, c->pair_rec, np * PR_STRIDE));
+  RC(ensure(c, c->pair_info, 4 * np));
   if (n_pairs > 0) {
     HIPCHK(c, hipMemcpyAsync(c->pairs.p, pairs, sizeof(int32_t) * 2 * n_pairs, hipMemcpyHostToDevice, c->stream));
     if (max_time)
@@ -1815,19 +2450,48 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_pairs = n_pairs;
+  c->n_clusters = 0;  // clusters describe a pair list
+  c->last_mode = -1;
+  c->ccd_valid = false;
   return D2_OK;
 }
 
+int d2_clusters_upload(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters) {
+  if (!c) return D2_ERR_ARG;
+  if (n_clusters <= 0 || (c->n_pairs > 0 && !cluster_of_pair)) return fail(c, D2_ERR_ARG, "d2_clusters_upload: bad arguments");
+  for (int p = 0; p < c->n_pairs; ++p)
+    if (cluster_of_pair[p] < 0 || cluster_of_pair[p] >= n_clusters)
+      return fail(c, D2_ERR_ARG, "d2_clusters_upload: cluster index out of range");
+  (void)hipSetDevice(c->device);
+  RC(ensure(c, c->cluster_of_pair, std::max(c->n_pairs, 1)));
+  if (c->n_pairs > 0)
+    HIPCHK(c, hipMemcpyAsync(c->cluster_of_pair.p, cluster_of_pair, sizeof(int32_t) * c->n_pairs, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->n_clusters = n_clusters;
+  return D2_OK;
+}
 
 int d2_narrow_run(d2_ctx* c, int filter, int64_t* n_contacts_out) {
   if (!c) return D2_ERR_ARG;
-  c->last_filter = filter ? 1 : 0;
-  return traverse(c, 1, n_contacts_out);
+  return run(c, RUN_NARROW, filter ? 1 : 0, 0.f, n_contacts_out);
 }
 
 int d2_ccd_run(d2_ctx* c, int64_t* n_contacts_out) {
   if (!c) return D2_ERR_ARG;
-  return traverse(c, 0, n_contacts_out);
+  return run(c, RUN_CCD, 0, 0.f, n_contacts_out);
+}
+
+int d2_step_run(d2_ctx* c, float dt, int filter, double* cluster_step_out, int64_t* n_contacts_out) {
+  if (!c) return D2_ERR_ARG;
+  if (!(dt > 0.f)) return fail(c, D2_ERR_ARG, "d2_step_run: dt must be > 0");
+  RC(run(c, RUN_STEP, filter ? 1 : 0, dt, n_contacts_out));
+  if (cluster_step_out) {
+    HIPCHK(c, hipMemcpyAsync(cluster_step_out, c->cluster_step.p, sizeof(double) * c->n_clusters, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return D2_OK;
 }
 
 int d2_narrow(d2_ctx* c, int n_pairs, const int32_t* pairs, int filter, int64_t* n_contacts_out) {
@@ -1844,14 +2508,20 @@ int d2_ccd(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* max_time,
 int d2_result_contacts(d2_ctx* c, int32_t* offsets, int32_t* ints, double* reals) {
   if (!c) return D2_ERR_ARG;
   if (c->last_mode < 0) return fail(c, D2_ERR_STATE, "no result: run the path first");
+  const int np = c->n_pairs;
+  const int* outoff = c->out_filtered ? c->out_off.p : c->pair_off.p;
   if ((ints || reals) && c->n_out > 0) {  // the CSR total and the record count must agree
     int total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, c->out_off.p + c->n_pairs, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&total, outoff + np, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if ((long long)total != c->n_out) return fail(c, D2_ERR_STATE, "result offsets and contact count disagree");
+    RC(ensure(c, c->out_i, (size_t)c->n_out * 8)); RC(ensure(c, c->out_r, (size_t)c->n_out * 11));
+    const int wblocks = std::max(1, std::min(np, c->cus * 16));
+    hipLaunchKernelGGL(k_gather_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
+                       c->perm.p, c->keep.p, outoff, c->out_filtered ? 1 : 0, c->out_i.p, c->out_r.p);
+    HIPCHK(c, hipGetLastError());
   }
-  if (offsets)
-    HIPCHK(c, hipMemcpyAsync(offsets, c->out_off.p, sizeof(int) * (c->n_pairs + 1), hipMemcpyDeviceToHost, c->stream));
+  if (offsets) HIPCHK(c, hipMemcpyAsync(offsets, outoff, sizeof(int) * (np + 1), hipMemcpyDeviceToHost, c->stream));
   if (ints && c->n_out)
     HIPCHK(c, hipMemcpyAsync(ints, c->out_i.p, sizeof(int32_t) * 8 * (size_t)c->n_out, hipMemcpyDeviceToHost, c->stream));
   if (reals && c->n_out)
@@ -1862,7 +2532,7 @@ int d2_result_contacts(d2_ctx* c, int32_t* offsets, int32_t* ints, double* reals
 
 int d2_result_pairs(d2_ctx* c, double* per_pair) {
   if (!c || !per_pair) return D2_ERR_ARG;
-  if (c->last_mode != 1) return fail(c, D2_ERR_STATE, "per-pair results exist after d2_ccd_run");
+  if (!c->ccd_valid) return fail(c, D2_ERR_STATE, "per-pair results exist after d2_ccd_run or d2_step_run");
   std::vector<double> sc(c->n_pairs), mt(c->n_pairs);
   if (c->n_pairs) {
     HIPCHK(c, hipMemcpyAsync(sc.data(), c->scale.p, sizeof(double) * c->n_pairs, hipMemcpyDeviceToHost, c->stream));
@@ -1873,48 +2543,18 @@ int d2_result_pairs(d2_ctx* c, double* per_pair) {
   return D2_OK;
 }
 
-// The cluster step is a minimum over pair steps and bodies' last steps:
-// every update of separate_clusters.cpp:541-553 takes a minimum with the
-// pair's max_time * scale, and the cluster minimum (:570-598) reads one body
-// per pair, so the result does not depend on the order of the sequential
-// loop.  Bodies, pairs and the per-pair scales are the resident ones.
 int d2_cluster_steps(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, double* out) {
   if (!c || !cluster_of_pair || !out || n_clusters <= 0) return D2_ERR_ARG;
   if (c->last_mode != 1) return fail(c, D2_ERR_STATE, "d2_cluster_steps needs a preceding d2_ccd_run");
   const int np = c->n_pairs;
-  std::vector<double> per(2 * (size_t)np), bodies((size_t)c->n_bodies * 40);
-  std::vector<int32_t> pairs(2 * (size_t)np), off(np + 1);
-  int rc = d2_result_pairs(c, per.data());
-  if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(bodies.data(), c->bodies.p, sizeof(double) * bodies.size(), hipMemcpyDeviceToHost, c->stream));
-  if (np) HIPCHK(c, hipMemcpyAsync(pairs.data(), c->pairs.p, sizeof(int32_t) * 2 * np, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(off.data(), c->out_off.p, sizeof(int) * (np + 1), hipMemcpyDeviceToHost, c->stream));
+  for (int p = 0; p < np; ++p)
+    if (cluster_of_pair[p] < 0 || cluster_of_pair[p] >= n_clusters)
+      return fail(c, D2_ERR_ARG, "d2_cluster_steps: cluster index out of range");
+  RC(ensure_clusters(c, n_clusters)); RC(ensure(c, c->tmp_cluster, std::max(np, 1)));
+  if (np) HIPCHK(c, hipMemcpyAsync(c->tmp_cluster.p, cluster_of_pair, sizeof(int32_t) * np, hipMemcpyHostToDevice, c->stream));
+  RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false));
+  HIPCHK(c, hipMemcpyAsync(out, c->cluster_step.p, sizeof(double) * n_clusters, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  std::vector<double> step(n_clusters, -1.0);
-  std::vector<int> first_body(n_clusters, -1);
-  for (int p = 0; p < np; ++p) {
-    int k = cluster_of_pair[p];
-    if (k < 0 || k >= n_clusters) return fail(c, D2_ERR_ARG, "d2_cluster_steps: cluster index out of range");
-    int a = pairs[2 * p], b = pairs[2 * p + 1];
-    const double* ra = &bodies[40 * (size_t)a];
-    const double* rb = &bodies[40 * (size_t)b];
-    bool sa = ra[38] != 0.0, sb = rb[38] != 0.0;
-    if (first_body[k] < 0) first_body[k] = a;
-    double mt = std::min(ra[39], rb[39]);
-    double v = 1e300;
-    if (off[p + 1] > off[p] && !(sa && sb)) v = std::min(v, mt * per[2 * p]);
-    if (!(sa && sb)) {
-      int id = (!sa) ? a : b;                 // separate_clusters.cpp:575-586
-      double last = step[k] < 0.0 ? ra[39] : bodies[40 * (size_t)id + 39];  // first pair reads a (:589-590)
-      v = std::min(v, last);
-      step[k] = step[k] < 0.0 ? v : std::min(step[k], v);
-    }
-  }
-  for (int k = 0; k < n_clusters; ++k) {
-    double s = step[k];
-    if (s < 0.0) s = first_body[k] >= 0 ? bodies[40 * (size_t)first_body[k] + 39] : (double)dt;
-    out[k] = std::max(1e-3, std::min(s, (double)dt));
-  }
   return D2_OK;
 }
 

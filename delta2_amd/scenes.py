@@ -224,6 +224,33 @@ def poisson_disk(rng, radii, box, max_tries=2000):
     return pos
 
 
+def clusters(pairs, n_bodies, records=None):
+    """Connected components of the pair graph, as separate_clusters.cpp:355-459
+    forms clusters (static bodies do not join clusters together).  Returns
+    (cluster_of_pair, n_clusters) with clusters numbered by first pair."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    pairs = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+    if pairs.shape[0] == 0:
+        return np.zeros(0, dtype=np.int32), 1
+    a, b = pairs[:, 0].copy(), pairs[:, 1].copy()
+    if records is not None:  # a static body links nothing: give each of its pairs a private node
+        static = records[:, 38] != 0
+        extra = n_bodies + np.arange(pairs.shape[0])
+        a = np.where(static[a], extra, a)
+        b = np.where(static[b], extra, b)
+    n = n_bodies + pairs.shape[0]
+    g = coo_matrix((np.ones(a.size), (a, b)), shape=(n, n))
+    _, label = connected_components(g, directed=False)
+    lab = label[a]  # a pair's endpoints share a component
+    _, first = np.unique(lab, return_index=True)
+    order = np.argsort(first)
+    remap = np.empty(order.size, dtype=np.int32)
+    remap[order] = np.arange(order.size, dtype=np.int32)
+    _, inv = np.unique(lab, return_inverse=True)
+    return remap[inv].astype(np.int32), int(order.size)
+
+
 def broad_phase(records, radii, geo_eps=GEO_EPS, scene_of_body=None):
     """Pairs whose bounding spheres (radius + geo_eps) meet within the step, by
     collideFuncIndividualTime's rule (collision_detection/broad_phase_embree.cpp:

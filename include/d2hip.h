@@ -74,6 +74,26 @@ int d2_set_precision(d2_ctx* ctx, int precision);
 /* Capacity of the per-level work lists, in entries (default 1<<22). */
 int d2_set_capacity(d2_ctx* ctx, int64_t entries);
 
+/* Launch shapes (no reference counterpart: the reference runs on the CPU).
+ * chunk_blocks: blocks of the chunked list kernels (1-4096; a list is split
+ * into that many contiguous ranges); stream_blocks: blocks of the
+ * grid-stride kernels (1-16384); post_blocks: one-wavefront blocks of the
+ * per-pair post-processing (1-16384).  Defaults: 4, 8 and 16 per CU.
+ * Results do not depend on them; delta2_amd.autotune picks them by timing. */
+typedef struct d2_tune {
+  int chunk_blocks;
+  int stream_blocks;
+  int post_blocks;
+} d2_tune;
+int d2_set_tune(d2_ctx* ctx, const d2_tune* tune);
+int d2_get_tune(const d2_ctx* ctx, d2_tune* tune);
+
+/* Graph mode (no reference counterpart).  When on, each kind of run (narrow,
+ * CCD, step) is captured once into a HIP graph with the full level budget of
+ * the uploaded trees and replayed on later calls; uploads that change sizes
+ * or launch parameters recapture.  Results are identical to eager runs. */
+int d2_set_graph(d2_ctx* ctx, int on);
+
 /* ---- surrogate trees ---------------------------------------------------- */
 /* Builds one surrogate tree on the host from a triangle mesh, as
  * model::SurrogateTree(V, F, opt) does (model/surrogate_tree.cpp:9-18,39-112,
@@ -133,6 +153,29 @@ int d2_ccd_run(d2_ctx* ctx, int64_t* n_contacts_out);
  * from the last d2_ccd_run.  cluster_of_pair[p] in [0, n_clusters); dt is
  * Options::time_step_size (a float, common/cli_options.h:11). */
 int d2_cluster_steps(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_clusters, float dt, double* cluster_step_out);
+
+/* Cluster of every resident pair (the connected components of
+ * separate_clusters.cpp:355-459, computed by the caller); needed by
+ * d2_step_run.  cluster_of_pair[p] in [0, n_clusters). */
+int d2_clusters_upload(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_clusters);
+
+/* One contact-detection step over the resident pairs, as
+ * fineCollisionClustersWithTimeStepSelection and PDEExplicit::step sequence
+ * it (separate_clusters.cpp:462-599, strategies/PDE/PDE_explicit.cpp:49-95),
+ * all on the device:
+ *   1. CCD with max_time = min(last_time_step_size of a, b) and the
+ *      TOI -> step rule (separate_clusters.cpp:479-566);
+ *   2. the clamped cluster steps (separate_clusters.cpp:570-598,
+ *      time_step_selection_dynamic_continuous.cpp:17-21), written to
+ *      cluster_step_out[n_clusters] when it is not NULL;
+ *   3. future = current.extrapolate(cluster step) for every non-static body
+ *      in a pair, dt for the others (PDE_explicit.cpp:49-56; static bodies
+ *      keep their future state);
+ *   4. the discrete narrow phase and, when filter != 0, filterContacts.
+ * Afterwards d2_result_contacts returns the discrete contacts and
+ * d2_result_pairs the CCD values of step 1.  The bodies' future states are
+ * overwritten (as in the reference); current states and last steps are not. */
+int d2_step_run(d2_ctx* ctx, float dt, int filter, double* cluster_step_out, int64_t* n_contacts_out);
 
 /* Results of the last run: CSR offsets (n_pairs+1), contact records, and
  * per-pair values (CCD: scale, min_time_out; discrete: unused).

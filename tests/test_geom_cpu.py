@@ -20,3 +20,14 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
         assert int(rows[name][2]) == 0, out
         assert int(rows[name][1]) >= 10000
     assert "bisection cases" in out
+
+
+def test_partition_rank_sort_matches_std_sort(tmp_path):
+    """The post-processing kernel replays std::sort (the contact filter's
+    unstable sort, contact.h:44-69) by partition ranks; the formulation is
+    checked against libstdc++ on tie-heavy keys."""
+    exe = tmp_path / "cpu_sort_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "cpu_sort_check.cpp"), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out[0] == "sort" and int(out[1]) >= 1000 and int(out[2]) == 0, out

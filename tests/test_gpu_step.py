@@ -1,0 +1,206 @@
+"""GPU: the composite step, graph replays and launch shapes.
+
+- d2_step_run (CCD + step rule + cluster steps + projection + discrete narrow
+  phase + filter, all on the device) against the oracle: cluster steps to
+  1e-12 relative, future states to 1e-14 (device and glibc sin/cos may differ
+  in the last bit), and the contacts bit-for-bit against the oracle's narrow
+  phase on the states the device used.
+- Graph mode: every replay equals the eager run, with eager work and
+  synchronisations between replays.
+- Launch shapes (d2_tune) change nothing: one block per chunked kernel makes
+  the in-kernel scans loop over many tiles.
+"""
+import numpy as np
+import pytest
+
+import delta2_amd as d2
+from gpu_cases import c1_scene, dense_scene, oracle_scene, with_future
+from delta2_amd import scenes
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+
+
+def close(a, b, tol=TOL):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0)
+
+
+def same_result(a, b, label):
+    assert np.array_equal(a.offsets, b.offsets), f"{label}: offsets differ"
+    assert np.array_equal(a.ints, b.ints), f"{label}: index tuples differ"
+    assert np.array_equal(a.reals, b.reals), f"{label}: reals differ"
+    if a.per_pair is not None and b.per_pair is not None:
+        assert np.array_equal(a.per_pair, b.per_pair), f"{label}: per-pair values differ"
+
+
+def load(ctx, s, records):
+    ctx.upload_trees(s.trees)
+    ctx.upload_bodies(s.tree_of_body, records)
+
+
+def clusters_of(pairs, n_bodies):
+    """Connected components of the pair graph (separate_clusters.cpp:355-459 is
+    the caller's job): every body belongs to exactly one cluster."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import connected_components
+    g = coo_matrix((np.ones(pairs.shape[0]), (pairs[:, 0], pairs[:, 1])), shape=(n_bodies, n_bodies))
+    _, label = connected_components(g, directed=False)
+    used, cl = np.unique(label[pairs[:, 0]], return_inverse=True)
+    return cl.astype(np.int32), len(used)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = d2.Context(0, d2.D2_FP64)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name,make", [("c1", c1_scene), ("dense", dense_scene)])
+def test_step_matches_oracle(ctx, name, make):
+    s = make()
+    print(f"[step] scene={name} pairs={s.pairs.shape[0]}", flush=True)
+    rec = s.records.copy()
+    rec[::7, 39] = scenes.DT / 4  # uneven last steps: max_time = min of the two bodies
+    load(ctx, s, rec)
+    pairs = s.pairs
+    cl, k = clusters_of(pairs, s.n_bodies)
+    ctx.upload_pairs(pairs)
+    ctx.upload_clusters(cl, k)
+    n, steps = ctx.step_run(scenes.DT, filter=True, want_steps=True)
+    print(f"[step] clusters={k} contacts={n}", flush=True)
+    # cluster steps: the oracle runs fineCollisionClustersWithTimeStepSelection per cluster
+    osc = oracle_scene(s, rec)
+    want = np.array([osc.cluster_step(pairs[cl == j], np.float32(scenes.DT)) for j in range(k)])
+    assert np.allclose(steps, want, rtol=1e-12, atol=0), (steps, want)
+    # future states: cluster step for bodies in pairs, dt for the others
+    body_step = np.full(s.n_bodies, float(np.float32(scenes.DT)))
+    body_step[pairs[:, 0]] = steps[cl]
+    body_step[pairs[:, 1]] = steps[cl]
+    got_bodies = ctx.download_bodies()
+    want_bodies = with_future(rec, body_step)
+    assert np.allclose(got_bodies[:, 14:28], want_bodies[:, 14:28], rtol=0, atol=1e-14)
+    # contacts: the oracle's narrow phase + filter on the states the device used
+    gpu = ctx.result(with_pairs=True)
+    assert gpu.n == n
+    o_off, o_ints, o_reals = oracle_scene(s, got_bodies).narrow(pairs, filter=True)
+    assert np.array_equal(gpu.offsets, o_off)
+    assert np.array_equal(gpu.ints, o_ints)
+    assert close(gpu.reals, o_reals).all()
+    # the per-pair CCD values of the step equal a plain CCD run
+    mt = np.minimum(rec[pairs[:, 0], 39], rec[pairs[:, 1], 39])
+    _, _, _, o_pp = oracle_scene(s, rec).ccd(pairs, mt)
+    assert close(gpu.per_pair, o_pp).all()
+
+
+@pytest.mark.parametrize("precision", [d2.D2_FP64, d2.D2_FP32])
+def test_graph_replays_equal_eager(precision):
+    s = dense_scene()
+    print(f"[graph] precision={precision}", flush=True)
+    c = d2.Context(0, precision)
+    try:
+        rec = with_future(s.records, scenes.DT)
+        load(c, s, rec)
+        pairs = s.pairs
+        mt = np.minimum(rec[pairs[:, 0], 39], rec[pairs[:, 1], 39])
+        cl, k = clusters_of(pairs, s.n_bodies)
+        # eager references
+        c.upload_pairs(pairs, mt)
+        c.narrow_run(True)
+        e_narrow = c.result()
+        c.ccd_run()
+        e_ccd = c.result(with_pairs=True)
+        c.upload_clusters(cl, k)
+        _, e_steps = c.step_run(scenes.DT, True, want_steps=True)
+        e_step = c.result(with_pairs=True)
+        e_bodies = c.download_bodies()
+        c.set_graph(True)
+        for rep in range(3):
+            load(c, s, rec)  # eager work (uploads, synchronisation) between replays
+            c.upload_pairs(pairs, mt)
+            c.upload_clusters(cl, k)
+            c.narrow_run(True)
+            same_result(c.result(), e_narrow, f"narrow replay {rep}")
+            c.ccd_run()
+            same_result(c.result(with_pairs=True), e_ccd, f"ccd replay {rep}")
+            c.project_future(np.full(s.n_bodies, scenes.DT))  # an eager kernel between replays
+            _, st = c.step_run(scenes.DT, True, want_steps=True)
+            assert np.array_equal(st, e_steps), f"cluster steps, replay {rep}"
+            same_result(c.result(with_pairs=True), e_step, f"step replay {rep}")
+            assert np.array_equal(c.download_bodies(), e_bodies)
+        c.set_graph(False)
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 7, 5), (4096, 16384, 16384)])
+def test_launch_shapes_change_nothing(ctx, shape):
+    s = dense_scene()
+    print(f"[tune] shape={shape}", flush=True)
+    rec = with_future(s.records, scenes.DT)
+    load(ctx, s, rec)
+    pairs = s.pairs
+    mt = np.full(pairs.shape[0], 4 * scenes.DT)
+    default = ctx.get_tune()
+    try:
+        ctx.upload_pairs(pairs, mt)
+        ctx.narrow_run(True)
+        ref_n = ctx.result()
+        ctx.ccd_run()
+        ref_c = ctx.result(with_pairs=True)
+        ctx.set_tune(*shape)
+        ctx.narrow_run(True)
+        same_result(ctx.result(), ref_n, f"narrow {shape}")
+        ctx.ccd_run()
+        same_result(ctx.result(with_pairs=True), ref_c, f"ccd {shape}")
+    finally:
+        ctx.set_tune(*default)
+
+
+def test_step_needs_clusters(ctx):
+    s = c1_scene()
+    load(ctx, s, s.records)
+    ctx.upload_pairs(s.pairs)  # uploading pairs clears the clusters
+    with pytest.raises(d2.D2Error):
+        ctx.step_run(scenes.DT)
+    with pytest.raises(d2.D2Error):
+        ctx.upload_clusters(np.full(s.pairs.shape[0], 5, dtype=np.int32), 2)
+
+
+def test_graph_continues_past_captured_levels():
+    """A graph captured on a one-level pair list, replayed on a deep one: the
+    host continues past the captured levels and the result equals eager."""
+    s = dense_scene()
+    rec = with_future(s.records, scenes.DT)
+    pos = rec[:, 4:7]
+    d = np.linalg.norm(pos[:, None, :] - pos[None, :, :], axis=-1)
+    i, j = np.unravel_index(np.argmax(d), d.shape)
+    far = np.array([[min(i, j), max(i, j)]], dtype=np.int32)
+    pairs = s.pairs
+    mt = np.minimum(rec[pairs[:, 0], 39], rec[pairs[:, 1], 39])
+    eager = d2.Context(0, d2.D2_FP64)
+    graph = d2.Context(0, d2.D2_FP64)
+    try:
+        for c in (eager, graph):
+            load(c, s, rec)
+        eager.upload_pairs(pairs, mt)
+        eager.narrow_run(True)
+        e_n = eager.result()
+        eager.ccd_run()
+        e_c = eager.result(with_pairs=True)
+        graph.set_graph(True)
+        for _ in range(2):  # observe (eager), then capture with one level per phase
+            graph.upload_pairs(far, np.full(1, scenes.DT))
+            assert graph.narrow_run(True) == 0
+            assert graph.ccd_run() == 0
+        for rep in range(2):  # deeper lists: continuation, then a recapture with more levels
+            print(f"[graph-continue] rep={rep}", flush=True)
+            graph.upload_pairs(pairs, mt)
+            graph.narrow_run(True)
+            same_result(graph.result(), e_n, f"narrow continuation {rep}")
+            graph.ccd_run()
+            same_result(graph.result(with_pairs=True), e_c, f"ccd continuation {rep}")
+    finally:
+        eager.close()
+        graph.close()
