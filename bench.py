@@ -86,12 +86,112 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
+def timed_steps(ctx, dt, steps, warmup):
+    for _ in range(warmup):
+        ctx.step_run(dt, True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step_run(dt, True)
+    ctx.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_c3(args):
+    """Batched workload: 256 independent scenes (16-512 bodies, C1 meshes).
+    Variants on the same inputs, each a full step over every scene:
+      batched_fp32_graph : one d2_step_run over all scenes' pairs, FP32 cull,
+                           graph replay, autotuned launch shapes (the value)
+      batched_fp64_graph : the same in FP64, default shapes
+      plain_fp64_scenes  : the plain path as the reference calls it, one
+                           scene at a time (pairs + clusters uploaded per
+                           scene, eager FP64 d2_step_run)"""
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    t0 = time.time()
+    sc = scenes.scene_c3(n_scenes=args.scenes)
+    n_pairs = int(sc.pairs.shape[0])
+    log(f"[c3] {len(sc.counts)} scenes, {sc.n_bodies} bodies, {sc.n_triangles()} triangles, {n_pairs} pairs, "
+        f"built in {time.time() - t0:.1f}s")
+    cl, n_cl = scenes.clusters(sc.pairs, sc.n_bodies, sc.records)
+    out = {}
+    for name, prec, graph in (("batched_fp32_graph", d2.D2_FP32, True), ("batched_fp64_graph", d2.D2_FP64, True)):
+        ctx = d2.Context(0, prec)  # capacities grow on demand
+        ctx.upload_trees(sc.trees)
+        ctx.upload_bodies(sc.tree_of_body, sc.records)
+        ctx.upload_pairs(sc.pairs)
+        ctx.upload_clusters(cl, n_cl)
+        ctx.set_graph(graph)
+        ctx.step_run(scenes.DT, True)  # observe the level counts
+        tune = None
+        if prec == d2.D2_FP32:
+            tune, ms = ctx.autotune("step", reps=3)
+            log(f"[c3] autotuned shape {tune}: {ms:.3f} ms")
+        sec = timed_steps(ctx, scenes.DT, args.steps, args.warmup)
+        out[name] = {"ms_per_step": 1e3 * sec, "value": n_pairs / sec, "tune": tune or ctx.get_tune()}
+        log(f"[c3] {name}: {1e3 * sec:.3f} ms/step, {n_pairs / sec:.0f} pair queries/s")
+        if prec == d2.D2_FP32:
+            ctx.set_graph(False)
+            ctx.reset_stats()
+            ctx.set_profiling(True)
+            for _ in range(args.profile_steps):
+                ctx.step_run(scenes.DT, True)
+            ctx.sync()
+            ctx.set_profiling(False)
+            for k, (n, ms, items) in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1][1]):
+                if n:
+                    log(f"[c3]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")
+        ctx.close()
+    # plain: one scene at a time
+    ctx = d2.Context(0, d2.D2_FP64)
+    ctx.upload_trees(sc.trees)
+    ctx.upload_bodies(sc.tree_of_body, sc.records)
+    sob = sc.scene_of_body[sc.pairs[:, 0]]
+    per_scene = []
+    for k in range(len(sc.counts)):
+        idx = np.nonzero(sob == k)[0]
+        if idx.size:
+            used, lcl = np.unique(cl[idx], return_inverse=True)
+            per_scene.append((sc.pairs[idx], lcl.astype(np.int32), used.size))
+    reps = max(1, args.steps // 4)
+
+    def plain_pass():
+        for p, c_, n in per_scene:
+            ctx.upload_pairs(p)
+            ctx.upload_clusters(c_, n)
+            ctx.step_run(scenes.DT, True)
+    plain_pass()
+    ctx.sync()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    ctx.sync()
+    sec = (time.perf_counter() - t1) / reps
+    ctx.close()
+    out["plain_fp64_scenes"] = {"ms_per_step": 1e3 * sec, "value": n_pairs / sec}
+    log(f"[c3] plain_fp64_scenes: {1e3 * sec:.3f} ms/step, {n_pairs / sec:.0f} pair queries/s")
+    best = out["batched_fp32_graph"]
+    res = {
+        "metric": METRIC, "value": best["value"], "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(sc.counts)} independent scenes, log-uniform 16-512 bodies each, subdiv-2 "
+                               "icospheres U(0.5,1.5), C1 density, v~N(0,1), dt=1/60; one step over all scenes",
+                   "bodies": sc.n_bodies, "triangles": sc.n_triangles(), "pairs": n_pairs, "clusters": n_cl},
+        "variants": out,
+        "speedup_vs_plain_fp64_scenes": best["value"] / out["plain_fp64_scenes"]["value"],
+        "speedup_vs_batched_fp64": best["value"] / out["batched_fp64_graph"]["value"],
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--scenes", type=int, default=256, help="c3: number of independent scenes")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--graph", default="on", choices=["on", "off"])
     ap.add_argument("--profile-steps", type=int, default=5, help="eager steps timed per kernel for the roofline")
@@ -110,6 +210,9 @@ def main():
     import delta2_amd as d2
     from delta2_amd import scenes
     d2.lib()
+    if args.workload == "c3":
+        run_c3(args)
+        return
     t0 = time.time()
     scene = scenes.cached(args.workload, scenes.scene_c2 if args.workload == "c2" else scenes.scene_c1)
     log(f"[bench] scene {scene.name}: {scene.n_bodies} bodies, {scene.n_triangles()} triangles, "

@@ -68,6 +68,7 @@ def lib():
         "d2_last_error": ([_vp], ctypes.c_char_p),
         "d2_set_precision": ([_vp, ctypes.c_int], ctypes.c_int),
         "d2_set_capacity": ([_vp, ctypes.c_int64], ctypes.c_int),
+        "d2_set_auto_grow": ([_vp, ctypes.c_int], ctypes.c_int),
         "d2_tree_build": ([_dp, ctypes.c_int, _ip, ctypes.c_int, _u64p, ctypes.c_int, _ip, _dp,
                            ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "d2_pcg_seed": ([ctypes.c_uint64, ctypes.c_uint64, _u64p], ctypes.c_int),
@@ -89,9 +90,12 @@ def lib():
         "d2_kernel_stats": ([_vp, ctypes.c_int, _i64p, _dp, _dp], ctypes.c_int),
         "d2_reset_stats": ([_vp], ctypes.c_int),
         "d2_sync": ([_vp], ctypes.c_int),
+        "d2_run_info": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
         "d2_set_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_get_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_set_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_autotune": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune), _dp],
+                        ctypes.c_int),
         "d2_clusters_upload": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
         "d2_step_run": ([_vp, ctypes.c_float, ctypes.c_int, _dp, _i64p], ctypes.c_int),
     }
@@ -210,6 +214,9 @@ class Context:
     def set_capacity(self, entries):
         self._check(lib().d2_set_capacity(self.h, int(entries)))
 
+    def set_auto_grow(self, on=True):
+        self._check(lib().d2_set_auto_grow(self.h, int(bool(on))))
+
     def upload_trees(self, trees):
         """trees: list of (nodes, buckets)."""
         offs = np.zeros(len(trees) + 1, dtype=np.int32)
@@ -289,6 +296,17 @@ class Context:
         t = Tune(int(chunk_blocks), int(stream_blocks), int(post_blocks))
         self._check(lib().d2_set_tune(self.h, ctypes.byref(t)))
 
+    def autotune(self, kind="step", dt=None, filter=True, reps=3):
+        """Picks the launch shapes by timing runs of `kind` ("narrow", "ccd",
+        "step") on the resident inputs; returns (shape, ms per run)."""
+        from .scenes import DT
+        k = {"narrow": 0, "ccd": 1, "step": 2}[kind]
+        t = Tune()
+        ms = ctypes.c_double(0)
+        self._check(lib().d2_autotune(self.h, k, float(DT if dt is None else dt), int(bool(filter)), int(reps),
+                                      ctypes.byref(t), ctypes.byref(ms)))
+        return t.as_tuple(), ms.value
+
     def get_tune(self):
         t = Tune()
         self._check(lib().d2_get_tune(self.h, ctypes.byref(t)))
@@ -316,6 +334,14 @@ class Context:
             self._check(L.d2_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(items)))
             out[L.d2_kernel_name(i).decode()] = (n.value, ms.value, items.value)
         return out
+
+    def run_info(self):
+        """(global counters[8], per-level sizes[2, 17, 8]) of the last run:
+        [0] the discrete phase (the CCD phase after ccd_run), [1] the last
+        CCD phase."""
+        buf = np.zeros(8 + 2 * 17 * 8, dtype=np.int64)
+        lib().d2_run_info(self.h, buf.ctypes.data_as(_i64p), buf.size)
+        return buf[:8], buf[8:].reshape(2, 17, 8)
 
     def sync(self):
         self._check(lib().d2_sync(self.h))

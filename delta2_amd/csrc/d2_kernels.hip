@@ -30,6 +30,8 @@
 
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -298,6 +300,13 @@ __device__ __forceinline__ void fold_pass(const Dev& D, const double* pair_rec, 
     pass_flag[p] = 0;
   }
 }
+// Keeps the CCD phase's sizes (and its contact count in the reserved field of
+// level 0) next to the discrete phase's, for sizing the capacities.
+__global__ void k_save_ccd_sizes(Dev D) {
+  const int n = (MAX_LEVELS + 1) * LV_STRIDE;
+  for (int i = threadIdx.x; i < n; i += blockDim.x)
+    D.lvl[n + i] = i == LV_STRIDE - 1 ? D.cnt[C_CONTACTS] : D.lvl[i];
+}
 __global__ void k_fold_final(Dev D) {
   fold_pass(D, D.pair_rec, D.pass_min_s, D.pass_flag_s);
   fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
@@ -442,7 +451,7 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D) {
     D.pass_flag_s[np.pair] = 1;
     unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
     if ((long long)slot >= D.cap_contacts) {
-      D.cnt[C_OVERFLOW] = 1;
+      atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
       continue;
     }
     ContactRec c;
@@ -467,7 +476,7 @@ __device__ __forceinline__ void emit_leaf(Dev& D, const NP& np, const double* pr
                                           v3<double> A, v3<double> B, double toc) {
   unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
   if ((long long)slot >= D.cap_contacts) {
-    D.cnt[C_OVERFLOW] = 1;
+    atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
     return;
   }
   ContactRec c;
@@ -688,7 +697,7 @@ __global__ void __launch_bounds__(256) k_mask_enum(Dev D, long long cap) {
   partial_prefix(D.part_a, before, total);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     D.lvl[D.level * LV_STRIDE + LV_ITEMS] = (unsigned long long)total;
-    if (total > cap) D.cnt[C_OVERFLOW] = 1;
+    if (total > cap) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
     atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)total);
   }
   long long carry = before;
@@ -735,7 +744,7 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
     if (na.inner || nb.inner) continue;
     unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
     if ((long long)slot >= D.cap_contacts) {
-      D.cnt[C_OVERFLOW] = 1;
+      atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
       continue;
     }
     ContactRec c;
@@ -822,7 +831,7 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP] = (unsigned long long)ts;
       D.lvl[(D.level + 1) * LV_STRIDE + LV_CONN] = D.discrete ? 0ull : (unsigned long long)tc;
-      if (ts > D.cap_np || tc > D.cap_np) D.cnt[C_OVERFLOW] = 1;
+      if (ts > D.cap_np || tc > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
     }
     for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
       const long long w = t0 + threadIdx.x;
@@ -859,7 +868,7 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       D.lvl[D.level * LV_STRIDE + LV_PT] = (unsigned long long)tp;
       D.lvl[D.level * LV_STRIDE + LV_EE] = (unsigned long long)te;
-      if (tp > D.cap_items || te > D.cap_items) D.cnt[C_OVERFLOW] = 1;
+      if (tp > D.cap_items || te > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
     }
     for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
       const long long w = t0 + threadIdx.x;
@@ -897,7 +906,7 @@ __global__ void k_post_init(int n_pairs, int* pair_cnt, int* pair_fill, int* kep
     pair_cnt[p] = 0;
     kept_cnt[p] = 0;
     if (p < n_pairs) pair_fill[p] = 0;
-    if (p == 0) *n_big = 0;
+    if (p == 0) n_big[0] = 0, n_big[1] = 0;
   }
 }
 __global__ void k_count(const unsigned long long* cnt, long long cap, const ContactRec* c, int* pair_cnt) {
@@ -1055,15 +1064,28 @@ __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off,
                                 int* kept_cnt, const double* pair_rec, double* scale) {
   {
     int b = pair_off[p], e = pair_off[p + 1];
-    for (int i = b + 1; i < e; ++i) {
-      int v = perm[i];
-      unsigned long long k = c[v].key;
-      int j = i - 1;
-      while (j >= b && c[perm[j]].key > k) {
-        perm[j + 1] = perm[j];
-        --j;
+    {  // heapsort of perm[b, e) by the (unique) emission key
+      const int n = e - b;
+      int* a = perm + b;
+      auto sift = [&](int root, int len) {
+        const int v = a[root];
+        const unsigned long long kv = c[v].key;
+        int hole = root;
+        for (int ch = 2 * hole + 1; ch < len; ch = 2 * hole + 1) {
+          if (ch + 1 < len && c[a[ch + 1]].key > c[a[ch]].key) ++ch;
+          if (c[a[ch]].key <= kv) break;
+          a[hole] = a[ch];
+          hole = ch;
+        }
+        a[hole] = v;
+      };
+      for (int r = n / 2 - 1; r >= 0; --r) sift(r, n);
+      for (int len = n - 1; len > 0; --len) {
+        const int t = a[0];
+        a[0] = a[len];
+        a[len] = t;
+        sift(0, len);
       }
-      perm[j + 1] = v;
     }
     if (mode == 1) {  // filterContacts(.., 0.0)
       Sorter s{perm, c};
@@ -1132,8 +1154,9 @@ __global__ void k_pair_finish_big(const int* big, const int* n_big, const Contac
     pair_finish_one(big[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
 }
 
-// One wavefront per pair, contacts staged in LDS (pairs with more than
-// POST_CAP contacts go to k_pair_finish, one lane per pair).
+// One wavefront per pair, contacts staged in LDS (pairs with more than the
+// kernel's capacity go on to the next pass; the last pass, k_pair_finish_big,
+// takes one lane per pair).
 //   1. bitonic sort by emission key -> the reference's contact order
 //   2a. discrete + filter: the order std::sort(compareContacts) produces.  When
 //       no two contacts have equal (eps_a+eps_b, |A-B|) every correct sort
@@ -1142,7 +1165,8 @@ __global__ void k_pair_finish_big(const int* big, const int* n_big, const Contac
 //       greedy pass of filterContacts, each candidate tested against the kept
 //       set across the lanes.
 //   2b. CCD: the step rule, lanes over contacts, min-reduced.
-constexpr int POST_CAP = 512;
+constexpr int POST_CAP_WAVE = 512;   // first pass: 30 KB of LDS per one-wavefront block
+constexpr int POST_CAP_LARGE = 2048; // second pass: 120 KB of LDS, one block per CU
 struct LdsKey {
   double eps, dist;
 };
@@ -1385,9 +1409,15 @@ __device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int*
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const ContactRec* c, const int* pair_off, int* perm,
-                                                         int mode, int* keep, int* kept_cnt, const double* pair_rec,
-                                                         double* scale, int* big, int* n_big) {
+// CAP: contacts a pair may have to be handled here (LDS 60 B each); pairs
+// with more go to the `over` list.  `list` (device count *n_list) selects
+// the pairs, or all n_pairs when it is null.
+template <int CAP>
+__global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int* list, const int* n_list,
+                                                         const ContactRec* c, const int* pair_off, int* perm, int mode,
+                                                         int* keep, int* kept_cnt, const double* pair_rec,
+                                                         double* scale, int* over, int* n_over) {
+  constexpr int POST_CAP = CAP;
   __shared__ unsigned long long skey[POST_CAP];
   __shared__ int sidx[POST_CAP];
   __shared__ LdsKey sk[POST_CAP];
@@ -1396,10 +1426,12 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const Cont
   __shared__ int kept_list[POST_CAP];
   __shared__ int stk[3 * 64];
   const int lane = threadIdx.x;
-  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+  const int n_items = list ? *n_list : n_pairs;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int p = list ? list[it] : it;
     const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
     if (n > POST_CAP) {
-      if (lane == 0) big[atomicAdd(n_big, 1)] = p;
+      if (lane == 0) over[atomicAdd(n_over, 1)] = p;
       continue;
     }
     int n2 = 1;
@@ -1694,12 +1726,11 @@ constexpr int MAX_SEGMENTS = 3;
 struct Sig {
   int n_pairs = -1, n_bodies = -1, n_clusters = -1, max_depth = -1, precision = -1, filter = -1;
   float dt = -1.f;
-  long long cap = -1;
   d2_tune tune{0, 0, 0};
   int levels[2] = {-1, -1};  // graph levels of each phase
   bool operator==(const Sig& o) const {
     return levels[0] == o.levels[0] && levels[1] == o.levels[1] && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
-           precision == o.precision && filter == o.filter && dt == o.dt && cap == o.cap &&
+           precision == o.precision && filter == o.filter && dt == o.dt &&
            tune.chunk_blocks == o.tune.chunk_blocks && tune.stream_blocks == o.tune.stream_blocks &&
            tune.post_blocks == o.tune.post_blocks;
   }
@@ -1717,9 +1748,13 @@ struct d2_ctx {
   int precision = D2_FP64;
   hipStream_t stream = nullptr;
   std::string err;
-  long long cap = 1ll << 22;  // node pairs / items per list
+  // capacities: node pairs per list, triangle-pair / sub-test items per
+  // level, contacts per run, CCD soup items with result slots (15 x 64 B)
+  long long cap = 1ll << 22;
+  long long cap_items = 1ll << 24;
   long long cap_contacts = 1ll << 22;
-  long long cap_res = 1ll << 20;  // CCD soup items with result slots
+  long long cap_res = 1ll << 20;
+  bool auto_grow = true;  // grow the capacities and re-run on overflow
   int cus = 256;
   d2_tune tune{1024, 2048, 4096};  // 4, 8, 16 per CU of a 256-CU device; reset at creation
   // trees / bodies
@@ -1744,7 +1779,7 @@ struct d2_ctx {
   DBuf<int> pass_flag_s, pass_flag_c;
   DBuf<double> min_time, scale;
   // post
-  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, n_big, scan_part;
+  DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, huge, n_big, scan_part;
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
   // clusters
@@ -1894,7 +1929,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.min_time = c->min_time.p;
   D.n_pairs = c->n_pairs;
   D.cap_np = c->cap;
-  D.cap_items = c->cap;
+  D.cap_items = c->cap_items;
   D.cap_res = c->cap_res;
   D.cap_contacts = c->cap_contacts;
   D.level = level;
@@ -1912,18 +1947,18 @@ int prepare(d2_ctx* c) {
     RC(ensure(c, c->soup_list[k], c->cap));
     RC(ensure(c, c->conn_list[k], c->cap));
   }
-  RC(ensure(c, c->items, c->cap)); RC(ensure(c, c->pt_items, c->cap));
-  RC(ensure(c, c->ee_items, c->cap)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
+  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->contacts, c->cap_contacts)); RC(ensure(c, c->cnt, C_COUNT));
-  RC(ensure(c, c->lvl, (size_t)(MAX_LEVELS + 1) * LV_STRIDE));
+  RC(ensure(c, c->lvl, (size_t)2 * (MAX_LEVELS + 1) * LV_STRIDE));
   RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
   RC(ensure(c, c->pass_min_s, np)); RC(ensure(c, c->pass_min_c, np)); RC(ensure(c, c->pass_flag_s, np));
   RC(ensure(c, c->pass_flag_c, np)); RC(ensure(c, c->min_time, np)); RC(ensure(c, c->scale, np));
   RC(ensure(c, c->pair_cnt, np + 1)); RC(ensure(c, c->pair_off, np + 1)); RC(ensure(c, c->pair_fill, np));
   RC(ensure(c, c->kept_cnt, np + 1)); RC(ensure(c, c->out_off, np + 1)); RC(ensure(c, c->big, np));
-  RC(ensure(c, c->n_big, 1)); RC(ensure(c, c->perm, c->cap_contacts)); RC(ensure(c, c->keep, c->cap_contacts));
+  RC(ensure(c, c->n_big, 2)); RC(ensure(c, c->huge, np)); RC(ensure(c, c->perm, c->cap_contacts)); RC(ensure(c, c->keep, c->cap_contacts));
   RC(ensure(c, c->scan_part, g));
   return D2_OK;
 }
@@ -1943,7 +1978,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     {
       Timer tm(c, K_TRI_ENUM, 0.0);
       hipLaunchKernelGGL(k_tri_mask, dim3(cg), dim3(256), 0, c->stream, D, c->precision == D2_FP32 ? 1 : 0);
-      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap);
+      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap_items);
     }
     {
       Timer tm(c, K_TRI_DIST, 0.0);
@@ -2004,10 +2039,14 @@ int enqueue_post(d2_ctx* c, int mode) {
   {
     Timer tm(c, K_POST_PAIRS, (double)np);
     const int wblocks = std::max(1, std::min(np, c->tune.post_blocks));
-    hipLaunchKernelGGL(k_pair_finish_wave, dim3(wblocks), dim3(64), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
-    hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->big.p,
-                       c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
+    hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_WAVE>, dim3(wblocks), dim3(64), 0, c->stream, np, nullptr, nullptr,
+                       c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p,
+                       c->scale.p, c->big.p, c->n_big.p);
+    hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(64), 0,
+                       c->stream, np, c->big.p, c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p,
+                       c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p, c->n_big.p + 1);
+    hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->huge.p,
+                       c->n_big.p + 1, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
                        c->pair_rec.p, c->scale.p);
   }
   if (filtered) {
@@ -2128,6 +2167,7 @@ int enqueue_end(d2_ctx* c, int kind, const PhaseSpec& ph, int levels, float dt) 
   if (!ph.discrete) {
     Dev D = make_dev(c, levels, 0);
     hipLaunchKernelGGL(k_fold_final, dim3(grid_for(c, std::max(c->n_pairs, 1))), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_save_ccd_sizes, dim3(1), dim3(256), 0, c->stream, D);
   }
   RC(enqueue_post(c, ph.post_mode));
   if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true));
@@ -2189,7 +2229,7 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   const int budget = level_budget(c);
   Sig sig;
   sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
-  sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt, sig.cap = c->cap;
+  sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
   sig.tune = c->tune;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
   std::vector<std::function<int()>> ops;
@@ -2226,7 +2266,7 @@ int ensure_clusters(d2_ctx* c, int n_clusters) {
 // Waits for the run, reads the counters, checks capacity.
 int finish_run(d2_ctx* c, int kind, int filter) {
   HIPCHK(c, hipMemcpyAsync(c->host_cnt, c->cnt.p, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->host_lvl, c->lvl.p, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE,
+  HIPCHK(c, hipMemcpyAsync(c->host_lvl, c->lvl.p, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE,
                            hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipGetLastError());
@@ -2245,7 +2285,55 @@ int finish_run(d2_ctx* c, int kind, int filter) {
   return D2_OK;
 }
 
+// Capacities large enough for the sizes the failed run recorded (exact for
+// the levels it reached; a deeper level may need another round).  The
+// discrete phase's sizes are in the first half of the level table, the CCD
+// phase's in the second; the overflow counter says which phase overflowed.
+void grow_capacities(d2_ctx* c, int kind) {
+  const int half = (MAX_LEVELS + 1) * LV_STRIDE;
+  long long np = c->n_pairs, items = 0, res = 0, contacts = 0;
+  auto scan = [&](const unsigned long long* t, bool ccd) {
+    for (int l = 0; l <= MAX_LEVELS; ++l) {
+      const unsigned long long* v = t + l * LV_STRIDE;
+      np = std::max<long long>(np, (long long)std::max(v[LV_SOUP], v[LV_CONN]));
+      if (ccd) {
+        res = std::max<long long>(res, (long long)v[LV_ITEMS]);
+        items = std::max<long long>(items, (long long)std::max(v[LV_PT], v[LV_EE]));
+      } else {
+        items = std::max<long long>(items, (long long)v[LV_ITEMS]);
+      }
+    }
+  };
+  if (kind != RUN_CCD) scan(c->host_lvl, false), contacts = (long long)c->host_cnt[C_CONTACTS];
+  if (kind != RUN_NARROW) scan(c->host_lvl + half, true),
+      contacts = std::max<long long>(contacts, (long long)c->host_lvl[half + LV_STRIDE - 1]);
+  auto grown = [](long long cur, long long need) { return need > cur ? std::max(2 * cur, need + need / 4) : cur; };
+  bool any = false;
+  long long v;
+  if ((v = grown(c->cap, np)) != c->cap) c->cap = v, any = true;
+  if ((v = grown(c->cap_items, items)) != c->cap_items) c->cap_items = v, any = true;
+  if ((v = grown(c->cap_res, res)) != c->cap_res) c->cap_res = v, any = true;
+  if ((v = grown(c->cap_contacts, contacts)) != c->cap_contacts) c->cap_contacts = v, any = true;
+  if (!any) {  // sizes beyond the recorded ones (an overflow cut a level short): double the phase's lists
+    const unsigned long long ov = c->host_cnt[C_OVERFLOW];
+    c->cap *= 2, c->cap_contacts *= 2;
+    if (ov & 2ull) c->cap_items *= 2;
+    if (ov & 1ull) c->cap_res *= 2, c->cap_items *= 2;
+  }
+}
+
+int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out);
+
 int run(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
+  int rc = run_once(c, kind, filter, dt, n_contacts_out);
+  for (int attempt = 0; rc == D2_ERR_CAPACITY && c->auto_grow && attempt < 8; ++attempt) {
+    grow_capacities(c, kind);
+    rc = run_once(c, kind, filter, dt, n_contacts_out);
+  }
+  return rc;
+}
+
+int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
   if (c->n_pairs < 0 || c->n_bodies <= 0 || c->n_trees <= 0)
     return fail(c, D2_ERR_STATE, "upload trees, bodies and pairs first");
   if (2 * c->max_depth + 1 > MAX_LEVELS) return fail(c, D2_ERR_ARG, "trees deeper than 7 levels are not supported");
@@ -2286,14 +2374,14 @@ int d2_ctx_create(int device, int precision, d2_ctx** out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
   c->tune = d2_tune{4 * c->cus, 8 * c->cus, 16 * c->cus};
   if (hipHostMalloc((void**)&c->host_cnt, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c->host_lvl, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE,
+      hipHostMalloc((void**)&c->host_lvl, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE,
                     hipHostMallocDefault) != hipSuccess) {
     if (c->host_cnt) (void)hipHostFree(c->host_cnt);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return D2_ERR_HIP;
   }
-  std::memset(c->host_lvl, 0, sizeof(unsigned long long) * (MAX_LEVELS + 1) * LV_STRIDE);
+  std::memset(c->host_lvl, 0, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE);
   *out = c;
   return D2_OK;
 }
@@ -2314,7 +2402,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->min_time), release(c->scale);
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
-  release(c->big), release(c->n_big);
+  release(c->big), release(c->huge), release(c->n_big);
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
@@ -2338,8 +2426,16 @@ int d2_set_capacity(d2_ctx* c, int64_t entries) {
   if (entries < 1024) return fail(c, D2_ERR_ARG, "capacity must be >= 1024");
   drop_graphs(c);
   c->cap = entries;
+  c->cap_items = entries;
   c->cap_contacts = entries;
   c->cap_res = std::min<long long>(entries, 1ll << 20);
+  c->auto_grow = false;
+  return D2_OK;
+}
+
+int d2_set_auto_grow(d2_ctx* c, int on) {
+  if (!c) return D2_ERR_ARG;
+  c->auto_grow = on != 0;
   return D2_OK;
 }
 
@@ -2354,6 +2450,51 @@ int d2_set_tune(d2_ctx* c, const d2_tune* t) {
 int d2_get_tune(const d2_ctx* c, d2_tune* t) {
   if (!c || !t) return D2_ERR_ARG;
   *t = c->tune;
+  return D2_OK;
+}
+
+// Coordinate descent over the launch shapes, timing whole runs of the given
+// kind (graph or eager, as the context is set) on the resident inputs.
+int d2_autotune(d2_ctx* c, int kind, float dt, int filter, int reps, d2_tune* best_out, double* best_ms_out) {
+  if (!c || kind < 0 || kind >= RUN_KINDS || reps < 1) return D2_ERR_ARG;
+  const d2_tune start = c->tune;
+  auto time_it = [&](const d2_tune& t, double* ms) -> int {
+    c->tune = t;
+    RC(run(c, kind, filter, dt, nullptr));  // warm-up (and capture)
+    std::vector<double> v;
+    for (int r = 0; r < reps; ++r) {
+      auto t0 = std::chrono::steady_clock::now();
+      RC(run(c, kind, filter, dt, nullptr));
+      v.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    }
+    std::sort(v.begin(), v.end());
+    *ms = v[v.size() / 2];
+    return D2_OK;
+  };
+  const int u = c->cus;
+  const std::vector<int> chunk = {u, 2 * u, 4 * u, 8 * u, 16 * u};
+  const std::vector<int> stream = {2 * u, 4 * u, 8 * u, 16 * u, 32 * u};
+  const std::vector<int> post = {4 * u, 16 * u, 64 * u};
+  d2_tune best = start;
+  double best_ms = 0.0;
+  int rc = time_it(best, &best_ms);
+  for (int sweep = 0; sweep < 2 && !rc; ++sweep)
+    for (int dim = 0; dim < 3 && !rc; ++dim) {
+      const std::vector<int>& vals = dim == 0 ? chunk : dim == 1 ? stream : post;
+      for (int v : vals) {
+        d2_tune t = best;
+        int* f = dim == 0 ? &t.chunk_blocks : dim == 1 ? &t.stream_blocks : &t.post_blocks;
+        if (*f == v || v > (dim == 0 ? 4096 : 16384)) continue;
+        *f = v;
+        double ms = 0.0;
+        if ((rc = time_it(t, &ms))) break;
+        if (ms < best_ms) best_ms = ms, best = t;
+      }
+    }
+  c->tune = rc ? start : best;
+  if (rc) return rc;
+  if (best_out) *best_out = best;
+  if (best_ms_out) *best_ms_out = best_ms;
   return D2_OK;
 }
 
@@ -2578,6 +2719,13 @@ int d2_reset_stats(d2_ctx* c) {
   collect_timers(c);
   for (int i = 0; i < K_NUM; ++i) c->k_launch[i] = 0, c->k_ms[i] = 0.0, c->k_items[i] = 0.0;
   return D2_OK;
+}
+int d2_run_info(const d2_ctx* c, int64_t* info, int n) {
+  if (!c || !info || n < 0) return D2_ERR_ARG;
+  const int total = C_COUNT + 2 * (MAX_LEVELS + 1) * LV_STRIDE;
+  for (int i = 0; i < n && i < total; ++i)
+    info[i] = (int64_t)(i < C_COUNT ? c->host_cnt[i] : c->host_lvl[i - C_COUNT]);
+  return total;
 }
 int d2_sync(d2_ctx* c) {
   if (!c) return D2_ERR_ARG;

@@ -321,6 +321,68 @@ def scene_c2(seed=1002, n_bodies=4096, packing=0.30, vel_sigma=2.0):
     return s
 
 
+def build_trees_parallel(meshes, stream_base, threads=None):
+    """Trees of many meshes on a thread pool (the library call releases the
+    GIL); mesh k uses its own PCG32 stream (42, stream_base + k), so the
+    result does not depend on the thread count."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+
+    def one(k):
+        V, F = meshes[k]
+        return tree_build(V, F, pcg_seed(42, stream_base + k))
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, range(len(meshes)), chunksize=64))
+
+
+def scene_c3(seed=1003, n_scenes=256, lo=16, hi=512, threads=None):
+    """Batched workload: n_scenes independent scenes with round(exp(U(ln lo,
+    ln hi))) bodies each; C1 meshes (subdivision-2 icospheres scaled
+    U(0.5,1.5)), random rotations, positions U([0, side]^3) with side =
+    10 (N/100)^(1/3) (the C1 density), velocities N(0,1), a random gravity
+    direction per scene (|g| = 9.8; it acts between steps, not inside the
+    contact-detection step).  Scenes share space but never share pairs: the
+    broad phase runs per scene.  Trees use one PCG32 stream per body
+    (42, 10000 + body) so they can be built in parallel."""
+    rng = np.random.default_rng(seed)
+    counts = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), size=n_scenes))).astype(np.int64)
+    V0, F0 = icosphere(2, 1.0)
+    _, _, I0 = mass_properties(V0, F0)
+    r0 = float(np.max(np.linalg.norm(V0, axis=1)))
+    meshes, recs, radii, sob = [], [], [], []
+    gravity = np.zeros((n_scenes, 3))
+    for k, n in enumerate(counts):
+        g = rng.normal(size=3)
+        gravity[k] = 9.8 * g / np.linalg.norm(g)
+        side = 10.0 * (n / 100.0) ** (1.0 / 3.0)
+        for _ in range(n):
+            sc = rng.uniform(0.5, 1.5)
+            meshes.append((V0 * sc, F0))
+            q = random_rotation(rng)
+            pos = rng.uniform(0.0, side, size=3)
+            v = rng.normal(0.0, 1.0, size=3)
+            recs.append(body_record(q, pos, v, np.linalg.inv(I0 * sc ** 5)))
+            radii.append(r0 * sc)
+            sob.append(k)
+    trees = build_trees_parallel(meshes, 10000, threads)
+    rec = np.array(recs)
+    radii = np.array(radii)
+    sob = np.array(sob, dtype=np.int32)
+    s = Scene("c3", meshes, trees, rec, radii, sob)
+    s.gravity = gravity
+    s.counts = counts
+    first = np.concatenate([[0], np.cumsum(counts)])
+    parts = []
+    for k in range(n_scenes):
+        p = broad_phase(rec[first[k]:first[k + 1]], radii[first[k]:first[k + 1]])
+        parts.append(p + first[k])
+    s.pairs = np.concatenate(parts).astype(np.int32) if parts else np.zeros((0, 2), dtype=np.int32)
+    s.scene_first = first
+    return s
+
+
 def advance(records, dt=DT):
     """Ballistic drift of the current state by dt (no contact response: that is
     the solver's job, outside this path)."""

@@ -71,8 +71,15 @@ int d2_ctx_create(int device, int precision, d2_ctx** out);
 int d2_ctx_destroy(d2_ctx* ctx);
 const char* d2_last_error(const d2_ctx* ctx);
 int d2_set_precision(d2_ctx* ctx, int precision);
-/* Capacity of the per-level work lists, in entries (default 1<<22). */
+/* Work-list capacities.  By default they start at 1<<22 node pairs per list,
+ * 1<<24 items per level, 1<<22 contacts and 1<<20 CCD soup items, and grow
+ * on demand (a run that overflows is repeated with capacities sized from its
+ * recorded counts).  d2_set_capacity fixes every capacity at `entries` (CCD
+ * soup items at min(entries, 1<<20)); an overflow then fails with
+ * D2_ERR_CAPACITY. */
 int d2_set_capacity(d2_ctx* ctx, int64_t entries);
+/* Turns growth on demand on or off (on by default; d2_set_capacity turns it off). */
+int d2_set_auto_grow(d2_ctx* ctx, int on);
 
 /* Launch shapes (no reference counterpart: the reference runs on the CPU).
  * chunk_blocks: blocks of the chunked list kernels (1-4096; a list is split
@@ -87,6 +94,11 @@ typedef struct d2_tune {
 } d2_tune;
 int d2_set_tune(d2_ctx* ctx, const d2_tune* tune);
 int d2_get_tune(const d2_ctx* ctx, d2_tune* tune);
+/* Autotuner: coordinate descent over the three block counts, timing whole
+ * runs on the resident inputs (kind 0: d2_narrow_run(filter), 1: d2_ccd_run,
+ * 2: d2_step_run(dt, filter); graph or eager as set), median of reps runs per
+ * candidate.  Leaves the best shape set and returns it with its time (ms). */
+int d2_autotune(d2_ctx* ctx, int kind, float dt, int filter, int reps, d2_tune* best, double* best_ms);
 
 /* Graph mode (no reference counterpart).  When on, each kind of run (narrow,
  * CCD, step) is captured once into a HIP graph with the full level budget of
@@ -196,6 +208,15 @@ int d2_num_kernels(void);
 const char* d2_kernel_name(int i);
 int d2_kernel_stats(d2_ctx* ctx, int i, int64_t* launches, double* total_ms, double* items);
 int d2_reset_stats(d2_ctx* ctx);
+/* Counters of the last run (read back when it finished, also after a
+ * failure): 8 global counters (contacts, overflow flags: 1 CCD phase,
+ * 2 discrete phase, triangle-pair items, output contacts, 4 reserved), then
+ * per level 0..16 eight sizes (soup node pairs, connected node pairs, items,
+ * point and edge sub-tests, 3 reserved) of the discrete phase (of the CCD
+ * phase for d2_ccd_run), then the same table of the last CCD phase (its
+ * contact count in the last field of level 0).  Copies min(n, total) values;
+ * returns the total. */
+int d2_run_info(const d2_ctx* ctx, int64_t* info, int n);
 /* Synchronise the context's stream. */
 int d2_sync(d2_ctx* ctx);
 

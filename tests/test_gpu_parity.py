@@ -175,6 +175,29 @@ def test_capacity_error_is_reported():
             d2.compare_trees_full(c, s.pairs, filter=False)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_capacities_grow_on_demand(ctx, graph):
+    """From tiny capacities with growth on, narrow and CCD runs repeat with
+    larger lists and return exactly the default-capacity results."""
+    s = dense_scene()
+    rec = with_future(s.records, scenes.DT)
+    load(ctx, s, rec)
+    mt = np.full(s.pairs.shape[0], 4 * scenes.DT)
+    ref_n = d2.compare_trees_full(ctx, s.pairs, filter=True)
+    ref_c = d2.compare_trees_full_continuous(ctx, s.pairs, mt)
+    with d2.Context(0) as c:
+        c.set_capacity(1024)
+        c.set_auto_grow(True)
+        c.set_graph(graph)
+        load(c, s, rec)
+        for _ in range(2):
+            got = d2.compare_trees_full(c, s.pairs, filter=True)
+            assert np.array_equal(got.ints, ref_n.ints) and np.array_equal(got.reals, ref_n.reals)
+            got = d2.compare_trees_full_continuous(c, s.pairs, mt)
+            assert np.array_equal(got.ints, ref_c.ints) and np.array_equal(got.reals, ref_c.reals)
+            assert np.array_equal(got.per_pair, ref_c.per_pair)
+
+
 def test_empty_run_after_contacts_reports_zero(ctx):
     """Regression: an empty pair list right after a run with contacts must not
     reuse the previous run's contact count (it once overflowed host buffers)."""
