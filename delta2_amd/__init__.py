@@ -30,6 +30,7 @@ BUCKET_STRIDE = 88
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _i64p = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(ctypes.c_float)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _vp = ctypes.c_void_p
 _lib = None
@@ -91,6 +92,16 @@ def lib():
         "d2_reset_stats": ([_vp], ctypes.c_int),
         "d2_sync": ([_vp], ctypes.c_int),
         "d2_run_info": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
+        "d2_mlp_num_params": ([], ctypes.c_int),
+        "d2_mlp_set_params": ([_vp, _fp], ctypes.c_int),
+        "d2_mlp_get_params": ([_vp, _fp], ctypes.c_int),
+        "d2_mlp_infer": ([_vp, _fp, ctypes.c_int, _fp], ctypes.c_int),
+        "d2_mlp_loss_grad": ([_vp, _fp, _fp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp, _fp],
+                             ctypes.c_int),
+        "d2_mlp_train": ([_vp, _fp, _fp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                          ctypes.c_float, ctypes.c_float, ctypes.c_uint64, _fp], ctypes.c_int),
+        "d2_pair_features": ([_vp, _fp], ctypes.c_int),
+        "d2_set_learned": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float], ctypes.c_int),
         "d2_set_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_get_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_set_graph": ([_vp, ctypes.c_int], ctypes.c_int),
@@ -334,6 +345,55 @@ class Context:
             self._check(L.d2_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(items)))
             out[L.d2_kernel_name(i).decode()] = (n.value, ms.value, items.value)
         return out
+
+    # --- learned pair gate ---------------------------------------------------
+    def mlp_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        assert p.size == lib().d2_mlp_num_params()
+        self._check(lib().d2_mlp_set_params(self.h, p.ctypes.data_as(_fp)))
+
+    def mlp_get_params(self):
+        p = np.zeros(lib().d2_mlp_num_params(), dtype=np.float32)
+        self._check(lib().d2_mlp_get_params(self.h, p.ctypes.data_as(_fp)))
+        return p
+
+    def mlp_infer(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 24)
+        Y = np.zeros((X.shape[0], 2), dtype=np.float32)
+        self._check(lib().d2_mlp_infer(self.h, X.ctypes.data_as(_fp), X.shape[0], Y.ctypes.data_as(_fp)))
+        return Y
+
+    def mlp_loss_grad(self, X, T, lw=(1.0, 1.0), delta=0.1):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 24)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, 2)
+        loss = np.zeros(1, dtype=np.float32)
+        g = np.zeros(lib().d2_mlp_num_params(), dtype=np.float32)
+        self._check(lib().d2_mlp_loss_grad(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp), X.shape[0],
+                                           float(lw[0]), float(lw[1]), float(delta), loss.ctypes.data_as(_fp),
+                                           g.ctypes.data_as(_fp)))
+        return float(loss[0]), g
+
+    def mlp_train(self, X, T, epochs, batch=4096, lr=1e-3, lw=(1.0, 1.0), delta=0.1, seed=0):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 24)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1, 2)
+        out = np.zeros(epochs, dtype=np.float32)
+        self._check(lib().d2_mlp_train(self.h, X.ctypes.data_as(_fp), T.ctypes.data_as(_fp), X.shape[0], int(epochs),
+                                       int(batch), float(lr), float(lw[0]), float(lw[1]), float(delta), int(seed),
+                                       out.ctypes.data_as(_fp)))
+        return out
+
+    def pair_features(self):
+        X = np.zeros((self.n_pairs, 24), dtype=np.float32)
+        self._check(lib().d2_pair_features(self.h, X.ctypes.data_as(_fp)))
+        return X
+
+    def set_learned(self, on=True, t_gate=1.1, s_gate=0.99):
+        self._check(lib().d2_set_learned(self.h, int(bool(on)), float(t_gate), float(s_gate)))
+
+    def gate_counts(self):
+        """(proposed, certified) pairs of the CCD and of the discrete phase of the last run."""
+        g, _ = self.run_info()
+        return {"ccd": (int(g[4]), int(g[5])), "discrete": (int(g[6]), int(g[7]))}
 
     def run_info(self):
         """(global counters[8], per-level sizes[2, 17, 8]) of the last run:

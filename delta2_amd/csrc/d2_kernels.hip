@@ -31,7 +31,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <random>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -76,7 +78,11 @@ constexpr int RES_SLOTS = 15;  // sub-results per soup triangle pair
 // Global counters (atomics) and per-level sizes.  Level L reads its list sizes
 // from lvl[L*LV_STRIDE + ...] and the level's write kernel stores the sizes of
 // level L+1; no size ever travels through the host.
-enum Counter : int { C_CONTACTS = 0, C_OVERFLOW = 1, C_LVL_ITEMS = 2, C_OUT = 3, C_COUNT = 8 };
+enum Counter : int {
+  C_CONTACTS = 0, C_OVERFLOW = 1, C_LVL_ITEMS = 2, C_OUT = 3,
+  C_GATED_CCD = 4, C_CERT_CCD = 5, C_GATED_DISC = 6, C_CERT_DISC = 7,  // learned gate: proposed / certified pairs
+  C_COUNT = 8
+};
 enum LevelSlot : int { LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_STRIDE = 8 };
 constexpr int MAX_LEVELS = 16;  // contact keys hold 16 base-8 path digits
 // Key layout: level(5) | list(1) | path(48) | sub(10)
@@ -280,8 +286,8 @@ __global__ void k_roots(Dev D, const double* pair_rec, double* scale, int keep_c
     }
   }
   if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)
-      if (!(keep_ccd && i == C_OVERFLOW)) D.cnt[i] = 0;  // a step reports an overflow of its CCD phase too
+    for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)  // a step keeps its CCD phase's flags and gate counts
+      if (!(keep_ccd && (i == C_OVERFLOW || i == C_GATED_CCD || i == C_CERT_CCD))) D.cnt[i] = 0;
     for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)
       D.lvl[i] = i == LV_SOUP ? (unsigned long long)n_pairs : 0ull;
   }
@@ -1688,6 +1694,8 @@ __global__ void k_body_step_pairs(int n_pairs, const int32_t* pairs, const int32
   }
 }
 
+#include "d2_learned.hpp"
+
 // Body future states: future = current.extrapolate(step).
 // skip_static: static bodies keep their future state (PDE_explicit.cpp:49-56).
 __global__ void k_project(int n, double* bodies, const double* step, int skip_static) {
@@ -1709,9 +1717,10 @@ using namespace d2;
 namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
-                              "leaf_point", "leaf_edge", "tri_enum", "tri_dist", "post", "post_pairs", "cluster"};
+                              "leaf_point", "leaf_edge", "tri_enum", "tri_dist", "post", "post_pairs", "cluster",
+                              "gate"};
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
-           K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_NUM };
+           K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
@@ -1728,8 +1737,11 @@ struct Sig {
   float dt = -1.f;
   d2_tune tune{0, 0, 0};
   int levels[2] = {-1, -1};  // graph levels of each phase
+  int learned = -1;
+  float gate_t = 0.f, gate_s = 0.f;
   bool operator==(const Sig& o) const {
-    return levels[0] == o.levels[0] && levels[1] == o.levels[1] && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
+    return levels[0] == o.levels[0] && levels[1] == o.levels[1] && learned == o.learned && gate_t == o.gate_t &&
+           gate_s == o.gate_s && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
            precision == o.precision && filter == o.filter && dt == o.dt &&
            tune.chunk_blocks == o.tune.chunk_blocks && tune.stream_blocks == o.tune.stream_blocks &&
            tune.post_blocks == o.tune.post_blocks;
@@ -1787,6 +1799,17 @@ struct d2_ctx {
   DBuf<int> first_any, first_ns;
   DBuf<double> cluster_step, body_step;
   DBuf<int32_t> tmp_cluster;
+  // learned gate
+  DBuf<float> tree_radius, tree_eps_out, hv, feat, pred;
+  DBuf<int32_t> hv_off;
+  DBuf<int> cert;
+  DBuf<float> mlp_w, mlp_m, mlp_v, mlp_wf, mlp_grad;
+  DBuf<short> mlp_wb;
+  DBuf<float> tr_x, tr_t, tr_act, tr_dlt, tr_loss;
+  DBuf<int> tr_idx;
+  bool mlp_ready = false, learned_on = false, has_max_time = false;
+  float gate_t = 1.1f, gate_s = 0.99f;
+  long long mlp_step = 0;
   // host mirrors (pinned)
   unsigned long long* host_cnt = nullptr;
   unsigned long long* host_lvl = nullptr;
@@ -1960,6 +1983,9 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->kept_cnt, np + 1)); RC(ensure(c, c->out_off, np + 1)); RC(ensure(c, c->big, np));
   RC(ensure(c, c->n_big, 2)); RC(ensure(c, c->huge, np)); RC(ensure(c, c->perm, c->cap_contacts)); RC(ensure(c, c->keep, c->cap_contacts));
   RC(ensure(c, c->scan_part, g));
+  if (c->learned_on) {
+    RC(ensure(c, c->feat, np * MLP_IN)); RC(ensure(c, c->pred, np * MLP_OUT)); RC(ensure(c, c->cert, np));
+  }
   return D2_OK;
 }
 
@@ -2114,6 +2140,22 @@ void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
   Dev D = make_dev(c, 0, ph.discrete);
   hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, D, c->pair_rec.p,
                      c->scale.p, ph.keep_ccd);
+  if (c->learned_on && c->mlp_ready && np > 0) {
+    Timer tg(c, K_GATE, np);
+    if (!ph.keep_ccd) {  // proposals once per run (a step's discrete phase reuses them)
+      const double* mt = (ph.mt_from_bodies || !c->has_max_time) ? nullptr : c->max_time.p;
+      hipLaunchKernelGGL(k_pair_features, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, c->bodies.p,
+                         c->body_tree.p, c->tree_radius.p, mt, c->feat.p);
+      const int tiles = (np + 15) / 16;
+      hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0, c->stream,
+                         np, c->feat.p, c->mlp_wb.p, c->pred.p);
+    }
+    hipLaunchKernelGGL(k_certify, dim3(grid_for(c, (long long)np * 64)), dim3(256), 0, c->stream, np, c->pair_info.p,
+                       c->body_tree.p, c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p, c->hv.p,
+                       c->tree_eps_out.p, ph.discrete ? 1 : 0, c->cert.p,
+                       c->cnt.p + (ph.discrete ? C_GATED_DISC : C_GATED_CCD));
+    hipLaunchKernelGGL(k_roots_compact, dim3(1), dim3(1024), 0, c->stream, np, c->cert.p, D.soup, c->lvl.p);
+  }
 }
 
 // Items of level l per kernel, from its sizes (profiling).
@@ -2231,6 +2273,7 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
   sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
   sig.tune = c->tune;
+  sig.learned = (c->learned_on && c->mlp_ready) ? 1 : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
   std::vector<std::function<int()>> ops;
   int seg = 0;
@@ -2405,6 +2448,10 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->big), release(c->huge), release(c->n_big);
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster);
+  release(c->tree_radius), release(c->tree_eps_out), release(c->hv), release(c->feat), release(c->pred), release(c->hv_off);
+  release(c->cert), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
+  release(c->mlp_wb), release(c->tr_x), release(c->tr_t), release(c->tr_act), release(c->tr_dlt), release(c->tr_loss);
+  release(c->tr_idx);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
   if (c->host_lvl) (void)hipHostFree(c->host_lvl);
   (void)hipStreamDestroy(c->stream);
@@ -2528,6 +2575,51 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   HIPCHK(c, hipMemcpyAsync(c->nodes.p, nodes, sizeof(NodeRec) * total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->buckets.p, buckets, sizeof(double) * BUCKET * (size_t)total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->tree_off.p, tree_node_offset, sizeof(int32_t) * (n_trees + 1), hipMemcpyHostToDevice, c->stream));
+  // Per tree, for the learned gate's certificate: the distinct vertices of the
+  // leaf triangles (body coordinates), the bounding radius, and the largest
+  // distance a float-barycentric "inside" test can accept a point outside a
+  // triangle: 64 eps_f kappa L (kappa = d00 d11 / (d00 d11 - d01^2), L the
+  // longest edge; 4x the 16 eps_f kappa L rounding bound).
+  std::vector<float> hv, radius(n_trees), eps_out(n_trees);
+  std::vector<int32_t> hv_off(n_trees + 1, 0);
+  for (int t = 0; t < n_trees; ++t) {
+    std::vector<std::array<double, 3>> vs;
+    double rmax = 0.0, eo = 0.0;
+    for (int i = tree_node_offset[t]; i < tree_node_offset[t + 1]; ++i) {
+      const int32_t* r = nodes + 8 * (size_t)i;
+      if (r[2]) continue;
+      const double* bk = buckets + (size_t)BUCKET * (tree_node_offset[t] + r[1]);
+      for (int k = 0; k < r[7]; ++k) {
+        std::array<double, 3> v[3];
+        for (int q = 0; q < 3; ++q) v[q] = {bk[(3 * q) * 8 + k], bk[(3 * q + 1) * 8 + k], bk[(3 * q + 2) * 8 + k]};
+        double e0[3], e1[3], e2[3];
+        for (int d = 0; d < 3; ++d) e0[d] = v[1][d] - v[0][d], e1[d] = v[2][d] - v[0][d], e2[d] = v[2][d] - v[1][d];
+        const double d00 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+        const double d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+        const double d01 = e0[0] * e1[0] + e0[1] * e1[1] + e0[2] * e1[2];
+        const double d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+        const double den = d00 * d11 - d01 * d01;
+        const double kappa = den > 0.0 ? d00 * d11 / den : 1e30;
+        eo = std::max(eo, 64.0 * 5.9604645e-08 * kappa * std::sqrt(std::max(d00, std::max(d11, d22))));
+        for (int q = 0; q < 3; ++q) {
+          vs.push_back(v[q]);
+          rmax = std::max(rmax, std::sqrt(v[q][0] * v[q][0] + v[q][1] * v[q][1] + v[q][2] * v[q][2]));
+        }
+      }
+    }
+    std::sort(vs.begin(), vs.end());
+    vs.erase(std::unique(vs.begin(), vs.end()), vs.end());
+    for (auto& v : vs) hv.push_back((float)v[0]), hv.push_back((float)v[1]), hv.push_back((float)v[2]);
+    hv_off[t + 1] = hv_off[t] + (int32_t)vs.size();
+    radius[t] = (float)rmax * (1.0f + 1e-6f);
+    eps_out[t] = (float)std::min(eo, 1e30);
+  }
+  RC(ensure(c, c->hv, std::max<size_t>(hv.size(), 3))); RC(ensure(c, c->hv_off, n_trees + 1));
+  RC(ensure(c, c->tree_radius, n_trees)); RC(ensure(c, c->tree_eps_out, n_trees));
+  HIPCHK(c, hipMemcpyAsync(c->hv.p, hv.data(), sizeof(float) * hv.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->hv_off.p, hv_off.data(), sizeof(int32_t) * (n_trees + 1), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->tree_radius.p, radius.data(), sizeof(float) * n_trees, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->tree_eps_out.p, eps_out.data(), sizeof(float) * n_trees, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_trees = n_trees;
   c->max_depth = max_depth;
@@ -2592,6 +2684,7 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_pairs = n_pairs;
   c->n_clusters = 0;  // clusters describe a pair list
+  c->has_max_time = max_time != nullptr;
   c->last_mode = -1;
   c->ccd_valid = false;
   return D2_OK;
@@ -2696,6 +2789,144 @@ int d2_cluster_steps(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
   RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false));
   HIPCHK(c, hipMemcpyAsync(out, c->cluster_step.p, sizeof(double) * n_clusters, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+// ----------------------------------------------------------- learned gate
+namespace {
+int mlp_alloc(d2_ctx* c) {
+  RC(ensure(c, c->mlp_w, MLP_PARAMS)); RC(ensure(c, c->mlp_m, MLP_PARAMS)); RC(ensure(c, c->mlp_v, MLP_PARAMS));
+  RC(ensure(c, c->mlp_wf, MLP_PARAMS)); RC(ensure(c, c->mlp_grad, MLP_PARAMS)); RC(ensure(c, c->mlp_wb, MLP_PARAMS));
+  return D2_OK;
+}
+// Minibatch forward/backward + parameter gradient (grad in c->mlp_grad,
+// per-sample losses in c->tr_loss).
+void mlp_batch_grad(d2_ctx* c, const int* idx, int nb, float lw0, float lw1, float delta) {
+  hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3((nb + 255) / 256), dim3(256), 0, c->stream, nb, idx, c->tr_x.p, c->tr_t.p,
+                     c->mlp_wf.p, lw0, lw1, delta, c->tr_act.p, c->tr_dlt.p, c->tr_loss.p);
+  hipLaunchKernelGGL(k_mlp_grad, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, c->stream, nb, c->tr_act.p, c->tr_dlt.p,
+                     c->mlp_grad.p);
+}
+int mlp_upload_data(d2_ctx* c, const float* X, const float* T, int n, int batch) {
+  RC(ensure(c, c->tr_x, (size_t)n * MLP_IN)); RC(ensure(c, c->tr_t, (size_t)n * MLP_OUT));
+  RC(ensure(c, c->tr_act, (size_t)batch * MLP_ACT)); RC(ensure(c, c->tr_dlt, (size_t)batch * MLP_DELTA));
+  RC(ensure(c, c->tr_loss, batch)); RC(ensure(c, c->tr_idx, n));
+  HIPCHK(c, hipMemcpyAsync(c->tr_x.p, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  if (T) HIPCHK(c, hipMemcpyAsync(c->tr_t.p, T, sizeof(float) * MLP_OUT * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  return D2_OK;
+}
+}  // namespace
+
+int d2_mlp_num_params(void) { return MLP_PARAMS; }
+
+int d2_mlp_set_params(d2_ctx* c, const float* params) {
+  if (!c || !params) return D2_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  RC(mlp_alloc(c));
+  HIPCHK(c, hipMemcpyAsync(c->mlp_w.p, params, sizeof(float) * MLP_PARAMS, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->mlp_m.p, 0, sizeof(float) * MLP_PARAMS, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->mlp_v.p, 0, sizeof(float) * MLP_PARAMS, c->stream));
+  hipLaunchKernelGGL(k_mlp_round, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->mlp_w.p, c->mlp_wb.p,
+                     c->mlp_wf.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mlp_ready = true;
+  c->mlp_step = 0;
+  return D2_OK;
+}
+
+int d2_mlp_get_params(d2_ctx* c, float* params) {
+  if (!c || !params) return D2_ERR_ARG;
+  if (!c->mlp_ready) return fail(c, D2_ERR_STATE, "no network parameters set");
+  HIPCHK(c, hipMemcpyAsync(params, c->mlp_w.p, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_mlp_infer(d2_ctx* c, const float* X, int n, float* Y) {
+  if (!c || n < 0 || (n > 0 && (!X || !Y))) return D2_ERR_ARG;
+  if (!c->mlp_ready) return fail(c, D2_ERR_STATE, "no network parameters set");
+  if (n == 0) return D2_OK;
+  RC(ensure(c, c->tr_x, (size_t)n * MLP_IN)); RC(ensure(c, c->tr_t, (size_t)n * MLP_OUT));
+  HIPCHK(c, hipMemcpyAsync(c->tr_x.p, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  const int tiles = (n + 15) / 16;
+  hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0, c->stream, n,
+                     c->tr_x.p, c->mlp_wb.p, c->tr_t.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(Y, c->tr_t.p, sizeof(float) * MLP_OUT * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_mlp_loss_grad(d2_ctx* c, const float* X, const float* T, int n, float lw0, float lw1, float delta, float* loss,
+                     float* grad) {
+  if (!c || !X || !T || n <= 0 || n > (1 << 20)) return D2_ERR_ARG;
+  if (!c->mlp_ready) return fail(c, D2_ERR_STATE, "no network parameters set");
+  RC(mlp_upload_data(c, X, T, n, n));
+  mlp_batch_grad(c, nullptr, n, lw0, lw1, delta);
+  HIPCHK(c, hipGetLastError());
+  std::vector<float> l(n);
+  HIPCHK(c, hipMemcpyAsync(l.data(), c->tr_loss.p, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+  if (grad) HIPCHK(c, hipMemcpyAsync(grad, c->mlp_grad.p, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double sum = 0.0;
+  for (float v : l) sum += v;
+  if (loss) *loss = (float)(sum / n);
+  return D2_OK;
+}
+
+int d2_mlp_train(d2_ctx* c, const float* X, const float* T, int n, int epochs, int batch, float lr, float lw0,
+                 float lw1, float delta, uint64_t seed, float* epoch_loss) {
+  if (!c || !X || !T || n <= 0 || epochs <= 0 || batch <= 0 || !(lr > 0.f)) return D2_ERR_ARG;
+  if (!c->mlp_ready) return fail(c, D2_ERR_STATE, "set initial network parameters first");
+  batch = std::min(batch, n);
+  RC(mlp_upload_data(c, X, T, n, batch));
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::mt19937_64 rng(seed);
+  std::vector<float> l(batch);
+  for (int e = 0; e < epochs; ++e) {
+    std::shuffle(idx.begin(), idx.end(), rng);
+    HIPCHK(c, hipMemcpyAsync(c->tr_idx.p, idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, c->stream));
+    double sum = 0.0;
+    for (int b0 = 0; b0 < n; b0 += batch) {
+      const int nb = std::min(batch, n - b0);
+      mlp_batch_grad(c, c->tr_idx.p + b0, nb, lw0, lw1, delta);
+      ++c->mlp_step;
+      const float b1 = 0.9f, b2 = 0.999f;
+      const float c1 = 1.f - std::pow(b1, (float)c->mlp_step), c2 = 1.f - std::pow(b2, (float)c->mlp_step);
+      hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_PARAMS + 255) / 256), dim3(256), 0, c->stream, c->mlp_w.p, c->mlp_m.p,
+                         c->mlp_v.p, c->mlp_grad.p, lr, b1, b2, 1e-8f, c1, c2, c->mlp_wb.p, c->mlp_wf.p);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(l.data(), c->tr_loss.p, sizeof(float) * nb, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int i = 0; i < nb; ++i) sum += l[i];
+    }
+    if (epoch_loss) epoch_loss[e] = (float)(sum / n);
+  }
+  drop_graphs(c);  // captured inferences read the weights in place; shapes are unchanged, but be explicit
+  return D2_OK;
+}
+
+int d2_pair_features(d2_ctx* c, float* X) {
+  if (!c || !X) return D2_ERR_ARG;
+  if (c->n_pairs <= 0) return D2_OK;
+  RC(ensure(c, c->feat, (size_t)c->n_pairs * MLP_IN));
+  hipLaunchKernelGGL(k_pair_features, dim3(grid_for(c, c->n_pairs)), dim3(256), 0, c->stream, c->n_pairs, c->pairs.p,
+                     c->bodies.p, c->body_tree.p, c->tree_radius.p, c->has_max_time ? c->max_time.p : nullptr,
+                     c->feat.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(X, c->feat.p, sizeof(float) * MLP_IN * (size_t)c->n_pairs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_set_learned(d2_ctx* c, int on, float t_gate, float s_gate) {
+  if (!c) return D2_ERR_ARG;
+  if (on && !c->mlp_ready) return fail(c, D2_ERR_STATE, "set network parameters before turning the gate on");
+  c->learned_on = on != 0;
+  c->gate_t = t_gate;
+  c->gate_s = s_gate;
   return D2_OK;
 }
 

@@ -189,6 +189,37 @@ int d2_clusters_upload(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_cluste
  * overwritten (as in the reference); current states and last steps are not. */
 int d2_step_run(d2_ctx* ctx, float dt, int filter, double* cluster_step_out, int64_t* n_contacts_out);
 
+/* ---- learned pair gate (no reference counterpart) ------------------------
+ * A 24 -> 64 -> 64 -> 2 MLP (ReLU; bf16 weights, FP32 accumulation on MFMA)
+ * proposes per pair (y0) its step scale and (y1) a lower bound on its time of
+ * impact as a fraction of the CCD interval.  With the gate on, a pair with
+ * y0 >= s_gate and y1 >= t_gate is checked by a sound separation certificate
+ * (vertex sets separated by more than the search distance plus rounding and
+ * float-barycentric margins, in every frame the sub-tests use); certified
+ * pairs skip the traversal and take the plain path's result for such pairs
+ * (no contact, scale 1, min_time = max_time); all other pairs, including
+ * failed certificates, run the plain path.  Results are identical either way.
+ * Parameters: FP32 master weights, layout W1[24][64] b1[64] W2[64][64] b2[64]
+ * W3[64][2] b3[2] (row = input), d2_mlp_num_params() floats. */
+int d2_mlp_num_params(void);
+int d2_mlp_set_params(d2_ctx* ctx, const float* params);
+int d2_mlp_get_params(d2_ctx* ctx, float* params);
+/* Inference (the MFMA kernel the gate uses) of n feature rows X[n][24] -> Y[n][2]. */
+int d2_mlp_infer(d2_ctx* ctx, const float* X, int n, float* Y);
+/* Mean loss and its gradient over n samples (one batch; the training kernels),
+ * loss per sample sum_o lw_o * huber_delta(y_o - t_o). */
+int d2_mlp_loss_grad(d2_ctx* ctx, const float* X, const float* T, int n, float lw0, float lw1, float delta,
+                     float* loss, float* grad);
+/* Adam training on the device (forward, backward and update kernels),
+ * minibatches of `batch` shuffled with `seed`; mean loss per epoch out. */
+int d2_mlp_train(d2_ctx* ctx, const float* X, const float* T, int n, int epochs, int batch, float lr, float lw0,
+                 float lw1, float delta, uint64_t seed, float* epoch_loss);
+/* The 24 features of every resident pair (relative pose and motion in body
+ * a's frame, lengths in units of r_a + r_b, times in units of the interval). */
+int d2_pair_features(d2_ctx* ctx, float* X);
+/* Turns the gate on or off for later runs (needs parameters). */
+int d2_set_learned(d2_ctx* ctx, int on, float t_gate, float s_gate);
+
 /* Results of the last run: CSR offsets (n_pairs+1), contact records, and
  * per-pair values (CCD: scale, min_time_out; discrete: unused).
  * Any pointer may be NULL. */
@@ -210,7 +241,8 @@ int d2_kernel_stats(d2_ctx* ctx, int i, int64_t* launches, double* total_ms, dou
 int d2_reset_stats(d2_ctx* ctx);
 /* Counters of the last run (read back when it finished, also after a
  * failure): 8 global counters (contacts, overflow flags: 1 CCD phase,
- * 2 discrete phase, triangle-pair items, output contacts, 4 reserved), then
+ * 2 discrete phase, triangle-pair items, output contacts, then the learned
+ * gate's proposed and certified pairs of the CCD and of the discrete phase), then
  * per level 0..16 eight sizes (soup node pairs, connected node pairs, items,
  * point and edge sub-tests, 3 reserved) of the discrete phase (of the CCD
  * phase for d2_ccd_run), then the same table of the last CCD phase (its

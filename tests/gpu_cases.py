@@ -54,3 +54,11 @@ def hull_scene(seed=11, n=40, box=4.0):
 
 def oracle_scene(s, records):
     return d2o.Scene(s.trees, s.tree_of_body, records)
+
+
+@functools.lru_cache(maxsize=None)
+def sparse_scene(seed=1202, n=256):
+    """The 4096-body workload's generator at 256 bodies (half subdivision-3
+    icospheres, half ellipsoid hulls, Poisson-disk placement at 30% packing):
+    many broad-phase pairs are separated, some touch."""
+    return scenes.scene_c2(seed=seed, n_bodies=n)
