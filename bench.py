@@ -86,6 +86,89 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 3}))
 
 
+def gate_weights():
+    """The committed gate weights, or a quick model trained on the device."""
+    from delta2_amd import learned
+    w = learned.load_weights()
+    if w is not None:
+        return w, "delta2_amd/weights/pair_gate_mlp.npy"
+    import delta2_amd as d2
+    with d2.Context(0, d2.D2_FP32) as c:
+        pool = learned.TrainingBodies(n_scenes=8)
+        X, T = learned.make_dataset(c, pool, 200_000, seed=1)
+        w, _ = learned.train(c, X, T, epochs=9)
+    return w, "trained in this run on 200k pairs"
+
+
+def step_results(ctx):
+    r = ctx.result(with_pairs=True)
+    return r.offsets.copy(), r.ints.copy(), r.reals.copy(), r.per_pair.copy()
+
+
+def same_results(a, b):
+    return all(np.array_equal(x, y) for x, y in zip(a, b))
+
+
+def make_ctx(scene, prec, graph, weights=None, clusters=None, tune=True):
+    """A context holding the scene; launch shapes (and the gate's phase mask)
+    picked by the library's autotuner on this step."""
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    ctx = d2.Context(0, prec)
+    ctx.upload_trees(scene.trees)
+    ctx.upload_bodies(scene.tree_of_body, scene.records)
+    ctx.upload_pairs(scene.pairs)
+    cl, n_cl = clusters
+    ctx.upload_clusters(cl, n_cl)
+    if weights is not None:
+        ctx.mlp_set_params(weights)
+        ctx.set_learned(True)
+    ctx.set_graph(graph)
+    ctx.step_run(scenes.DT, True)  # sizes the buffers, observes the level counts
+    if tune:
+        shape, ms = ctx.autotune("step", reps=3)
+        log(f"[bench] autotuned {shape}: {ms:.3f} ms")
+    return ctx
+
+
+def run_c4(args):
+    """Learned vs plain at equal TOI error on the held-out 4096-body C1-density
+    scene: identical step results (contacts, per-pair TOI/scale, cluster
+    steps) are checked, so the speedup is at zero TOI difference."""
+    import delta2_amd as d2
+    from delta2_amd import learned, scenes
+    t0 = time.time()
+    sc = scenes.scene_c4()
+    cl = scenes.clusters(sc.pairs, sc.n_bodies, sc.records)
+    n_pairs = int(sc.pairs.shape[0])
+    log(f"[c4] {sc.n_bodies} bodies, {n_pairs} pairs, built in {time.time() - t0:.1f}s")
+    w, src = gate_weights()
+    out, ref = {}, None
+    for name, prec, wt in (("plain_fp64", d2.D2_FP64, None), ("learned_fp64", d2.D2_FP64, w),
+                           ("plain_fp32", d2.D2_FP32, None), ("learned_fp32", d2.D2_FP32, w)):
+        ctx = make_ctx(sc, prec, True, wt, cl)
+        sec = timed_steps(ctx, scenes.DT, args.steps, args.warmup)
+        res = step_results(ctx)
+        ref = res if ref is None else ref
+        gate = ctx.gate_counts() if wt is not None else None
+        out[name] = {"ms_per_step": 1e3 * sec, "value": n_pairs / sec, "identical_to_plain_fp64": same_results(res, ref),
+                     "gate": gate}
+        log(f"[c4] {name}: {1e3 * sec:.3f} ms/step, identical {out[name]['identical_to_plain_fp64']}, gate {gate}")
+        ctx.close()
+    best = out["learned_fp32"]
+    res = {"metric": METRIC, "value": best["value"], "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": "held-out 4096 subdiv-2 icospheres U(0.5,1.5) at the C1 density (34.5 m box), "
+                                  "v~N(0,1), dt=1/60; one step (CCD+step rule+cluster steps+discrete+filter)",
+                      "pairs": n_pairs, "weights": src},
+           "variants": out,
+           "learned_vs_plain_fp64": best["value"] / out["plain_fp64"]["value"],
+           "learned_vs_plain_same_precision": {"fp64": out["learned_fp64"]["value"] / out["plain_fp64"]["value"],
+                                               "fp32": best["value"] / out["plain_fp32"]["value"]}}
+    print(json.dumps(res), flush=True)
+
+
 def timed_steps(ctx, dt, steps, warmup):
     for _ in range(warmup):
         ctx.step_run(dt, True)
@@ -190,9 +273,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--scenes", type=int, default=256, help="c3: number of independent scenes")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
+    ap.add_argument("--learned", default="on", choices=["on", "off"],
+                    help="the learned pair gate (weights from delta2_amd/weights; trained here when absent)")
     ap.add_argument("--graph", default="on", choices=["on", "off"])
     ap.add_argument("--profile-steps", type=int, default=5, help="eager steps timed per kernel for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -213,6 +298,9 @@ def main():
     if args.workload == "c3":
         run_c3(args)
         return
+    if args.workload == "c4":
+        run_c4(args)
+        return
     t0 = time.time()
     scene = scenes.cached(args.workload, scenes.scene_c2 if args.workload == "c2" else scenes.scene_c1)
     log(f"[bench] scene {scene.name}: {scene.n_bodies} bodies, {scene.n_triangles()} triangles, "
@@ -220,38 +308,29 @@ def main():
     pairs = scene.pairs
     max_time = np.minimum(scene.records[pairs[:, 0], 39], scene.records[pairs[:, 1], 39])
     prec = d2.D2_FP64 if args.precision == "fp64" else d2.D2_FP32
-    ctx = d2.Context(0, prec)
-    ctx.upload_trees(scene.trees)
-    ctx.upload_bodies(scene.tree_of_body, scene.records)
-    ctx.upload_pairs(pairs)
-    cl, n_cl = scenes.clusters(pairs, scene.n_bodies, scene.records)
-    ctx.upload_clusters(cl, n_cl)
-    ctx.set_graph(args.graph == "on")
-    n_contacts = 0
-
-    def step():
-        nonlocal n_contacts
-        n_contacts = ctx.step_run(scenes.DT, True)
-
-    for _ in range(args.warmup):
-        step()
-    ctx.sync()
-    ts = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.sync()
-    te = time.perf_counter()
-    wall = te - ts
-    ms_per_step = 1e3 * wall / args.steps
-    value = pairs.shape[0] * args.steps / wall
-    log(f"[bench] {args.steps} steps in {wall:.3f}s = {ms_per_step:.3f} ms/step ({n_cl} clusters, "
-        f"{n_contacts} filtered contacts, graph {args.graph})")
+    clusters = scenes.clusters(pairs, scene.n_bodies, scene.records)
+    n_cl = clusters[1]
+    weights, wsrc = gate_weights() if args.learned == "on" else (None, None)
+    # the plain FP64 path on the same inputs (reference point of the speedup)
+    plain = make_ctx(scene, d2.D2_FP64, True, None, clusters)
+    plain_sec = timed_steps(plain, scenes.DT, args.steps, args.warmup)
+    plain_res = step_results(plain)
+    plain.close()
+    ctx = make_ctx(scene, prec, args.graph == "on", weights, clusters)
+    n_contacts = ctx.step_run(scenes.DT, True)
+    sec = timed_steps(ctx, scenes.DT, args.steps, args.warmup)
+    identical = same_results(step_results(ctx), plain_res)
+    ms_per_step = 1e3 * sec
+    value = pairs.shape[0] / sec
+    gate = ctx.gate_counts() if weights is not None else None
+    log(f"[bench] plain fp64: {1e3 * plain_sec:.3f} ms/step; this form: {ms_per_step:.3f} ms/step "
+        f"({n_cl} clusters, {n_contacts} filtered contacts, graph {args.graph}, gate {gate}, identical {identical})")
     # per-kernel times: eager steps, every kernel bracketed by HIP events on the context's stream
     ctx.set_graph(False)
     ctx.reset_stats()
     ctx.set_profiling(True)
     for _ in range(args.profile_steps):
-        step()
+        ctx.step_run(scenes.DT, True)
     ctx.sync()
     ctx.set_profiling(False)
     stats = ctx.kernel_stats()
@@ -277,6 +356,10 @@ def main():
         "metric": METRIC, "value": value, "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64" if prec == d2.D2_FP64 else "f32+f64", "data": "synthetic",
+        "form": {"precision": args.precision, "learned_gate": args.learned == "on", "graph": args.graph == "on",
+                 "weights": wsrc},
+        "plain_fp64": {"value": pairs.shape[0] / plain_sec, "ms_per_step": 1e3 * plain_sec},
+        "speedup_vs_plain_fp64": plain_sec / sec, "identical_to_plain_fp64": identical, "gate": gate,
         "config": {"workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
                                "Poisson-disk 30% packing; v~N(0,2); dt=1/60; CCD+step rule then discrete+filter"
                    if scene.name == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60",

@@ -49,10 +49,11 @@ class CapacityError(D2Error):
 class Tune(ctypes.Structure):
     """Launch shapes (d2_tune): blocks of the chunked list kernels, of the
     grid-stride kernels and of the per-pair post-processing."""
-    _fields_ = [("chunk_blocks", ctypes.c_int), ("stream_blocks", ctypes.c_int), ("post_blocks", ctypes.c_int)]
+    _fields_ = [("chunk_blocks", ctypes.c_int), ("stream_blocks", ctypes.c_int), ("post_blocks", ctypes.c_int),
+                ("gate_phases", ctypes.c_int)]
 
     def as_tuple(self):
-        return (self.chunk_blocks, self.stream_blocks, self.post_blocks)
+        return (self.chunk_blocks, self.stream_blocks, self.post_blocks, self.gate_phases)
 
 
 def lib():
@@ -303,8 +304,8 @@ class Context:
     def set_graph(self, on=True):
         self._check(lib().d2_set_graph(self.h, int(bool(on))))
 
-    def set_tune(self, chunk_blocks, stream_blocks, post_blocks):
-        t = Tune(int(chunk_blocks), int(stream_blocks), int(post_blocks))
+    def set_tune(self, chunk_blocks, stream_blocks, post_blocks, gate_phases=3):
+        t = Tune(int(chunk_blocks), int(stream_blocks), int(post_blocks), int(gate_phases))
         self._check(lib().d2_set_tune(self.h, ctypes.byref(t)))
 
     def autotune(self, kind="step", dt=None, filter=True, reps=3):
@@ -387,8 +388,9 @@ class Context:
         self._check(lib().d2_pair_features(self.h, X.ctypes.data_as(_fp)))
         return X
 
-    def set_learned(self, on=True, t_gate=1.1, s_gate=0.99):
-        self._check(lib().d2_set_learned(self.h, int(bool(on)), float(t_gate), float(s_gate)))
+    def set_learned(self, on=True, t_gate=1.1, s_gate=0.99, phases=3):
+        """on: gate on/off; phases: mask of the phases it applies to (1 CCD, 2 discrete)."""
+        self._check(lib().d2_set_learned(self.h, int(phases) if on else 0, float(t_gate), float(s_gate)))
 
     def gate_counts(self):
         """(proposed, certified) pairs of the CCD and of the discrete phase of the last run."""

@@ -147,6 +147,10 @@ struct Dev {
   unsigned long long *pass_min_s, *pass_min_c;
   int *pass_flag_s, *pass_flag_c;
   double* min_time;
+  const float* tree_eps_out;   // float-barycentric acceptance bound per tree
+  unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
+  const int32_t* body_tree;
+  int fp32;                    // FP32 culls of the CCD leaf sub-tests
   int n_pairs;
   long long cap_np, cap_items, cap_res, cap_contacts;  // cap_res: CCD soup items (result slots)
   int level;
@@ -651,6 +655,91 @@ __device__ int cull_class(const double* pa, const double* pb, const tri<double>&
   return 1;
 }
 
+// The same classification split in two: the per-triangle terms are computed
+// once per node pair (lanes 0-7: triangles of a, lanes 8-15: triangles of b)
+// and shared by shuffles with the 64 triangle-pair lanes.  Per triangle:
+// vertices (relative frame of cull_class), normal, kappa, longest squared
+// edge, own-normal extent, vertex sum, largest coordinate, and whether its
+// barycentric system is well conditioned.
+struct CullTri {
+  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok;
+};
+__device__ __forceinline__ void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
+  const v3<double> vv[3] = {t.a, t.b, t.c};
+  float crd = 0.f;
+  for (int v = 0; v < 3; ++v) {
+    const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
+    for (int r = 0; r < 3; ++r) {
+      o.v[3 * v + r] = ((rot[3 * r] * x + rot[3 * r + 1] * y) + rot[3 * r + 2] * z) + tr[r];
+      crd = fmaxf(crd, fabsf(o.v[3 * v + r]));
+    }
+  }
+  float e0[3], e1[3], e2[3];
+  for (int r = 0; r < 3; ++r)
+    e0[r] = o.v[3 + r] - o.v[r], e1[r] = o.v[6 + r] - o.v[r], e2[r] = o.v[6 + r] - o.v[3 + r];
+  o.n[0] = e0[1] * e1[2] - e0[2] * e1[1], o.n[1] = e0[2] * e1[0] - e0[0] * e1[2], o.n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+  const float d00 = f3dot(e0, e0), d11 = f3dot(e1, e1), d01 = f3dot(e0, e1);
+  const float den = d00 * d11 - d01 * d01;
+  o.ok = (den > 1e-4f * d00 * d11) ? 1.f : 0.f;
+  o.kap = o.ok != 0.f ? d00 * d11 / den : 0.f;
+  o.L2 = fmaxf(fmaxf(d00, d11), f3dot(e2, e2));
+  o.nn = f3dot(o.n, o.n);
+  const float p0 = f3dot(o.v, o.n), p1 = f3dot(o.v + 3, o.n), p2 = f3dot(o.v + 6, o.n);
+  o.pmin = fminf(p0, fminf(p1, p2)), o.pmax = fmaxf(p0, fmaxf(p1, p2));
+  for (int r = 0; r < 3; ++r) o.cen[r] = (o.v[r] + o.v[3 + r]) + o.v[6 + r];
+  o.crd = crd;
+}
+__device__ __forceinline__ void cull_fetch(const CullTri& mine, int src, CullTri& o) {
+  const float* m = reinterpret_cast<const float*>(&mine);
+  float* d = reinterpret_cast<float*>(&o);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(CullTri) / sizeof(float)); ++k) d[k] = __shfl(m[k], src);
+}
+// 0 certain miss, 1 needs the FP64 test, 2 certain hit (the bounds of cull_class).
+__device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, float thr) {
+  const float EPSF = 5.9604645e-08f;
+  if (A.ok == 0.f || B.ok == 0.f) return 1;
+  const float kappa = fmaxf(A.kap, B.kap);
+  const float L = sqrtf(fmaxf(A.L2, B.L2));
+  const float margin = 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * fmaxf(A.crd, B.crd) + EPSF * thr;
+  float lb = 0.f;
+  {  // axis nA
+    const float b0 = f3dot(B.v, A.n), b1 = f3dot(B.v + 3, A.n), b2 = f3dot(B.v + 6, A.n);
+    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - A.pmax, A.pmin - fmaxf(b0, fmaxf(b1, b2)));
+    if (A.nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(A.nn));
+  }
+  {  // axis nB
+    const float a0 = f3dot(A.v, B.n), a1 = f3dot(A.v + 3, B.n), a2 = f3dot(A.v + 6, B.n);
+    const float g = fmaxf(B.pmin - fmaxf(a0, fmaxf(a1, a2)), fminf(a0, fminf(a1, a2)) - B.pmax);
+    if (B.nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(B.nn));
+  }
+  {  // centroid direction
+    const float cd[3] = {B.cen[0] - A.cen[0], B.cen[1] - A.cen[1], B.cen[2] - A.cen[2]};
+    const float nn = f3dot(cd, cd);
+    const float a0 = f3dot(A.v, cd), a1 = f3dot(A.v + 3, cd), a2 = f3dot(A.v + 6, cd);
+    const float b0 = f3dot(B.v, cd), b1 = f3dot(B.v + 3, cd), b2 = f3dot(B.v + 6, cd);
+    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - fmaxf(a0, fmaxf(a1, a2)),
+                          fminf(a0, fminf(a1, a2)) - fmaxf(b0, fmaxf(b1, b2)));
+    if (nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(nn));
+  }
+  if (lb > thr + margin) return 0;
+  float ub2 = 3.4e38f;
+  float dd[3][3];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      const float t[3] = {A.v[3 * x] - B.v[3 * y], A.v[3 * x + 1] - B.v[3 * y + 1], A.v[3 * x + 2] - B.v[3 * y + 2]};
+      dd[x][y] = f3dot(t, t);
+    }
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      const int x1 = x == 2 ? 0 : x + 1, y1 = y == 2 ? 0 : y + 1;
+      ub2 = fminf(ub2, fmaxf(fmaxf(dd[x][y], dd[x][y1]), fmaxf(dd[x1][y], dd[x1][y1])));
+    }
+  const float lim = thr - margin;
+  if (lim > 0.f && ub2 < lim * lim) return 2;
+  return 1;
+}
+
 // One wavefront per node pair (chunked ranges): which triangle pairs need the
 // FP64 test.  FP64 mode keeps every valid pair; FP32 mode drops certain
 // misses, and for node pairs with an inner node (they emit no contacts, only
@@ -669,12 +758,27 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
     int i = lane >> 3, j = lane & 7;
     bool valid = i < na.ntris && j < nb.ntris;
     int cls = valid ? 1 : 0;
-    if (fp32 && valid) {
+    if (fp32) {  // wave-uniform
       const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
       const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
       const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-      double thr = (pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j];
-      cls = cull_class(pr + PR_TA1, pr + PR_TB1, load_tri<double>(ba, i), load_tri<double>(bb, j), thr);
+      CullTri mine{}, ta, tb;
+      const int side = (lane >> 3) & 1, k = lane & 7;
+      if (lane < 16 && k < (side ? nb.ntris : na.ntris)) {
+        const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
+        float rot[9], tr[3];
+        for (int r = 0; r < 3; ++r) {
+          for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
+          tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
+        }
+        cull_prep(rot, tr, load_tri<double>(side ? bb : ba, k), mine);
+      }
+      cull_fetch(mine, i, ta);
+      cull_fetch(mine, 8 + j, tb);
+      if (valid) {
+        const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
+        cls = cull_pair(ta, tb, thr);
+      }
     }
     bool inner = na.inner || nb.inner;
     unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
@@ -770,6 +874,199 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
 __device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
   return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
 }
+// ------------------------------------------- FP32 culls of CCD leaf sub-tests
+// A culled sub-test provably reports no distance below the search distance.
+//   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
+//     point moves on the segment p0 -> p1 in the triangle's frame and the
+//     reported distance is 0 (segment crosses the triangle), the plane
+//     distance at t = 0 or 1 (projection inside), or infinite.  Culled when
+//     p0 and p1 lie on one side of the plane farther than the search distance
+//     plus margin (no crossing; both plane distances large), or both lie
+//     outside one edge's half-plane by more than the margin plus the
+//     float-barycentric acceptance bound (no projection or crossing inside).
+//   edge-edge (edgeEdgeCCD, :565-673): the reported distance is |P - Q| for
+//     points of the two lerped segments at one t, so it is at least the
+//     separation of the two segments along an axis that separates them on the
+//     same side at t = 0 and at t = 1 (point offsets are linear in t).  Axes:
+//     the midpoint offsets and the edge-direction cross products at both ends.
+// margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products).
+struct M34f {
+  float m[12];
+};
+__device__ __forceinline__ M34f to_m34f(const double* m, const double* org) {
+  M34f o;
+  for (int r = 0; r < 3; ++r) {
+    o.m[4 * r] = (float)m[4 * r], o.m[4 * r + 1] = (float)m[4 * r + 1], o.m[4 * r + 2] = (float)m[4 * r + 2];
+    o.m[4 * r + 3] = (float)(m[4 * r + 3] - (org ? org[4 * r + 3] : 0.0));
+  }
+  return o;
+}
+__device__ __forceinline__ void xf(const M34f& M, const v3<double>& p, float* o) {
+  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
+  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
+  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
+  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
+}
+__device__ __forceinline__ float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
+// Separation of segment Q above segment P along u at one instant (negative
+// when none), scaled by 1/|u|.
+__device__ __forceinline__ float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0,
+                                         const float* q1) {
+  const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
+  return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
+}
+__device__ bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
+                               const float* bs1, const float* be0, const float* be1, float thr) {
+  const float EPSF = 5.9604645e-08f;
+  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
+                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
+  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
+  float ax[4][3];
+  for (int r = 0; r < 3; ++r) {
+    ax[0][r] = 0.5f * ((bs0[r] + bs1[r]) - (as0[r] + as1[r]));
+    ax[1][r] = 0.5f * ((be0[r] + be1[r]) - (ae0[r] + ae1[r]));
+  }
+  const float da[3] = {as1[0] - as0[0], as1[1] - as0[1], as1[2] - as0[2]};
+  const float db[3] = {bs1[0] - bs0[0], bs1[1] - bs0[1], bs1[2] - bs0[2]};
+  const float ea[3] = {ae1[0] - ae0[0], ae1[1] - ae0[1], ae1[2] - ae0[2]};
+  const float eb[3] = {be1[0] - be0[0], be1[1] - be0[1], be1[2] - be0[2]};
+  ax[2][0] = da[1] * db[2] - da[2] * db[1], ax[2][1] = da[2] * db[0] - da[0] * db[2], ax[2][2] = da[0] * db[1] - da[1] * db[0];
+  ax[3][0] = ea[1] * eb[2] - ea[2] * eb[1], ax[3][1] = ea[2] * eb[0] - ea[0] * eb[2], ax[3][2] = ea[0] * eb[1] - ea[1] * eb[0];
+  for (int k = 0; k < 4; ++k) {
+    const float nu = sqrtf(f3dot(ax[k], ax[k]));
+    if (!(nu > 1e-20f)) continue;
+    const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
+    const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
+    if (fmaxf(up, dn) > lim) return true;
+  }
+  return false;
+}
+// Per connected node pair, shared across its wavefront: lane v < 24 holds
+// vertex v of a's leaf (triangle v/3, corner v%3), lanes 24-47 those of b:
+// the vertex at both ends in world coordinates (relative to b's start
+// position) and in the other body's frame; lanes 48-55 / 56-63 hold
+// triangle k of a / b in its own frame: unit normal, plane offset, outward
+// unit edge normals and offsets.  A sub-test then reads what it needs by
+// shuffles.
+struct LeafShare {
+  float w0[3], w1[3], f0[3], f1[3];  // vertex lanes
+  float n[3], d, m[9], o[3];         // triangle lanes
+};
+struct LeafInfo {
+  int n, m;
+  float thr, eo_a, eo_b, margin;
+};
+__device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
+  const int lane = lane_id();
+  const int32_t* pi = D.pair_info + 4 * np.pair;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+  const NodeRec na = D.nodes[pi[2] + np.na];
+  const NodeRec nb = D.nodes[pi[3] + np.nb];
+  const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+  const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+  LeafInfo I;
+  I.n = na.ntris, I.m = nb.ntris;
+  I.thr = (float)pr[PR_SEARCH];
+  I.eo_a = D.tree_eps_out[D.body_tree[pi[0]]], I.eo_b = D.tree_eps_out[D.body_tree[pi[1]]];
+  float mag = 0.f;
+  for (int k = 0; k < 3; ++k) sh.w0[k] = sh.w1[k] = sh.f0[k] = sh.f1[k] = sh.n[k] = sh.o[k] = 0.f;
+  for (int k = 0; k < 9; ++k) sh.m[k] = 0.f;
+  sh.d = 0.f;
+  if (lane < 48) {
+    const bool of_b = lane >= 24;
+    const int v = of_b ? lane - 24 : lane;
+    if (v / 3 < (of_b ? I.m : I.n)) {
+      const v3<double> p = corner(load_tri<double>(of_b ? bb : ba, v / 3), v % 3);
+      const M34f W0 = to_m34f(of_b ? pr + PR_TB0 : pr + PR_TA0, pr + PR_TB0);
+      const M34f W1 = to_m34f(of_b ? pr + PR_TB1 : pr + PR_TA1, pr + PR_TB0);
+      const M34f F0 = to_m34f(of_b ? pr + PR_MBA0 : pr + PR_MAB0, nullptr);
+      const M34f F1 = to_m34f(of_b ? pr + PR_MBA1 : pr + PR_MAB1, nullptr);
+      xf(W0, p, sh.w0), xf(W1, p, sh.w1), xf(F0, p, sh.f0), xf(F1, p, sh.f1);
+      mag = fmaxf(fmaxf(amax3(sh.w0), amax3(sh.w1)), fmaxf(amax3(sh.f0), amax3(sh.f1)));
+    }
+  } else {
+    const bool of_b = lane >= 56;
+    const int k = lane & 7;
+    if (k < (of_b ? I.m : I.n)) {
+      const tri<double> T = load_tri<double>(of_b ? bb : ba, k);
+      const float va[3] = {(float)T.a.x, (float)T.a.y, (float)T.a.z}, vb[3] = {(float)T.b.x, (float)T.b.y, (float)T.b.z},
+                  vc[3] = {(float)T.c.x, (float)T.c.y, (float)T.c.z};
+      mag = fmaxf(amax3(va), fmaxf(amax3(vb), amax3(vc)));
+      const float e0[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
+      const float e1[3] = {vc[0] - va[0], vc[1] - va[1], vc[2] - va[2]};
+      float n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+      const float nl = sqrtf(f3dot(n, n));
+      if (nl > 1e-12f) {  // a degenerate triangle keeps n = 0: never culled
+        for (int r = 0; r < 3; ++r) sh.n[r] = n[r] / nl;
+        sh.d = f3dot(sh.n, va);
+        const float* V[3] = {va, vb, vc};
+        for (int q = 0; q < 3; ++q) {
+          const float* x = V[q];
+          const float* y = V[(q + 1) % 3];
+          const float* z = V[(q + 2) % 3];
+          const float ed[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
+          float mm[3] = {ed[1] * sh.n[2] - ed[2] * sh.n[1], ed[2] * sh.n[0] - ed[0] * sh.n[2],
+                         ed[0] * sh.n[1] - ed[1] * sh.n[0]};
+          const float zx[3] = {z[0] - x[0], z[1] - x[1], z[2] - x[2]};
+          if (f3dot(mm, zx) > 0.f) mm[0] = -mm[0], mm[1] = -mm[1], mm[2] = -mm[2];
+          const float ml = sqrtf(f3dot(mm, mm));
+          if (ml > 1e-12f) {
+            for (int r = 0; r < 3; ++r) sh.m[3 * q + r] = mm[r] / ml;
+            sh.o[q] = f3dot(sh.m + 3 * q, x);
+          } else {
+            sh.o[q] = 3.4e38f;  // never outside
+          }
+        }
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_xor(mag, off));
+  I.margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
+  return I;
+}
+__device__ __forceinline__ void sh3(const float* v, int src, float* o) {
+  o[0] = __shfl(v[0], src), o[1] = __shfl(v[1], src), o[2] = __shfl(v[2], src);
+}
+// Point-triangle sub-test s survives (the bounds above: plane side, edge half-plane).
+// Every lane of the wavefront calls this (the shuffles need all lanes).
+__device__ bool leaf_point_keep(const LeafShare& sh, const LeafInfo& I, int s, bool active) {
+  const int n = I.n, m = I.m;
+  int vl, tl;
+  float eo;
+  if (s < 3 * n * m) {
+    const int v = s / m, j = s % m;
+    vl = v, tl = 56 + j, eo = I.eo_b;  // vertex of a in b's frame, triangle of b
+  } else {
+    const int s2 = s - 3 * n * m, v = s2 / n, i = s2 % n;
+    vl = 24 + v, tl = 48 + i, eo = I.eo_a;
+  }
+  if (!active) vl = 0, tl = 48;
+  float p0[3], p1[3], nn[3], mm[9], oo[3];
+  sh3(sh.f0, vl, p0), sh3(sh.f1, vl, p1), sh3(sh.n, tl, nn);
+  const float d = __shfl(sh.d, tl);
+  for (int q = 0; q < 3; ++q) sh3(sh.m + 3 * q, tl, mm + 3 * q), oo[q] = __shfl(sh.o[q], tl);
+  if (!active) return false;
+  if (f3dot(nn, nn) == 0.f) return true;
+  const float h0 = f3dot(nn, p0) - d, h1 = f3dot(nn, p1) - d;
+  const float lim = I.thr + I.margin;
+  if ((h0 > lim && h1 > lim) || (h0 < -lim && h1 < -lim)) return false;
+  const float lat = I.margin + eo;
+  for (int q = 0; q < 3; ++q)
+    if (f3dot(mm + 3 * q, p0) - oo[q] > lat && f3dot(mm + 3 * q, p1) - oo[q] > lat) return false;
+  return true;
+}
+__device__ bool leaf_edge_keep(const LeafShare& sh, const LeafInfo& I, int s, bool active) {
+  const int ea = active ? s / (3 * I.m) : 0, eb = active ? s % (3 * I.m) : 0;
+  const int a0 = 3 * (ea / 3) + ea % 3, a1 = 3 * (ea / 3) + (ea % 3 + 1) % 3;
+  const int b0 = 24 + 3 * (eb / 3) + eb % 3, b1 = 24 + 3 * (eb / 3) + (eb % 3 + 1) % 3;
+  float as0[3], as1[3], ae0[3], ae1[3], bs0[3], bs1[3], be0[3], be1[3];
+  sh3(sh.w0, a0, as0), sh3(sh.w0, a1, as1), sh3(sh.w1, a0, ae0), sh3(sh.w1, a1, ae1);
+  sh3(sh.w0, b0, bs0), sh3(sh.w0, b1, bs1), sh3(sh.w1, b0, be0), sh3(sh.w1, b1, be1);
+  if (!active) return false;
+  return !cull_edge_edge(as0, as1, ae0, ae1, bs0, bs1, be0, be1, I.thr);
+}
+constexpr int CONN_MASK_WORDS = 16;  // 6 words of point sub-tests (<= 384), 9 of edge sub-tests (<= 576)
+
 __global__ void __launch_bounds__(256) k_level_count(Dev D) {
   {  // soup: children per destination list
     const long long n = D.n_soup();
@@ -809,14 +1106,34 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
       const long long n = D.n_conn();
       long long rb, re;
       chunk_of(n, rb, re);
-      long long ap = 0, ae = 0;
-      for (long long w = rb + threadIdx.x; w < re; w += blockDim.x) {
-        NP np = D.conn[w];
-        const int32_t* pi = D.pair_info + 4 * np.pair;
-        int na = D.nodes[pi[2] + np.na].ntris, nb = D.nodes[pi[3] + np.nb].ntris;
-        D.cnt_pt[w] = 6 * na * nb;
-        D.cnt_ee[w] = 9 * na * nb;
-        ap += 6 * na * nb, ae += 9 * na * nb;
+      long long ap = 0, ae = 0;  // lane 0 of each wavefront
+      const int lane = lane_id();
+      for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {  // one wavefront per node pair
+        const NP np = D.conn[w];
+        LeafShare sh;
+        const LeafInfo L = leaf_share(D, np, sh);
+        const int npt = 6 * L.n * L.m, nee = 9 * L.n * L.m;
+        int kp = 0, ke = 0;
+        unsigned long long* mk = D.conn_mask + (size_t)CONN_MASK_WORDS * w;
+        for (int b0 = 0, wd = 0; b0 < npt; b0 += 64, ++wd) {
+          const int sb = b0 + lane;
+          const bool keep = D.fp32 ? leaf_point_keep(sh, L, sb, sb < npt) : sb < npt;
+          const unsigned long long bits = __ballot(keep);
+          if (lane == 0) mk[wd] = bits;
+          kp += __popcll(bits);
+        }
+        for (int b0 = 0, wd = 6; b0 < nee; b0 += 64, ++wd) {
+          const int sb = b0 + lane;
+          const bool keep = D.fp32 ? leaf_edge_keep(sh, L, sb, sb < nee) : sb < nee;
+          const unsigned long long bits = __ballot(keep);
+          if (lane == 0) mk[wd] = bits;
+          ke += __popcll(bits);
+        }
+        if (lane == 0) {
+          D.cnt_pt[w] = kp;
+          D.cnt_ee[w] = ke;
+          ap += kp, ae += ke;
+        }
       }
       const long long tp = block_sum(ap), te = block_sum(ae);
       if (threadIdx.x == 0) D.part_pt[blockIdx.x] = (int)tp, D.part_ee[blockIdx.x] = (int)te;
@@ -887,12 +1204,28 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
       const int in_tile = (int)((re - t0) < (long long)blockDim.x ? (re - t0) : (long long)blockDim.x);
       for (int j = wid; j < in_tile; j += nw) {  // one wavefront per connected node pair
         const long long ww = t0 + j;
-        const int npt = D.cnt_pt[ww], nee = D.cnt_ee[ww];
-        const long long op = bp + toff_s[j], oe = be + toff_c[j];
-        for (int q = lane; q < npt; q += 64)
-          if (op + q < D.cap_items) D.pt_items[op + q] = Item{(uint32_t)ww, (uint32_t)q};
-        for (int q = lane; q < nee; q += 64)
-          if (oe + q < D.cap_items) D.ee_items[oe + q] = Item{(uint32_t)ww, (uint32_t)q};
+        const NP cnp = D.conn[ww];
+        const int32_t* cpi = D.pair_info + 4 * cnp.pair;
+        const int nn = D.nodes[cpi[2] + cnp.na].ntris, mm = D.nodes[cpi[3] + cnp.nb].ntris;
+        const int npt = 6 * nn * mm, nee = 9 * nn * mm;
+        const unsigned long long* mkw = D.conn_mask + (size_t)CONN_MASK_WORDS * ww;
+        long long op = bp + toff_s[j], oe = be + toff_c[j];
+        for (int b0 = 0, wd = 0; b0 < npt; b0 += 64, ++wd) {  // surviving sub-tests in sub-test order
+          const int q = b0 + lane;
+          const unsigned long long mk = mkw[wd];
+          const bool keep = (mk >> lane) & 1ull;
+          const long long slot = op + __popcll(mk & lanes_below());
+          if (keep && slot < D.cap_items) D.pt_items[slot] = Item{(uint32_t)ww, (uint32_t)q};
+          op += __popcll(mk);
+        }
+        for (int b0 = 0, wd = 6; b0 < nee; b0 += 64, ++wd) {
+          const int q = b0 + lane;
+          const unsigned long long mk = mkw[wd];
+          const bool keep = (mk >> lane) & 1ull;
+          const long long slot = oe + __popcll(mk & lanes_below());
+          if (keep && slot < D.cap_items) D.ee_items[slot] = Item{(uint32_t)ww, (uint32_t)q};
+          oe += __popcll(mk);
+        }
       }
       bp += ttp, be += tte;
       __syncthreads();
@@ -1735,7 +2068,7 @@ constexpr int MAX_SEGMENTS = 3;
 struct Sig {
   int n_pairs = -1, n_bodies = -1, n_clusters = -1, max_depth = -1, precision = -1, filter = -1;
   float dt = -1.f;
-  d2_tune tune{0, 0, 0};
+  d2_tune tune{0, 0, 0, 0};
   int levels[2] = {-1, -1};  // graph levels of each phase
   int learned = -1;
   float gate_t = 0.f, gate_s = 0.f;
@@ -1768,7 +2101,7 @@ struct d2_ctx {
   long long cap_res = 1ll << 20;
   bool auto_grow = true;  // grow the capacities and re-run on overflow
   int cus = 256;
-  d2_tune tune{1024, 2048, 4096};  // 4, 8, 16 per CU of a 256-CU device; reset at creation
+  d2_tune tune{1024, 2048, 4096, 3};  // 4, 8, 16 per CU of a 256-CU device; reset at creation
   // trees / bodies
   DBuf<NodeRec> nodes;
   DBuf<double> buckets;
@@ -1785,7 +2118,7 @@ struct d2_ctx {
   DBuf<double> res;
   DBuf<int> np_hit;
   DBuf<ContactRec> contacts;
-  DBuf<unsigned long long> cnt, lvl, masks;
+  DBuf<unsigned long long> cnt, lvl, masks, conn_mask;
   DBuf<int> cnt_a, cnt_s, cnt_c, cnt_pt, cnt_ee, parts;
   DBuf<unsigned long long> pass_min_s, pass_min_c;
   DBuf<int> pass_flag_s, pass_flag_c;
@@ -1802,7 +2135,8 @@ struct d2_ctx {
   // learned gate
   DBuf<float> tree_radius, tree_eps_out, hv, feat, pred;
   DBuf<int32_t> hv_off;
-  DBuf<int> cert;
+  DBuf<int> cert, cert_off;  // per pair: 1 = traverse (not certified); scanned offsets
+  int gate_phases = 3;       // bit 0: CCD phase, bit 1: discrete phase
   DBuf<float> mlp_w, mlp_m, mlp_v, mlp_wf, mlp_grad;
   DBuf<short> mlp_wb;
   DBuf<float> tr_x, tr_t, tr_act, tr_dlt, tr_loss;
@@ -1950,6 +2284,10 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.pass_min_s = c->pass_min_s.p, D.pass_min_c = c->pass_min_c.p;
   D.pass_flag_s = c->pass_flag_s.p, D.pass_flag_c = c->pass_flag_c.p;
   D.min_time = c->min_time.p;
+  D.tree_eps_out = c->tree_eps_out.p;
+  D.conn_mask = c->conn_mask.p;
+  D.body_tree = c->body_tree.p;
+  D.fp32 = c->precision == D2_FP32 ? 1 : 0;
   D.n_pairs = c->n_pairs;
   D.cap_np = c->cap;
   D.cap_items = c->cap_items;
@@ -1974,6 +2312,7 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
+  RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
   RC(ensure(c, c->contacts, c->cap_contacts)); RC(ensure(c, c->cnt, C_COUNT));
   RC(ensure(c, c->lvl, (size_t)2 * (MAX_LEVELS + 1) * LV_STRIDE));
   RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
@@ -1985,6 +2324,7 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->scan_part, g));
   if (c->learned_on) {
     RC(ensure(c, c->feat, np * MLP_IN)); RC(ensure(c, c->pred, np * MLP_OUT)); RC(ensure(c, c->cert, np));
+    RC(ensure(c, c->cert_off, np + 1));
   }
   return D2_OK;
 }
@@ -2140,9 +2480,18 @@ void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
   Dev D = make_dev(c, 0, ph.discrete);
   hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, D, c->pair_rec.p,
                      c->scale.p, ph.keep_ccd);
-  if (c->learned_on && c->mlp_ready && np > 0) {
+}
+
+// The learned gate of a phase: proposals (once per run), certificates, and
+// the phase's root list without the certified pairs.
+void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
+  const int np = c->n_pairs;
+  const int bit = ph.discrete ? 2 : 1;
+  if (!(c->learned_on && c->mlp_ready && np > 0 && (c->gate_phases & bit))) return;
+  {
     Timer tg(c, K_GATE, np);
-    if (!ph.keep_ccd) {  // proposals once per run (a step's discrete phase reuses them)
+    Dev D = make_dev(c, 0, ph.discrete);
+    if (!ph.keep_ccd || !(c->gate_phases & 1)) {  // a step's discrete phase reuses its CCD phase's proposals
       const double* mt = (ph.mt_from_bodies || !c->has_max_time) ? nullptr : c->max_time.p;
       hipLaunchKernelGGL(k_pair_features, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, c->bodies.p,
                          c->body_tree.p, c->tree_radius.p, mt, c->feat.p);
@@ -2150,11 +2499,13 @@ void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
       hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0, c->stream,
                          np, c->feat.p, c->mlp_wb.p, c->pred.p);
     }
-    hipLaunchKernelGGL(k_certify, dim3(grid_for(c, (long long)np * 64)), dim3(256), 0, c->stream, np, c->pair_info.p,
-                       c->body_tree.p, c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p, c->hv.p,
-                       c->tree_eps_out.p, ph.discrete ? 1 : 0, c->cert.p,
+    hipLaunchKernelGGL(k_certify, dim3(std::max(1, std::min(np, c->cus * 8))), dim3(256), 0, c->stream, np,
+                       c->pair_info.p, c->body_tree.p, c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p,
+                       c->hv.p, c->tree_eps_out.p, ph.discrete ? 1 : 0, c->cert.p,
                        c->cnt.p + (ph.discrete ? C_GATED_DISC : C_GATED_CCD));
-    hipLaunchKernelGGL(k_roots_compact, dim3(1), dim3(1024), 0, c->stream, np, c->cert.p, D.soup, c->lvl.p);
+    scan_host_n(c, c->cert.p, c->cert_off.p, np, c->lvl.p + LV_SOUP);  // level-0 size = pairs kept
+    hipLaunchKernelGGL(k_roots_scatter, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->cert.p, c->cert_off.p,
+                       D.soup);
   }
 }
 
@@ -2222,6 +2573,7 @@ int run_eager(d2_ctx* c, int kind, int filter, float dt) {
   const int n = phases_of(kind, filter, ph);
   for (int i = 0; i < n; ++i) {
     enqueue_begin(c, ph[i]);
+    enqueue_gate(c, ph[i]);
     HIPCHK(c, hipGetLastError());
     int used = 0;
     RC(eager_levels(c, ph[i], 0, c->n_pairs, 0, &used));
@@ -2273,14 +2625,18 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
   sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
   sig.tune = c->tune;
-  sig.learned = (c->learned_on && c->mlp_ready) ? 1 : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
+  sig.learned = (c->learned_on && c->mlp_ready) ? c->gate_phases : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
   std::vector<std::function<int()>> ops;
   int seg = 0;
   for (int i = 0; i < n; ++i) {
     const PhaseSpec p = ph[i];
     const int L = sig.levels[i];
-    ops.push_back([c, p] { enqueue_begin(c, p); return D2_OK; });
+    ops.push_back([c, p] {
+      enqueue_begin(c, p);
+      enqueue_gate(c, p);
+      return D2_OK;
+    });
     ops.push_back([c, p, L] { enqueue_levels(c, p, 0, L); return D2_OK; });
     int used = L;
     if (L < budget) {  // end of a segment: check the next level on the host
@@ -2415,7 +2771,7 @@ int d2_ctx_create(int device, int precision, d2_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cus = prop.multiProcessorCount;
-  c->tune = d2_tune{4 * c->cus, 8 * c->cus, 16 * c->cus};
+  c->tune = d2_tune{4 * c->cus, 8 * c->cus, 16 * c->cus, 3};
   if (hipHostMalloc((void**)&c->host_cnt, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->host_lvl, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE,
                     hipHostMallocDefault) != hipSuccess) {
@@ -2439,7 +2795,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
   release(c->items), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit);
-  release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks);
+  release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
   release(c->min_time), release(c->scale);
@@ -2449,7 +2805,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster);
   release(c->tree_radius), release(c->tree_eps_out), release(c->hv), release(c->feat), release(c->pred), release(c->hv_off);
-  release(c->cert), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
+  release(c->cert), release(c->cert_off), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
   release(c->mlp_wb), release(c->tr_x), release(c->tr_t), release(c->tr_act), release(c->tr_dlt), release(c->tr_loss);
   release(c->tr_idx);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
@@ -2489,14 +2845,16 @@ int d2_set_auto_grow(d2_ctx* c, int on) {
 int d2_set_tune(d2_ctx* c, const d2_tune* t) {
   if (!c || !t) return D2_ERR_ARG;
   if (t->chunk_blocks < 1 || t->chunk_blocks > 4096 || t->stream_blocks < 1 || t->stream_blocks > 16384 ||
-      t->post_blocks < 1 || t->post_blocks > 16384)
+      t->post_blocks < 1 || t->post_blocks > 16384 || t->gate_phases < 1 || t->gate_phases > 3)
     return fail(c, D2_ERR_ARG, "d2_set_tune: value out of range");
   c->tune = *t;
+  if (c->learned_on) c->gate_phases = t->gate_phases;
   return D2_OK;
 }
 int d2_get_tune(const d2_ctx* c, d2_tune* t) {
   if (!c || !t) return D2_ERR_ARG;
   *t = c->tune;
+  t->gate_phases = c->gate_phases;
   return D2_OK;
 }
 
@@ -2504,9 +2862,11 @@ int d2_get_tune(const d2_ctx* c, d2_tune* t) {
 // kind (graph or eager, as the context is set) on the resident inputs.
 int d2_autotune(d2_ctx* c, int kind, float dt, int filter, int reps, d2_tune* best_out, double* best_ms_out) {
   if (!c || kind < 0 || kind >= RUN_KINDS || reps < 1) return D2_ERR_ARG;
-  const d2_tune start = c->tune;
+  d2_tune start = c->tune;
+  start.gate_phases = c->gate_phases;
   auto time_it = [&](const d2_tune& t, double* ms) -> int {
     c->tune = t;
+    if (c->learned_on) c->gate_phases = t.gate_phases;
     RC(run(c, kind, filter, dt, nullptr));  // warm-up (and capture)
     std::vector<double> v;
     for (int r = 0; r < reps; ++r) {
@@ -2525,12 +2885,14 @@ int d2_autotune(d2_ctx* c, int kind, float dt, int filter, int reps, d2_tune* be
   d2_tune best = start;
   double best_ms = 0.0;
   int rc = time_it(best, &best_ms);
+  const std::vector<int> gate = {1, 2, 3};
+  const int dims = c->learned_on ? 4 : 3;
   for (int sweep = 0; sweep < 2 && !rc; ++sweep)
-    for (int dim = 0; dim < 3 && !rc; ++dim) {
-      const std::vector<int>& vals = dim == 0 ? chunk : dim == 1 ? stream : post;
+    for (int dim = 0; dim < dims && !rc; ++dim) {
+      const std::vector<int>& vals = dim == 0 ? chunk : dim == 1 ? stream : dim == 2 ? post : gate;
       for (int v : vals) {
         d2_tune t = best;
-        int* f = dim == 0 ? &t.chunk_blocks : dim == 1 ? &t.stream_blocks : &t.post_blocks;
+        int* f = dim == 0 ? &t.chunk_blocks : dim == 1 ? &t.stream_blocks : dim == 2 ? &t.post_blocks : &t.gate_phases;
         if (*f == v || v > (dim == 0 ? 4096 : 16384)) continue;
         *f = v;
         double ms = 0.0;
@@ -2539,6 +2901,7 @@ int d2_autotune(d2_ctx* c, int kind, float dt, int filter, int reps, d2_tune* be
       }
     }
   c->tune = rc ? start : best;
+  if (c->learned_on) c->gate_phases = c->tune.gate_phases;
   if (rc) return rc;
   if (best_out) *best_out = best;
   if (best_ms_out) *best_ms_out = best_ms;
@@ -2922,9 +3285,10 @@ int d2_pair_features(d2_ctx* c, float* X) {
 }
 
 int d2_set_learned(d2_ctx* c, int on, float t_gate, float s_gate) {
-  if (!c) return D2_ERR_ARG;
+  if (!c || on < 0 || on > 3) return D2_ERR_ARG;
   if (on && !c->mlp_ready) return fail(c, D2_ERR_STATE, "set network parameters before turning the gate on");
   c->learned_on = on != 0;
+  c->gate_phases = on;
   c->gate_t = t_gate;
   c->gate_s = s_gate;
   return D2_OK;

@@ -282,150 +282,228 @@ __global__ void k_mlp_round(const float* w, short* wb, float* wf) {
 }
 
 // ------------------------------------------------------------- certificate
-// Lower bound of the distance between two vertex sets in one frame: the gap
-// of their projections on an axis (any axis bounds the distance from below);
-// the axis starts at the centre difference and is refined three times to the
-// difference of the support points.  Vertices are given in body coordinates
-// and mapped by 3x4 FP32 matrices (one or two poses per set: the sets of a
-// swept set are the union of both poses, which holds every linear blend).
-struct CertSet {
-  const float* v;  // xyz per vertex
-  int n;
-  float m[2][12];
-  int poses;
-};
-__device__ __forceinline__ void cert_xform(const float* m, const float* v, float* o) {
-  o[0] = (m[0] * v[0] + m[1] * v[1]) + (m[2] * v[2] + m[3]);
-  o[1] = (m[4] * v[0] + m[5] * v[1]) + (m[6] * v[2] + m[7]);
-  o[2] = (m[8] * v[0] + m[9] * v[1]) + (m[10] * v[2] + m[11]);
-}
-// support of set S along u (sign = +1: max, -1: min); returns the extreme
-// projection and the point; also the largest coordinate magnitude seen.
-__device__ float cert_support(const CertSet& S, const float* u, float sign, float* pt, float& mag) {
-  const int lane = lane_id();
-  float best = -3.4e38f, bp[3] = {0.f, 0.f, 0.f};
-  for (int k = lane; k < S.n * S.poses; k += 64) {
-    float o[3];
-    cert_xform(S.m[k / S.n], S.v + 3 * (k % S.n), o);
-    const float pr = sign * ((u[0] * o[0] + u[1] * o[1]) + u[2] * o[2]);
-    mag = fmaxf(mag, fmaxf(fabsf(o[0]), fmaxf(fabsf(o[1]), fabsf(o[2]))));
-    if (pr > best) best = pr, bp[0] = o[0], bp[1] = o[1], bp[2] = o[2];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    const float ob = __shfl_xor(best, off);
-    const float o0 = __shfl_xor(bp[0], off), o1 = __shfl_xor(bp[1], off), o2 = __shfl_xor(bp[2], off);
-    if (ob > best) best = ob, bp[0] = o0, bp[1] = o1, bp[2] = o2;
-  }
-  for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_xor(mag, off));
-  pt[0] = bp[0], pt[1] = bp[1], pt[2] = bp[2];
-  return sign * best;
-}
-// Best separation gap of P below Q found (negative when none), and the
-// magnitude of the coordinates involved.
-__device__ float cert_gap(const CertSet& P, const CertSet& Q, const float* u0, float& mag) {
-  float u[3] = {u0[0], u0[1], u0[2]};
-  float best = -3.4e38f;
-  for (int it = 0; it < 4; ++it) {
-    const float nu = sqrtf((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
-    if (!(nu > 1e-20f)) break;
-    float pp[3], qq[3];
-    const float hp = cert_support(P, u, 1.f, pp, mag);
-    const float lq = cert_support(Q, u, -1.f, qq, mag);
-    best = fmaxf(best, (lq - hp) / nu);
-    u[0] = qq[0] - pp[0], u[1] = qq[1] - pp[1], u[2] = qq[2] - pp[2];
-  }
-  return best;
-}
+// Lower bounds of the distances the reference's sub-tests can report, from
+// separating axes.  Along a unit axis u, set P lies below set Q by
+// min_Q u.q - max_P u.p, a lower bound of their distance.  Three bounds:
+//   world, per time: the CCD's edge-edge sub-tests lerp world positions, so a
+//     point pair's offset at time t is the lerp of its offsets at the ends and
+//     min(gap at start, gap at end) along one axis bounds every t;
+//   b's frame: a point of a lerps between its start and end positions in b's
+//     frame (point-triangle sub-tests) while b is fixed: the gap between the
+//     union of a's two poses and b;
+//   a's frame: the same with the bodies swapped.
+// The discrete phase needs one bound: the two bodies at the future poses.
+// One block per pair; a set's vertices are transformed into LDS once; axes
+// come from Gilbert's minimum-norm-point iteration on the difference set
+// (every iterate's axis gives a sound bound, the best one is kept).
+constexpr int CERT_MAXV = 1024;  // vertices per body held in LDS; larger bodies are not certified
 __device__ __forceinline__ void cert_mat(const double* m, const double* origin, float* o) {
   for (int r = 0; r < 3; ++r) {
     o[4 * r] = (float)m[4 * r], o[4 * r + 1] = (float)m[4 * r + 1], o[4 * r + 2] = (float)m[4 * r + 2];
     o[4 * r + 3] = (float)(m[4 * r + 3] - (origin ? origin[r] : 0.0));
   }
 }
+// LDS copy of n body-frame vertices under the 3x4 matrix m (identity when null).
+__device__ void cert_load(const float* v, int n, const float* m, float* out, float& mag) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    const float x = v[3 * k], y = v[3 * k + 1], z = v[3 * k + 2];
+    float o0 = x, o1 = y, o2 = z;
+    if (m) {
+      o0 = (m[0] * x + m[1] * y) + (m[2] * z + m[3]);
+      o1 = (m[4] * x + m[5] * y) + (m[6] * z + m[7]);
+      o2 = (m[8] * x + m[9] * y) + (m[10] * z + m[11]);
+    }
+    out[3 * k] = o0, out[3 * k + 1] = o1, out[3 * k + 2] = o2;
+    mag = fmaxf(mag, fmaxf(fabsf(o0), fmaxf(fabsf(o1), fabsf(o2))));
+  }
+}
+// Block-wide extreme of u.x over sets S[0..ns) (sign +1: max, -1: min) and
+// the point attaining it.
+__device__ float cert_extreme(const float* const* S, const int* n, int ns, const float* u, float sign, float* pt,
+                              float* red) {
+  float best = -3.4e38f, bp[3] = {0.f, 0.f, 0.f};
+  for (int s = 0; s < ns; ++s)
+    for (int k = threadIdx.x; k < n[s]; k += blockDim.x) {
+      const float* q = S[s] + 3 * k;
+      const float pr = sign * ((u[0] * q[0] + u[1] * q[1]) + u[2] * q[2]);
+      if (pr > best) best = pr, bp[0] = q[0], bp[1] = q[1], bp[2] = q[2];
+    }
+  for (int off = 32; off > 0; off >>= 1) {
+    const float ob = __shfl_xor(best, off);
+    const float o0 = __shfl_xor(bp[0], off), o1 = __shfl_xor(bp[1], off), o2 = __shfl_xor(bp[2], off);
+    if (ob > best) best = ob, bp[0] = o0, bp[1] = o1, bp[2] = o2;
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) {
+    float* r = red + 4 * (threadIdx.x >> 6);
+    r[0] = best, r[1] = bp[0], r[2] = bp[1], r[3] = bp[2];
+  }
+  __syncthreads();
+  int w = 0;
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i)
+    if (red[4 * i] > red[4 * w]) w = i;
+  pt[0] = red[4 * w + 1], pt[1] = red[4 * w + 2], pt[2] = red[4 * w + 3];
+  return sign * red[4 * w];
+}
+// Separation bound of P below Q along u: (min_Q u.q - max_P u.p) / |u|, and
+// the support points.
+__device__ float cert_axis(const float* const* P, const int* nP, int nsP, const float* const* Q, const int* nQ, int nsQ,
+                           const float* u, float* pp, float* qq, float* red) {
+  const float nu = sqrtf((u[0] * u[0] + u[1] * u[1]) + u[2] * u[2]);
+  const float hp = cert_extreme(P, nP, nsP, u, 1.f, pp, red);
+  const float lq = cert_extreme(Q, nQ, nsQ, u, -1.f, qq, red);
+  return nu > 1e-30f ? (lq - hp) / nu : -3.4e38f;
+}
+// Gilbert's minimum-norm-point iteration on the difference set Q - P from the
+// point x0 (a member of its hull): the support point s = argmin (x.d) over
+// Q - P bounds the distance by x.s / |x| at every step (kept as the best
+// bound), and x moves to the point of segment [x, s] closest to the origin.
+// Returns the best bound; u_out = its axis.
+__device__ float cert_gilbert(const float* const* P, const int* nP, int nsP, const float* const* Q, const int* nQ,
+                              int nsQ, const float* x0, int iters, float* u_out, float* red) {
+  float x[3] = {x0[0], x0[1], x0[2]};
+  float best = -3.4e38f;
+  u_out[0] = x[0], u_out[1] = x[1], u_out[2] = x[2];
+  for (int it = 0; it < iters; ++it) {
+    const float xx = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+    if (!(xx > 1e-30f)) break;
+    float pp[3], qq[3];
+    const float g = cert_axis(P, nP, nsP, Q, nQ, nsQ, x, pp, qq, red);
+    if (g > best) best = g, u_out[0] = x[0], u_out[1] = x[1], u_out[2] = x[2];
+    const float sd[3] = {qq[0] - pp[0], qq[1] - pp[1], qq[2] - pp[2]};
+    const float xs = (x[0] * sd[0] + x[1] * sd[1]) + x[2] * sd[2];
+    if (xx - xs <= 1e-7f * xx) break;  // x is (nearly) the minimum-norm point
+    const float e[3] = {x[0] - sd[0], x[1] - sd[1], x[2] - sd[2]};
+    const float ee = (e[0] * e[0] + e[1] * e[1]) + e[2] * e[2];
+    float lam = ee > 0.f ? (xx - xs) / ee : 0.f;  // x + lam (s - x) is closest to the origin
+    lam = fminf(fmaxf(lam, 0.f), 1.f);
+    x[0] += lam * (sd[0] - x[0]), x[1] += lam * (sd[1] - x[1]), x[2] += lam * (sd[2] - x[2]);
+  }
+  return best;
+}
+// Per-time world bound: min over the two instants of the separation along one
+// axis, the axis taken from each instant's own Gilbert iteration.
+__device__ float cert_gap_timed(const float* P0, const float* P1, int nP, const float* Q0, const float* Q1, int nQ,
+                                const float* x0, const float* x1, int iters, float* red) {
+  const float* Ps[2] = {P0, P1};
+  const float* Qs[2] = {Q0, Q1};
+  float ax[3][3];
+  for (int k = 0; k < 2; ++k) (void)cert_gilbert(Ps + k, &nP, 1, Qs + k, &nQ, 1, k ? x1 : x0, iters, ax[k], red);
+  const float n0 = sqrtf((ax[0][0] * ax[0][0] + ax[0][1] * ax[0][1]) + ax[0][2] * ax[0][2]);
+  const float n1 = sqrtf((ax[1][0] * ax[1][0] + ax[1][1] * ax[1][1]) + ax[1][2] * ax[1][2]);
+  for (int r = 0; r < 3; ++r) ax[2][r] = (n0 > 0.f ? ax[0][r] / n0 : 0.f) + (n1 > 0.f ? ax[1][r] / n1 : 0.f);
+  float best = -3.4e38f;
+  for (int c = 0; c < 3; ++c) {
+    float pp[3], qq[3];
+    const float g0 = cert_axis(Ps, &nP, 1, Qs, &nQ, 1, ax[c], pp, qq, red);
+    const float g1 = cert_axis(Ps + 1, &nP, 1, Qs + 1, &nQ, 1, ax[c], pp, qq, red);
+    best = fmaxf(best, fminf(g0, g1));
+  }
+  return best;
+}
+constexpr int CERT_ITERS = 16;
 
-// One wavefront per pair.  mode 0: the CCD interval (a swept against b in
-// world coordinates, a's vertices in b's frame at both ends against b, b's in
-// a's frame against a); mode 1: the poses of the discrete phase.  gate: the
-// MLP outputs (y0 >= s_gate and y1 >= t_gate), or every pair when null.
+// gate: the MLP outputs (y0 >= s_gate and y1 >= t_gate), or every pair when
+// null.  keep[p] = 0 for a certified pair, 1 otherwise.  counters[0..1]:
+// proposed / certified pairs.
 __global__ void __launch_bounds__(256) k_certify(int n_pairs, const int32_t* pair_info, const int32_t* body_tree,
                                                  const double* pair_rec, const float* Y, float t_gate, float s_gate,
                                                  const int32_t* hv_off, const float* hv, const float* tree_eps_out,
-                                                 int mode, int* cert, unsigned long long* counters) {
-  const int lane = lane_id();
-  const int wstride = gridDim.x * (blockDim.x >> 6);
-  for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); p < n_pairs; p += wstride) {
+                                                 int mode, int* keep, unsigned long long* counters) {
+  __shared__ float buf[4][3 * CERT_MAXV];
+  __shared__ float red[4 * 16];
+  __shared__ float mag_red[16];
+  const float EPSF = 5.9604645e-08f;
+  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const bool gated = !Y || (Y[2 * p] >= s_gate && Y[2 * p + 1] >= t_gate);
-    if (!gated) {
-      if (lane == 0) cert[p] = 0;
-      continue;
-    }
     const int a = pair_info[4 * p], b = pair_info[4 * p + 1];
     const int ta = body_tree[a], tb = body_tree[b];
+    const int na = hv_off[ta + 1] - hv_off[ta], nb = hv_off[tb + 1] - hv_off[tb];
+    if (!gated || na > CERT_MAXV || nb > CERT_MAXV) {
+      if (threadIdx.x == 0) {
+        keep[p] = 1;
+        if (gated) atomicAdd(&counters[0], 1ull);
+      }
+      continue;
+    }
     const double* pr = pair_rec + (size_t)PR_STRIDE * p;
     const float thr = (float)pr[PR_SEARCH] + tree_eps_out[ta] + tree_eps_out[tb];
-    CertSet A, B;
-    A.v = hv + 3 * (size_t)hv_off[ta], A.n = hv_off[ta + 1] - hv_off[ta];
-    B.v = hv + 3 * (size_t)hv_off[tb], B.n = hv_off[tb + 1] - hv_off[tb];
-    bool ok = true;
+    const float* va = hv + 3 * (size_t)hv_off[ta];
+    const float* vb = hv + 3 * (size_t)hv_off[tb];
+    float m0[12], m1[12], m2[12], m3[12], u[3];
     float mag = 0.f;
-    const double* org = pr + PR_TB0;  // world frame relative to b's start position
-    const double o3[3] = {org[3], org[7], org[11]};
-    float u[3];
+    bool ok = true;
+    auto finish_mag = [&]() {  // block maximum of the coordinate magnitudes
+      float m = mag;
+      for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) mag_red[threadIdx.x >> 6] = m;
+      __syncthreads();
+      float r = 0.f;
+      for (int i = 0; i < (int)(blockDim.x >> 6); ++i) r = fmaxf(r, mag_red[i]);
+      return r;
+    };
     if (mode == 0) {
-      // world, both bodies swept
-      cert_mat(pr + PR_TA0, o3, A.m[0]), cert_mat(pr + PR_TA1, o3, A.m[1]), A.poses = 2;
-      cert_mat(pr + PR_TB0, o3, B.m[0]), cert_mat(pr + PR_TB1, o3, B.m[1]), B.poses = 2;
-      for (int k = 0; k < 3; ++k)
-        u[k] = 0.5f * ((B.m[0][4 * k + 3] + B.m[1][4 * k + 3]) - (A.m[0][4 * k + 3] + A.m[1][4 * k + 3]));
-      float g = cert_gap(A, B, u, mag);
-      ok = g - (1e-6f + 64.f * 5.9604645e-08f * mag) > thr;
-      if (ok) {  // a's vertices in b's frame at both ends against b
-        cert_mat(pr + PR_MAB0, nullptr, A.m[0]), cert_mat(pr + PR_MAB1, nullptr, A.m[1]), A.poses = 2;
-        const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        cert_mat(I, nullptr, B.m[0]), B.poses = 1;
-        for (int k = 0; k < 3; ++k) u[k] = -0.5f * (A.m[0][4 * k + 3] + A.m[1][4 * k + 3]);
-        mag = 0.f;
-        g = cert_gap(A, B, u, mag);
-        ok = g - (1e-6f + 64.f * 5.9604645e-08f * mag) > thr;
+      const double o[3] = {pr[PR_TB0 + 3], pr[PR_TB0 + 7], pr[PR_TB0 + 11]};
+      cert_mat(pr + PR_TA0, o, m0), cert_mat(pr + PR_TA1, o, m1), cert_mat(pr + PR_TB0, o, m2), cert_mat(pr + PR_TB1, o, m3);
+      cert_load(va, na, m0, buf[0], mag), cert_load(va, na, m1, buf[1], mag);
+      cert_load(vb, nb, m2, buf[2], mag), cert_load(vb, nb, m3, buf[3], mag);
+      const float M = finish_mag();
+      {  // world, per instant: (a at start, b at start), (a at end, b at end)
+        float x0[3], x1[3];
+        for (int k = 0; k < 3; ++k) x0[k] = m2[4 * k + 3] - m0[4 * k + 3], x1[k] = m3[4 * k + 3] - m1[4 * k + 3];
+        ok = cert_gap_timed(buf[0], buf[1], na, buf[2], buf[3], nb, x0, x1, CERT_ITERS, red) -
+                 (1e-6f + 64.f * EPSF * M) > thr;
       }
-      if (ok) {  // b's vertices in a's frame at both ends against a
-        CertSet A2, B2;
-        A2.v = B.v, A2.n = B.n, B2.v = A.v, B2.n = A.n;
-        cert_mat(pr + PR_MBA0, nullptr, A2.m[0]), cert_mat(pr + PR_MBA1, nullptr, A2.m[1]), A2.poses = 2;
-        const double I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-        cert_mat(I, nullptr, B2.m[0]), B2.poses = 1;
-        for (int k = 0; k < 3; ++k) u[k] = -0.5f * (A2.m[0][4 * k + 3] + A2.m[1][4 * k + 3]);
+      if (ok) {  // a's two poses in b's frame against b
+        __syncthreads();
         mag = 0.f;
-        g = cert_gap(A2, B2, u, mag);
-        ok = g - (1e-6f + 64.f * 5.9604645e-08f * mag) > thr;
+        cert_mat(pr + PR_MAB0, nullptr, m0), cert_mat(pr + PR_MAB1, nullptr, m1);
+        cert_load(va, na, m0, buf[0], mag), cert_load(va, na, m1, buf[1], mag), cert_load(vb, nb, nullptr, buf[2], mag);
+        const float M2 = finish_mag();
+        for (int k = 0; k < 3; ++k) u[k] = -0.5f * (m0[4 * k + 3] + m1[4 * k + 3]);
+        const float* P[2] = {buf[0], buf[1]};
+        const float* Q[1] = {buf[2]};
+        const int nP[2] = {na, na}, nQ[1] = {nb};
+        float ax[3];
+        ok = cert_gilbert(P, nP, 2, Q, nQ, 1, u, CERT_ITERS, ax, red) - (1e-6f + 64.f * EPSF * M2) > thr;
+      }
+      if (ok) {  // b's two poses in a's frame against a
+        __syncthreads();
+        mag = 0.f;
+        cert_mat(pr + PR_MBA0, nullptr, m0), cert_mat(pr + PR_MBA1, nullptr, m1);
+        cert_load(vb, nb, m0, buf[0], mag), cert_load(vb, nb, m1, buf[1], mag), cert_load(va, na, nullptr, buf[2], mag);
+        const float M3 = finish_mag();
+        for (int k = 0; k < 3; ++k) u[k] = -0.5f * (m0[4 * k + 3] + m1[4 * k + 3]);
+        const float* P[2] = {buf[0], buf[1]};
+        const float* Q[1] = {buf[2]};
+        const int nP[2] = {nb, nb}, nQ[1] = {na};
+        float ax[3];
+        ok = cert_gilbert(P, nP, 2, Q, nQ, 1, u, CERT_ITERS, ax, red) - (1e-6f + 64.f * EPSF * M3) > thr;
       }
     } else {
-      const double* ob = pr + PR_TB1;
-      const double o1[3] = {ob[3], ob[7], ob[11]};
-      cert_mat(pr + PR_TA1, o1, A.m[0]), A.poses = 1;
-      cert_mat(pr + PR_TB1, o1, B.m[0]), B.poses = 1;
-      for (int k = 0; k < 3; ++k) u[k] = B.m[0][4 * k + 3] - A.m[0][4 * k + 3];
-      const float g = cert_gap(A, B, u, mag);
-      ok = g - (1e-6f + 64.f * 5.9604645e-08f * mag) > thr;
+      const double o[3] = {pr[PR_TB1 + 3], pr[PR_TB1 + 7], pr[PR_TB1 + 11]};
+      cert_mat(pr + PR_TA1, o, m0), cert_mat(pr + PR_TB1, o, m1);
+      cert_load(va, na, m0, buf[0], mag), cert_load(vb, nb, m1, buf[1], mag);
+      const float M = finish_mag();
+      for (int k = 0; k < 3; ++k) u[k] = m1[4 * k + 3] - m0[4 * k + 3];
+      const float* P[1] = {buf[0]};
+      const float* Q[1] = {buf[1]};
+      const int nP[1] = {na}, nQ[1] = {nb};
+      float ax[3];
+      ok = cert_gilbert(P, nP, 1, Q, nQ, 1, u, CERT_ITERS, ax, red) - (1e-6f + 64.f * EPSF * M) > thr;
     }
-    if (lane == 0) {
-      cert[p] = ok ? 1 : 0;
+    if (threadIdx.x == 0) {
+      keep[p] = ok ? 0 : 1;
       atomicAdd(&counters[0], 1ull);
       if (ok) atomicAdd(&counters[1], 1ull);
     }
+    __syncthreads();  // LDS reuse by the next pair
   }
 }
 
-// The phase's root list without the certified pairs (one block, pair order).
-__global__ void __launch_bounds__(1024) k_roots_compact(int n_pairs, const int* cert, NP* soup, unsigned long long* lvl) {
-  long long carry = 0;
-  for (int t0 = 0; t0 < n_pairs; t0 += blockDim.x) {
-    const int p = t0 + threadIdx.x;
-    const int keep = (p < n_pairs && !cert[p]) ? 1 : 0;
-    int tot;
-    const int ex = block_scan(keep, tot);
-    if (keep) soup[carry + ex] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
-    carry += tot;
-  }
-  if (threadIdx.x == 0) lvl[LV_SOUP] = (unsigned long long)carry;
+// The phase's root list: the kept pairs at their scanned offsets (pair order).
+__global__ void k_roots_scatter(int n_pairs, const int* keep, const int* off, NP* soup) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x)
+    if (keep[p]) soup[off[p]] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
 }

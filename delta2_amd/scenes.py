@@ -383,6 +383,32 @@ def scene_c3(seed=1003, n_scenes=256, lo=16, hi=512, threads=None):
     return s
 
 
+def scene_c4(seed=1004, n_bodies=4096, threads=None):
+    """Held-out evaluation scene of the learned gate: n_bodies C1 meshes
+    (subdivision-2 icospheres scaled U(0.5,1.5)) at the C1 density (box side
+    10 (n/100)^(1/3) m), random rotations, velocities N(0,1); trees from
+    per-body PCG32 streams (42, 30000 + body)."""
+    rng = np.random.default_rng(seed)
+    V0, F0 = icosphere(2, 1.0)
+    _, _, I0 = mass_properties(V0, F0)
+    r0 = float(np.max(np.linalg.norm(V0, axis=1)))
+    side = 10.0 * (n_bodies / 100.0) ** (1.0 / 3.0)
+    meshes, recs, radii = [], [], []
+    for _ in range(n_bodies):
+        sc = rng.uniform(0.5, 1.5)
+        meshes.append((V0 * sc, F0))
+        recs.append(body_record(random_rotation(rng), rng.uniform(0.0, side, size=3), rng.normal(0.0, 1.0, size=3),
+                                np.linalg.inv(I0 * sc ** 5)))
+        radii.append(r0 * sc)
+    trees = build_trees_parallel(meshes, 30000, threads)
+    rec = np.array(recs)
+    radii = np.array(radii)
+    s = Scene("c4", meshes, trees, rec, radii)
+    s.box = side
+    s.pairs = broad_phase(rec, radii)
+    return s
+
+
 def advance(records, dt=DT):
     """Ballistic drift of the current state by dt (no contact response: that is
     the solver's job, outside this path)."""

@@ -86,15 +86,17 @@ int d2_set_auto_grow(d2_ctx* ctx, int on);
  * into that many contiguous ranges); stream_blocks: blocks of the
  * grid-stride kernels (1-16384); post_blocks: one-wavefront blocks of the
  * per-pair post-processing (1-16384).  Defaults: 4, 8 and 16 per CU.
- * Results do not depend on them; delta2_amd.autotune picks them by timing. */
+ * Results do not depend on them; d2_autotune picks them by timing. */
 typedef struct d2_tune {
   int chunk_blocks;
   int stream_blocks;
   int post_blocks;
+  int gate_phases; /* phase mask of the learned gate when it is on (1 CCD, 2 discrete, 3 both) */
 } d2_tune;
 int d2_set_tune(d2_ctx* ctx, const d2_tune* tune);
 int d2_get_tune(const d2_ctx* ctx, d2_tune* tune);
-/* Autotuner: coordinate descent over the three block counts, timing whole
+/* Autotuner: coordinate descent over the three block counts (and, with the
+ * learned gate on, over its phase mask), timing whole
  * runs on the resident inputs (kind 0: d2_narrow_run(filter), 1: d2_ccd_run,
  * 2: d2_step_run(dt, filter); graph or eager as set), median of reps runs per
  * candidate.  Leaves the best shape set and returns it with its time (ms). */
@@ -217,7 +219,8 @@ int d2_mlp_train(d2_ctx* ctx, const float* X, const float* T, int n, int epochs,
 /* The 24 features of every resident pair (relative pose and motion in body
  * a's frame, lengths in units of r_a + r_b, times in units of the interval). */
 int d2_pair_features(d2_ctx* ctx, float* X);
-/* Turns the gate on or off for later runs (needs parameters). */
+/* The gate for later runs (needs parameters): on = 0 off, or a mask of the
+ * phases it applies to (1: CCD, 2: discrete, 3: both). */
 int d2_set_learned(d2_ctx* ctx, int on, float t_gate, float s_gate);
 
 /* Results of the last run: CSR offsets (n_pairs+1), contact records, and

@@ -239,3 +239,27 @@ def test_narrow_parity_fp32_cull(ctx32, name, make, filt):
     # node pairs seen by the mask kernel x up to 64 triangle pairs: the FP64
     # work after the cull must be a small part of the full cross product
     assert tested < 0.5 * 64 * st["tri_enum"][2], (tested, st["tri_enum"])
+
+
+@pytest.mark.parametrize("name,make", SCENES)
+def test_ccd_parity_fp32_culls(ctx32, name, make):
+    """FP32 culls of the CCD leaf sub-tests keep the FP64 result exactly and
+    drop most sub-tests (a 4-step interval reaches many leaf pairs)."""
+    s = make()
+    rec = s.records
+    load(ctx32, s, rec)
+    mt = np.full(s.pairs.shape[0], 4 * scenes.DT)
+    ctx32.reset_stats()
+    ctx32.set_profiling(True)
+    gpu = d2.compare_trees_full_continuous(ctx32, s.pairs, mt)
+    ctx32.set_profiling(False)
+    o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
+    assert_same(gpu, o_off, o_ints, o_reals, f"ccd fp32 {name}")
+    assert close(gpu.per_pair, o_pp).all()
+    st = ctx32.kernel_stats()
+    leaf_pairs = int(sum(lv[:, 1].sum() for lv in [ctx32.run_info()[1][1]]))
+    kept = st["leaf_point"][2] + st["leaf_edge"][2]
+    print(f"[ccd32] {name}: leaf node pairs {leaf_pairs}, sub-tests kept {kept:.0f}", flush=True)
+    if name == "dense":
+        full = 15 * 64 * leaf_pairs  # at most 6nm + 9nm sub-tests per leaf pair (n, m <= 8)
+        assert kept < 0.5 * full
