@@ -28,14 +28,26 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "narrow+CCD pair queries/s, 1 GPU; learned vs plain speedup at equal TOI error"
 
 # Algorithmic HBM bytes per unit of work of each kernel (unit = the item the
-# kernel's launch counts): the geometry a unit must read plus what it writes.
-#   soup_edge / leaf_edge : 4 edge endpoints (96 B) + one 8-double result (64 B)
-#   soup_point/leaf_point : a point (24 B) + a triangle (72 B) + a result (64 B)
-#   tri_dist              : two triangles (144 B) + their two eps (16 B)
-#   soup_cull / tri_enum  : two 8-slot buckets (2 x 576 B) + two node records (64 B)
-#   soup_reduce           : 15 results (960 B) read
-ALG_BYTES = {"soup_edge": 160.0, "leaf_edge": 160.0, "soup_point": 160.0, "leaf_point": 160.0,
-             "tri_dist": 160.0, "soup_cull": 1216.0, "tri_enum": 1216.0, "soup_reduce": 960.0}
+# kernel's launch counts): what one unit must read or write once, per-pair
+# records (poses, shared by all of a pair's units) not counted.
+#   tri_mask   : one node pair: its record (24 B), two node records (64 B),
+#                the pair's body/tree ids (16 B), the triangles (2 x 576 B)
+#                and eps (2 x 64 B) of both buckets; mask, count, hit flag out (16 B)
+#   tri_enum   : one node pair: mask and count in (12 B), its kept items out
+#                (counted under tri_dist)
+#   tri_dist   : one triangle pair: item (8 B), node pair (24 B), two node
+#                records (64 B), two triangles (144 B), two eps (16 B)
+#   soup_cull  : one soup node pair: as tri_mask (swept boxes of both buckets)
+#   soup_point / soup_edge : one sub-test: item (8 B), two triangles (144 B), one 64-B result
+#   leaf_point / leaf_edge : one sub-test: item (8 B), two triangles (144 B)
+#   soup_reduce: one triangle pair: 15 results (960 B)
+ALG_BYTES = {"tri_mask": 1400.0, "tri_enum": 12.0, "tri_dist": 256.0, "soup_cull": 1400.0, "soup_point": 216.0,
+             "soup_edge": 216.0, "leaf_point": 152.0, "leaf_edge": 152.0, "soup_reduce": 960.0}
+# Estimated FP32 operations per unit of the FP32 cull (tri_mask): per node
+# pair 16 triangle preparations (~80 ops) and 64 pair classifications (~130 ops).
+ALU_OPS = {"tri_mask": 16 * 80 + 64 * 130}
+FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 HBM_PEAK_GBS = 8000.0
 
 
@@ -281,6 +293,9 @@ def main():
     ap.add_argument("--graph", default="on", choices=["on", "off"])
     ap.add_argument("--profile-steps", type=int, default=5, help="eager steps timed per kernel for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="default launch shapes (profiling runs)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 reference leg (profiling runs of one configuration)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -312,18 +327,20 @@ def main():
     n_cl = clusters[1]
     weights, wsrc = gate_weights() if args.learned == "on" else (None, None)
     # the plain FP64 path on the same inputs (reference point of the speedup)
-    plain = make_ctx(scene, d2.D2_FP64, True, None, clusters)
-    plain_sec = timed_steps(plain, scenes.DT, args.steps, args.warmup)
-    plain_res = step_results(plain)
-    plain.close()
-    ctx = make_ctx(scene, prec, args.graph == "on", weights, clusters)
+    plain_sec, plain_res = None, None
+    if not args.no_plain:
+        plain = make_ctx(scene, d2.D2_FP64, True, None, clusters, tune=not args.no_autotune)
+        plain_sec = timed_steps(plain, scenes.DT, args.steps, args.warmup)
+        plain_res = step_results(plain)
+        plain.close()
+    ctx = make_ctx(scene, prec, args.graph == "on", weights, clusters, tune=not args.no_autotune)
     n_contacts = ctx.step_run(scenes.DT, True)
     sec = timed_steps(ctx, scenes.DT, args.steps, args.warmup)
-    identical = same_results(step_results(ctx), plain_res)
+    identical = same_results(step_results(ctx), plain_res) if plain_res else None
     ms_per_step = 1e3 * sec
     value = pairs.shape[0] / sec
     gate = ctx.gate_counts() if weights is not None else None
-    log(f"[bench] plain fp64: {1e3 * plain_sec:.3f} ms/step; this form: {ms_per_step:.3f} ms/step "
+    log(f"[bench] plain fp64: {1e3 * plain_sec if plain_sec else float('nan'):.3f} ms/step; this form: {ms_per_step:.3f} ms/step "
         f"({n_cl} clusters, {n_contacts} filtered contacts, graph {args.graph}, gate {gate}, identical {identical})")
     # per-kernel times: eager steps, every kernel bracketed by HIP events on the context's stream
     ctx.set_graph(False)
@@ -343,9 +360,19 @@ def main():
     avg_s = ms / n / 1e3
     bytes_per_launch = ALG_BYTES[dom] * items / n
     achieved = bytes_per_launch / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)
+        with open(TRAFFIC_FILE) as f:
+            key = f"{scene.name}/{args.precision}/{'learned' if weights is not None else 'plain'}"
+            t = json.load(f).get(key, {}).get(dom)
+        traffic = t.get("bytes_per_dispatch") if t else None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                 "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch}
+    if dom in ALU_OPS:
+        tf = ALU_OPS[dom] * items / n / avg_s / 1e12
+        roofline["alu"] = {"achieved_tflops_est": tf, "peak_fp32_tflops": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
+                           "ops_per_unit_est": ALU_OPS[dom]}
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
@@ -358,8 +385,9 @@ def main():
         "vs_baseline": None, "dtype": "f64" if prec == d2.D2_FP64 else "f32+f64", "data": "synthetic",
         "form": {"precision": args.precision, "learned_gate": args.learned == "on", "graph": args.graph == "on",
                  "weights": wsrc},
-        "plain_fp64": {"value": pairs.shape[0] / plain_sec, "ms_per_step": 1e3 * plain_sec},
-        "speedup_vs_plain_fp64": plain_sec / sec, "identical_to_plain_fp64": identical, "gate": gate,
+        "plain_fp64": {"value": pairs.shape[0] / plain_sec, "ms_per_step": 1e3 * plain_sec} if plain_sec else None,
+        "speedup_vs_plain_fp64": plain_sec / sec if plain_sec else None, "identical_to_plain_fp64": identical,
+        "gate": gate,
         "config": {"workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
                                "Poisson-disk 30% packing; v~N(0,2); dt=1/60; CCD+step rule then discrete+filter"
                    if scene.name == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60",

@@ -28,6 +28,7 @@
 // use atomics and the order is restored per pair afterwards.
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_ext.h>
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -2050,10 +2051,10 @@ using namespace d2;
 namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
-                              "leaf_point", "leaf_edge", "tri_enum", "tri_dist", "post", "post_pairs", "cluster",
-                              "gate"};
+                              "leaf_point", "leaf_edge", "tri_mask", "tri_enum", "tri_dist", "post", "post_pairs",
+                              "cluster", "gate"};
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
-           K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_NUM };
+           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
@@ -2258,6 +2259,22 @@ void collect_timers(d2_ctx* c) {
   c->pending_items.clear();
 }
 
+// Launch of one kernel that, when profiling, is timed by events the GPU
+// stamps at the kernel's own start and end (no host submission gaps).
+template <typename F, typename... Args>
+void timed_launch(d2_ctx* c, int kid, F kernel, dim3 grid, dim3 block, Args... args) {
+  if (c->profiling && !c->capturing) {
+    hipEvent_t a = nullptr, b = nullptr;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    hipExtLaunchKernelGGL(kernel, grid, block, 0, c->stream, a, b, 0, args...);
+    c->pending.push_back({kid, {a, b}});
+    c->pending_items.push_back(0.0);
+  } else {
+    hipLaunchKernelGGL(kernel, grid, block, 0, c->stream, args...);
+  }
+}
+
 Dev make_dev(d2_ctx* c, int level, int discrete) {
   Dev D;
   D.nodes = c->nodes.p;
@@ -2341,33 +2358,15 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   Dev D = make_dev(c, level, discrete);
   const int cg = chunk_grid(c), sg = stream_grid(c);
   if (discrete) {
-    {
-      Timer tm(c, K_TRI_ENUM, 0.0);
-      hipLaunchKernelGGL(k_tri_mask, dim3(cg), dim3(256), 0, c->stream, D, c->precision == D2_FP32 ? 1 : 0);
-      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap_items);
-    }
-    {
-      Timer tm(c, K_TRI_DIST, 0.0);
-      hipLaunchKernelGGL(k_tri_dist, dim3(sg), dim3(256), 0, c->stream, D);
-    }
+    timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, c->precision == D2_FP32 ? 1 : 0);
+    timed_launch(c, K_TRI_ENUM, k_mask_enum, dim3(cg), dim3(256), D, (long long)c->cap_items);
+    timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
   } else {
-    {
-      Timer tm(c, K_SOUP_CULL, 0.0);
-      hipLaunchKernelGGL(k_soup_cull, dim3(cg), dim3(256), 0, c->stream, D);
-      hipLaunchKernelGGL(k_mask_enum, dim3(cg), dim3(256), 0, c->stream, D, (long long)c->cap_res);
-    }
-    {
-      Timer tm(c, K_SOUP_POINT, 0.0);
-      hipLaunchKernelGGL(k_soup_point, dim3(sg), dim3(256), 0, c->stream, D);
-    }
-    {
-      Timer tm(c, K_SOUP_EDGE, 0.0);
-      hipLaunchKernelGGL(k_soup_edge, dim3(sg), dim3(256), 0, c->stream, D);
-    }
-    {
-      Timer tm(c, K_SOUP_REDUCE, 0.0);
-      hipLaunchKernelGGL(k_soup_reduce, dim3(sg), dim3(256), 0, c->stream, D);
-    }
+    timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
+    timed_launch(c, K_TRI_ENUM, k_mask_enum, dim3(cg), dim3(256), D, (long long)c->cap_res);
+    timed_launch(c, K_SOUP_POINT, k_soup_point, dim3(sg), dim3(256), D);
+    timed_launch(c, K_SOUP_EDGE, k_soup_edge, dim3(sg), dim3(256), D);
+    timed_launch(c, K_SOUP_REDUCE, k_soup_reduce, dim3(sg), dim3(256), D);
   }
   {
     Timer tm(c, K_EXPAND, 0.0);
@@ -2375,14 +2374,8 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     hipLaunchKernelGGL(k_level_write, dim3(cg), dim3(256), 0, c->stream, D);
   }
   if (!discrete) {
-    {
-      Timer tm(c, K_LEAF_POINT, 0.0);
-      hipLaunchKernelGGL(k_leaf_point, dim3(sg), dim3(256), 0, c->stream, D);
-    }
-    {
-      Timer tm(c, K_LEAF_EDGE, 0.0);
-      hipLaunchKernelGGL(k_leaf_edge, dim3(sg), dim3(256), 0, c->stream, D);
-    }
+    timed_launch(c, K_LEAF_POINT, k_leaf_point, dim3(sg), dim3(256), D);
+    timed_launch(c, K_LEAF_EDGE, k_leaf_edge, dim3(sg), dim3(256), D);
   }
 }
 
@@ -2513,6 +2506,7 @@ void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
 void add_level_items(d2_ctx* c, int level, int discrete) {
   const unsigned long long* s = c->host_lvl + level * LV_STRIDE;
   if (discrete) {
+    c->k_items[K_TRI_MASK] += (double)s[LV_SOUP];
     c->k_items[K_TRI_ENUM] += (double)s[LV_SOUP];
     c->k_items[K_TRI_DIST] += (double)s[LV_ITEMS];
   } else {
