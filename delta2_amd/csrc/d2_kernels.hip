@@ -2108,7 +2108,9 @@ struct d2_ctx {
   DBuf<double> buckets;
   DBuf<int32_t> tree_off, body_tree;
   DBuf<double> bodies;
-  int n_trees = 0, n_bodies = 0, max_depth = 0;
+  int n_trees = 0, n_bodies = 0;
+  int pair_depth = 0;                       // max over pairs of depth(tree a) + depth(tree b)
+  std::vector<int> tree_depth, host_body_tree;
   // pairs and clusters
   DBuf<int32_t> pairs, pair_info, cluster_of_pair;
   DBuf<double> max_time, pair_rec;
@@ -2315,7 +2317,9 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   return D;
 }
 
-int level_budget(const d2_ctx* c) { return std::min(MAX_LEVELS, 2 * c->max_depth + 1); }
+// A node pair's level is the sum of its nodes' depths, so a pair list needs
+// (deepest depth sum) + 1 levels.
+int level_budget(const d2_ctx* c) { return std::min(MAX_LEVELS, c->pair_depth + 1); }
 
 // Every buffer a run touches, sized before any launch (and before capture).
 int prepare(d2_ctx* c) {
@@ -2617,7 +2621,7 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   const int budget = level_budget(c);
   Sig sig;
   sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
-  sig.max_depth = c->max_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
+  sig.max_depth = c->pair_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
   sig.tune = c->tune;
   sig.learned = (c->learned_on && c->mlp_ready) ? c->gate_phases : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
@@ -2729,7 +2733,8 @@ int run(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
 int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
   if (c->n_pairs < 0 || c->n_bodies <= 0 || c->n_trees <= 0)
     return fail(c, D2_ERR_STATE, "upload trees, bodies and pairs first");
-  if (2 * c->max_depth + 1 > MAX_LEVELS) return fail(c, D2_ERR_ARG, "trees deeper than 7 levels are not supported");
+  if (c->pair_depth + 1 > MAX_LEVELS)
+    return fail(c, D2_ERR_ARG, "a pair's trees are too deep: depth(a) + depth(b) must stay below 16");
   if (kind == RUN_STEP && c->n_clusters <= 0) return fail(c, D2_ERR_STATE, "upload the pairs' clusters first");
   (void)hipSetDevice(c->device);
   if (c->n_pairs > c->cap) return fail(c, D2_ERR_CAPACITY, "more pairs than the work-list capacity");
@@ -2914,7 +2919,7 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   if (n_trees <= 0 || !tree_node_offset || !nodes || !buckets) return fail(c, D2_ERR_ARG, "d2_trees_upload: bad arguments");
   int total = tree_node_offset[n_trees];
   if (total <= 0 || tree_node_offset[0] != 0) return fail(c, D2_ERR_ARG, "d2_trees_upload: bad offsets");
-  int max_depth = 0;
+  std::vector<int> depth(n_trees, 0);
   for (int t = 0; t < n_trees; ++t) {
     int b = tree_node_offset[t], e = tree_node_offset[t + 1];
     if (e <= b) return fail(c, D2_ERR_ARG, "d2_trees_upload: empty tree");
@@ -2924,7 +2929,7 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
       if (r[1] < 0 || r[1] >= cnt || r[7] < 0 || r[7] > 8 || r[3] < 0 ||
           (r[2] && (r[5] < 0 || r[5] + r[6] > cnt || r[6] < 1 || r[6] > 8)))
         return fail(c, D2_ERR_ARG, "d2_trees_upload: inconsistent node record");
-      max_depth = std::max(max_depth, (int)r[3]);
+      depth[t] = std::max(depth[t], (int)r[3]);
     }
   }
   (void)hipSetDevice(c->device);
@@ -2979,7 +2984,9 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   HIPCHK(c, hipMemcpyAsync(c->tree_eps_out.p, eps_out.data(), sizeof(float) * n_trees, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_trees = n_trees;
-  c->max_depth = max_depth;
+  c->tree_depth = depth;
+  c->pair_depth = std::min(MAX_LEVELS - 1, 2 * *std::max_element(depth.begin(), depth.end()));  // until pairs come
+  c->host_body_tree.clear();
   return D2_OK;
 }
 
@@ -2995,6 +3002,7 @@ int d2_bodies_upload(d2_ctx* c, int n_bodies, const int32_t* tree_of_body, const
   HIPCHK(c, hipMemcpyAsync(c->body_tree.p, tree_of_body, sizeof(int32_t) * n_bodies, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_bodies = n_bodies;
+  c->host_body_tree.assign(tree_of_body, tree_of_body + n_bodies);
   return D2_OK;
 }
 
@@ -3040,6 +3048,11 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_pairs = n_pairs;
+  int deepest = 0;
+  for (int p = 0; p < n_pairs; ++p)
+    deepest = std::max(deepest, c->tree_depth[c->host_body_tree[pairs[2 * p]]] +
+                                    c->tree_depth[c->host_body_tree[pairs[2 * p + 1]]]);
+  c->pair_depth = deepest;
   c->n_clusters = 0;  // clusters describe a pair list
   c->has_max_time = max_time != nullptr;
   c->last_mode = -1;

@@ -409,6 +409,100 @@ def scene_c4(seed=1004, n_bodies=4096, threads=None):
     return s
 
 
+def funnel_mesh(r_top=20.0, r_bottom=4.0, height=40.0, n_theta=256, n_z=64):
+    """Open cone frustum (axis z, outlet at z = 0), 2 n_theta n_z triangles,
+    normals facing the axis."""
+    th = np.linspace(0.0, 2.0 * np.pi, n_theta, endpoint=False)
+    zs = np.linspace(0.0, height, n_z + 1)
+    rs = r_bottom + (r_top - r_bottom) * zs / height
+    V = np.array([[r * np.cos(t), r * np.sin(t), z] for z, r in zip(zs, rs) for t in th])
+    F = []
+    for i in range(n_z):
+        for j in range(n_theta):
+            a, b = i * n_theta + j, i * n_theta + (j + 1) % n_theta
+            c, d = a + n_theta, b + n_theta
+            F.append([a, c, b])
+            F.append([b, c, d])
+    return V, np.array(F, dtype=np.int32)
+
+
+def funnel_distance(p, r_top=20.0, r_bottom=4.0, height=40.0):
+    """Distance of points p (n, 3) from the funnel's cone surface (its segment
+    in the (radius, z) half-plane)."""
+    rr = np.hypot(p[:, 0], p[:, 1])
+    a = np.array([r_bottom, 0.0])
+    b = np.array([r_top, height])
+    ab = b - a
+    q = np.stack([rr, p[:, 2]], axis=1) - a
+    t = np.clip(q @ ab / (ab @ ab), 0.0, 1.0)
+    return np.linalg.norm(q - t[:, None] * ab, axis=1)
+
+
+def scene_c5(seed=1005, n_bodies=65536, threads=None, density=0.1):
+    """Large pile: n_bodies C1 meshes (subdivision-2 icospheres scaled
+    U(0.5,1.5), 320 triangles each) at the C1 number density (0.1 per m^3)
+    inside a static 40 m funnel (top radius 20 m, outlet radius 4 m) and in
+    the column of radius 20 m above it, falling at v ~ N(0,1) + (0,0,-3) m/s.
+    Body 0 is the funnel (static, 32768 triangles).  A body pairs with the
+    funnel when its bounding sphere comes within reach of the cone surface
+    (what a triangle BVH broad phase decides); body pairs use the swept
+    bounding-sphere test."""
+    rng = np.random.default_rng(seed)
+    V0, F0 = icosphere(2, 1.0)
+    _, _, I0 = mass_properties(V0, F0)
+    r0 = float(np.max(np.linalg.norm(V0, axis=1)))
+    funnel_v = np.pi * 40.0 / 3.0 * (20.0 ** 2 + 20.0 * 4.0 + 4.0 ** 2)
+    col_h = max(0.0, (n_bodies / density - funnel_v) / (np.pi * 20.0 ** 2))
+    pos = np.zeros((n_bodies, 3))
+    k = 0
+    while k < n_bodies:  # uniform in funnel + column (rejection sampling)
+        m = (n_bodies - k) * 2
+        c = np.stack([rng.uniform(-20, 20, m), rng.uniform(-20, 20, m), rng.uniform(0, 40 + col_h, m)], axis=1)
+        rr = np.hypot(c[:, 0], c[:, 1])
+        rmax = np.where(c[:, 2] < 40.0, 4.0 + 16.0 * c[:, 2] / 40.0, 20.0)
+        c = c[rr < rmax - 0.3]
+        take = min(len(c), n_bodies - k)
+        pos[k:k + take] = c[:take]
+        k += take
+    FV, FF = funnel_mesh()
+    meshes = [(FV, FF)]
+    recs = [body_record(np.array([1.0, 0.0, 0.0, 0.0]), np.zeros(3), np.zeros(3), np.zeros((3, 3)), static=True)]
+    radii = [float(np.max(np.linalg.norm(FV, axis=1)))]
+    for i in range(n_bodies):
+        sc = rng.uniform(0.5, 1.5)
+        meshes.append((V0 * sc, F0))
+        v = rng.normal(0.0, 1.0, size=3) + np.array([0.0, 0.0, -3.0])
+        recs.append(body_record(random_rotation(rng), pos[i], v, np.linalg.inv(I0 * sc ** 5)))
+        radii.append(r0 * sc)
+    trees = build_trees_parallel(meshes, 40000, threads)
+    rec = np.array(recs)
+    radii = np.array(radii)
+    s = Scene("c5", meshes, trees, rec, radii)
+    s.pairs = c5_pairs(rec, radii)
+    return s
+
+
+def c5_pairs(rec, radii, geo_eps=GEO_EPS):
+    """Body-body pairs by the swept-sphere test, plus (0, body) for bodies
+    within reach of the funnel wall."""
+    body = broad_phase(rec[1:], radii[1:], geo_eps) + 1
+    reach = radii[1:] + 2 * geo_eps + np.linalg.norm(rec[1:, 7:10], axis=1) * rec[1:, 39]
+    near = np.nonzero(funnel_distance(rec[1:, 4:7]) < reach)[0] + 1
+    wall = np.stack([np.zeros(near.size, dtype=np.int64), near], axis=1)
+    return np.concatenate([wall, body]).astype(np.int32)
+
+
+def advance_gravity(records, g=(0.0, 0.0, -9.8), dt=DT):
+    """Ballistic step of the dynamic bodies under gravity (no contact response)."""
+    r = records.copy()
+    dyn = r[:, 38] == 0
+    r[dyn, 7:10] += dt * np.asarray(g)
+    r[dyn, 4:7] += dt * r[dyn, 7:10]
+    r[:, 13] += dt
+    r[:, 14:28] = r[:, 0:14]
+    return r
+
+
 def advance(records, dt=DT):
     """Ballistic drift of the current state by dt (no contact response: that is
     the solver's job, outside this path)."""
