@@ -110,6 +110,14 @@ def lib():
                         ctypes.c_int),
         "d2_clusters_upload": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
         "d2_step_run": ([_vp, ctypes.c_float, ctypes.c_int, _dp, _i64p], ctypes.c_int),
+        "d2_tree_bounds_upload": ([_vp, _fp, _dp], ctypes.c_int),
+        "d2_tree_bounds_download": ([_vp, _fp, _dp], ctypes.c_int),
+        "d2_bodies_last_upload": ([_vp, _dp], ctypes.c_int),
+        "d2_bodies_groups_upload": ([_vp, _ip], ctypes.c_int),
+        "d2_broad_phase": ([_vp, ctypes.c_int, ctypes.c_double, _i64p], ctypes.c_int),
+        "d2_pairs_download": ([_vp, _ip, _dp], ctypes.c_int),
+        "d2_clusters_build": ([_vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "d2_clusters_download": ([_vp, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -407,6 +415,58 @@ class Context:
 
     def sync(self):
         self._check(lib().d2_sync(self.h))
+
+    # --- broad phase and clusters ---------------------------------------------
+    def upload_tree_bounds(self, aabb6, radius):
+        box = np.ascontiguousarray(np.asarray(aabb6, dtype=np.float32).ravel())
+        rad = _f64(radius)
+        self._check(lib().d2_tree_bounds_upload(self.h, box.ctypes.data_as(_fp), _dptr(rad)))
+
+    def tree_bounds(self, n_trees):
+        box = np.zeros((n_trees, 6), dtype=np.float32)
+        rad = np.zeros(n_trees)
+        self._check(lib().d2_tree_bounds_download(self.h, box.ctypes.data_as(_fp), _dptr(rad)))
+        return box, rad
+
+    def upload_last_states(self, last14):
+        st = _f64(last14)
+        self._check(lib().d2_bodies_last_upload(self.h, _dptr(st)))
+
+    def upload_groups(self, groups):
+        g = None if groups is None else _i32(groups)
+        self._check(lib().d2_bodies_groups_upload(self.h, None if g is None else _iptr(g)))
+
+    def broad_phase(self, mode=1, t=0.0):
+        """broadPhaseEmbree on the device (mode 0: with step t; mode 1: the
+        individual-time form); the pairs become the resident pair list."""
+        n = ctypes.c_int64(0)
+        self._check(lib().d2_broad_phase(self.h, int(mode), float(t), ctypes.byref(n)))
+        self.n_pairs = n.value
+        return n.value
+
+    def download_pairs(self):
+        pairs = np.zeros((self.n_pairs, 2), dtype=np.int32)
+        mt = np.zeros(self.n_pairs)
+        self._check(lib().d2_pairs_download(self.h, _iptr(pairs), _dptr(mt)))
+        return pairs, mt
+
+    def build_clusters(self):
+        """separateCollisionClusters on the device over the resident pairs."""
+        k = ctypes.c_int(0)
+        self._check(lib().d2_clusters_build(self.h, ctypes.byref(k)))
+        self.n_clusters = k.value
+        return k.value
+
+    def download_clusters(self):
+        k = self.n_clusters
+        cop = np.zeros(max(self.n_pairs, 1), dtype=np.int32)
+        cob = np.zeros(self.n_bodies, dtype=np.int32)
+        first = np.zeros(max(k, 1), dtype=np.int32)
+        mt = np.zeros(max(k, 1))
+        st = np.zeros(max(k, 1))
+        self._check(lib().d2_clusters_download(self.h, _iptr(cop), _iptr(cob), _iptr(first), _dptr(mt), _dptr(st)))
+        return {"n": k, "of_pair": cop[:self.n_pairs], "of_body": cob, "first": first[:k], "min_time": mt[:k],
+                "step": st[:k]}
 
 
 # --- reference-named mirrors -------------------------------------------------

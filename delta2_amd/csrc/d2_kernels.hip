@@ -2003,21 +2003,32 @@ __global__ void k_cluster_min(int n_pairs, const int32_t* pairs, const int32_t* 
     atomicMin(&step_bits[k], (unsigned long long)__double_as_longlong(v));
   }
 }
+// A cluster without pairs keeps its first particle's last step (:594-596);
+// first_member (clusters built on the device) names it, otherwise dt.
 __global__ void k_cluster_final(int n_clusters, float dt, const int32_t* pairs, const double* bodies,
-                                const unsigned long long* step_bits, const int* first_any, double* out) {
+                                const unsigned long long* step_bits, const int* first_any, const int* first_member,
+                                double* out) {
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_clusters; k += gridDim.x * blockDim.x) {
     double s;
     if (step_bits[k] != ~0ull) s = __longlong_as_double((long long)step_bits[k]);
     else if (first_any[k] != 0x7fffffff) s = bodies[40 * (size_t)pairs[2 * first_any[k]] + 39];
+    else if (first_member) s = bodies[40 * (size_t)first_member[k] + 39];
     else s = (double)dt;
     const double hi = (double)dt;
     s = (hi < s) ? hi : s;        // std::min(s, dt)
     out[k] = (s < 1e-3) ? 1e-3 : s;  // std::max(1e-3, .)
   }
 }
-// Per-body step: its cluster's step, dt for bodies in no pair.
-__global__ void k_body_step_fill(int n_bodies, float dt, double* body_step) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_bodies; i += gridDim.x * blockDim.x) body_step[i] = (double)dt;
+// Per-body step: its cluster's step; a body in no pair is a cluster of its
+// own without interactions, whose step is its clamped last step
+// (separate_clusters.cpp:594-596, time_step_selection_dynamic_continuous.cpp:19).
+__global__ void k_body_step_fill(int n_bodies, float dt, const double* bodies, double* body_step) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_bodies; i += gridDim.x * blockDim.x) {
+    double s = bodies[40 * (size_t)i + 39];
+    const double hi = (double)dt;
+    s = (hi < s) ? hi : s;
+    body_step[i] = (s < 1e-3) ? 1e-3 : s;
+  }
 }
 __global__ void k_body_step_pairs(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair,
                                   const double* cluster_step, double* body_step) {
@@ -2029,6 +2040,7 @@ __global__ void k_body_step_pairs(int n_pairs, const int32_t* pairs, const int32
 }
 
 #include "d2_learned.hpp"
+#include "d2_broad.hpp"
 
 // Body future states: future = current.extrapolate(step).
 // skip_static: static bodies keep their future state (PDE_explicit.cpp:49-56).
@@ -2073,8 +2085,9 @@ struct Sig {
   int levels[2] = {-1, -1};  // graph levels of each phase
   int learned = -1;
   float gate_t = 0.f, gate_s = 0.f;
+  int cl_dev = -1;  // clusters built on the device (first members known)
   bool operator==(const Sig& o) const {
-    return levels[0] == o.levels[0] && levels[1] == o.levels[1] && learned == o.learned && gate_t == o.gate_t &&
+    return cl_dev == o.cl_dev && levels[0] == o.levels[0] && levels[1] == o.levels[1] && learned == o.learned && gate_t == o.gate_t &&
            gate_s == o.gate_s && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
            precision == o.precision && filter == o.filter && dt == o.dt &&
            tune.chunk_blocks == o.tune.chunk_blocks && tune.stream_blocks == o.tune.stream_blocks &&
@@ -2135,6 +2148,26 @@ struct d2_ctx {
   DBuf<int> first_any, first_ns;
   DBuf<double> cluster_step, body_step;
   DBuf<int32_t> tmp_cluster;
+  // broad phase (per tree: float AABB of the mesh and bounding radius)
+  DBuf<float> tb_box;
+  DBuf<double> tb_rad, last_state, bp_R;
+  DBuf<int32_t> tree_depth_d, group;
+  bool has_groups = false;
+  std::vector<int> host_static;
+  int n_items = 0;
+  DBuf<int> bp_statics, bp_item_body, bp_item_seg, bp_item_cnt, bp_item_sorted, bp_item_off, bp_meta, bp_slot,
+      bp_rank, bp_counts, bp_start, bp_members, bp_red;
+  DBuf<long long> bp_item_pos;
+  DBuf<unsigned long long> bp_keys, bp_fill;
+  DBuf<Box8> bp_box;
+  DBuf<int2> bp_chunk;
+  long long bp_cap = 0;
+  int* host_red = nullptr;  // pinned: BR_N reductions + pair total
+  // clusters built on the device
+  DBuf<int> cc_parent, cc_label, cc_root, cc_root_off, cc_of_body, cc_first;
+  DBuf<long long> cc_tmin;
+  DBuf<unsigned long long> cc_smin;
+  bool clusters_on_device = false;
   // learned gate
   DBuf<float> tree_radius, tree_eps_out, hv, feat, pred;
   DBuf<int32_t> hv_off;
@@ -2420,7 +2453,8 @@ int enqueue_post(d2_ctx* c, int mode) {
   return D2_OK;
 }
 
-int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, bool project) {
+int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, bool project,
+                     bool first_member) {
   const int np = c->n_pairs;
   Timer tm(c, K_CLUSTER, np);
   hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters,
@@ -2432,10 +2466,11 @@ int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
                        c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->step_bits.p);
   }
   hipLaunchKernelGGL(k_cluster_final, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters, dt,
-                     c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p, c->cluster_step.p);
+                     c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p,
+                     first_member ? c->cc_first.p : (const int*)nullptr, c->cluster_step.p);
   if (project) {
     hipLaunchKernelGGL(k_body_step_fill, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, dt,
-                       c->body_step.p);
+                       c->bodies.p, c->body_step.p);
     if (np > 0)
       hipLaunchKernelGGL(k_body_step_pairs, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p,
                          cluster_of_pair, c->cluster_step.p, c->body_step.p);
@@ -2561,7 +2596,7 @@ int enqueue_end(d2_ctx* c, int kind, const PhaseSpec& ph, int levels, float dt) 
     hipLaunchKernelGGL(k_save_ccd_sizes, dim3(1), dim3(256), 0, c->stream, D);
   }
   RC(enqueue_post(c, ph.post_mode));
-  if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true));
+  if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true, c->clusters_on_device));
   HIPCHK(c, hipGetLastError());
   return D2_OK;
 }
@@ -2624,6 +2659,7 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   sig.max_depth = c->pair_depth, sig.precision = c->precision, sig.filter = filter, sig.dt = dt;
   sig.tune = c->tune;
   sig.learned = (c->learned_on && c->mlp_ready) ? c->gate_phases : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
+  sig.cl_dev = kind == RUN_STEP && c->clusters_on_device ? 1 : 0;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
   std::vector<std::function<int()>> ops;
   int seg = 0;
@@ -2773,8 +2809,10 @@ int d2_ctx_create(int device, int precision, d2_ctx** out) {
   c->tune = d2_tune{4 * c->cus, 8 * c->cus, 16 * c->cus, 3};
   if (hipHostMalloc((void**)&c->host_cnt, sizeof(unsigned long long) * C_COUNT, hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c->host_lvl, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE,
-                    hipHostMallocDefault) != hipSuccess) {
+                    hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->host_red, sizeof(int) * (BR_N + 2), hipHostMallocDefault) != hipSuccess) {
     if (c->host_cnt) (void)hipHostFree(c->host_cnt);
+    if (c->host_lvl) (void)hipHostFree(c->host_lvl);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return D2_ERR_HIP;
@@ -2807,6 +2845,15 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->cert), release(c->cert_off), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
   release(c->mlp_wb), release(c->tr_x), release(c->tr_t), release(c->tr_act), release(c->tr_dlt), release(c->tr_loss);
   release(c->tr_idx);
+  release(c->tb_box), release(c->tb_rad), release(c->last_state), release(c->bp_R), release(c->tree_depth_d),
+      release(c->group);
+  release(c->bp_statics), release(c->bp_item_body), release(c->bp_item_seg), release(c->bp_item_cnt),
+      release(c->bp_item_sorted), release(c->bp_item_off), release(c->bp_meta), release(c->bp_slot), release(c->bp_rank),
+      release(c->bp_counts), release(c->bp_start), release(c->bp_members), release(c->bp_red);
+  release(c->bp_item_pos), release(c->bp_keys), release(c->bp_fill), release(c->bp_box), release(c->bp_chunk);
+  release(c->cc_parent), release(c->cc_label), release(c->cc_root), release(c->cc_root_off), release(c->cc_of_body),
+      release(c->cc_first), release(c->cc_tmin), release(c->cc_smin);
+  if (c->host_red) (void)hipHostFree(c->host_red);
   if (c->host_cnt) (void)hipHostFree(c->host_cnt);
   if (c->host_lvl) (void)hipHostFree(c->host_lvl);
   (void)hipStreamDestroy(c->stream);
@@ -2976,6 +3023,32 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
     radius[t] = (float)rmax * (1.0f + 1e-6f);
     eps_out[t] = (float)std::min(eo, 1e30);
   }
+  // Per tree for the broad phase, as MeshData::initBoudingData (model/mesh.cpp:43-50)
+  // computes it over the mesh vertices (here: the leaf triangles' vertices,
+  // which are every vertex of a mesh whose vertices all belong to a face):
+  // the float AABB and the largest vertex norm.
+  std::vector<float> tbox(6 * (size_t)n_trees);
+  std::vector<double> trad(n_trees, 0.0);
+  for (int t = 0; t < n_trees; ++t) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = tree_node_offset[t]; i < tree_node_offset[t + 1]; ++i) {
+      const int32_t* r = nodes + 8 * (size_t)i;
+      if (r[2]) continue;
+      const double* bk = buckets + (size_t)BUCKET * (tree_node_offset[t] + r[1]);
+      for (int k = 0; k < r[7]; ++k)
+        for (int q = 0; q < 3; ++q) {
+          const double x = bk[(3 * q) * 8 + k], y = bk[(3 * q + 1) * 8 + k], z = bk[(3 * q + 2) * 8 + k];
+          const float f[3] = {(float)x, (float)y, (float)z};
+          for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], f[d]), hi[d] = std::max(hi[d], f[d]);
+          trad[t] = std::max(trad[t], std::sqrt((x * x + y * y) + z * z));
+        }
+    }
+    for (int d = 0; d < 3; ++d) tbox[6 * t + d] = lo[d], tbox[6 * t + 3 + d] = hi[d];
+  }
+  RC(ensure(c, c->tb_box, 6 * (size_t)n_trees)); RC(ensure(c, c->tb_rad, n_trees)); RC(ensure(c, c->tree_depth_d, n_trees));
+  HIPCHK(c, hipMemcpyAsync(c->tb_box.p, tbox.data(), sizeof(float) * tbox.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->tb_rad.p, trad.data(), sizeof(double) * n_trees, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->tree_depth_d.p, depth.data(), sizeof(int32_t) * n_trees, hipMemcpyHostToDevice, c->stream));
   RC(ensure(c, c->hv, std::max<size_t>(hv.size(), 3))); RC(ensure(c, c->hv_off, n_trees + 1));
   RC(ensure(c, c->tree_radius, n_trees)); RC(ensure(c, c->tree_eps_out, n_trees));
   HIPCHK(c, hipMemcpyAsync(c->hv.p, hv.data(), sizeof(float) * hv.size(), hipMemcpyHostToDevice, c->stream));
@@ -3000,8 +3073,35 @@ int d2_bodies_upload(d2_ctx* c, int n_bodies, const int32_t* tree_of_body, const
   RC(ensure(c, c->bodies, (size_t)n_bodies * 40)); RC(ensure(c, c->body_tree, n_bodies));
   HIPCHK(c, hipMemcpyAsync(c->bodies.p, records, sizeof(double) * 40 * (size_t)n_bodies, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->body_tree.p, tree_of_body, sizeof(int32_t) * n_bodies, hipMemcpyHostToDevice, c->stream));
+  // last states start equal to the current ones (d2_bodies_last_upload sets them)
+  std::vector<double> last(14 * (size_t)n_bodies);
+  for (int i = 0; i < n_bodies; ++i) std::memcpy(&last[14 * (size_t)i], records + 40 * (size_t)i, 14 * sizeof(double));
+  RC(ensure(c, c->last_state, 14 * (size_t)n_bodies));
+  HIPCHK(c, hipMemcpyAsync(c->last_state.p, last.data(), sizeof(double) * last.size(), hipMemcpyHostToDevice, c->stream));
+  // broad-phase work items: one per non-static body, one per BP_SEG later
+  // bodies for a static one (it sweeps all of them)
+  c->host_static.clear();
+  std::vector<int> ib, is;
+  for (int i = 0; i < n_bodies; ++i) {
+    if (records[40 * (size_t)i + 38] != 0.0) {
+      c->host_static.push_back(i);
+      const int segs = std::max(1, (n_bodies - 1 - i + BP_SEG - 1) / BP_SEG);
+      for (int q = 0; q < segs; ++q) ib.push_back(i), is.push_back(q);
+    } else {
+      ib.push_back(i), is.push_back(0);
+    }
+  }
+  c->n_items = (int)ib.size();
+  RC(ensure(c, c->bp_item_body, ib.size())); RC(ensure(c, c->bp_item_seg, is.size()));
+  RC(ensure(c, c->bp_statics, std::max<size_t>(c->host_static.size(), 1)));
+  HIPCHK(c, hipMemcpyAsync(c->bp_item_body.p, ib.data(), sizeof(int) * ib.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->bp_item_seg.p, is.data(), sizeof(int) * is.size(), hipMemcpyHostToDevice, c->stream));
+  if (!c->host_static.empty())
+    HIPCHK(c, hipMemcpyAsync(c->bp_statics.p, c->host_static.data(), sizeof(int) * c->host_static.size(),
+                             hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_bodies = n_bodies;
+  c->has_groups = false;
   c->host_body_tree.assign(tree_of_body, tree_of_body + n_bodies);
   return D2_OK;
 }
@@ -3054,6 +3154,7 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
                                     c->tree_depth[c->host_body_tree[pairs[2 * p + 1]]]);
   c->pair_depth = deepest;
   c->n_clusters = 0;  // clusters describe a pair list
+  c->clusters_on_device = false;
   c->has_max_time = max_time != nullptr;
   c->last_mode = -1;
   c->ccd_valid = false;
@@ -3073,6 +3174,7 @@ int d2_clusters_upload(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters
                              c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_clusters = n_clusters;
+  c->clusters_on_device = false;
   return D2_OK;
 }
 
@@ -3156,9 +3258,214 @@ int d2_cluster_steps(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
       return fail(c, D2_ERR_ARG, "d2_cluster_steps: cluster index out of range");
   RC(ensure_clusters(c, n_clusters)); RC(ensure(c, c->tmp_cluster, std::max(np, 1)));
   if (np) HIPCHK(c, hipMemcpyAsync(c->tmp_cluster.p, cluster_of_pair, sizeof(int32_t) * np, hipMemcpyHostToDevice, c->stream));
-  RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false));
+  RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false, false));
   HIPCHK(c, hipMemcpyAsync(out, c->cluster_step.p, sizeof(double) * n_clusters, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+// ------------------------------------------------ broad phase, clusters
+int d2_tree_bounds_upload(d2_ctx* c, const float* aabb6, const double* radius) {
+  if (!c || !aabb6 || !radius) return D2_ERR_ARG;
+  if (c->n_trees <= 0) return fail(c, D2_ERR_STATE, "d2_tree_bounds_upload: upload trees first");
+  for (int t = 0; t < c->n_trees; ++t)
+    if (!(radius[t] >= 0.0)) return fail(c, D2_ERR_ARG, "d2_tree_bounds_upload: radius must be >= 0");
+  HIPCHK(c, hipMemcpyAsync(c->tb_box.p, aabb6, sizeof(float) * 6 * c->n_trees, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->tb_rad.p, radius, sizeof(double) * c->n_trees, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_tree_bounds_download(d2_ctx* c, float* aabb6, double* radius) {
+  if (!c || !aabb6 || !radius) return D2_ERR_ARG;
+  if (c->n_trees <= 0) return fail(c, D2_ERR_STATE, "no trees uploaded");
+  HIPCHK(c, hipMemcpyAsync(aabb6, c->tb_box.p, sizeof(float) * 6 * c->n_trees, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(radius, c->tb_rad.p, sizeof(double) * c->n_trees, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_bodies_last_upload(d2_ctx* c, const double* last14) {
+  if (!c || !last14) return D2_ERR_ARG;
+  if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_bodies_last_upload: upload bodies first");
+  HIPCHK(c, hipMemcpyAsync(c->last_state.p, last14, sizeof(double) * 14 * (size_t)c->n_bodies, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_bodies_groups_upload(d2_ctx* c, const int32_t* group_of_body) {
+  if (!c) return D2_ERR_ARG;
+  if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_bodies_groups_upload: upload bodies first");
+  if (!group_of_body) {
+    c->has_groups = false;
+    return D2_OK;
+  }
+  for (int i = 0; i < c->n_bodies; ++i)
+    if (group_of_body[i] < 0 || group_of_body[i] > (1 << 29))
+      return fail(c, D2_ERR_ARG, "d2_bodies_groups_upload: group out of range");
+  RC(ensure(c, c->group, c->n_bodies));
+  HIPCHK(c, hipMemcpyAsync(c->group.p, group_of_body, sizeof(int32_t) * c->n_bodies, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->has_groups = true;
+  return D2_OK;
+}
+
+int d2_broad_phase(d2_ctx* c, int mode, double t, int64_t* n_pairs_out) {
+  if (!c) return D2_ERR_ARG;
+  if (mode != 0 && mode != 1) return fail(c, D2_ERR_ARG, "d2_broad_phase: mode must be 0 or 1");
+  if (mode == 0 && !(t >= 0.0 && t < 1e300)) return fail(c, D2_ERR_ARG, "d2_broad_phase: t must be finite and >= 0");
+  if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_broad_phase: upload bodies first");
+  (void)hipSetDevice(c->device);
+  // the resident pair list is replaced (and invalid until this call succeeds)
+  c->n_pairs = 0;
+  c->n_clusters = 0;
+  c->clusters_on_device = false;
+  c->last_mode = -1;
+  c->ccd_valid = false;
+  const int n = c->n_bodies;
+  long long T = 64;
+  while (T < 2ll * n) T <<= 1;
+  RC(ensure(c, c->bp_box, n)); RC(ensure(c, c->bp_R, n)); RC(ensure(c, c->bp_meta, n));
+  RC(ensure(c, c->bp_slot, n)); RC(ensure(c, c->bp_rank, n)); RC(ensure(c, c->bp_members, n));
+  RC(ensure(c, c->bp_keys, T)); RC(ensure(c, c->bp_counts, T)); RC(ensure(c, c->bp_start, T + 1));
+  RC(ensure(c, c->bp_red, BR_N + 2)); RC(ensure(c, c->bp_fill, 2));
+  RC(ensure(c, c->bp_item_cnt, c->n_items)); RC(ensure(c, c->bp_item_pos, c->n_items));
+  RC(ensure(c, c->bp_item_sorted, c->n_items)); RC(ensure(c, c->bp_item_off, c->n_items + 1));
+  RC(ensure(c, c->scan_part, chunk_grid(c)));
+  if (c->bp_cap <= 0) c->bp_cap = std::max<long long>(1 << 16, 8ll * n);
+  const int* grp = c->has_groups ? c->group.p : nullptr;
+  const int g = grid_for(c, n);
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    RC(ensure(c, c->bp_chunk, c->bp_cap)); RC(ensure(c, c->pairs, 2 * c->bp_cap)); RC(ensure(c, c->max_time, c->bp_cap));
+    hipLaunchKernelGGL(k_bp_clear, dim3(grid_for(c, T)), dim3(256), 0, c->stream, T, c->bp_keys.p, c->bp_counts.p,
+                       c->bp_red.p);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->bp_fill.p, 2ll, 0ull);
+    hipLaunchKernelGGL(k_bp_bounds, dim3(g), dim3(256), 0, c->stream, n, c->bodies.p, c->last_state.p, c->body_tree.p,
+                       c->tb_box.p, c->tb_rad.p, grp, mode, t, c->bp_box.p, c->bp_R.p, c->bp_meta.p, c->bp_red.p);
+    hipLaunchKernelGGL(k_bp_insert, dim3(g), dim3(256), 0, c->stream, n, c->bp_box.p, c->bp_meta.p, c->bp_red.p, T,
+                       c->bp_keys.p, c->bp_counts.p, c->bp_slot.p, c->bp_rank.p);
+    scan_host_n(c, c->bp_counts.p, c->bp_start.p, T, nullptr);
+    hipLaunchKernelGGL(k_bp_scatter, dim3(g), dim3(256), 0, c->stream, n, c->bp_meta.p, c->bp_slot.p, c->bp_rank.p,
+                       c->bp_start.p, c->bp_members.p);
+    BpIn in;
+    in.n = n, in.mode = mode, in.t = t, in.bodies = c->bodies.p, in.last = c->last_state.p, in.box = c->bp_box.p;
+    in.R = c->bp_R.p, in.meta = c->bp_meta.p, in.statics = c->bp_statics.p, in.n_static = (int)c->host_static.size();
+    in.red = c->bp_red.p, in.T = T, in.keys = c->bp_keys.p, in.counts = c->bp_counts.p, in.start = c->bp_start.p;
+    in.members = c->bp_members.p;
+    const int gi = (int)std::min<long long>((c->n_items + 3) / 4, (long long)c->cus * 16);
+    hipLaunchKernelGGL(k_bp_pairs, dim3(std::max(gi, 1)), dim3(256), 0, c->stream, in, c->n_items, c->bp_item_body.p,
+                       c->bp_item_seg.p, c->bp_cap, c->bp_chunk.p, c->bp_fill.p, c->bp_item_cnt.p, c->bp_item_pos.p,
+                       c->bp_item_sorted.p, c->bp_red.p);
+    scan_host_n(c, c->bp_item_cnt.p, c->bp_item_off.p, c->n_items, c->bp_fill.p + 1);
+    hipLaunchKernelGGL(k_bp_gather, dim3(std::max(gi, 1)), dim3(256), 0, c->stream, c->n_items, c->bp_item_cnt.p,
+                       c->bp_item_pos.p, c->bp_item_sorted.p, c->bp_item_off.p, c->bp_chunk.p, c->bodies.p,
+                       c->body_tree.p, c->tree_depth_d.p, c->pairs.p, c->max_time.p, c->bp_red.p);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->host_red, c->bp_red.p, sizeof(int) * BR_N, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->host_red + BR_N, c->bp_item_off.p + c->n_items, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->host_red[BR_ERR] & 1)
+      return fail(c, D2_ERR_RUNTIME, "d2_broad_phase: Invalid args for interpolation (state.cpp:56-58)");
+    const long long total = c->host_red[BR_N];
+    if (c->host_red[BR_OVF]) {
+      c->bp_cap = std::max(c->bp_cap * 2, total + total / 4 + 1024);
+      continue;
+    }
+    const int np = (int)total;
+    RC(ensure(c, c->pair_rec, (size_t)std::max(np, 1) * PR_STRIDE)); RC(ensure(c, c->pair_info, 4 * (size_t)std::max(np, 1)));
+    c->n_pairs = np;
+    c->pair_depth = c->host_red[BR_DEPTH];
+    c->n_clusters = 0;
+    c->clusters_on_device = false;
+    c->has_max_time = true;
+    c->last_mode = -1;
+    c->ccd_valid = false;
+    if (n_pairs_out) *n_pairs_out = np;
+    return D2_OK;
+  }
+  return fail(c, D2_ERR_CAPACITY, "d2_broad_phase: pair list kept overflowing");
+}
+
+int d2_pairs_download(d2_ctx* c, int32_t* pairs, double* max_time) {
+  if (!c) return D2_ERR_ARG;
+  if (c->n_pairs > 0 && pairs)
+    HIPCHK(c, hipMemcpyAsync(pairs, c->pairs.p, sizeof(int32_t) * 2 * c->n_pairs, hipMemcpyDeviceToHost, c->stream));
+  if (c->n_pairs > 0 && max_time)
+    HIPCHK(c, hipMemcpyAsync(max_time, c->max_time.p, sizeof(double) * c->n_pairs, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_clusters_build(d2_ctx* c, int* n_clusters_out) {
+  if (!c) return D2_ERR_ARG;
+  if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_clusters_build: upload bodies first");
+  (void)hipSetDevice(c->device);
+  const int n = c->n_bodies, np = c->n_pairs;
+  c->n_clusters = 0;  // invalid until this call succeeds
+  c->clusters_on_device = false;
+  RC(ensure(c, c->cc_parent, n)); RC(ensure(c, c->cc_label, n)); RC(ensure(c, c->cc_root, n));
+  RC(ensure(c, c->cc_root_off, n + 1)); RC(ensure(c, c->cc_of_body, n)); RC(ensure(c, c->cc_first, n));
+  RC(ensure(c, c->cc_tmin, n)); RC(ensure(c, c->cc_smin, n)); RC(ensure(c, c->bp_red, BR_N + 2));
+  RC(ensure(c, c->cluster_of_pair, std::max(np, 1))); RC(ensure(c, c->scan_part, chunk_grid(c)));
+  RC(ensure(c, c->bp_fill, 2));
+  const int g = grid_for(c, n), gp = grid_for(c, std::max(np, 1));
+  hipLaunchKernelGGL(k_bp_clear, dim3(1), dim3(64), 0, c->stream, 0ll, c->bp_keys.p, c->bp_counts.p, c->bp_red.p);
+  hipLaunchKernelGGL(k_cc_init, dim3(g), dim3(256), 0, c->stream, n, c->cc_parent.p);
+  if (np > 0)
+    hipLaunchKernelGGL(k_cc_union, dim3(gp), dim3(256), 0, c->stream, np, c->pairs.p, c->bodies.p, c->cc_parent.p,
+                       c->bp_red.p);
+  hipLaunchKernelGGL(k_cc_roots, dim3(g), dim3(256), 0, c->stream, n, c->bodies.p, c->cc_parent.p, c->cc_label.p,
+                     c->cc_root.p);
+  scan_host_n(c, c->cc_root.p, c->cc_root_off.p, n, nullptr);
+  hipLaunchKernelGGL(k_cc_init_cl, dim3(g), dim3(256), 0, c->stream, n, c->cc_tmin.p, c->cc_smin.p);
+  hipLaunchKernelGGL(k_cc_assign, dim3(g), dim3(256), 0, c->stream, n, c->bodies.p, c->cc_label.p, c->cc_root_off.p,
+                     c->cc_of_body.p, c->cc_first.p, c->cc_tmin.p, c->cc_smin.p);
+  if (np > 0)
+    hipLaunchKernelGGL(k_cc_pairs, dim3(gp), dim3(256), 0, c->stream, np, c->pairs.p, c->bodies.p, c->cc_of_body.p,
+                       c->cluster_of_pair.p);
+  hipLaunchKernelGGL(k_cc_rollback, dim3(g), dim3(256), 0, c->stream, n, c->bodies.p, c->last_state.p, c->cc_of_body.p,
+                     c->cc_tmin.p, c->bp_red.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->host_red, c->bp_red.p, sizeof(int) * BR_N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->host_red + BR_N, c->cc_root_off.p + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->host_red[BR_ERR] & 2)
+    return fail(c, D2_ERR_ARG, "d2_clusters_build: a pair of two static bodies belongs to no cluster");
+  if (c->host_red[BR_ERR] & 1)
+    return fail(c, D2_ERR_RUNTIME, "d2_clusters_build: Invalid args for interpolation (state.cpp:56-58)");
+  const int k = c->host_red[BR_N];
+  RC(ensure_clusters(c, std::max(k, 1)));
+  c->n_clusters = k;
+  c->clusters_on_device = true;
+  c->ccd_valid = false;
+  if (n_clusters_out) *n_clusters_out = k;
+  return D2_OK;
+}
+
+int d2_clusters_download(d2_ctx* c, int32_t* cluster_of_pair, int32_t* cluster_of_body, int32_t* first,
+                         double* min_time, double* step) {
+  if (!c) return D2_ERR_ARG;
+  if (!c->clusters_on_device) return fail(c, D2_ERR_STATE, "d2_clusters_download needs d2_clusters_build");
+  const int n = c->n_bodies, k = c->n_clusters;
+  if (cluster_of_pair && c->n_pairs > 0)
+    HIPCHK(c, hipMemcpyAsync(cluster_of_pair, c->cluster_of_pair.p, sizeof(int32_t) * c->n_pairs, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (cluster_of_body)
+    HIPCHK(c, hipMemcpyAsync(cluster_of_body, c->cc_of_body.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->stream));
+  if (first && k) HIPCHK(c, hipMemcpyAsync(first, c->cc_first.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost, c->stream));
+  std::vector<long long> tm(min_time ? k : 0);
+  std::vector<unsigned long long> sm(step ? k : 0);
+  if (min_time && k) HIPCHK(c, hipMemcpyAsync(tm.data(), c->cc_tmin.p, sizeof(long long) * k, hipMemcpyDeviceToHost, c->stream));
+  if (step && k) HIPCHK(c, hipMemcpyAsync(sm.data(), c->cc_smin.p, sizeof(long long) * k, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < (min_time ? k : 0); ++i) {
+    long long o = tm[i];
+    o = o >= 0 ? o : o ^ 0x7fffffffffffffffll;
+    std::memcpy(&min_time[i], &o, sizeof(double));
+  }
+  for (int i = 0; i < (step ? k : 0); ++i) std::memcpy(&step[i], &sm[i], sizeof(double));
   return D2_OK;
 }
 

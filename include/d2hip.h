@@ -183,13 +183,63 @@ int d2_clusters_upload(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_cluste
  *      time_step_selection_dynamic_continuous.cpp:17-21), written to
  *      cluster_step_out[n_clusters] when it is not NULL;
  *   3. future = current.extrapolate(cluster step) for every non-static body
- *      in a pair, dt for the others (PDE_explicit.cpp:49-56; static bodies
- *      keep their future state);
+ *      in a pair; a body in no pair is a cluster of its own and steps with
+ *      its clamped last step (PDE_explicit.cpp:49-56, separate_clusters.cpp:
+ *      594-596; static bodies keep their future state);
  *   4. the discrete narrow phase and, when filter != 0, filterContacts.
  * Afterwards d2_result_contacts returns the discrete contacts and
  * d2_result_pairs the CCD values of step 1.  The bodies' future states are
  * overwritten (as in the reference); current states and last steps are not. */
 int d2_step_run(d2_ctx* ctx, float dt, int filter, double* cluster_step_out, int64_t* n_contacts_out);
+
+/* ---- broad phase and clusters -------------------------------------------- */
+/* Per-tree mesh bounds the broad phase reads: the float AABB (lo xyz, hi xyz)
+ * and the bounding radius of MeshData::initBoudingData (model/mesh.cpp:43-50,
+ * getAABB :132-153, getSphereBounds :155-158, getBoundingRadius :160-162).
+ * d2_trees_upload computes them from the leaf triangles (every vertex of a
+ * mesh whose vertices all belong to a face); the upload overrides them. */
+int d2_tree_bounds_upload(d2_ctx* ctx, const float* aabb6, const double* radius);
+int d2_tree_bounds_download(d2_ctx* ctx, float* aabb6, double* radius);
+/* Particle::last_state of every body (14 doubles each, state layout; the
+ * individual-time broad phase and the cluster roll-back read it).  Until it is
+ * uploaded a body's last state equals its current state. */
+int d2_bodies_last_upload(d2_ctx* ctx, const double* last14);
+/* Independent scenes in one context (no reference counterpart): bodies of
+ * different groups never pair.  NULL puts every body in one group. */
+int d2_bodies_groups_upload(d2_ctx* ctx, const int32_t* group_of_body);
+
+/* Broad phase on the device: broadPhaseEmbree (collision_detection/
+ * broad_phase_embree.cpp:248-289) without Embree.  mode 0 is
+ * broadPhaseEmbree(particles, t) (boundsFunc :60-84, collideFunc :143-176);
+ * mode 1 is broadPhaseEmbree(particles) (boundsFuncIndividualTime :86-112,
+ * collideFuncIndividualTime :178-226; t unused).  Float bounding boxes that
+ * overlap (touching included) and pass the sphere test of
+ * sphereSphereFirstIntersect (continuous_comparison.h:12-36) give a pair
+ * (a, b), a < b, not both static, same group.  The pairs become the resident
+ * pair list (sorted by a, then b; Embree's order is not deterministic) with
+ * max_time = min(last_time_step_size of a, b) (separate_clusters.cpp:490).
+ * D2_ERR_RUNTIME when State::interpolate would throw (state.cpp:56-58). */
+int d2_broad_phase(d2_ctx* ctx, int mode, double t, int64_t* n_pairs_out);
+/* Copies the resident pair list (and its max_time when not NULL). */
+int d2_pairs_download(d2_ctx* ctx, int32_t* pairs, double* max_time);
+
+/* Clusters of the resident pairs on the device: separateCollisionClusters
+ * (collision_detection/separate_clusters.cpp:355-459) without sleeping
+ * bodies.  Connected components of the pairs between two non-static bodies,
+ * numbered as libigl's connected_components numbers them (by smallest body
+ * index), components without a non-static body dropped; a pair belongs to the
+ * cluster of its non-static body.  Every non-static member is rolled back to
+ * its cluster's earliest current time (Particle::rollBackState,
+ * model/particle.cpp:96-98).  Afterwards d2_step_run uses these clusters; a
+ * cluster without pairs steps with its first particle's clamped last step.
+ * D2_ERR_ARG for a pair of two static bodies (it belongs to no cluster). */
+int d2_clusters_build(d2_ctx* ctx, int* n_clusters_out);
+/* Any output may be NULL: cluster of each resident pair, of each body (-1
+ * for static bodies), and per cluster its first particle, earliest current
+ * time and smallest last_time_step_size (Cluster::min_current_time /
+ * step_size before time-step selection). */
+int d2_clusters_download(d2_ctx* ctx, int32_t* cluster_of_pair, int32_t* cluster_of_body, int32_t* first,
+                         double* min_time, double* step);
 
 /* ---- learned pair gate (no reference counterpart) ------------------------
  * A 24 -> 64 -> 64 -> 2 MLP (ReLU; bf16 weights, FP32 accumulation on MFMA)

@@ -85,6 +85,15 @@ def lib():
         L.d2o_scene_filter_result.restype = ctypes.c_int
         L.d2o_step_scale.argtypes = [_dp, ctypes.c_int, _dp, _dp, ctypes.c_double]
         L.d2o_step_scale.restype = ctypes.c_double
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.d2o_mesh_bounds.argtypes = [_dp, ctypes.c_int, _fp, _dp]
+        L.d2o_broad_phase.argtypes = [ctypes.c_int, _dp, _dp, _fp, _dp, _ip, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_int, _ip]
+        L.d2o_broad_phase.restype = ctypes.c_int
+        L.d2o_clusters.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
+        L.d2o_clusters.restype = ctypes.c_int
+        L.d2o_interpolate.argtypes = [_dp, _dp, ctypes.c_double, _dp]
+        L.d2o_interpolate.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -273,3 +282,70 @@ def step_scale(reals, body_a, body_b, max_time):
     """TOI -> step rule on explicit contact reals (n x 11)."""
     r = _d(reals)
     return lib().d2o_step_scale(_p(r), r.size // 11, _p(_d(body_a, 40)), _p(_d(body_b, 40)), float(max_time))
+
+
+# --- broad phase and clusters ------------------------------------------------
+def _opt(a, dtype, ptype):
+    if a is None:
+        return None, None
+    a = np.ascontiguousarray(np.asarray(a, dtype=dtype).ravel())
+    return a, a.ctypes.data_as(ptype)
+
+
+def mesh_bounds(V):
+    """MeshData::initBoudingData: (float32 AABB [lo xyz, hi xyz], radius)."""
+    V = _d(V)
+    box = np.zeros(6, dtype=np.float32)
+    r = ctypes.c_double(0)
+    lib().d2o_mesh_bounds(_p(V), V.size // 3, box.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(r))
+    return box, r.value
+
+
+def broad_phase(records, aabb6, radius, mode, t=0.0, last=None, groups=None):
+    """broadPhaseEmbree by brute force: mode 0 = (particles, t), 1 = (particles).
+    aabb6/radius are per body.  Returns int32 pairs (a < b, sorted); raises
+    RuntimeError when State::interpolate throws."""
+    records = _d(records)
+    n = records.size // 40
+    last, lp = _opt(last, np.float64, _dp)
+    box, bp = _opt(aabb6, np.float32, ctypes.POINTER(ctypes.c_float))
+    rad, rp = _opt(radius, np.float64, _dp)
+    grp, gp = _opt(groups, np.int32, _ip)
+    cap = 1 << 16
+    while True:
+        out = np.zeros(2 * cap, dtype=np.int32)
+        k = lib().d2o_broad_phase(n, _p(records), lp, bp, rp, gp, int(mode), float(t), cap, out.ctypes.data_as(_ip))
+        if k < 0:
+            raise RuntimeError("Invalid args for interpolation")
+        if k <= cap:
+            return out[:2 * k].reshape(-1, 2)
+        cap = k
+
+
+def clusters(records, pairs, last=None):
+    """separateCollisionClusters: dict with n, of_body, of_pair, first,
+    min_time, step and the rolled-back current states (n x 14)."""
+    records = _d(records)
+    n = records.size // 40
+    pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int32).reshape(-1, 2))
+    last, lp = _opt(last, np.float64, _dp)
+    ob = np.zeros(n, dtype=np.int32)
+    op = np.zeros(max(pairs.shape[0], 1), dtype=np.int32)
+    first = np.zeros(n, dtype=np.int32)
+    mt = np.zeros(n)
+    st = np.zeros(n)
+    cur = np.zeros(14 * n)
+    k = lib().d2o_clusters(n, _p(records), lp, pairs.shape[0], pairs.ctypes.data_as(_ip), ob.ctypes.data_as(_ip),
+                           op.ctypes.data_as(_ip), first.ctypes.data_as(_ip), _p(mt), _p(st), _p(cur))
+    if k < 0:
+        raise RuntimeError("Invalid args for interpolation")
+    return {"n": k, "of_body": ob, "of_pair": op[:pairs.shape[0]], "first": first[:k], "min_time": mt[:k],
+            "step": st[:k], "current": cur.reshape(n, 14)}
+
+
+def interpolate(cur14, last14, time):
+    """State::interpolate; raises RuntimeError when the reference throws."""
+    out = np.zeros(14)
+    if lib().d2o_interpolate(_p(_d(cur14)), _p(_d(last14)), float(time), _p(out)) < 0:
+        raise RuntimeError("Invalid args for interpolation")
+    return out

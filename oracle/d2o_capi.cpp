@@ -20,6 +20,7 @@
 #include <thread>
 #include <atomic>
 
+#include "d2o_broad.hpp"
 #include "d2o_tree.hpp"
 
 using namespace d2o;
@@ -561,4 +562,91 @@ extern "C" double d2o_step_scale(const double* reals, int n, const double* body_
     cs[i].toc = r[10];
   }
   return pairStepScale(cs, a, b, max_time);
+}
+
+// ---------------------------------------------------- broad phase, clusters --
+// Body inputs of d2o_broad_phase / d2o_clusters: n body records (40 doubles),
+// last states (14 doubles each, state layout; NULL: last = current), per body
+// the mesh's float AABB (lo xyz, hi xyz) and bounding radius, and a group per
+// body (NULL: one group).
+namespace {
+std::vector<BpBody> bpBodies(int n, const double* records, const double* last14, const float* aabb6,
+                             const double* radius, const int32_t* groups) {
+  std::vector<BpBody> bs(n);
+  for (int i = 0; i < n; ++i) {
+    const double* r = records + 40 * (size_t)i;
+    BpBody& b = bs[i];
+    b.current = stateFrom(r);
+    b.future = stateFrom(r + 14);
+    b.last = last14 ? stateFrom(last14 + 14 * (size_t)i) : b.current;
+    b.geo_eps = r[37];
+    b.is_static = r[38] != 0.0;
+    b.last_time_step_size = r[39];
+    if (aabb6)
+      for (int k = 0; k < 3; ++k) b.mesh.aabb.lo[k] = aabb6[6 * i + k], b.mesh.aabb.hi[k] = aabb6[6 * i + 3 + k];
+    b.mesh.radius = radius ? radius[i] : 0.0;
+    b.group = groups ? groups[i] : 0;
+  }
+  return bs;
+}
+}  // namespace
+
+// MeshData::initBoudingData (model/mesh.cpp:43-50) of one vertex list.
+extern "C" void d2o_mesh_bounds(const double* Vp, int nv, float* aabb6, double* radius) {
+  std::vector<V> vs(nv);
+  for (int i = 0; i < nv; ++i) vs[i] = V(Vp[3 * i], Vp[3 * i + 1], Vp[3 * i + 2]);
+  MeshBounds m = meshBounds(vs);
+  for (int k = 0; k < 3; ++k) aabb6[k] = m.aabb.lo[k], aabb6[3 + k] = m.aabb.hi[k];
+  *radius = m.radius;
+}
+
+// broadPhaseEmbree: mode 0 = (particles, t), mode 1 = (particles).  Writes at
+// most max_pairs pairs (a < b, sorted) and returns their total count, or -1
+// when State::interpolate throws.
+extern "C" int d2o_broad_phase(int n, const double* records, const double* last14, const float* aabb6,
+                               const double* radius, const int32_t* groups, int mode, double t, int max_pairs,
+                               int32_t* pairs_out) {
+  std::vector<BpBody> bs = bpBodies(n, records, last14, aabb6, radius, groups);
+  std::vector<std::pair<int, int>> ps;
+  try {
+    ps = broadPhase(bs, mode, t);
+  } catch (const std::runtime_error&) {
+    return -1;
+  }
+  for (int i = 0; i < (int)ps.size() && i < max_pairs; ++i) pairs_out[2 * i] = ps[i].first, pairs_out[2 * i + 1] = ps[i].second;
+  return (int)ps.size();
+}
+
+// separateCollisionClusters over a pair list.  Outputs (arrays sized n_bodies
+// or n_pairs): cluster of each body (-1 static) and pair (-1 static-static),
+// per cluster its first particle, smallest current time and smallest last
+// step; the current states after the roll-back (14 doubles per body).
+// Returns the cluster count, or -1 when State::interpolate throws.
+extern "C" int d2o_clusters(int n, const double* records, const double* last14, int n_pairs, const int32_t* pairs,
+                            int32_t* of_body, int32_t* of_pair, int32_t* first, double* min_time, double* step,
+                            double* current14_out) {
+  std::vector<BpBody> bs = bpBodies(n, records, last14, nullptr, nullptr, nullptr);
+  std::vector<std::pair<int, int>> ps(n_pairs);
+  for (int p = 0; p < n_pairs; ++p) ps[p] = {pairs[2 * p], pairs[2 * p + 1]};
+  Clusters C;
+  try {
+    C = separateClusters(bs, ps);
+  } catch (const std::runtime_error&) {
+    return -1;
+  }
+  for (int i = 0; i < n; ++i) of_body[i] = C.of_body[i], stateTo(bs[i].current, current14_out + 14 * (size_t)i);
+  for (int p = 0; p < n_pairs; ++p) of_pair[p] = C.of_pair[p];
+  for (int c = 0; c < C.n; ++c) first[c] = C.first[c], min_time[c] = C.min_current_time[c], step[c] = C.step_size[c];
+  return C.n;
+}
+
+// State::interpolate (model/state.cpp:55-70) of one state; returns 0, or -1
+// when it throws.
+extern "C" int d2o_interpolate(const double* cur14, const double* last14, double time, double* out14) {
+  try {
+    stateTo(interpolate(stateFrom(cur14), stateFrom(last14), time), out14);
+  } catch (const std::runtime_error&) {
+    return -1;
+  }
+  return 0;
 }

@@ -74,8 +74,9 @@ def test_step_matches_oracle(ctx, name, make):
     osc = oracle_scene(s, rec)
     want = np.array([osc.cluster_step(pairs[cl == j], np.float32(scenes.DT)) for j in range(k)])
     assert np.allclose(steps, want, rtol=1e-12, atol=0), (steps, want)
-    # future states: cluster step for bodies in pairs, dt for the others
-    body_step = np.full(s.n_bodies, float(np.float32(scenes.DT)))
+    # future states: cluster step for bodies in pairs; a body in no pair is a
+    # cluster of its own and steps with its clamped last step
+    body_step = np.clip(rec[:, 39], 1e-3, float(np.float32(scenes.DT)))
     body_step[pairs[:, 0]] = steps[cl]
     body_step[pairs[:, 1]] = steps[cl]
     got_bodies = ctx.download_bodies()
