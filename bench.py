@@ -260,6 +260,58 @@ def timed_steps(ctx, dt, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def end_to_end(ctx, scene, clusters, steps=5):
+    """The same step with host buffers on both sides: bodies, pairs and
+    clusters uploaded, contacts and per-pair CCD values downloaded."""
+    from delta2_amd import scenes
+    cl, n_cl = clusters
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.upload_bodies(scene.tree_of_body, scene.records)
+        ctx.upload_pairs(scene.pairs)
+        ctx.upload_clusters(cl, n_cl)
+        ctx.step_run(scenes.DT, True)
+        ctx.result(with_pairs=True)
+    sec = (time.perf_counter() - t0) / steps
+    return {"value": scene.pairs.shape[0] / sec, "unit": "pair queries/s", "ms_per_step": 1e3 * sec, "steps": steps,
+            "includes": "H2D of body records, pairs, clusters; D2H of contacts and per-pair CCD values"}
+
+
+def device_pipeline(scene, prec, weights, steps=5):
+    """Broad phase, cluster separation and the step, all on the device, over
+    the scene's bodies (the pair list never leaves the GPU)."""
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    ctx = d2.Context(0, prec)
+    ctx.upload_trees(scene.trees)
+    ctx.upload_bodies(scene.tree_of_body, scene.records)
+    ctx.project_future(np.minimum(2.0 * scene.records[:, 39], scenes.DT))  # broad_phase_embree_cluster.cpp:121-127
+    if weights is not None:
+        ctx.mlp_set_params(weights)
+        ctx.set_learned(True)
+    ctx.set_graph(True)
+    t_bp = t_cl = t_st = 0.0
+    for k in range(steps + 1):
+        t0 = time.perf_counter()
+        n_pairs = ctx.broad_phase(mode=1)
+        t1 = time.perf_counter()
+        n_cl = ctx.build_clusters()
+        t2 = time.perf_counter()
+        ctx.step_run(scenes.DT, True)
+        ctx.sync()
+        t3 = time.perf_counter()
+        if k:  # the first step sizes buffers and captures graphs
+            t_bp += t1 - t0
+            t_cl += t2 - t1
+            t_st += t3 - t2
+    ctx.close()
+    ms = lambda x: 1e3 * x / steps  # noqa: E731
+    total = ms(t_bp + t_cl + t_st)
+    return {"bodies": scene.n_bodies, "pairs": n_pairs, "clusters": n_cl, "broad_phase_ms": ms(t_bp),
+            "clusters_ms": ms(t_cl), "step_ms": ms(t_st), "total_ms": total,
+            "bodies_per_s_broad_phase": scene.n_bodies / (t_bp / steps), "pair_queries_per_s": n_pairs / (total / 1e3)}
+
+
 def run_c3(args):
     """Batched workload: 256 independent scenes (16-512 bodies, C1 meshes).
     Variants on the same inputs, each a full step over every scene:
@@ -444,6 +496,12 @@ def main():
         tf = ALU_OPS[dom] * items / n / avg_s / 1e12
         roofline["alu"] = {"achieved_tflops_est": tf, "peak_fp32_tflops": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
                            "ops_per_unit_est": ALU_OPS[dom]}
+    ctx.set_graph(args.graph == "on")
+    e2e = end_to_end(ctx, scene, clusters)
+    log(f"[bench] end to end (host buffers): {e2e['ms_per_step']:.3f} ms/step")
+    dev = device_pipeline(scene, prec, weights)
+    log(f"[bench] device pipeline: broad phase {dev['broad_phase_ms']:.3f} ms ({dev['pairs']} pairs), clusters "
+        f"{dev['clusters_ms']:.3f} ms ({dev['clusters']}), step {dev['step_ms']:.3f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         t0 = time.time()
@@ -464,7 +522,7 @@ def main():
                    if scene.name == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60",
                    "bodies": scene.n_bodies, "triangles": scene.n_triangles(), "pairs": int(pairs.shape[0]),
                    "precision": args.precision},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "end_to_end": e2e, "device_pipeline": dev,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
