@@ -269,11 +269,6 @@ __device__ __forceinline__ bool bp_collide(int a, int b, int mode, double t, con
 // one atomic and records the chunk position, the count and whether the chunk
 // is sorted (a static body's sweep and a list that fit in LDS are; a longer
 // list is placed by rank when it is gathered).
-__device__ __forceinline__ void wave_sync_lds() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 __device__ __forceinline__ void lds_rank_sort(int* buf, int cnt) {
   // distinct body indices: a value's rank is the number of smaller values
   constexpr int PER = BP_WAVE_CAP / 64;

@@ -38,6 +38,17 @@ template <class R> D2_HD v3<R> unit(v3<R> a) {
   R z = dot(a, a);
   return z > R(0) ? dvs(a, (R)sqrt(z)) : a;
 }
+// vlen(d) < search without the square root in all but borderline cases: the
+// squared comparison decides whenever |d|^2 is more than 1e-12 (relative)
+// away from search^2, far beyond the rounding of either side.
+D2_HD bool near_mid(const v3<double>& d, double search) {
+  const double s2 = dot(d, d), q = search * search;
+  if (q > 1e-280 && q < 1e280) {
+    if (s2 < q * (1.0 - 1e-12)) return true;
+    if (s2 > q * (1.0 + 1e-12)) return false;
+  }
+  return sqrt(s2) < search;
+}
 template <class R> D2_HD bool same(v3<R> a, v3<R> b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
 template <class R> D2_HD R clamp01(R t) { return t < R(0) ? R(0) : (t > R(1) ? R(1) : t); }
 template <class R> D2_HD v3<R> lerp(v3<R> a, v3<R> b, R t) { return add(scl(a, R(1) - t), scl(b, t)); }

@@ -96,6 +96,13 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+// Orders one wavefront's LDS writes before its later LDS reads (the other
+// wavefronts of the block are not waited for).
+__device__ __forceinline__ void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 __device__ __forceinline__ unsigned long long lanes_below() { return (1ull << lane_id()) - 1ull; }
 
 // One atomic per wavefront; returns this lane's slot (or -1 when !pred).
@@ -1514,10 +1521,12 @@ __device__ __forceinline__ bool lds_less(const LdsKey& x, const LdsKey& y) {
   if (x.eps == y.eps) return x.dist < y.dist;
   return x.eps < y.eps;
 }
-struct LdsSorter {  // Sorter over LDS keys (same algorithm, LDS operands)
+// Sorter over LDS operands (same algorithm): keys are integer ranks, equal
+// exactly when the (eps, dist) keys are equal, so comparisons are unchanged.
+struct LdsSorter {
   int* a;
-  const LdsKey* k;
-  __device__ bool lt(int x, int y) const { return lds_less(k[x], k[y]); }
+  const int* k;
+  __device__ bool lt(int x, int y) const { return k[x] < k[y]; }
   __device__ void swp(int i, int j) {
     int t = a[i];
     a[i] = a[j];
@@ -1640,25 +1649,25 @@ struct LdsSorter {  // Sorter over LDS keys (same algorithm, LDS operands)
 // K = #{k : A_k < B_k}, and returns min(A_K, B_{K-1}).  The closing insertion
 // sort never moves an element across a leaf segment's border, so it equals a
 // stable sort of every leaf segment; heapsort (depth exhausted) stays on lane 0.
-__device__ int wave_partition(int lo, int hi, int* ord, const LdsKey* sk, int* Apos, int* Bpos) {
+__device__ int wave_partition(int lo, int hi, int* ord, const int* rk, int* Apos, int* Bpos) {
   const int lane = lane_id();
-  const LdsKey P = sk[ord[lo - 1]];
+  const int P = rk[ord[lo - 1]];
   int nA = 0, nB = 0;
   for (int base = lo; base < hi; base += 64) {
     const int i = base + lane;
-    const bool ge = i < hi && !lds_less(sk[ord[i]], P);
+    const bool ge = i < hi && !(rk[ord[i]] < P);
     const unsigned long long m = __ballot(ge);
     if (ge) Apos[nA + __popcll(m & lanes_below())] = i;
     nA += __popcll(m);
   }
   for (int top = hi - 1; top >= lo - 1; top -= 64) {
     const int i = top - lane;
-    const bool le = i >= lo - 1 && !lds_less(P, sk[ord[i]]);
+    const bool le = i >= lo - 1 && !(P < rk[ord[i]]);
     const unsigned long long m = __ballot(le);
     if (le) Bpos[nB + __popcll(m & lanes_below())] = i;
     nB += __popcll(m);
   }
-  __syncthreads();
+  wave_sync_lds();
   const int lim = nA < nB ? nA : nB;
   int K = 0;
   for (int base = 0; base < lim; base += 64) {  // A_k < B_k holds for a prefix of k
@@ -1674,13 +1683,13 @@ __device__ int wave_partition(int lo, int hi, int* ord, const LdsKey* sk, int* A
     ord[Apos[q]] = y;
     ord[Bpos[q]] = x;
   }
-  __syncthreads();
+  wave_sync_lds();
   return ak < bk1 ? ak : bk1;
 }
 
 // ord[0, n) := the order std::sort gives.  mark, Apos, Bpos hold n ints each,
 // stk 3 * 64 ints (the segment stack is at most 2*log2(n)+1 deep).
-__device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int* Bpos, int* mark, int* stk) {
+__device__ void wave_std_sort(int n, int* ord, const int* rk, int* Apos, int* Bpos, int* mark, int* stk) {
   const int lane = lane_id();
   for (int i = lane; i < n; i += 64) mark[i] = 0;
   if (lane == 0) stk[0] = 0, stk[1] = n, stk[2] = 2 * (63 - __clzll((long long)(n > 0 ? n : 1)));
@@ -1695,7 +1704,7 @@ __device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int*
     while (l - f > 16) {
       if (d == 0) {
         if (lane == 0) {
-          LdsSorter srt{ord, sk};
+          LdsSorter srt{ord, rk};
           srt.heap_sort(f, l);
         }
         heaped = true;
@@ -1705,10 +1714,10 @@ __device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int*
       --d;
       {  // median of three to the front (uniform decision, lane 0 swaps)
         const int x = f + 1, y = f + (l - f) / 2, z = l - 1;
-        const LdsKey kx = sk[ord[x]], ky = sk[ord[y]], kz = sk[ord[z]];
+        const int kx = rk[ord[x]], ky = rk[ord[y]], kz = rk[ord[z]];
         int s;
-        if (lds_less(kx, ky)) s = lds_less(ky, kz) ? y : (lds_less(kx, kz) ? z : x);
-        else s = lds_less(kx, kz) ? x : (lds_less(ky, kz) ? z : y);
+        if (kx < ky) s = ky < kz ? y : (kx < kz ? z : x);
+        else s = kx < kz ? x : (ky < kz ? z : y);
         __syncthreads();
         if (lane == 0) {
           const int t = ord[f];
@@ -1717,7 +1726,7 @@ __device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int*
         }
         __syncthreads();
       }
-      const int cut = wave_partition(f + 1, l, ord, sk, Apos, Bpos);
+      const int cut = wave_partition(f + 1, l, ord, rk, Apos, Bpos);
       if (lane == 0) stk[3 * sp] = cut, stk[3 * sp + 1] = l, stk[3 * sp + 2] = d;
       ++sp;
       l = cut;
@@ -1734,11 +1743,11 @@ __device__ void wave_std_sort(int n, int* ord, const LdsKey* sk, int* Apos, int*
     if (mark[f] == 1) {
       int l = j + 1;
       while (l < n && mark[l] == 0) ++l;
-      const LdsKey kj = sk[ord[j]];
+      const int kj = rk[ord[j]];
       int r = 0;
       for (int i = f; i < l; ++i) {
-        const LdsKey ki = sk[ord[i]];
-        r += lds_less(ki, kj) || (i < j && !lds_less(kj, ki));
+        const int ki = rk[ord[i]];
+        r += ki < kj || (i < j && !(kj < ki));
       }
       dst = f + r;
     }
@@ -1764,6 +1773,7 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int*
   __shared__ int ord[POST_CAP];
   __shared__ double smid[POST_CAP * 3];
   __shared__ int kept_list[POST_CAP];
+  __shared__ int rk[POST_CAP];
   __shared__ int stk[3 * 64];
   const int lane = threadIdx.x;
   const int n_items = list ? *n_list : n_pairs;
@@ -1869,41 +1879,348 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int*
           }
           __syncthreads();
         }
+      // dense integer rank of every key (equal keys share a rank) from the
+      // sorted order; a repeated key means std::sort's order among the tied
+      // contacts must be replayed
       bool tie = false;
-      for (int i = lane; i + 1 < n; i += 64) {
-        const LdsKey x = sk[ord[i]], y = sk[ord[i + 1]];
-        tie = tie || (x.eps == y.eps && x.dist == y.dist);
+      int run = 0;
+      for (int t0 = 0; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        bool step = false;
+        if (t < n && t > 0) {
+          const LdsKey x = sk[ord[t - 1]], y = sk[ord[t]];
+          step = !(x.eps == y.eps && x.dist == y.dist);
+          tie = tie || !step;
+        }
+        const unsigned long long m = __ballot(step);
+        if (t < n) rk[ord[t]] = run + __popcll(m & (lanes_below() | (1ull << lane)));
+        run += __popcll(m);
       }
+      __syncthreads();
       if (__ballot(tie)) {
-        __syncthreads();
         for (int i = lane; i < n; i += 64) ord[i] = i;
         __syncthreads();
         // skey (no longer needed) holds the two rank lists
-        wave_std_sort(n, ord, sk, reinterpret_cast<int*>(skey), reinterpret_cast<int*>(skey) + POST_CAP, kept_list,
+        wave_std_sort(n, ord, rk, reinterpret_cast<int*>(skey), reinterpret_cast<int*>(skey) + POST_CAP, kept_list,
                       stk);
       }
-      // greedy pass in sorted order
+      // greedy pass in sorted order, 64 candidates at a time: each lane tests
+      // its candidate against the contacts kept before the block and against
+      // the earlier candidates of the block; the block's decisions are then
+      // made in order on the wave's scalar unit
       int nk = 0;
-      for (int t = 0; t < n; ++t) {
-        int ci = ord[t];
-        double mx = smid[3 * ci], my = smid[3 * ci + 1], mz = smid[3 * ci + 2];
-        double search = sk[ci].eps + 0.0;
+      for (int t0 = 0; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        const bool valid = t < n;
+        const int ci = valid ? ord[t] : 0;
+        const double mx = smid[3 * ci], my = smid[3 * ci + 1], mz = smid[3 * ci + 2];
+        const double search = sk[ci].eps + 0.0;
         bool near = false;
-        for (int q = lane; q < nk; q += 64) {
-          int h = kept_list[q];
-          v3<double> dd{mx - smid[3 * h], my - smid[3 * h + 1], mz - smid[3 * h + 2]};
-          near = near || (vlen(dd) < search);
+        for (int q0 = 0; q0 < nk; q0 += 64) {
+          const int h = q0 + lane < nk ? kept_list[q0 + lane] : 0;
+          const double hx = smid[3 * h], hy = smid[3 * h + 1], hz = smid[3 * h + 2];
+          const int m = nk - q0 < 64 ? nk - q0 : 64;
+          for (int u = 0; u < m; ++u) {
+            const v3<double> dd{mx - __shfl(hx, u), my - __shfl(hy, u), mz - __shfl(hz, u)};
+            near = near || near_mid(dd, search);
+          }
         }
-        bool fresh = __ballot(near) == 0ull;
-        if (fresh && lane == 0) kept_list[nk] = ci;
-        if (lane == 0) keep[b + t] = fresh;
-        nk += fresh;
+        unsigned long long conf = 0;  // bit u: an earlier candidate u of the block is near this one
+        const int m = n - t0 < 64 ? n - t0 : 64;
+        for (int u = 0; u < m; ++u) {
+          const v3<double> dd{mx - __shfl(mx, u), my - __shfl(my, u), mz - __shfl(mz, u)};
+          if (u < lane && near_mid(dd, search)) conf |= 1ull << u;
+        }
+        const unsigned long long blocked = __ballot(near);
+        unsigned long long kept = 0;
+        for (int i = 0; i < m; ++i) {
+          const unsigned long long ci_conf = __shfl(conf, i);
+          if (!((blocked >> i) & 1ull) && !(ci_conf & kept)) kept |= 1ull << i;
+        }
+        const bool fresh = (kept >> lane) & 1ull;
+        if (valid) keep[b + t] = fresh;
+        if (fresh) kept_list[nk + __popcll(kept & lanes_below())] = ci;
+        nk += __popcll(kept);
         __syncthreads();
       }
       for (int i = lane; i < n; i += 64) perm[b + i] = sidx[ord[i]];
       if (lane == 0) kept_cnt[p] = nk;
     } else {
       for (int i = lane; i < n; i += 64) perm[b + i] = sidx[i];
+    }
+    __syncthreads();
+  }
+}
+
+// ---- one block (4 wavefronts) per pair: pairs with at most BLK_CAP contacts
+constexpr int BLK_CAP = 512;
+constexpr int BLK_SEGS = 3 * 64;  // queued segments per round (> 16 elements each: at most BLK_CAP / 17)
+
+// ord[0, n) := the order std::sort gives (as wave_std_sort), with the
+// partitions of one round's segments spread over the block's wavefronts:
+// partitions of disjoint segments commute, and every segment keeps its own
+// depth budget, so the order equals the sequential introsort's.  Apos/Bpos
+// are indexed by position (a segment's lists stay inside its range).
+__device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* Bpos, int* mark, int* segs,
+                               int* s_cnt) {
+  const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+  for (int i = tid; i < n; i += blockDim.x) mark[i] = 0;
+  if (tid == 0) {  // thread 0 also zeroed mark[0] above
+    s_cnt[0] = 0, s_cnt[1] = 0;
+    if (n > 16) segs[0] = 0, segs[1] = n, segs[2] = 2 * (63 - __clzll((long long)n)), s_cnt[0] = 1;
+    else if (n > 0) mark[0] = 1;
+  }
+  __syncthreads();
+  if (n <= 1) return;
+  int cur = 0;
+  for (;;) {
+    const int m = s_cnt[cur];
+    if (m == 0) break;
+    const int* L = segs + cur * BLK_SEGS;
+    int* N = segs + (cur ^ 1) * BLK_SEGS;
+    for (int q = w; q < m; q += (int)(blockDim.x >> 6)) {
+      const int f = L[3 * q], l = L[3 * q + 1];
+      int d = L[3 * q + 2];
+      if (d == 0) {  // depth exhausted: heapsort (libstdc++ __partial_sort) on one lane
+        if (lane == 0) {
+          LdsSorter srt{ord, rk};
+          srt.heap_sort(f, l);
+          mark[f] = 2;
+        }
+        wave_sync_lds();
+        continue;
+      }
+      --d;
+      {  // median of three to the front
+        const int x = f + 1, y = f + (l - f) / 2, z = l - 1;
+        const int kx = rk[ord[x]], ky = rk[ord[y]], kz = rk[ord[z]];
+        int sel;
+        if (kx < ky) sel = ky < kz ? y : (kx < kz ? z : x);
+        else sel = kx < kz ? x : (ky < kz ? z : y);
+        wave_sync_lds();
+        if (lane == 0) {
+          const int t = ord[f];
+          ord[f] = ord[sel];
+          ord[sel] = t;
+        }
+        wave_sync_lds();
+      }
+      const int cut = wave_partition(f + 1, l, ord, rk, Apos + f, Bpos + f);
+      if (lane == 0) {
+        // left part [f, cut) and right part [cut, l), both with depth d
+        const int cf[2] = {f, cut}, cl[2] = {cut, l};
+        for (int k = 0; k < 2; ++k) {
+          if (cl[k] - cf[k] > 16) {
+            const int slot = atomicAdd(&s_cnt[cur ^ 1], 1);
+            N[3 * slot] = cf[k], N[3 * slot + 1] = cl[k], N[3 * slot + 2] = d;
+          } else if (cl[k] > cf[k]) {
+            mark[cf[k]] = 1;
+          }
+        }
+      }
+      wave_sync_lds();
+    }
+    __syncthreads();
+    if (tid == 0) s_cnt[cur] = 0;
+    cur ^= 1;
+    __syncthreads();
+  }
+  // leaf segments tile [0, n): stable-sort each (<= 16 elements)
+  for (int j = tid; j < n; j += blockDim.x) {
+    int f = j;
+    while (mark[f] == 0) --f;
+    int dst = j;
+    if (mark[f] == 1) {
+      int l = j + 1;
+      while (l < n && mark[l] == 0) ++l;
+      const int kj = rk[ord[j]];
+      int r = 0;
+      for (int i = f; i < l; ++i) {
+        const int ki = rk[ord[i]];
+        r += ki < kj || (i < j && !(kj < ki));
+      }
+      dst = f + r;
+    }
+    Bpos[dst] = ord[j];
+  }
+  __syncthreads();
+  for (int j = tid; j < n; j += blockDim.x) ord[j] = Bpos[j];
+  __syncthreads();
+}
+
+// The first post-processing pass (as k_pair_finish_wave, same results) with a
+// 256-thread block per pair: orders by rank counting instead of sorting
+// networks (emission keys are unique; (eps, dist) ranks equal exactly for tied
+// keys), the tie replay over the block's wavefronts, and the greedy pass of
+// filterContacts 64 candidates at a time with the distance tests spread over
+// the wavefronts and the decisions taken in order by the first one.
+__global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const ContactRec* c, const int* pair_off,
+                                                           int* perm, int mode, int* keep, int* kept_cnt,
+                                                           const double* pair_rec, double* scale, int* over,
+                                                           int* n_over) {
+  __shared__ unsigned long long skey[BLK_CAP];  // emission keys, then the partition lists of the tie replay
+  __shared__ int sidx[BLK_CAP];
+  __shared__ int eo[BLK_CAP];  // contact at each emission position
+  __shared__ LdsKey sk[BLK_CAP];
+  __shared__ double smid[3 * BLK_CAP];
+  __shared__ int rk[BLK_CAP];
+  __shared__ int ord[BLK_CAP];
+  __shared__ int mark[BLK_CAP];  // leaf marks of the tie replay, then the kept list
+  __shared__ int segs[2 * BLK_SEGS];
+  __shared__ unsigned long long conf_w[4][64];
+  __shared__ int near_w[4][64];
+  __shared__ double s_min[4];
+  __shared__ int s_misc[4];
+  const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
+  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+    const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
+    if (n > BLK_CAP) {
+      if (tid == 0) over[atomicAdd(n_over, 1)] = p;
+      continue;
+    }
+    for (int i = tid; i < n; i += 256) {
+      const int idx = perm[b + i];
+      sidx[i] = idx;
+      skey[i] = c[idx].key;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += 256) {  // emission position = rank of the (unique) key
+      const unsigned long long k = skey[i];
+      int r = 0;
+      for (int j = 0; j < n; ++j) r += skey[j] < k ? 1 : 0;
+      eo[r] = sidx[i];
+    }
+    __syncthreads();
+    if (mode == 2) {  // step rule, separate_clusters.cpp:496-564
+      const double* pr = pair_rec + (size_t)PR_STRIDE * p;
+      double mt = pr[PR_MAXT], ms = 1.0;
+      v3<double> va{pr[PR_VA], pr[PR_VA + 1], pr[PR_VA + 2]}, wa{pr[PR_WA], pr[PR_WA + 1], pr[PR_WA + 2]};
+      v3<double> xa{pr[PR_XA], pr[PR_XA + 1], pr[PR_XA + 2]};
+      v3<double> vb{pr[PR_VB], pr[PR_VB + 1], pr[PR_VB + 2]}, wb{pr[PR_WB], pr[PR_WB + 1], pr[PR_WB + 2]};
+      v3<double> xb{pr[PR_XB], pr[PR_XB + 1], pr[PR_XB + 2]};
+      for (int i = tid; i < n; i += 256) {
+        const ContactRec& ct = c[eo[i]];
+        v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
+        v3<double> nrm = sub(A, B);
+        double inter = ct.r[6] + ct.r[7], toc = ct.r[10];
+        v3<double> rel = sub(add(va, cross(wa, sub(A, xa))), add(vb, cross(wb, sub(B, xb))));
+        double len = vlen(nrm);
+        if (!(dot(unit(nrm), unit(rel)) <= 0.0 || len < 1e-4)) continue;
+        double sc;
+        if (len < 1e-4) {
+          if (toc == 0.0) continue;
+          double st = toc - (vlen(rel) * mt) / inter;
+          st = st > 0.0 ? st : 0.0;
+          sc = lerp(st, toc, 0.75);
+        } else {
+          if (!(toc < 1.0)) continue;
+          double pv = -(dot(unit(nrm), rel));
+          double depth = inter - len;
+          double pd = (pv * toc) * mt;
+          if (!(pd > 1e-6)) continue;
+          if (depth < pd) {
+            sc = lerp(toc * (1.0 - depth / pd), toc, 0.5);
+          } else {
+            double fp = toc + ((1.0 - toc) * inter) / (pv * mt);
+            sc = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);
+          }
+        }
+        if (sc < ms) ms = sc;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(ms, off);
+        if (o < ms) ms = o;
+      }
+      if (lane == 0) s_min[w] = ms;
+      __syncthreads();
+      if (tid == 0) {
+        double m = s_min[0];
+        for (int k = 1; k < 4; ++k) m = (s_min[k] < m) ? s_min[k] : m;
+        scale[p] = m;
+      }
+      for (int i = tid; i < n; i += 256) perm[b + i] = eo[i];
+    } else if (mode == 1) {  // filterContacts(.., 0.0)
+      for (int i = tid; i < n; i += 256) {
+        const ContactRec& ct = c[eo[i]];
+        const v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
+        sk[i].eps = ct.r[6] + ct.r[7];
+        sk[i].dist = vlen(sub(A, B));
+        const v3<double> m = dvs(add(A, B), 2.0);
+        smid[3 * i] = m.x, smid[3 * i + 1] = m.y, smid[3 * i + 2] = m.z;
+      }
+      if (tid == 0) s_misc[0] = 0;
+      __syncthreads();
+      bool tie = false;
+      for (int i = tid; i < n; i += 256) {  // rank of the (eps, dist) key; equal keys share it
+        const LdsKey x = sk[i];
+        int r = 0, eq = 0;
+        for (int j = 0; j < n; ++j) {
+          const LdsKey y = sk[j];
+          if (lds_less(y, x)) ++r;
+          else if (!lds_less(x, y)) ++eq;
+        }
+        rk[i] = r;
+        tie = tie || eq > 1;
+      }
+      if (tie) s_misc[0] = 1;
+      __syncthreads();
+      if (s_misc[0] == 0) {
+        for (int i = tid; i < n; i += 256) ord[rk[i]] = i;
+        __syncthreads();
+      } else {
+        for (int i = tid; i < n; i += 256) ord[i] = i;
+        __syncthreads();
+        int* lists = reinterpret_cast<int*>(skey);
+        block_std_sort(n, ord, rk, lists, lists + BLK_CAP, mark, segs, s_misc + 2);
+      }
+      // greedy pass in sorted order
+      int nk = 0;
+      for (int t0 = 0; t0 < n; t0 += 64) {
+        const int m = n - t0 < 64 ? n - t0 : 64;
+        const int t = t0 + lane;
+        const bool valid = t < n;
+        const int pi = valid ? ord[t] : 0;
+        const double mx = smid[3 * pi], my = smid[3 * pi + 1], mz = smid[3 * pi + 2];
+        const double search = sk[pi].eps + 0.0;
+        bool near = false;
+        for (int q0 = 0; q0 < nk; q0 += 64) {  // contacts kept before the block
+          const int h = q0 + lane < nk ? mark[q0 + lane] : 0;
+          const double hx = smid[3 * h], hy = smid[3 * h + 1], hz = smid[3 * h + 2];
+          const int mm = nk - q0 < 64 ? nk - q0 : 64;
+          for (int u = w; u < mm; u += 4) {
+            const v3<double> dd{mx - __shfl(hx, u), my - __shfl(hy, u), mz - __shfl(hz, u)};
+            near = near || near_mid(dd, search);
+          }
+        }
+        unsigned long long conf = 0;  // bit u: candidate u (< this one) of the block is near
+        for (int u = w; u < m; u += 4) {
+          const v3<double> dd{mx - __shfl(mx, u), my - __shfl(my, u), mz - __shfl(mz, u)};
+          if (u < lane && near_mid(dd, search)) conf |= 1ull << u;
+        }
+        near_w[w][lane] = near ? 1 : 0;
+        conf_w[w][lane] = conf;
+        __syncthreads();
+        if (w == 0) {
+          const bool nr = (near_w[0][lane] | near_w[1][lane] | near_w[2][lane] | near_w[3][lane]) != 0;
+          const unsigned long long cf = conf_w[0][lane] | conf_w[1][lane] | conf_w[2][lane] | conf_w[3][lane];
+          const unsigned long long blocked = __ballot(nr);
+          unsigned long long kept = 0;
+          for (int i = 0; i < m; ++i) {
+            const unsigned long long ci = __shfl(cf, i);
+            if (!((blocked >> i) & 1ull) && !(ci & kept)) kept |= 1ull << i;
+          }
+          const bool fresh = (kept >> lane) & 1ull;
+          if (valid) keep[b + t] = fresh ? 1 : 0;
+          if (fresh) mark[nk + __popcll(kept & lanes_below())] = pi;
+          if (lane == 0) s_misc[1] = nk + __popcll(kept);
+        }
+        __syncthreads();
+        nk = s_misc[1];
+      }
+      for (int t = tid; t < n; t += 256) perm[b + t] = eo[ord[t]];
+      if (tid == 0) kept_cnt[p] = nk;
+    } else {
+      for (int i = tid; i < n; i += 256) perm[b + i] = eo[i];
     }
     __syncthreads();
   }
@@ -2435,9 +2752,8 @@ int enqueue_post(d2_ctx* c, int mode) {
   {
     Timer tm(c, K_POST_PAIRS, (double)np);
     const int wblocks = std::max(1, std::min(np, c->tune.post_blocks));
-    hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_WAVE>, dim3(wblocks), dim3(64), 0, c->stream, np, nullptr, nullptr,
-                       c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p,
-                       c->scale.p, c->big.p, c->n_big.p);
+    hipLaunchKernelGGL(k_pair_finish_block, dim3(wblocks), dim3(256), 0, c->stream, np, c->contacts.p, c->pair_off.p,
+                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
     hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(64), 0,
                        c->stream, np, c->big.p, c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p,
                        c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p, c->n_big.p + 1);

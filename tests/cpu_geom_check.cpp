@@ -2,6 +2,7 @@
 // __host__ __device__) checked bit-for-bit against the oracle on random
 // inputs.  Built and run by tests/test_geom_cpu.py; prints one line per
 // primitive: "<name> <cases> <mismatches>".
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <random>
@@ -125,5 +126,22 @@ int main() {
     if (!ok) ++bad;
   }
   printf("extrapolate %d %d\n", N, bad);
+
+  // the filter's distance test without the square root equals vlen(d) < search,
+  // also when search sits within a few ulps of |d|
+  bad = 0;
+  int cases = 0;
+  for (int k = 0; k < N; ++k) {
+    d2::v3<double> d{U(-1, 1), U(-1, 1), U(-1, 1)};
+    const double sc = std::pow(10.0, U(-6, 3));
+    d = d2::scl(d, sc);
+    const double len = d2::vlen(d);
+    for (int j = -6; j <= 6; ++j) {
+      const double search = j == 6 ? U(0, 2) * len : std::nextafter(len, j < 0 ? 0.0 : 1e300) * (1.0 + j * 1e-16);
+      ++cases;
+      if (d2::near_mid(d, search) != (d2::vlen(d) < search)) ++bad;
+    }
+  }
+  printf("near_mid %d %d\n", cases, bad);
   return 0;
 }

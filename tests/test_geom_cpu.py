@@ -16,7 +16,7 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
                     "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
     rows = {l.split()[0]: l.split() for l in out.strip().splitlines()}
-    for name in ("tri_tri_dist", "point_tri", "edge_edge", "extrapolate"):
+    for name in ("tri_tri_dist", "point_tri", "edge_edge", "extrapolate", "near_mid"):
         assert int(rows[name][2]) == 0, out
         assert int(rows[name][1]) >= 10000
     assert "bisection cases" in out
