@@ -96,6 +96,18 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+// Value of lane l (wave-uniform) without going through LDS.
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(bits & 0xffffffffll), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(v & 0xffffffffull), l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
 // Orders one wavefront's LDS writes before its later LDS reads (the other
 // wavefronts of the block are not waited for).
 __device__ __forceinline__ void wave_sync_lds() {
@@ -752,51 +764,71 @@ __device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, flo
 // FP64 test.  FP64 mode keeps every valid pair; FP32 mode drops certain
 // misses, and for node pairs with an inner node (they emit no contacts, only
 // "any hit" matters) a certain hit settles the node pair without FP64 work.
+__device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
-  const int lane = lane_id();
+  const int lane = lane_id(), nw = blockDim.x >> 6;
   const long long n = D.n_soup();
   long long rb, re;
   chunk_of(n, rb, re);
   long long acc = 0;
-  for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {
-    NP np = D.soup[w];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    int i = lane >> 3, j = lane & 7;
-    bool valid = i < na.ntris && j < nb.ntris;
-    int cls = valid ? 1 : 0;
-    if (fp32) {  // wave-uniform
-      const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-      const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-      const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-      CullTri mine{}, ta, tb;
-      const int side = (lane >> 3) & 1, k = lane & 7;
-      if (lane < 16 && k < (side ? nb.ntris : na.ntris)) {
-        const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
-        float rot[9], tr[3];
-        for (int r = 0; r < 3; ++r) {
-          for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
-          tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
-        }
-        cull_prep(rot, tr, load_tri<double>(side ? bb : ba, k), mine);
-      }
-      cull_fetch(mine, i, ta);
-      cull_fetch(mine, 8 + j, tb);
-      if (valid) {
-        const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
-        cls = cull_pair(ta, tb, thr);
-      }
+  // The wave's node pairs come in batches of 64: lane l fetches the record,
+  // node offsets and node records of the batch's l-th node pair in one go, so
+  // the per-node-pair dependent loads reduce to its pose and buckets.
+  for (long long w0 = rb + (threadIdx.x >> 6); w0 < re; w0 += 64ll * nw) {
+    const long long wl = w0 + (long long)nw * lane;
+    int l_pair = 0, l_offa = 0, l_offb = 0, l_bka = 0, l_bkb = 0, l_nta = 0, l_ntb = 0, l_inner = 0;
+    if (wl < re) {
+      const NP np = D.soup[wl];
+      const int32_t* pi = D.pair_info + 4 * np.pair;
+      l_pair = (int)np.pair, l_offa = pi[2], l_offb = pi[3];
+      const NodeRec na = D.nodes[l_offa + np.na];
+      const NodeRec nb = D.nodes[l_offb + np.nb];
+      l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
+      l_inner = (na.inner || nb.inner) ? 1 : 0;
     }
-    bool inner = na.inner || nb.inner;
-    unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
-    bool settled = inner && __ballot(cls == 2) != 0ull;
-    if (settled) keep = 0ull;
-    if (lane == 0) {
-      D.masks[w] = keep;
-      D.cnt_a[w] = __popcll(keep);
-      D.np_hit[w] = settled ? 1 : 0;
-      acc += __popcll(keep);
+    const long long left = (re - w0 + nw - 1) / nw;
+    const int batch = left < 64 ? (int)left : 64;
+    unsigned long long my_keep = 0;
+    int my_settled = 0;
+    for (int k = 0; k < batch; ++k) {
+      const int pair = rl(l_pair, k), offa = rl(l_offa, k), offb = rl(l_offb, k);
+      const int nta = rl(l_nta, k), ntb = rl(l_ntb, k);
+      const bool inner = rl(l_inner, k) != 0;
+      int i = lane >> 3, j = lane & 7;
+      bool valid = i < nta && j < ntb;
+      int cls = valid ? 1 : 0;
+      if (fp32) {  // wave-uniform
+        const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
+        const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
+        const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
+        CullTri mine{}, ta, tb;
+        const int side = (lane >> 3) & 1, kk = lane & 7;
+        if (lane < 16 && kk < (side ? ntb : nta)) {
+          const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
+          float rot[9], tr[3];
+          for (int r = 0; r < 3; ++r) {
+            for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
+            tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
+          }
+          cull_prep(rot, tr, load_tri<double>(side ? bb : ba, kk), mine);
+        }
+        cull_fetch(mine, i, ta);
+        cull_fetch(mine, 8 + j, tb);
+        if (valid) {
+          const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
+          cls = cull_pair(ta, tb, thr);
+        }
+      }
+      unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
+      const bool settled = inner && __ballot(cls == 2) != 0ull;
+      if (settled) keep = 0ull;
+      if (lane == k) my_keep = keep, my_settled = settled ? 1 : 0;
+    }
+    if (lane < batch) {
+      D.masks[wl] = my_keep;
+      D.cnt_a[wl] = __popcll(my_keep);
+      D.np_hit[wl] = my_settled;
+      acc += __popcll(my_keep);
     }
   }
   const long long tot = block_sum(acc);
@@ -2086,8 +2118,12 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
     __syncthreads();
     for (int i = tid; i < n; i += 256) {  // emission position = rank of the (unique) key
       const unsigned long long k = skey[i];
-      int r = 0;
-      for (int j = 0; j < n; ++j) r += skey[j] < k ? 1 : 0;
+      int r = 0, j = 0;
+      for (; j + 4 <= n; j += 4) {  // four independent LDS reads in flight
+        const unsigned long long k0 = skey[j], k1 = skey[j + 1], k2 = skey[j + 2], k3 = skey[j + 3];
+        r += (k0 < k ? 1 : 0) + (k1 < k ? 1 : 0) + (k2 < k ? 1 : 0) + (k3 < k ? 1 : 0);
+      }
+      for (; j < n; ++j) r += skey[j] < k ? 1 : 0;
       eo[r] = sidx[i];
     }
     __syncthreads();
@@ -2153,8 +2189,15 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
       bool tie = false;
       for (int i = tid; i < n; i += 256) {  // rank of the (eps, dist) key; equal keys share it
         const LdsKey x = sk[i];
-        int r = 0, eq = 0;
-        for (int j = 0; j < n; ++j) {
+        int r = 0, eq = 0, j = 0;
+        for (; j + 4 <= n; j += 4) {
+          const LdsKey y0 = sk[j], y1 = sk[j + 1], y2 = sk[j + 2], y3 = sk[j + 3];
+          r += (lds_less(y0, x) ? 1 : 0) + (lds_less(y1, x) ? 1 : 0) + (lds_less(y2, x) ? 1 : 0) +
+               (lds_less(y3, x) ? 1 : 0);
+          eq += (!lds_less(y0, x) && !lds_less(x, y0) ? 1 : 0) + (!lds_less(y1, x) && !lds_less(x, y1) ? 1 : 0) +
+                (!lds_less(y2, x) && !lds_less(x, y2) ? 1 : 0) + (!lds_less(y3, x) && !lds_less(x, y3) ? 1 : 0);
+        }
+        for (; j < n; ++j) {
           const LdsKey y = sk[j];
           if (lds_less(y, x)) ++r;
           else if (!lds_less(x, y)) ++eq;
@@ -2188,13 +2231,13 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
           const double hx = smid[3 * h], hy = smid[3 * h + 1], hz = smid[3 * h + 2];
           const int mm = nk - q0 < 64 ? nk - q0 : 64;
           for (int u = w; u < mm; u += 4) {
-            const v3<double> dd{mx - __shfl(hx, u), my - __shfl(hy, u), mz - __shfl(hz, u)};
+            const v3<double> dd{mx - readlane_d(hx, u), my - readlane_d(hy, u), mz - readlane_d(hz, u)};
             near = near || near_mid(dd, search);
           }
         }
         unsigned long long conf = 0;  // bit u: candidate u (< this one) of the block is near
         for (int u = w; u < m; u += 4) {
-          const v3<double> dd{mx - __shfl(mx, u), my - __shfl(my, u), mz - __shfl(mz, u)};
+          const v3<double> dd{mx - readlane_d(mx, u), my - readlane_d(my, u), mz - readlane_d(mz, u)};
           if (u < lane && near_mid(dd, search)) conf |= 1ull << u;
         }
         near_w[w][lane] = near ? 1 : 0;
@@ -2206,7 +2249,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
           const unsigned long long blocked = __ballot(nr);
           unsigned long long kept = 0;
           for (int i = 0; i < m; ++i) {
-            const unsigned long long ci = __shfl(cf, i);
+            const unsigned long long ci = readlane_u64(cf, i);
             if (!((blocked >> i) & 1ull) && !(ci & kept)) kept |= 1ull << i;
           }
           const bool fresh = (kept >> lane) & 1ull;
