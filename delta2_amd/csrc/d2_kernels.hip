@@ -169,6 +169,7 @@ struct Dev {
   double* min_time;
   const float* tree_eps_out;   // float-barycentric acceptance bound per tree
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
+  int* exp_desc;                  // per discrete node pair: expansion_desc
   const int32_t* body_tree;
   int fp32;                    // FP32 culls of the CCD leaf sub-tests
   int n_pairs;
@@ -764,17 +765,33 @@ __device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, flo
 // FP64 test.  FP64 mode keeps every valid pair; FP32 mode drops certain
 // misses, and for node pairs with an inner node (they emit no contacts, only
 // "any hit" matters) a certain hit settles the node pair without FP64 work.
+__device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
+  return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
+}
+// Expansion of a node pair of the discrete list, packed for the level's
+// count/write kernels: bit 0 advance a, bits 1-4 child count, bits 5-31 first
+// child (0: two leaves, no children).
+__device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& nb) {
+  if (!(na.inner || nb.inner)) return 0;
+  const bool adv_a = expand_side(na, nb);
+  const NodeRec& par = adv_a ? na : nb;
+  return (adv_a ? 1 : 0) | (par.nchild << 1) | (par.child_first << 5);
+}
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
-  const int lane = lane_id(), nw = blockDim.x >> 6;
+  __shared__ int s_tot[2][4];
+  __shared__ unsigned long long s_base[2];
+  const int lane = lane_id(), nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
   const long long n = D.n_soup();
   long long rb, re;
   chunk_of(n, rb, re);
-  long long acc = 0;
-  // The wave's node pairs come in batches of 64: lane l fetches the record,
-  // node offsets and node records of the batch's l-th node pair in one go, so
-  // the per-node-pair dependent loads reduce to its pose and buckets.
-  for (long long w0 = rb + (threadIdx.x >> 6); w0 < re; w0 += 64ll * nw) {
+  // The block walks its range 64 * nw node pairs at a time, wavefront wid
+  // taking rb + wid + nw * k; lane l fetches the record, node offsets and node
+  // records of the wave's l-th node pair in one go, so the per-node-pair
+  // dependent loads reduce to its pose and buckets.
+  int it = 0;
+  for (long long b0 = rb; b0 < re; b0 += 64ll * nw, ++it) {
+    const long long w0 = b0 + wid;
     const long long wl = w0 + (long long)nw * lane;
     int l_pair = 0, l_offa = 0, l_offb = 0, l_bka = 0, l_bkb = 0, l_nta = 0, l_ntb = 0, l_inner = 0;
     if (wl < re) {
@@ -785,8 +802,9 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
       const NodeRec nb = D.nodes[l_offb + np.nb];
       l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
       l_inner = (na.inner || nb.inner) ? 1 : 0;
+      D.exp_desc[wl] = expansion_desc(na, nb);
     }
-    const long long left = (re - w0 + nw - 1) / nw;
+    const long long left = w0 < re ? (re - w0 + nw - 1) / nw : 0;
     const int batch = left < 64 ? (int)left : 64;
     unsigned long long my_keep = 0;
     int my_settled = 0;
@@ -824,15 +842,42 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
       if (settled) keep = 0ull;
       if (lane == k) my_keep = keep, my_settled = settled ? 1 : 0;
     }
-    if (lane < batch) {
-      D.masks[wl] = my_keep;
-      D.cnt_a[wl] = __popcll(my_keep);
-      D.np_hit[wl] = my_settled;
-      acc += __popcll(my_keep);
+    if (lane < batch) D.np_hit[wl] = my_settled;
+    // The kept triangle pairs go straight to the level's item list: one slot
+    // range per block and round, one atomic (item order does not matter:
+    // contacts carry their emission keys).
+    const int mine = lane < batch ? __popcll(my_keep) : 0;
+    int incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    const int par = it & 1;
+    if (lane == 63) s_tot[par][wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int sum = 0;
+      for (int v = 0; v < nw; ++v) sum += s_tot[par][v];
+      unsigned long long base = 0;
+      if (sum) {
+        base = atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_ITEMS], (unsigned long long)sum);
+        if ((long long)(base + sum) > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+      }
+      s_base[par] = base;
+    }
+    __syncthreads();
+    long long base = (long long)s_base[par];
+    for (int v = 0; v < wid; ++v) base += s_tot[par][v];
+    const int excl = incl - mine;
+    for (int k = 0; k < batch; ++k) {
+      const unsigned long long m = readlane_u64(my_keep, k);
+      if (m == 0ull) continue;
+      const long long slot = base + rl(excl, k) + __popcll(m & lanes_below());
+      if (((m >> lane) & 1ull) && slot < D.cap_items)
+        D.items[slot] = Item{(uint32_t)(w0 + (long long)nw * k), (uint32_t)lane};
     }
   }
-  const long long tot = block_sum(acc);
-  if (threadIdx.x == 0) D.part_a[blockIdx.x] = (int)tot;
 }
 
 // Kept triangle pairs of every node pair to the item list at scanned offsets
@@ -874,6 +919,7 @@ __global__ void __launch_bounds__(256) k_mask_enum(Dev D, long long cap) {
 // FP64 triangle-pair distances of the level's items (grid-stride).
 __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   const long long count = D.n_lv(LV_ITEMS);
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)D.lv(D.level, LV_ITEMS));
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
     Item item = D.items[t];
     NP np = D.soup[item.np];
@@ -911,9 +957,6 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
 // CCD, the sub-tests of the connected (leaf-leaf) list, in two chunked
 // passes: counts + block partials, then writes at scanned offsets, which
 // keeps every list in the reference's breadth-first order.
-__device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
-  return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
-}
 // ------------------------------------------- FP32 culls of CCD leaf sub-tests
 // A culled sub-test provably reports no distance below the search distance.
 //   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
@@ -1115,7 +1158,9 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
     long long as = 0, ac = 0;
     for (long long w = rb + threadIdx.x; w < re; w += blockDim.x) {
       int ns = 0, nc = 0;
-      if (D.np_hit[w]) {
+      if (D.discrete) {
+        if (D.np_hit[w]) ns = (D.exp_desc[w] >> 1) & 15;
+      } else if (D.np_hit[w]) {
         NP np = D.soup[w];
         const int32_t* pi = D.pair_info + 4 * np.pair;
         const NodeRec na = D.nodes[pi[2] + np.na];
@@ -1201,7 +1246,19 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
       int tts, ttc;
       const int es = block_scan(w < re ? D.cnt_s[w] : 0, tts);
       const int ec = block_scan(w < re ? D.cnt_c[w] : 0, ttc);
-      if (w < re && D.cnt_s[w] + D.cnt_c[w] > 0) {
+      if (w < re && D.discrete && D.cnt_s[w] > 0) {  // children from the node pair's descriptor
+        const NP np = D.soup[w];
+        const int desc = D.exp_desc[w];
+        const bool adv_a = desc & 1;
+        const int nchild = (desc >> 1) & 15, first = desc >> 5;
+        long long ps = bs + es;
+        for (int c = 0; c < nchild; ++c) {
+          const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
+          const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
+          const long long slot = ps++;
+          if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
+        }
+      } else if (w < re && D.cnt_s[w] + D.cnt_c[w] > 0) {
         NP np = D.soup[w];
         const int32_t* pi = D.pair_info + 4 * np.pair;
         const NodeRec na = D.nodes[pi[2] + np.na];
@@ -2492,7 +2549,7 @@ struct d2_ctx {
   DBuf<NP> soup_list[2], conn_list[2];  // level l reads [l & 1] and writes [(l + 1) & 1]
   DBuf<Item> items, pt_items, ee_items;
   DBuf<double> res;
-  DBuf<int> np_hit;
+  DBuf<int> np_hit, exp_desc;
   DBuf<ContactRec> contacts;
   DBuf<unsigned long long> cnt, lvl, masks, conn_mask;
   DBuf<int> cnt_a, cnt_s, cnt_c, cnt_pt, cnt_ee, parts;
@@ -2698,6 +2755,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.min_time = c->min_time.p;
   D.tree_eps_out = c->tree_eps_out.p;
   D.conn_mask = c->conn_mask.p;
+  D.exp_desc = c->exp_desc.p;
   D.body_tree = c->body_tree.p;
   D.fp32 = c->precision == D2_FP32 ? 1 : 0;
   D.n_pairs = c->n_pairs;
@@ -2723,7 +2781,7 @@ int prepare(d2_ctx* c) {
     RC(ensure(c, c->conn_list[k], c->cap));
   }
   RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
-  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
@@ -2756,7 +2814,6 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   const int cg = chunk_grid(c), sg = stream_grid(c);
   if (discrete) {
     timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, c->precision == D2_FP32 ? 1 : 0);
-    timed_launch(c, K_TRI_ENUM, k_mask_enum, dim3(cg), dim3(256), D, (long long)c->cap_items);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
   } else {
     timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
@@ -3190,7 +3247,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
-  release(c->items), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit);
+  release(c->items), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->exp_desc);
   release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
