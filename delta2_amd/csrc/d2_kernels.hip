@@ -58,6 +58,9 @@ struct NP {  // node pair work entry
 struct Item {
   uint32_t np, sub;
 };
+struct DItem {  // discrete triangle-pair item: node pair, pair, global buckets, sub | inner << 8
+  uint32_t np, pair, ga, gb, sub;
+};
 struct ContactRec {
   unsigned long long key;
   int32_t pair, na, nb, kind, ia, ib, level, pad;
@@ -151,6 +154,7 @@ struct Dev {
   NP* soup_next;
   NP* conn_next;
   Item* items;
+  DItem* ditems;
   Item* pt_items;
   Item* ee_items;
   double* res;  // [items][15][8]
@@ -875,7 +879,9 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
       if (m == 0ull) continue;
       const long long slot = base + rl(excl, k) + __popcll(m & lanes_below());
       if (((m >> lane) & 1ull) && slot < D.cap_items)
-        D.items[slot] = Item{(uint32_t)(w0 + (long long)nw * k), (uint32_t)lane};
+        D.ditems[slot] = DItem{(uint32_t)(w0 + (long long)nw * k), (uint32_t)rl(l_pair, k),
+                               (uint32_t)(rl(l_offa, k) + rl(l_bka, k)), (uint32_t)(rl(l_offb, k) + rl(l_bkb, k)),
+                               (uint32_t)lane | (rl(l_inner, k) ? 256u : 0u)};
     }
   }
 }
@@ -921,15 +927,12 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   const long long count = D.n_lv(LV_ITEMS);
   if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)D.lv(D.level, LV_ITEMS));
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
-    Item item = D.items[t];
-    NP np = D.soup[item.np];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-    int i = item.sub >> 3, j = item.sub & 7;
+    const DItem item = D.ditems[t];
+    const double* pr = D.pair_rec + (size_t)PR_STRIDE * item.pair;
+    const double* ba = D.buckets + (size_t)BUCKET * item.ga;
+    const double* bb = D.buckets + (size_t)BUCKET * item.gb;
+    const int sub = (int)(item.sub & 63u);
+    int i = sub >> 3, j = sub & 7;
     tri<double> A = xform(pr + PR_TA1, load_tri<double>(ba, i));
     tri<double> B = xform(pr + PR_TB1, load_tri<double>(bb, j));
     v3<double> P, Q;
@@ -937,14 +940,15 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
     double ea = ba[72 + i], eb = bb[72 + j];
     if (!(d < (pr[PR_SEARCH] + ea) + eb)) continue;
     D.np_hit[item.np] = 1;
-    if (na.inner || nb.inner) continue;
+    if (item.sub & 256u) continue;  // an inner node: the hit only expands the node pair
     unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
     if ((long long)slot >= D.cap_contacts) {
       atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
       continue;
     }
+    const NP np = D.soup[item.np];
     ContactRec c;
-    c.key = make_key(D.level, 0, np.key, (int)item.sub);
+    c.key = make_key(D.level, 0, np.key, sub);
     c.pair = (int32_t)np.pair, c.na = (int32_t)np.na, c.nb = (int32_t)np.nb, c.kind = 0;
     c.ia = i, c.ib = j, c.level = D.level, c.pad = 0;
     c.r[0] = P.x, c.r[1] = P.y, c.r[2] = P.z, c.r[3] = Q.x, c.r[4] = Q.y, c.r[5] = Q.z;
@@ -2548,6 +2552,7 @@ struct d2_ctx {
   // work
   DBuf<NP> soup_list[2], conn_list[2];  // level l reads [l & 1] and writes [(l + 1) & 1]
   DBuf<Item> items, pt_items, ee_items;
+  DBuf<DItem> ditems;
   DBuf<double> res;
   DBuf<int> np_hit, exp_desc;
   DBuf<ContactRec> contacts;
@@ -2738,6 +2743,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.soup_next = c->soup_list[(level + 1) & 1].p;
   D.conn_next = c->conn_list[(level + 1) & 1].p;
   D.items = c->items.p;
+  D.ditems = c->ditems.p;
   D.pt_items = c->pt_items.p;
   D.ee_items = c->ee_items.p;
   D.res = c->res.p;
@@ -2781,7 +2787,7 @@ int prepare(d2_ctx* c) {
     RC(ensure(c, c->conn_list[k], c->cap));
   }
   RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
-  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->ditems, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
@@ -3247,7 +3253,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
-  release(c->items), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->exp_desc);
+  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->exp_desc);
   release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
