@@ -32,16 +32,17 @@ METRIC = "narrow+CCD pair queries/s, 1 GPU; learned vs plain speedup at equal TO
 # records (poses, shared by all of a pair's units) not counted.
 #   tri_mask   : one node pair: its record (24 B), two node records (64 B),
 #                the pair's body/tree ids (16 B), the triangles (2 x 576 B)
-#                and eps (2 x 64 B) of both buckets; mask, count, hit flag out (16 B)
-#   tri_enum   : one node pair: mask and count in (12 B), its kept items out
+#                and eps (2 x 64 B) of both buckets; hit flag and expansion
+#                descriptor out (8 B); its kept items out (counted under tri_dist)
+#   tri_enum   : one CCD soup node pair: mask and count in (12 B), its kept items out
 #                (counted under tri_dist)
-#   tri_dist   : one triangle pair: item (8 B), node pair (24 B), two node
-#                records (64 B), two triangles (144 B), two eps (16 B)
+#   tri_dist   : one triangle pair: item (20 B: node pair, pair, buckets),
+#                two triangles (144 B), two eps (16 B)
 #   soup_cull  : one soup node pair: as tri_mask (swept boxes of both buckets)
 #   soup_point / soup_edge : one sub-test: item (8 B), two triangles (144 B), one 64-B result
 #   leaf_point / leaf_edge : one sub-test: item (8 B), two triangles (144 B)
 #   soup_reduce: one triangle pair: 15 results (960 B)
-ALG_BYTES = {"tri_mask": 1400.0, "tri_enum": 12.0, "tri_dist": 256.0, "soup_cull": 1400.0, "soup_point": 216.0,
+ALG_BYTES = {"tri_mask": 1392.0, "tri_enum": 12.0, "tri_dist": 180.0, "soup_cull": 1400.0, "soup_point": 216.0,
              "soup_edge": 216.0, "leaf_point": 152.0, "leaf_edge": 152.0, "soup_reduce": 960.0}
 # Estimated FP32 operations per unit of the FP32 cull (tri_mask): per node
 # pair 16 triangle preparations (~80 ops) and 64 pair classifications (~130 ops).

@@ -686,8 +686,8 @@ __device__ int cull_class(const double* pa, const double* pb, const tri<double>&
 // vertices (relative frame of cull_class), normal, kappa, longest squared
 // edge, own-normal extent, vertex sum, largest coordinate, and whether its
 // barycentric system is well conditioned.
-struct CullTri {
-  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok;
+struct __align__(16) CullTri {
+  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok, pad[2];
 };
 __device__ __forceinline__ void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
   const v3<double> vv[3] = {t.a, t.b, t.c};
@@ -784,6 +784,7 @@ __device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& 
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
   __shared__ int s_tot[2][4];
+  __shared__ CullTri s_tri[4][16];  // per wavefront: the node pair's 8 + 8 prepared triangles
   __shared__ unsigned long long s_base[2];
   const int lane = lane_id(), nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
   const long long n = D.n_soup();
@@ -823,19 +824,23 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
         const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
         const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
         const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
-        CullTri mine{}, ta, tb;
         const int side = (lane >> 3) & 1, kk = lane & 7;
-        if (lane < 16 && kk < (side ? ntb : nta)) {
-          const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
-          float rot[9], tr[3];
-          for (int r = 0; r < 3; ++r) {
-            for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
-            tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
+        if (lane < 16) {
+          CullTri mine{};
+          if (kk < (side ? ntb : nta)) {
+            const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
+            float rot[9], tr[3];
+            for (int r = 0; r < 3; ++r) {
+              for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
+              tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
+            }
+            cull_prep(rot, tr, load_tri<double>(side ? bb : ba, kk), mine);
           }
-          cull_prep(rot, tr, load_tri<double>(side ? bb : ba, kk), mine);
+          s_tri[wid][lane] = mine;
         }
-        cull_fetch(mine, i, ta);
-        cull_fetch(mine, 8 + j, tb);
+        wave_sync_lds();
+        const CullTri ta = s_tri[wid][i], tb = s_tri[wid][8 + j];
+        wave_sync_lds();  // read before the next node pair overwrites
         if (valid) {
           const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
           cls = cull_pair(ta, tb, thr);
@@ -1247,9 +1252,9 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
     }
     for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
       const long long w = t0 + threadIdx.x;
-      int tts, ttc;
+      int tts, ttc = 0;
       const int es = block_scan(w < re ? D.cnt_s[w] : 0, tts);
-      const int ec = block_scan(w < re ? D.cnt_c[w] : 0, ttc);
+      const int ec = D.discrete ? 0 : block_scan(w < re ? D.cnt_c[w] : 0, ttc);  // no connected list when discrete
       if (w < re && D.discrete && D.cnt_s[w] > 0) {  // children from the node pair's descriptor
         const NP np = D.soup[w];
         const int desc = D.exp_desc[w];
