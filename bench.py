@@ -238,6 +238,38 @@ def run_c5(args):
                      "cluster_step_range_s": [float(allst.min()), float(allst.max())]}
         log(f"[c5] {name}: {out[name]['ms_per_step']:.1f} ms/step, identical {same}")
         ctx.close()
+    # the same steps with broad phase and clusters on the device, as the
+    # reference's cluster stepper forms them (broadPhaseEmbree(particles): the
+    # static funnel pairs with every body inside its box)
+    ctx = d2.Context(0, d2.D2_FP32)
+    ctx.upload_trees(sc.trees)
+    ctx.mlp_set_params(w)
+    ctx.set_learned(True)
+    dev = {"broad_phase_ms": [], "clusters_ms": [], "step_ms": [], "pairs": [], "clusters": []}
+    for k, (r, _, _) in enumerate(steps):
+        for rep in range(2 if k == 0 else 1):  # the first pass sizes buffers
+            ctx.upload_bodies(sc.tree_of_body, r)
+            ctx.project_future(np.minimum(2.0 * r[:, 39], scenes.DT))  # broad_phase_embree_cluster.cpp:121-127
+            ctx.sync()
+            t1 = time.perf_counter()
+            n_pairs = ctx.broad_phase(mode=1)
+            t2 = time.perf_counter()
+            n_cl = ctx.build_clusters()
+            t3 = time.perf_counter()
+            ctx.step_run(scenes.DT, True)
+            ctx.sync()
+            t4 = time.perf_counter()
+        dev["broad_phase_ms"].append(1e3 * (t2 - t1))
+        dev["clusters_ms"].append(1e3 * (t3 - t2))
+        dev["step_ms"].append(1e3 * (t4 - t3))
+        dev["pairs"].append(int(n_pairs))
+        dev["clusters"].append(int(n_cl))
+    ctx.close()
+    dev["pair_queries_per_s"] = sum(dev["pairs"]) / (sum(dev["broad_phase_ms"]) + sum(dev["clusters_ms"]) +
+                                                     sum(dev["step_ms"])) * 1e3
+    log(f"[c5] device pipeline: pairs {dev['pairs']}, broad phase {dev['broad_phase_ms']} ms, clusters "
+        f"{dev['clusters_ms']} ms, step {dev['step_ms']} ms")
+    out["device_pipeline_fp32"] = dev
     best = out["learned_fp32"]
     res = {"metric": METRIC, "value": best["value"], "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
            "warmup": 1, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
