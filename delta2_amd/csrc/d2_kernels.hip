@@ -839,12 +839,11 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
           s_tri[wid][lane] = mine;
         }
         wave_sync_lds();
-        const CullTri ta = s_tri[wid][i], tb = s_tri[wid][8 + j];
-        wave_sync_lds();  // read before the next node pair overwrites
-        if (valid) {
+        if (valid) {  // operands read from LDS where they are used (fewer live registers)
           const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
-          cls = cull_pair(ta, tb, thr);
+          cls = cull_pair(s_tri[wid][i], s_tri[wid][8 + j], thr);
         }
+        wave_sync_lds();  // read before the next node pair overwrites
       }
       unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
       const bool settled = inner && __ballot(cls == 2) != 0ull;
