@@ -2511,8 +2511,9 @@ struct Sig {
   int learned = -1;
   float gate_t = 0.f, gate_s = 0.f;
   int cl_dev = -1;  // clusters built on the device (first members known)
+  unsigned long long grids = 0;  // hash of the per-level launch grids
   bool operator==(const Sig& o) const {
-    return cl_dev == o.cl_dev && levels[0] == o.levels[0] && levels[1] == o.levels[1] && learned == o.learned && gate_t == o.gate_t &&
+    return grids == o.grids && cl_dev == o.cl_dev && levels[0] == o.levels[0] && levels[1] == o.levels[1] && learned == o.learned && gate_t == o.gate_t &&
            gate_s == o.gate_s && n_pairs == o.n_pairs && n_bodies == o.n_bodies && n_clusters == o.n_clusters && max_depth == o.max_depth &&
            precision == o.precision && filter == o.filter && dt == o.dt &&
            tune.chunk_blocks == o.tune.chunk_blocks && tune.stream_blocks == o.tune.stream_blocks &&
@@ -2619,6 +2620,10 @@ struct d2_ctx {
   bool capturing = false;
   Captured graphs[RUN_KINDS][MAX_SEGMENTS];
   int phase_levels[4] = {-1, -1, -1, -1};  // deepest level count seen per phase (narrow, ccd, step ccd, step narrow)
+  // level sizes of the last run of each phase (0 discrete, 1 CCD): launch
+  // grids of the next runs' levels are sized from them
+  long long prev_lv[2][MAX_LEVELS + 1][LV_STRIDE] = {};
+  bool prev_ok[2] = {false, false};
   // profiling
   bool profiling = false;
   long long k_launch[K_NUM] = {};
@@ -2819,9 +2824,45 @@ void scan_host_n(d2_ctx* c, const int* in, int* out, long long n, unsigned long 
 }
 
 // One breadth-first level of a phase.
+// Launch grids of one level: the tuned block counts are the caps; a level
+// that was small in the last run of its phase gets fewer blocks (at least 16,
+// in powers of two, with 25% headroom): the culling kernels keep about one
+// node pair per wavefront (they are latency-bound, so as many wavefronts as
+// there is work), the count/write pair one soup node pair per thread or one
+// connected node pair per wavefront, the stream kernels one unit per thread.
+// Any grid gives the same results; only the time changes.
+struct LevelGrids {
+  int cull, expand, stream;
+};
+LevelGrids level_grids(const d2_ctx* c, int discrete, int level) {
+  LevelGrids g{chunk_grid(c), chunk_grid(c), stream_grid(c)};
+  const int ph = discrete ? 0 : 1;
+  if (!c->prev_ok[ph] || level > MAX_LEVELS) return g;
+  const long long* v = c->prev_lv[ph][level];
+  const long long ns = v[LV_SOUP], nc = v[LV_CONN];
+  const long long units = discrete ? v[LV_ITEMS] : std::max(std::max(9 * v[LV_ITEMS], v[LV_PT]), v[LV_EE]);
+  auto fit = [](long long n, long long per, int hi) {
+    long long b = 16;
+    while (b < hi && b * per < n + n / 4) b <<= 1;
+    return (int)std::min<long long>(b, hi);
+  };
+  g.cull = fit(ns, 4, g.cull);
+  g.expand = fit(std::max((ns + 63) / 64, nc), 4, g.expand);
+  g.stream = fit(units, 256, g.stream);
+  return g;
+}
+unsigned long long grids_hash(const d2_ctx* c, int discrete, int levels) {
+  unsigned long long h = 1469598103934665603ull;
+  for (int l = 0; l < levels; ++l) {
+    const LevelGrids g = level_grids(c, discrete, l);
+    h = (h ^ (unsigned long long)((g.cull * 131071ll + g.expand) * 131071ll + g.stream)) * 1099511628211ull;
+  }
+  return h;
+}
 void enqueue_level(d2_ctx* c, int level, int discrete) {
   Dev D = make_dev(c, level, discrete);
-  const int cg = chunk_grid(c), sg = stream_grid(c);
+  const LevelGrids G = level_grids(c, discrete, level);
+  const int cg = G.cull, eg = G.expand, sg = G.stream;
   if (discrete) {
     timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, c->precision == D2_FP32 ? 1 : 0);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
@@ -2834,8 +2875,8 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   }
   {
     Timer tm(c, K_EXPAND, 0.0);
-    hipLaunchKernelGGL(k_level_count, dim3(cg), dim3(256), 0, c->stream, D);
-    hipLaunchKernelGGL(k_level_write, dim3(cg), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_level_count, dim3(eg), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
   }
   if (!discrete) {
     timed_launch(c, K_LEAF_POINT, k_leaf_point, dim3(sg), dim3(256), D);
@@ -3087,6 +3128,7 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   sig.learned = (c->learned_on && c->mlp_ready) ? c->gate_phases : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
   sig.cl_dev = kind == RUN_STEP && c->clusters_on_device ? 1 : 0;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
+  for (int i = 0; i < n; ++i) sig.grids = sig.grids * 31 + grids_hash(c, ph[i].discrete, sig.levels[i]);
   std::vector<std::function<int()>> ops;
   int seg = 0;
   for (int i = 0; i < n; ++i) {
@@ -3136,6 +3178,17 @@ int finish_run(d2_ctx* c, int kind, int filter) {
     return fail(c, D2_ERR_CAPACITY, "work list capacity exceeded; raise it with d2_set_capacity");
   if (last[LV_SOUP] || last[LV_CONN])
     return fail(c, D2_ERR_RUNTIME, "traversal deeper than the level budget of the uploaded trees");
+  {
+    const int half = (MAX_LEVELS + 1) * LV_STRIDE;
+    auto keep = [&](int ph, const unsigned long long* src) {
+      for (int l = 0; l <= MAX_LEVELS; ++l)
+        for (int k = 0; k < LV_STRIDE; ++k) c->prev_lv[ph][l][k] = (long long)src[l * LV_STRIDE + k];
+      c->prev_ok[ph] = true;
+    };
+    if (kind == RUN_CCD) keep(1, c->host_lvl);
+    if (kind == RUN_NARROW) keep(0, c->host_lvl);
+    if (kind == RUN_STEP) keep(0, c->host_lvl), keep(1, c->host_lvl + half);
+  }
   c->n_out = (long long)c->host_cnt[C_OUT];
   c->last_mode = kind == RUN_CCD ? 1 : 0;
   c->ccd_valid = kind != RUN_NARROW;
