@@ -359,17 +359,22 @@ __device__ float cert_axis(const float* const* P, const int* nP, int nsP, const 
 // Q - P bounds the distance by x.s / |x| at every step (kept as the best
 // bound), and x moves to the point of segment [x, s] closest to the origin.
 // Returns the best bound; u_out = its axis.
+// With stop >= 0 the iteration ends as soon as the answer to "best > stop"
+// is known: a gap above stop, or an iterate x with |x| <= stop (x lies in the
+// hull of the difference set, so no axis can separate it by more than |x|).
 __device__ float cert_gilbert(const float* const* P, const int* nP, int nsP, const float* const* Q, const int* nQ,
-                              int nsQ, const float* x0, int iters, float* u_out, float* red) {
+                              int nsQ, const float* x0, int iters, float* u_out, float* red, float stop = -1.f) {
   float x[3] = {x0[0], x0[1], x0[2]};
   float best = -3.4e38f;
   u_out[0] = x[0], u_out[1] = x[1], u_out[2] = x[2];
   for (int it = 0; it < iters; ++it) {
     const float xx = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
     if (!(xx > 1e-30f)) break;
+    if (stop >= 0.f && it > 0 && xx <= stop * stop) break;
     float pp[3], qq[3];
     const float g = cert_axis(P, nP, nsP, Q, nQ, nsQ, x, pp, qq, red);
     if (g > best) best = g, u_out[0] = x[0], u_out[1] = x[1], u_out[2] = x[2];
+    if (stop >= 0.f && best > stop) break;
     const float sd[3] = {qq[0] - pp[0], qq[1] - pp[1], qq[2] - pp[2]};
     const float xs = (x[0] * sd[0] + x[1] * sd[1]) + x[2] * sd[2];
     if (xx - xs <= 1e-7f * xx) break;  // x is (nearly) the minimum-norm point
@@ -491,7 +496,8 @@ __global__ void __launch_bounds__(256) k_certify(int n_pairs, const int32_t* pai
       const float* Q[1] = {buf[1]};
       const int nP[1] = {na}, nQ[1] = {nb};
       float ax[3];
-      ok = cert_gilbert(P, nP, 1, Q, nQ, 1, u, CERT_ITERS, ax, red) - (1e-6f + 64.f * EPSF * M) > thr;
+      const float margin = 1e-6f + 64.f * EPSF * M;
+      ok = cert_gilbert(P, nP, 1, Q, nQ, 1, u, CERT_ITERS, ax, red, thr + margin) - margin > thr;
     }
     if (threadIdx.x == 0) {
       keep[p] = ok ? 0 : 1;
