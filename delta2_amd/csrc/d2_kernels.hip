@@ -243,64 +243,69 @@ __device__ void partial_prefix(const int* part, long long& before, long long& to
 
 // ----------------------------------------------------------------- set-up
 // Per pair: poses, frame changes, kinematics (CCD: future = extrapolate(max_time)).
+// Four lanes per pair split the record: lane q computes the poses and the
+// frame change it writes (q = 0: TA0, inv(TB0) TA0; 1: TA1, inv(TB1) TA1 and
+// a's kinematics; 2: TB0, inv(TA0) TB0; 3: TB1, inv(TA1) TB1 and b's
+// kinematics), so the longest chain holds two extrapolations, one inverse
+// and one product instead of all of them.
 __global__ void k_setup(int n_pairs, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
                         int ccd, int mt_from_bodies) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 4ll * n_pairs;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int p = (int)(t >> 2), q = (int)(t & 3);
     int a = pairs[2 * p], b = pairs[2 * p + 1];
     const double* ra = bodies + 40 * (size_t)a;
     const double* rb = bodies + 40 * (size_t)b;
-    body_state ca = load_state(ra), cb = load_state(rb);
-    body_state fa, fb;
     // max_time: given per pair, or min(last steps of a, b) (separate_clusters.cpp:490)
-    double mt = !ccd ? 0.0 : mt_from_bodies ? ((rb[39] < ra[39]) ? rb[39] : ra[39]) : max_time[p];
-    if (ccd) {
-      fa = extrapolate(ca, mt, ra + 28);
-      fb = extrapolate(cb, mt, rb + 28);
+    const double mt = !ccd ? 0.0 : mt_from_bodies ? ((rb[39] < ra[39]) ? rb[39] : ra[39]) : max_time[p];
+    const bool end = q & 1;  // lanes 1, 3 work on the end poses
+    body_state sa, sb;
+    if (!end) {
+      sa = load_state(ra), sb = load_state(rb);
+    } else if (ccd) {
+      sa = extrapolate(load_state(ra), mt, ra + 28);
+      sb = extrapolate(load_state(rb), mt, rb + 28);
     } else {
-      fa = load_state(ra + 14);
-      fb = load_state(rb + 14);
+      sa = load_state(ra + 14), sb = load_state(rb + 14);
     }
     double* pr = pair_rec + (size_t)PR_STRIDE * p;
-    pose(ca.q, ca.t, pr + PR_TA0);
-    pose(fa.q, fa.t, pr + PR_TA1);
-    pose(cb.q, cb.t, pr + PR_TB0);
-    pose(fb.q, fb.t, pr + PR_TB1);
+    double Ta[16], Tb[16];
+    pose(sa.q, sa.t, Ta);
+    pose(sb.q, sb.t, Tb);
+    const bool of_a = q < 2;  // lanes 0, 1 own a's pose, lanes 2, 3 b's
+    const double* own = of_a ? Ta : Tb;
+    double* dst = pr + (of_a ? (end ? PR_TA1 : PR_TA0) : (end ? PR_TB1 : PR_TB0));
+#pragma unroll
+    for (int k = 0; k < 16; ++k) dst[k] = own[k];
     if (ccd) {
       double inv[16];
-      inv44(pr + PR_TB0, inv);
-      mul44(inv, pr + PR_TA0, pr + PR_MAB0);
-      inv44(pr + PR_TB1, inv);
-      mul44(inv, pr + PR_TA1, pr + PR_MAB1);
-      inv44(pr + PR_TA0, inv);
-      mul44(inv, pr + PR_TB0, pr + PR_MBA0);
-      inv44(pr + PR_TA1, inv);
-      mul44(inv, pr + PR_TB1, pr + PR_MBA1);
-      double Iw[9];
-      world_inv_inertia(fa.q, ra + 28, Iw);
-      v3<double> wa = mul33v(Iw, fa.L);
-      world_inv_inertia(fb.q, rb + 28, Iw);
-      v3<double> wb = mul33v(Iw, fb.L);
-      pr[PR_VA] = fa.v.x, pr[PR_VA + 1] = fa.v.y, pr[PR_VA + 2] = fa.v.z;
-      pr[PR_WA] = wa.x, pr[PR_WA + 1] = wa.y, pr[PR_WA + 2] = wa.z;
-      pr[PR_XA] = fa.t.x, pr[PR_XA + 1] = fa.t.y, pr[PR_XA + 2] = fa.t.z;
-      pr[PR_VB] = fb.v.x, pr[PR_VB + 1] = fb.v.y, pr[PR_VB + 2] = fb.v.z;
-      pr[PR_WB] = wb.x, pr[PR_WB + 1] = wb.y, pr[PR_WB + 2] = wb.z;
-      pr[PR_XB] = fb.t.x, pr[PR_XB + 1] = fb.t.y, pr[PR_XB + 2] = fb.t.z;
+      inv44(of_a ? Tb : Ta, inv);
+      mul44(inv, own, pr + (of_a ? (end ? PR_MAB1 : PR_MAB0) : (end ? PR_MBA1 : PR_MBA0)));
+      if (end) {
+        const body_state& s = of_a ? sa : sb;
+        double Iw[9];
+        world_inv_inertia(s.q, (of_a ? ra : rb) + 28, Iw);
+        const v3<double> w = mul33v(Iw, s.L);
+        double* k = pr + (of_a ? PR_VA : PR_VB);
+        k[0] = s.v.x, k[1] = s.v.y, k[2] = s.v.z;
+        k[3] = w.x, k[4] = w.y, k[5] = w.z;  // PR_WA / PR_WB
+        k[6] = s.t.x, k[7] = s.t.y, k[8] = s.t.z;  // PR_XA / PR_XB
+      }
     }
-    pr[PR_GEOA] = ra[37];
-    pr[PR_GEOB] = rb[37];
-    pr[PR_MAXT] = mt;
-    pr[PR_SEARCH] = (double)(float)(ra[37] + rb[37]);  // a float in the reference
-    pair_info[4 * p + 0] = a;
-    pair_info[4 * p + 1] = b;
-    pair_info[4 * p + 2] = tree_off[body_tree[a]];
-    pair_info[4 * p + 3] = tree_off[body_tree[b]];
+    if (q == 0) {
+      pr[PR_GEOA] = ra[37];
+      pr[PR_GEOB] = rb[37];
+      pr[PR_MAXT] = mt;
+      pr[PR_SEARCH] = (double)(float)(ra[37] + rb[37]);  // a float in the reference
+      pair_info[4 * p + 0] = a;
+      pair_info[4 * p + 1] = b;
+      pair_info[4 * p + 2] = tree_off[body_tree[a]];
+      pair_info[4 * p + 3] = tree_off[body_tree[b]];
+    }
   }
 }
 
-// Root node pairs, pass state, counters.  keep_ccd leaves min_time / scale
-// (the CCD results of the same step) untouched.
 __global__ void k_roots(Dev D, const double* pair_rec, double* scale, int keep_ccd) {
   const int n_pairs = D.n_pairs;
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
@@ -2973,7 +2978,7 @@ int phases_of(int kind, int filter, PhaseSpec* out) {
 void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
   const int np = c->n_pairs;
   Timer tm(c, K_SETUP, np);
-  hipLaunchKernelGGL(k_setup, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, np, c->pairs.p,
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(c, 4ll * std::max(np, 1))), dim3(256), 0, c->stream, np, c->pairs.p,
                      c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
                      ph.discrete ? 0 : 1, ph.mt_from_bodies);
   Dev D = make_dev(c, 0, ph.discrete);
