@@ -789,7 +789,7 @@ __device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& 
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
   __shared__ int s_tot[2][4];
-  __shared__ CullTri s_tri[4][16];  // per wavefront: the node pair's 8 + 8 prepared triangles
+  __shared__ CullTri s_tri[4][4][16];  // per wavefront: 4 node pairs' 8 + 8 prepared triangles
   __shared__ unsigned long long s_base[2];
   const int lane = lane_id(), nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
   const long long n = D.n_soup();
@@ -818,42 +818,53 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
     const int batch = left < 64 ? (int)left : 64;
     unsigned long long my_keep = 0;
     int my_settled = 0;
-    for (int k = 0; k < batch; ++k) {
-      const int pair = rl(l_pair, k), offa = rl(l_offa, k), offb = rl(l_offb, k);
-      const int nta = rl(l_nta, k), ntb = rl(l_ntb, k);
-      const bool inner = rl(l_inner, k) != 0;
-      int i = lane >> 3, j = lane & 7;
-      bool valid = i < nta && j < ntb;
-      int cls = valid ? 1 : 0;
-      if (fp32) {  // wave-uniform
-        const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
-        const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
-        const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
-        const int side = (lane >> 3) & 1, kk = lane & 7;
-        if (lane < 16) {
-          CullTri mine{};
-          if (kk < (side ? ntb : nta)) {
-            const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
-            float rot[9], tr[3];
-            for (int r = 0; r < 3; ++r) {
-              for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
-              tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
-            }
-            cull_prep(rot, tr, load_tri<double>(side ? bb : ba, kk), mine);
+    const int i = lane >> 3, j = lane & 7;
+    for (int k0 = 0; k0 < batch; k0 += 4) {
+      if (fp32) {
+        // the triangles of four node pairs at once: lane l prepares triangle
+        // (l & 15) of node pair k0 + (l >> 4) (lanes 0-7 of a group: bucket a,
+        // 8-15: bucket b), so no lane idles in the preparation
+        const int kp = k0 + (lane >> 4), slot = lane & 15, side = slot >> 3, kk = slot & 7;
+        const int src = kp < batch ? kp : k0;
+        // every lane takes part in every shuffle (a shuffle inside divergent
+        // code may read an inactive source lane)
+        const int pair = __shfl(l_pair, src), nta = __shfl(l_nta, src), ntb = __shfl(l_ntb, src);
+        const int ga = __shfl(l_offa, src) + __shfl(l_bka, src), gb = __shfl(l_offb, src) + __shfl(l_bkb, src);
+        const int gbk = side ? gb : ga;
+        CullTri mine{};
+        if (kp < batch && kk < (side ? ntb : nta)) {
+          const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
+          const double* pm = side ? pr + PR_TB1 : pr + PR_TA1;
+          float rot[9], tr[3];
+          for (int r = 0; r < 3; ++r) {
+            for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
+            tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
           }
-          s_tri[wid][lane] = mine;
+          cull_prep(rot, tr, load_tri<double>(D.buckets + (size_t)BUCKET * gbk, kk), mine);
         }
+        s_tri[wid][lane >> 4][slot] = mine;
         wave_sync_lds();
-        if (valid) {  // operands read from LDS where they are used (fewer live registers)
-          const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
-          cls = cull_pair(s_tri[wid][i], s_tri[wid][8 + j], thr);
-        }
-        wave_sync_lds();  // read before the next node pair overwrites
       }
-      unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
-      const bool settled = inner && __ballot(cls == 2) != 0ull;
-      if (settled) keep = 0ull;
-      if (lane == k) my_keep = keep, my_settled = settled ? 1 : 0;
+      const int kend = k0 + 4 < batch ? k0 + 4 : batch;
+      for (int k = k0; k < kend; ++k) {
+        const int pair = rl(l_pair, k), offa = rl(l_offa, k), offb = rl(l_offb, k);
+        const int nta = rl(l_nta, k), ntb = rl(l_ntb, k);
+        const bool inner = rl(l_inner, k) != 0;
+        const bool valid = i < nta && j < ntb;
+        int cls = valid ? 1 : 0;
+        if (fp32 && valid) {  // operands read from LDS where they are used (fewer live registers)
+          const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
+          const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
+          const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
+          const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
+          cls = cull_pair(s_tri[wid][k - k0][i], s_tri[wid][k - k0][8 + j], thr);
+        }
+        unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
+        const bool settled = inner && __ballot(cls == 2) != 0ull;
+        if (settled) keep = 0ull;
+        if (lane == k) my_keep = keep, my_settled = settled ? 1 : 0;
+      }
+      if (fp32) wave_sync_lds();  // read before the next group overwrites
     }
     if (lane < batch) D.np_hit[wl] = my_settled;
     // The kept triangle pairs go straight to the level's item list: one slot
