@@ -726,22 +726,28 @@ __device__ __forceinline__ void cull_fetch(const CullTri& mine, int src, CullTri
   for (int k = 0; k < (int)(sizeof(CullTri) / sizeof(float)); ++k) d[k] = __shfl(m[k], src);
 }
 // 0 certain miss, 1 needs the FP64 test, 2 certain hit (the bounds of cull_class).
-__device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, float thr) {
+// The lower bound compares g^2 > T'^2 |n|^2 per axis instead of g / |n| > T,
+// with T' = T (1 + 2^-20) covering the few roundings of the squared form.
+// The upper bound (certain hit) is only computed when want_hit: a certain
+// hit only matters for node pairs with an inner node.
+__device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, float thr, bool want_hit = true) {
   const float EPSF = 5.9604645e-08f;
   if (A.ok == 0.f || B.ok == 0.f) return 1;
   const float kappa = fmaxf(A.kap, B.kap);
   const float L = sqrtf(fmaxf(A.L2, B.L2));
   const float margin = 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * fmaxf(A.crd, B.crd) + EPSF * thr;
-  float lb = 0.f;
+  const float T = (thr + margin) * (1.f + 9.5367432e-07f);
+  const float T2 = T * T;
+  bool sep = false;
   {  // axis nA
     const float b0 = f3dot(B.v, A.n), b1 = f3dot(B.v + 3, A.n), b2 = f3dot(B.v + 6, A.n);
     const float g = fmaxf(fminf(b0, fminf(b1, b2)) - A.pmax, A.pmin - fmaxf(b0, fmaxf(b1, b2)));
-    if (A.nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(A.nn));
+    sep = sep || (A.nn > 0.f && g > 0.f && g * g > T2 * A.nn);
   }
   {  // axis nB
     const float a0 = f3dot(A.v, B.n), a1 = f3dot(A.v + 3, B.n), a2 = f3dot(A.v + 6, B.n);
     const float g = fmaxf(B.pmin - fmaxf(a0, fmaxf(a1, a2)), fminf(a0, fminf(a1, a2)) - B.pmax);
-    if (B.nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(B.nn));
+    sep = sep || (B.nn > 0.f && g > 0.f && g * g > T2 * B.nn);
   }
   {  // centroid direction
     const float cd[3] = {B.cen[0] - A.cen[0], B.cen[1] - A.cen[1], B.cen[2] - A.cen[2]};
@@ -750,9 +756,10 @@ __device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, flo
     const float b0 = f3dot(B.v, cd), b1 = f3dot(B.v + 3, cd), b2 = f3dot(B.v + 6, cd);
     const float g = fmaxf(fminf(b0, fminf(b1, b2)) - fmaxf(a0, fmaxf(a1, a2)),
                           fminf(a0, fminf(a1, a2)) - fmaxf(b0, fmaxf(b1, b2)));
-    if (nn > 0.f && g > 0.f) lb = fmaxf(lb, g / sqrtf(nn));
+    sep = sep || (nn > 0.f && g > 0.f && g * g > T2 * nn);
   }
-  if (lb > thr + margin) return 0;
+  if (sep) return 0;
+  if (!want_hit) return 1;
   float ub2 = 3.4e38f;
   float dd[3][3];
   for (int x = 0; x < 3; ++x)
@@ -857,7 +864,7 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
           const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
           const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
           const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
-          cls = cull_pair(s_tri[wid][k - k0][i], s_tri[wid][k - k0][8 + j], thr);
+          cls = cull_pair(s_tri[wid][k - k0][i], s_tri[wid][k - k0][8 + j], thr, inner);
         }
         unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
         const bool settled = inner && __ballot(cls == 2) != 0ull;
