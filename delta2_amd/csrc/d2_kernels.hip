@@ -18,7 +18,12 @@
 //                  vertex-triangle and edge-edge sub-tests
 //     leaf_point / leaf_edge  one thread per sub-test, hit -> contact
 //   Discrete level (compareTreesFull, full_tree_comparison.h:13-67)
-//     tri_enum / tri_dist / expand
+//     tri_mask     one wavefront per node pair, lane = triangle pair: FP32
+//                  cull (or keep-all in FP64 mode), expansion descriptor,
+//                  kept pairs to the item list
+//     tri_dist     one thread per item: FP64 _comparisonDist, hit -> contact
+//                  or node-pair hit flag
+//     disc_expand  one thread per node pair: children of hit node pairs
 //   Post-processing: per-pair ordering of contacts by their breadth-first key,
 //   the contact filter (contact.h:44-69, libstdc++ std::sort permutation) and
 //   the TOI -> step rule (separate_clusters.cpp:496-564).
@@ -984,10 +989,11 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   }
 }
 
-// Children of hit node pairs (full_tree_comparison.h:26-62, 99-150) and, in
-// CCD, the sub-tests of the connected (leaf-leaf) list, in two chunked
+// CCD levels: children of hit node pairs (full_tree_comparison.h:26-62,
+// 99-150) and the sub-tests of the connected (leaf-leaf) list, in two chunked
 // passes: counts + block partials, then writes at scanned offsets, which
-// keeps every list in the reference's breadth-first order.
+// keeps every list in the reference's breadth-first order (discrete levels
+// expand in one pass, k_disc_expand).
 // ------------------------------------------- FP32 culls of CCD leaf sub-tests
 // A culled sub-test provably reports no distance below the search distance.
 //   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
@@ -1189,9 +1195,7 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
     long long as = 0, ac = 0;
     for (long long w = rb + threadIdx.x; w < re; w += blockDim.x) {
       int ns = 0, nc = 0;
-      if (D.discrete) {
-        if (D.np_hit[w]) ns = (D.exp_desc[w] >> 1) & 15;
-      } else if (D.np_hit[w]) {
+      if (D.np_hit[w]) {
         NP np = D.soup[w];
         const int32_t* pi = D.pair_info + 4 * np.pair;
         const NodeRec na = D.nodes[pi[2] + np.na];
@@ -1200,12 +1204,9 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
           bool adv_a = expand_side(na, nb);
           const NodeRec& par = adv_a ? na : nb;
           for (int c = 0; c < par.nchild; ++c) {
-            bool to_soup = true;
-            if (!D.discrete) {
-              const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
-              const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
-              to_soup = x.inner || y.inner;
-            }
+            const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
+            const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
+            const bool to_soup = x.inner || y.inner;
             ns += to_soup, nc += !to_soup;
           }
         }
@@ -1217,7 +1218,7 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
     const long long ts = block_sum(as), tc = block_sum(ac);
     if (threadIdx.x == 0) D.part_s[blockIdx.x] = (int)ts, D.part_c[blockIdx.x] = (int)tc;
   }
-  if (!D.discrete) {
+  {
     {  // connected list: sub-test counts
       const long long n = D.n_conn();
       long long rb, re;
@@ -1269,27 +1270,15 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
     partial_prefix(D.part_c, bc, tc);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP] = (unsigned long long)ts;
-      D.lvl[(D.level + 1) * LV_STRIDE + LV_CONN] = D.discrete ? 0ull : (unsigned long long)tc;
-      if (ts > D.cap_np || tc > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
+      D.lvl[(D.level + 1) * LV_STRIDE + LV_CONN] = (unsigned long long)tc;
+      if (ts > D.cap_np || tc > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], 1ull);
     }
     for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
       const long long w = t0 + threadIdx.x;
-      int tts, ttc = 0;
+      int tts, ttc;
       const int es = block_scan(w < re ? D.cnt_s[w] : 0, tts);
-      const int ec = D.discrete ? 0 : block_scan(w < re ? D.cnt_c[w] : 0, ttc);  // no connected list when discrete
-      if (w < re && D.discrete && D.cnt_s[w] > 0) {  // children from the node pair's descriptor
-        const NP np = D.soup[w];
-        const int desc = D.exp_desc[w];
-        const bool adv_a = desc & 1;
-        const int nchild = (desc >> 1) & 15, first = desc >> 5;
-        long long ps = bs + es;
-        for (int c = 0; c < nchild; ++c) {
-          const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
-          const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
-          const long long slot = ps++;
-          if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-        }
-      } else if (w < re && D.cnt_s[w] + D.cnt_c[w] > 0) {
+      const int ec = block_scan(w < re ? D.cnt_c[w] : 0, ttc);
+      if (w < re && D.cnt_s[w] + D.cnt_c[w] > 0) {
         NP np = D.soup[w];
         const int32_t* pi = D.pair_info + 4 * np.pair;
         const NodeRec na = D.nodes[pi[2] + np.na];
@@ -1301,8 +1290,7 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
           uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
           uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
           NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-          bool to_soup = true;
-          if (!D.discrete) to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
+          const bool to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
           long long slot = to_soup ? ps++ : pc++;
           if (slot < D.cap_np) (to_soup ? D.soup_next : D.conn_next)[slot] = child;
         }
@@ -1310,7 +1298,7 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
       bs += tts, bc += ttc;
     }
   }
-  if (!D.discrete) {
+  {
     const long long n = D.n_conn();
     long long rb, re, bp, tp, be, te;
     chunk_of(n, rb, re);
@@ -1319,7 +1307,7 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       D.lvl[D.level * LV_STRIDE + LV_PT] = (unsigned long long)tp;
       D.lvl[D.level * LV_STRIDE + LV_EE] = (unsigned long long)te;
-      if (tp > D.cap_items || te > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
+      if (tp > D.cap_items || te > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 1ull);
     }
     for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
       const long long w = t0 + threadIdx.x;
@@ -1357,6 +1345,61 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
       }
       bp += ttp, be += tte;
       __syncthreads();
+    }
+  }
+}
+
+// Discrete levels in one pass: the children of every hit node pair (from its
+// expansion descriptor) to the next list, slots taken by one atomic per
+// block and round on the next level's size (zeroed with the level table when the
+// phase starts).  The discrete phase's contacts carry their emission keys
+// (child index paths) and are ordered by them afterwards, so its lists need
+// no breadth-first order.
+__global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
+  __shared__ int s_tot[4];
+  __shared__ unsigned long long s_base;
+  const long long n = D.n_soup();
+  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long* next = &D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {  // block-uniform
+    const long long w = b0 + threadIdx.x;
+    int desc = 0, nc = 0;
+    if (w < n && D.np_hit[w]) {
+      desc = D.exp_desc[w];
+      nc = (desc >> 1) & 15;
+    }
+    int incl = nc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(incl, d);
+      if (lane >= d) incl += o;
+    }
+    if (lane == 63) s_tot[wid] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int tot = 0;
+      for (int v = 0; v < nw; ++v) tot += s_tot[v];
+      unsigned long long base = 0;
+      if (tot) {
+        base = atomicAdd(next, (unsigned long long)tot);
+        if ((long long)(base + tot) > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+      }
+      s_base = base;
+    }
+    __syncthreads();
+    long long slot = (long long)s_base + incl - nc;
+    for (int v = 0; v < wid; ++v) slot += s_tot[v];
+    __syncthreads();  // s_tot and s_base are rewritten by the next round
+    if (nc > 0) {
+      const NP np = D.soup[w];
+      const bool adv_a = desc & 1;
+      const int first = desc >> 5;
+      for (int c = 0; c < nc; ++c, ++slot) {
+        const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
+        const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
+        if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
+      }
     }
   }
 }
@@ -2411,6 +2454,30 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int* in, int* out, lon
   }
 }
 
+// The same scan for a short array (n <= SCAN_SMALL) in one block of 1024
+// threads: thread t sums its run of ceil(n / 1024) elements, the block scans
+// the sums, and each thread writes its run.
+constexpr long long SCAN_SMALL = 16384;
+__global__ void __launch_bounds__(1024) k_scan_small(const int* in, int* out, long long n,
+                                                     unsigned long long* total_dst) {
+  const long long per = (n + blockDim.x - 1) / blockDim.x;
+  const long long b = (long long)threadIdx.x * per;
+  const long long e = b + per < n ? b + per : n;
+  int acc = 0;
+  for (long long i = b; i < e; ++i) acc += in[i];
+  int total;
+  int run = block_scan(acc, total);
+  for (long long i = b; i < e; ++i) {
+    const int v = in[i];
+    out[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) {
+    out[n] = total;
+    if (total_dst) *total_dst = (unsigned long long)total;
+  }
+}
+
 // ------------------------------------------------------ cluster step sizes
 // The cluster step (separate_clusters.cpp:541-598,
 // time_step_selection_dynamic_continuous.cpp:17-21) is a minimum over its
@@ -2418,9 +2485,22 @@ __global__ void __launch_bounds__(256) k_scan_apply(const int* in, int* out, lon
 // first pair reading body a (:589-590); a minimum does not depend on the
 // order of the sequential loop.  Steps are non-negative, so they order like
 // their bit patterns.
-__global__ void k_cluster_init(int n_clusters, unsigned long long* step_bits, int* first_any, int* first_ns) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_clusters; k += gridDim.x * blockDim.x)
-    step_bits[k] = ~0ull, first_any[k] = 0x7fffffff, first_ns[k] = 0x7fffffff;
+// Cluster accumulators; with body_step (a step's projection), every body's
+// step starts as its clamped last step: a body in no pair is a cluster of
+// its own without interactions (separate_clusters.cpp:594-596,
+// time_step_selection_dynamic_continuous.cpp:19).
+__global__ void k_cluster_init(int n_clusters, unsigned long long* step_bits, int* first_any, int* first_ns,
+                               int n_bodies, float dt, const double* bodies, double* body_step) {
+  const int n = body_step && n_bodies > n_clusters ? n_bodies : n_clusters;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (k < n_clusters) step_bits[k] = ~0ull, first_any[k] = 0x7fffffff, first_ns[k] = 0x7fffffff;
+    if (body_step && k < n_bodies) {
+      double s = bodies[40 * (size_t)k + 39];
+      const double hi = (double)dt;
+      s = (hi < s) ? hi : s;
+      body_step[k] = (s < 1e-3) ? 1e-3 : s;
+    }
+  }
 }
 __global__ void k_cluster_first(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
                                 int* first_any, int* first_ns) {
@@ -2453,37 +2533,31 @@ __global__ void k_cluster_min(int n_pairs, const int32_t* pairs, const int32_t* 
 }
 // A cluster without pairs keeps its first particle's last step (:594-596);
 // first_member (clusters built on the device) names it, otherwise dt.
+__device__ __forceinline__ double cluster_step_of(int k, float dt, const int32_t* pairs, const double* bodies,
+                                                  const unsigned long long* step_bits, const int* first_any,
+                                                  const int* first_member) {
+  double s;
+  if (step_bits[k] != ~0ull) s = __longlong_as_double((long long)step_bits[k]);
+  else if (first_any[k] != 0x7fffffff) s = bodies[40 * (size_t)pairs[2 * first_any[k]] + 39];
+  else if (first_member) s = bodies[40 * (size_t)first_member[k] + 39];
+  else s = (double)dt;
+  const double hi = (double)dt;
+  s = (hi < s) ? hi : s;         // std::min(s, dt)
+  return (s < 1e-3) ? 1e-3 : s;  // std::max(1e-3, .)
+}
+// Cluster steps, and with body_step the per-body steps of bodies in pairs
+// (their cluster's step, computed per pair from the same accumulators).
 __global__ void k_cluster_final(int n_clusters, float dt, const int32_t* pairs, const double* bodies,
                                 const unsigned long long* step_bits, const int* first_any, const int* first_member,
-                                double* out) {
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n_clusters; k += gridDim.x * blockDim.x) {
-    double s;
-    if (step_bits[k] != ~0ull) s = __longlong_as_double((long long)step_bits[k]);
-    else if (first_any[k] != 0x7fffffff) s = bodies[40 * (size_t)pairs[2 * first_any[k]] + 39];
-    else if (first_member) s = bodies[40 * (size_t)first_member[k] + 39];
-    else s = (double)dt;
-    const double hi = (double)dt;
-    s = (hi < s) ? hi : s;        // std::min(s, dt)
-    out[k] = (s < 1e-3) ? 1e-3 : s;  // std::max(1e-3, .)
-  }
-}
-// Per-body step: its cluster's step; a body in no pair is a cluster of its
-// own without interactions, whose step is its clamped last step
-// (separate_clusters.cpp:594-596, time_step_selection_dynamic_continuous.cpp:19).
-__global__ void k_body_step_fill(int n_bodies, float dt, const double* bodies, double* body_step) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_bodies; i += gridDim.x * blockDim.x) {
-    double s = bodies[40 * (size_t)i + 39];
-    const double hi = (double)dt;
-    s = (hi < s) ? hi : s;
-    body_step[i] = (s < 1e-3) ? 1e-3 : s;
-  }
-}
-__global__ void k_body_step_pairs(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair,
-                                  const double* cluster_step, double* body_step) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
-    const double s = cluster_step[cluster_of_pair[p]];
-    body_step[pairs[2 * p]] = s;
-    body_step[pairs[2 * p + 1]] = s;
+                                double* out, int n_pairs, const int32_t* cluster_of_pair, double* body_step) {
+  const int n = body_step && n_pairs > n_clusters ? n_pairs : n_clusters;
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    if (k < n_clusters) out[k] = cluster_step_of(k, dt, pairs, bodies, step_bits, first_any, first_member);
+    if (body_step && k < n_pairs) {
+      const double s = cluster_step_of(cluster_of_pair[k], dt, pairs, bodies, step_bits, first_any, first_member);
+      body_step[pairs[2 * k]] = s;
+      body_step[pairs[2 * k + 1]] = s;
+    }
   }
 }
 
@@ -2647,6 +2721,7 @@ struct d2_ctx {
   // grids of the next runs' levels are sized from them
   long long prev_lv[2][MAX_LEVELS + 1][LV_STRIDE] = {};
   bool prev_ok[2] = {false, false};
+  long long prev_contacts[2] = {-1, -1};  // contacts of the last run's discrete / CCD phase (-1: none yet)
   // profiling
   bool profiling = false;
   long long k_launch[K_NUM] = {};
@@ -2841,6 +2916,10 @@ int prepare(d2_ctx* c) {
 
 // Exclusive scan of n (host-known) ints on the context stream.
 void scan_host_n(d2_ctx* c, const int* in, int* out, long long n, unsigned long long* total_dst) {
+  if (n <= SCAN_SMALL) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, c->stream, in, out, n, total_dst);
+    return;
+  }
   const int g = chunk_grid(c);
   hipLaunchKernelGGL(k_scan_part, dim3(g), dim3(256), 0, c->stream, in, n, c->scan_part.p);
   hipLaunchKernelGGL(k_scan_apply, dim3(g), dim3(256), 0, c->stream, in, out, n, c->scan_part.p, total_dst);
@@ -2898,8 +2977,12 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   }
   {
     Timer tm(c, K_EXPAND, 0.0);
-    hipLaunchKernelGGL(k_level_count, dim3(eg), dim3(256), 0, c->stream, D);
-    hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
+    if (discrete) {
+      hipLaunchKernelGGL(k_disc_expand, dim3(eg), dim3(256), 0, c->stream, D);
+    } else {
+      hipLaunchKernelGGL(k_level_count, dim3(eg), dim3(256), 0, c->stream, D);
+      hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
+    }
   }
   if (!discrete) {
     timed_launch(c, K_LEAF_POINT, k_leaf_point, dim3(sg), dim3(256), D);
@@ -2907,11 +2990,23 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   }
 }
 
+// Grid of the per-contact kernels of the post-processing: one contact per
+// thread at the last run's count of the phase (mode 2: CCD) with 25%
+// headroom, at least 16 blocks, at most the tuned stream grid.
+int post_grid(const d2_ctx* c, int mode) {
+  const int hi = stream_grid(c);
+  const long long n = c->prev_contacts[mode == 2 ? 1 : 0];
+  if (n < 0) return hi;
+  long long b = 16;
+  while (b < hi && b * 256 < n + n / 4) b <<= 1;
+  return (int)std::min<long long>(b, hi);
+}
+
 // Post-processing of the contacts of the last phase: per-pair order, then the
 // contact filter (mode 1), the step rule (mode 2) or nothing (mode 0).
 int enqueue_post(d2_ctx* c, int mode) {
   const int np = c->n_pairs;
-  const int sg = stream_grid(c);
+  const int sg = post_grid(c, mode);
   const bool filtered = mode == 1;
   {
     Timer tm(c, K_POST, 0.0);
@@ -2947,26 +3042,23 @@ int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
                      bool first_member) {
   const int np = c->n_pairs;
   Timer tm(c, K_CLUSTER, np);
-  hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters,
-                     c->step_bits.p, c->first_any.p, c->first_ns.p);
+  double* body_step = project ? c->body_step.p : nullptr;
+  hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, project ? std::max(n_clusters, c->n_bodies) : n_clusters)),
+                     dim3(256), 0, c->stream, n_clusters, c->step_bits.p, c->first_any.p, c->first_ns.p, c->n_bodies, dt,
+                     c->bodies.p, body_step);
   if (np > 0) {
     hipLaunchKernelGGL(k_cluster_first, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
                        c->bodies.p, c->first_any.p, c->first_ns.p);
     hipLaunchKernelGGL(k_cluster_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
                        c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->step_bits.p);
   }
-  hipLaunchKernelGGL(k_cluster_final, dim3(grid_for(c, n_clusters)), dim3(256), 0, c->stream, n_clusters, dt,
-                     c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p,
-                     first_member ? c->cc_first.p : (const int*)nullptr, c->cluster_step.p);
-  if (project) {
-    hipLaunchKernelGGL(k_body_step_fill, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, dt,
-                       c->bodies.p, c->body_step.p);
-    if (np > 0)
-      hipLaunchKernelGGL(k_body_step_pairs, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p,
-                         cluster_of_pair, c->cluster_step.p, c->body_step.p);
+  hipLaunchKernelGGL(k_cluster_final, dim3(grid_for(c, project ? std::max(n_clusters, np) : n_clusters)), dim3(256), 0,
+                     c->stream, n_clusters, dt, c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p,
+                     first_member ? c->cc_first.p : (const int*)nullptr, c->cluster_step.p, np, cluster_of_pair,
+                     body_step);
+  if (project)
     hipLaunchKernelGGL(k_project, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p,
                        c->body_step.p, 1);
-  }
   HIPCHK(c, hipGetLastError());
   return D2_OK;
 }
@@ -3151,7 +3243,8 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   sig.learned = (c->learned_on && c->mlp_ready) ? c->gate_phases : 0, sig.gate_t = c->gate_t, sig.gate_s = c->gate_s;
   sig.cl_dev = kind == RUN_STEP && c->clusters_on_device ? 1 : 0;
   for (int i = 0; i < n; ++i) sig.levels[i] = std::min(c->phase_levels[ph[i].slot], budget);
-  for (int i = 0; i < n; ++i) sig.grids = sig.grids * 31 + grids_hash(c, ph[i].discrete, sig.levels[i]);
+  for (int i = 0; i < n; ++i)
+    sig.grids = (sig.grids * 31 + grids_hash(c, ph[i].discrete, sig.levels[i])) * 31 + post_grid(c, ph[i].post_mode);
   std::vector<std::function<int()>> ops;
   int seg = 0;
   for (int i = 0; i < n; ++i) {
@@ -3211,6 +3304,9 @@ int finish_run(d2_ctx* c, int kind, int filter) {
     if (kind == RUN_CCD) keep(1, c->host_lvl);
     if (kind == RUN_NARROW) keep(0, c->host_lvl);
     if (kind == RUN_STEP) keep(0, c->host_lvl), keep(1, c->host_lvl + half);
+    if (kind == RUN_CCD) c->prev_contacts[1] = (long long)c->host_cnt[C_CONTACTS];
+    else c->prev_contacts[0] = (long long)c->host_cnt[C_CONTACTS];
+    if (kind == RUN_STEP) c->prev_contacts[1] = (long long)c->host_lvl[half + LV_STRIDE - 1];
   }
   c->n_out = (long long)c->host_cnt[C_OUT];
   c->last_mode = kind == RUN_CCD ? 1 : 0;
