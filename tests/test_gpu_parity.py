@@ -76,6 +76,25 @@ def test_narrow_parity(ctx, name, make, filt):
         assert gpu.n > 100  # the dense scene must exercise many contacts
 
 
+@pytest.mark.parametrize("prec", [d2.D2_FP64, d2.D2_FP32])
+def test_narrow_repeat_runs(prec):
+    """A fresh context's first run launches every level with the default
+    grids; later runs size each level's grids (and the post-processing's)
+    from the previous run's sizes.  Every run equals the oracle."""
+    s = dense_scene()
+    rec = with_future(s.records, scenes.DT)
+    o_off, o_ints, o_reals = oracle_scene(s, rec).narrow(s.pairs, filter=True)
+    c = d2.Context(0, prec)
+    try:
+        load(c, s, rec)
+        for run in range(3):
+            print(f"[narrow repeat] precision={prec} run={run}", flush=True)
+            gpu = d2.compare_trees_full(c, s.pairs, filter=True)
+            assert_same(gpu, o_off, o_ints, o_reals, f"narrow repeat {prec} run {run}")
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("name,make", SCENES)
 def test_ccd_parity(ctx, name, make):
     s = make()
