@@ -19,11 +19,12 @@
 //     leaf_point / leaf_edge  one thread per sub-test, hit -> contact
 //   Discrete level (compareTreesFull, full_tree_comparison.h:13-67)
 //     tri_mask     one wavefront per node pair, lane = triangle pair: FP32
-//                  cull (or keep-all in FP64 mode), expansion descriptor,
-//                  kept pairs to the item list
+//                  cull (or keep-all in FP64 mode), kept pairs to the item
+//                  list, children of node pairs that hit or may hit to the
+//                  next list (dropped there if the parent did not hit)
 //     tri_dist     one thread per item: FP64 _comparisonDist, hit -> contact
-//                  or node-pair hit flag
-//     disc_expand  one thread per node pair: children of hit node pairs
+//                  (leaf-leaf) or the node pair's hit flag
+//     disc_expand  (FP64 mode) one thread per node pair: children of hits
 //   Post-processing: per-pair ordering of contacts by their breadth-first key,
 //   the contact filter (contact.h:44-69, libstdc++ std::sort permutation) and
 //   the TOI -> step rule (separate_clusters.cpp:496-564).
@@ -164,6 +165,8 @@ struct Dev {
   Item* ee_items;
   double* res;  // [items][15][8]
   int* np_hit;
+  int* exp_desc;               // per discrete node pair: expansion_desc (FP64 mode)
+  const int* np_hit_prev;      // the previous level's hit flags (discrete: parity buffers)
   ContactRec* contacts;
   unsigned long long* cnt;
   unsigned long long* lvl;
@@ -178,7 +181,6 @@ struct Dev {
   double* min_time;
   const float* tree_eps_out;   // float-barycentric acceptance bound per tree
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
-  int* exp_desc;                  // per discrete node pair: expansion_desc
   const int32_t* body_tree;
   int fp32;                    // FP32 culls of the CCD leaf sub-tests
   int n_pairs;
@@ -789,9 +791,8 @@ __device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, flo
 __device__ __forceinline__ bool expand_side(const NodeRec& na, const NodeRec& nb) {
   return na.inner && (!nb.inner || na.depth < nb.depth);  // true: advance a
 }
-// Expansion of a node pair of the discrete list, packed for the level's
-// count/write kernels: bit 0 advance a, bits 1-4 child count, bits 5-31 first
-// child (0: two leaves, no children).
+// Expansion of a node pair of the discrete list, packed: bit 0 advance a,
+// bits 1-4 child count, bits 5-31 first child (0: two leaves, no children).
 __device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& nb) {
   if (!(na.inner || nb.inner)) return 0;
   const bool adv_a = expand_side(na, nb);
@@ -799,10 +800,35 @@ __device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& 
   return (adv_a ? 1 : 0) | (par.nchild << 1) | (par.child_first << 5);
 }
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+// Inclusive prefix sum over the wavefront (every lane active).
+__device__ __forceinline__ int wave_incl_scan(int v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d);
+    if (lane_id() >= d) v += o;
+  }
+  return v;
+}
+// Discrete lists: the children of a node pair that hits or may hit, at slot,
+// slot + 1, ... of the next list, each naming its parent: parent 0 for a
+// certain hit, list index + 1 when the parent's FP64 tests (k_tri_dist, its
+// hit flag) decide; the next level drops the children of parents that did
+// not hit.  The discrete phase's contacts carry their emission keys (child
+// index paths) and are ordered by them afterwards, so its lists need no
+// breadth-first order and children take slots by atomics.
+__device__ __forceinline__ void write_children(const Dev& D, long long slot, const NP& np, int desc, uint32_t parent) {
+  const bool adv_a = desc & 1;
+  const int nc = (desc >> 1) & 15, first = desc >> 5;
+  for (int c = 0; c < nc; ++c, ++slot) {
+    const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
+    const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
+    if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, parent, np.key * 8ull + (unsigned long long)c};
+  }
+}
 __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
-  __shared__ int s_tot[2][4];
+  __shared__ int s_tot[2][4], s_ch[2][4];
   __shared__ CullTri s_tri[4][4][16];  // per wavefront: 4 node pairs' 8 + 8 prepared triangles
-  __shared__ unsigned long long s_base[2];
+  __shared__ unsigned long long s_base[2], s_chb[2];
   const int lane = lane_id(), nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
   const long long n = D.n_soup();
   long long rb, re;
@@ -815,16 +841,22 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
   for (long long b0 = rb; b0 < re; b0 += 64ll * nw, ++it) {
     const long long w0 = b0 + wid;
     const long long wl = w0 + (long long)nw * lane;
-    int l_pair = 0, l_offa = 0, l_offb = 0, l_bka = 0, l_bkb = 0, l_nta = 0, l_ntb = 0, l_inner = 0;
+    int l_pair = 0, l_offa = 0, l_offb = 0, l_bka = 0, l_bkb = 0, l_nta = 0, l_ntb = 0, l_inner = 0, l_desc = 0;
+    NP l_np{};
     if (wl < re) {
       const NP np = D.soup[wl];
       const int32_t* pi = D.pair_info + 4 * np.pair;
       l_pair = (int)np.pair, l_offa = pi[2], l_offb = pi[3];
       const NodeRec na = D.nodes[l_offa + np.na];
       const NodeRec nb = D.nodes[l_offb + np.nb];
-      l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
-      l_inner = (na.inner || nb.inner) ? 1 : 0;
-      D.exp_desc[wl] = expansion_desc(na, nb);
+      // a child of a parent that did not hit: no work, no children
+      if (np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) {
+        l_np = np;
+        l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
+        l_inner = (na.inner || nb.inner) ? 1 : 0;
+        l_desc = expansion_desc(na, nb);
+      }
+      if (!fp32) D.exp_desc[wl] = l_desc;
     }
     const long long left = w0 < re ? (re - w0 + nw - 1) / nw : 0;
     const int batch = left < 64 ? (int)left : 64;
@@ -879,30 +911,38 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
       if (fp32) wave_sync_lds();  // read before the next group overwrites
     }
     if (lane < batch) D.np_hit[wl] = my_settled;
-    // The kept triangle pairs go straight to the level's item list: one slot
-    // range per block and round, one atomic (item order does not matter:
-    // contacts carry their emission keys).
+    // The kept triangle pairs go straight to the level's item list, and the
+    // children of node pairs that hit (settled) or may hit (kept pairs) to
+    // the next list: one slot range each per block and round, one atomic
+    // each (order does not matter: contacts carry their emission keys).
     const int mine = lane < batch ? __popcll(my_keep) : 0;
-    int incl = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(incl, d);
-      if (lane >= d) incl += o;
-    }
+    // (FP32 mode; in FP64 mode every node pair keeps its pairs, so k_disc_expand
+    // expands the hit ones after k_tri_dist)
+    const int nch = fp32 && lane < batch && (my_settled || my_keep != 0ull) ? (l_desc >> 1) & 15 : 0;
+    const int incl = wave_incl_scan(mine), ich = wave_incl_scan(nch);
     const int par = it & 1;
-    if (lane == 63) s_tot[par][wid] = incl;
+    if (lane == 63) s_tot[par][wid] = incl, s_ch[par][wid] = ich;
     __syncthreads();
     if (threadIdx.x == 0) {
-      int sum = 0;
-      for (int v = 0; v < nw; ++v) sum += s_tot[par][v];
-      unsigned long long base = 0;
+      int sum = 0, sch = 0;
+      for (int v = 0; v < nw; ++v) sum += s_tot[par][v], sch += s_ch[par][v];
+      unsigned long long base = 0, chb = 0;
       if (sum) {
         base = atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_ITEMS], (unsigned long long)sum);
         if ((long long)(base + sum) > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
       }
-      s_base[par] = base;
+      if (sch) {
+        chb = atomicAdd(&D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP], (unsigned long long)sch);
+        if ((long long)(chb + sch) > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+      }
+      s_base[par] = base, s_chb[par] = chb;
     }
     __syncthreads();
+    if (nch > 0) {
+      long long slot = (long long)s_chb[par] + ich - nch;
+      for (int v = 0; v < wid; ++v) slot += s_ch[par][v];
+      write_children(D, slot, l_np, l_desc, my_settled ? 0u : (uint32_t)(wl + 1));
+    }
     long long base = (long long)s_base[par];
     for (int v = 0; v < wid; ++v) base += s_tot[par][v];
     const int excl = incl - mine;
@@ -993,7 +1033,7 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
 // 99-150) and the sub-tests of the connected (leaf-leaf) list, in two chunked
 // passes: counts + block partials, then writes at scanned offsets, which
 // keeps every list in the reference's breadth-first order (discrete levels
-// expand in one pass, k_disc_expand).
+// are expanded by k_tri_mask, or by k_disc_expand in FP64 mode).
 // ------------------------------------------- FP32 culls of CCD leaf sub-tests
 // A culled sub-test provably reports no distance below the search distance.
 //   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
@@ -1349,18 +1389,13 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
   }
 }
 
-// Discrete levels in one pass: the children of every hit node pair (from its
-// expansion descriptor) to the next list, slots taken by one atomic per
-// block and round on the next level's size (zeroed with the level table when the
-// phase starts).  The discrete phase's contacts carry their emission keys
-// (child index paths) and are ordered by them afterwards, so its lists need
-// no breadth-first order.
+// FP64-mode discrete levels: the children of every hit node pair (from its
+// expansion descriptor) to the next list, one atomic per block and round.
 __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
   __shared__ int s_tot[4];
   __shared__ unsigned long long s_base;
   const long long n = D.n_soup();
   const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  unsigned long long* next = &D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP];
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {  // block-uniform
     const long long w = b0 + threadIdx.x;
@@ -1369,12 +1404,7 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
       desc = D.exp_desc[w];
       nc = (desc >> 1) & 15;
     }
-    int incl = nc;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const int o = __shfl_up(incl, d);
-      if (lane >= d) incl += o;
-    }
+    const int incl = wave_incl_scan(nc);
     if (lane == 63) s_tot[wid] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -1382,7 +1412,7 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
       for (int v = 0; v < nw; ++v) tot += s_tot[v];
       unsigned long long base = 0;
       if (tot) {
-        base = atomicAdd(next, (unsigned long long)tot);
+        base = atomicAdd(&D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP], (unsigned long long)tot);
         if ((long long)(base + tot) > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
       }
       s_base = base;
@@ -1391,16 +1421,7 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
     long long slot = (long long)s_base + incl - nc;
     for (int v = 0; v < wid; ++v) slot += s_tot[v];
     __syncthreads();  // s_tot and s_base are rewritten by the next round
-    if (nc > 0) {
-      const NP np = D.soup[w];
-      const bool adv_a = desc & 1;
-      const int first = desc >> 5;
-      for (int c = 0; c < nc; ++c, ++slot) {
-        const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
-        const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
-        if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-      }
-    }
+    if (nc > 0) write_children(D, slot, D.soup[w], desc, 0u);
   }
 }
 
@@ -2656,7 +2677,7 @@ struct d2_ctx {
   DBuf<Item> items, pt_items, ee_items;
   DBuf<DItem> ditems;
   DBuf<double> res;
-  DBuf<int> np_hit, exp_desc;
+  DBuf<int> np_hit, np_hit2, exp_desc;
   DBuf<ContactRec> contacts;
   DBuf<unsigned long long> cnt, lvl, masks, conn_mask;
   DBuf<int> cnt_a, cnt_s, cnt_c, cnt_pt, cnt_ee, parts;
@@ -2854,7 +2875,9 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.pt_items = c->pt_items.p;
   D.ee_items = c->ee_items.p;
   D.res = c->res.p;
-  D.np_hit = c->np_hit.p;
+  D.np_hit = (level & 1) ? c->np_hit2.p : c->np_hit.p;
+  D.np_hit_prev = (level & 1) ? c->np_hit.p : c->np_hit2.p;
+  D.exp_desc = c->exp_desc.p;
   D.contacts = c->contacts.p;
   D.cnt = c->cnt.p;
   D.lvl = c->lvl.p;
@@ -2868,7 +2891,6 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.min_time = c->min_time.p;
   D.tree_eps_out = c->tree_eps_out.p;
   D.conn_mask = c->conn_mask.p;
-  D.exp_desc = c->exp_desc.p;
   D.body_tree = c->body_tree.p;
   D.fp32 = c->precision == D2_FP32 ? 1 : 0;
   D.n_pairs = c->n_pairs;
@@ -2894,7 +2916,7 @@ int prepare(d2_ctx* c) {
     RC(ensure(c, c->conn_list[k], c->cap));
   }
   RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
-  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->ditems, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->ditems, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->np_hit2, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
@@ -2975,14 +2997,13 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     timed_launch(c, K_SOUP_EDGE, k_soup_edge, dim3(sg), dim3(256), D);
     timed_launch(c, K_SOUP_REDUCE, k_soup_reduce, dim3(sg), dim3(256), D);
   }
-  {
+  if (discrete && c->precision != D2_FP32) {
     Timer tm(c, K_EXPAND, 0.0);
-    if (discrete) {
-      hipLaunchKernelGGL(k_disc_expand, dim3(eg), dim3(256), 0, c->stream, D);
-    } else {
-      hipLaunchKernelGGL(k_level_count, dim3(eg), dim3(256), 0, c->stream, D);
-      hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
-    }
+    hipLaunchKernelGGL(k_disc_expand, dim3(eg), dim3(256), 0, c->stream, D);
+  } else if (!discrete) {  // FP32 discrete levels are expanded by k_tri_mask
+    Timer tm(c, K_EXPAND, 0.0);
+    hipLaunchKernelGGL(k_level_count, dim3(eg), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
   }
   if (!discrete) {
     timed_launch(c, K_LEAF_POINT, k_leaf_point, dim3(sg), dim3(256), D);
@@ -3429,7 +3450,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
-  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->exp_desc);
+  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
   release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
