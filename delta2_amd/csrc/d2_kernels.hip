@@ -615,7 +615,12 @@ __global__ void __launch_bounds__(256) k_leaf_edge(Dev D) {
 // dist_ref <= the smallest such bound over the 9 edge pairs.
 // margin = 64*eps_f*kappa*L (4x the barycentric bound) + 16*eps_f*|coords|
 // (FP32 rounding of the relative coordinates and projections) + 1e-6 m.
-__device__ __forceinline__ float f3dot(const float* a, const float* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+// FP32 culls only decide which FP64 tests run, so their arithmetic may fuse:
+// an FMA rounds once where a multiply-add rounds twice, so the error bounds
+// the margins cover (gamma_n |a||b| for an n-term dot product) still hold.
+__device__ __forceinline__ float f3dot(const float* a, const float* b) {
+  return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
 __device__ int cull_class(const double* pa, const double* pb, const tri<double>& la, const tri<double>& lt, double thr_d) {
   const float EPSF = 5.9604645e-08f;
   float ra[9], rb[9], ta[3];
@@ -707,7 +712,7 @@ __device__ __forceinline__ void cull_prep(const float* rot, const float* tr, con
   for (int v = 0; v < 3; ++v) {
     const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
     for (int r = 0; r < 3; ++r) {
-      o.v[3 * v + r] = ((rot[3 * r] * x + rot[3 * r + 1] * y) + rot[3 * r + 2] * z) + tr[r];
+      o.v[3 * v + r] = fmaf(rot[3 * r + 2], z, fmaf(rot[3 * r + 1], y, fmaf(rot[3 * r], x, tr[r])));
       crd = fmaxf(crd, fabsf(o.v[3 * v + r]));
     }
   }
