@@ -165,6 +165,7 @@ struct Dev {
   Item* ee_items;
   double* res;  // [items][15][8]
   int* np_hit;
+  int* pair_cnt;               // contacts per pair, counted as they are written
   int* exp_desc;               // per discrete node pair: expansion_desc (FP64 mode)
   const int* np_hit_prev;      // the previous level's hit flags (discrete: parity buffers)
   ContactRec* contacts;
@@ -255,9 +256,23 @@ __device__ void partial_prefix(const int* part, long long& before, long long& to
 // a's kinematics; 2: TB0, inv(TA0) TB0; 3: TB1, inv(TA1) TB1 and b's
 // kinematics), so the longest chain holds two extrapolations, one inverse
 // and one product instead of all of them.
-__global__ void k_setup(int n_pairs, const int32_t* pairs, const double* max_time, const double* bodies,
+// The same kernel starts the phase: per pair the root node pair, the pass
+// accumulators, min_time and scale (a step's discrete phase keeps its CCD
+// phase's), the post-processing counters; block 0 clears the counters (a
+// step's discrete phase keeps the CCD phase's overflow flag and gate counts)
+// and sets the level table to one level of n_pairs roots.
+__global__ void k_setup(Dev D, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
-                        int ccd, int mt_from_bodies) {
+                        int ccd, int mt_from_bodies, int keep_ccd, double* scale, int* pair_fill, int* kept_cnt,
+                        int* n_big) {
+  const int n_pairs = D.n_pairs;
+  if (blockIdx.x == 0) {
+    for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)
+      if (!(keep_ccd && (i == C_OVERFLOW || i == C_GATED_CCD || i == C_CERT_CCD))) D.cnt[i] = 0;
+    for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)
+      D.lvl[i] = i == LV_SOUP ? (unsigned long long)n_pairs : 0ull;
+    if (threadIdx.x == 0) D.pair_cnt[n_pairs] = 0, kept_cnt[n_pairs] = 0, n_big[0] = 0, n_big[1] = 0;
+  }
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 4ll * n_pairs;
        t += (long long)gridDim.x * blockDim.x) {
     const int p = (int)(t >> 2), q = (int)(t & 3);
@@ -309,30 +324,15 @@ __global__ void k_setup(int n_pairs, const int32_t* pairs, const double* max_tim
       pair_info[4 * p + 1] = b;
       pair_info[4 * p + 2] = tree_off[body_tree[a]];
       pair_info[4 * p + 3] = tree_off[body_tree[b]];
+      D.soup[p] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
+      D.pass_min_s[p] = ~0ull, D.pass_min_c[p] = ~0ull;
+      D.pass_flag_s[p] = 0, D.pass_flag_c[p] = 0;
+      if (!keep_ccd) D.min_time[p] = mt, scale[p] = 1.0;
+      D.pair_cnt[p] = 0, kept_cnt[p] = 0, pair_fill[p] = 0;
     }
   }
 }
 
-__global__ void k_roots(Dev D, const double* pair_rec, double* scale, int keep_ccd) {
-  const int n_pairs = D.n_pairs;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
-    D.soup[p] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
-    D.pass_min_s[p] = ~0ull;
-    D.pass_min_c[p] = ~0ull;
-    D.pass_flag_s[p] = 0;
-    D.pass_flag_c[p] = 0;
-    if (!keep_ccd) {
-      D.min_time[p] = pair_rec[(size_t)PR_STRIDE * p + PR_MAXT];
-      scale[p] = 1.0;
-    }
-  }
-  if (blockIdx.x == 0) {
-    for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)  // a step keeps its CCD phase's flags and gate counts
-      if (!(keep_ccd && (i == C_OVERFLOW || i == C_GATED_CCD || i == C_CERT_CCD))) D.cnt[i] = 0;
-    for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)
-      D.lvl[i] = i == LV_SOUP ? (unsigned long long)n_pairs : 0ull;
-  }
-}
 
 // Folds one pass: "min toc of every hit of this pass" enters min_time when a
 // leaf-leaf node pair of the pair had hits (full_tree_comparison.h:106-108,167-169).
@@ -501,6 +501,7 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D) {
       atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
       continue;
     }
+    atomicAdd(&D.pair_cnt[np.pair], 1);
     ContactRec c;
     c.key = make_key(2 * D.level, 0, np.key, (int)item.sub);
     c.pair = (int32_t)np.pair, c.na = (int32_t)np.na, c.nb = (int32_t)np.nb, c.kind = 0;
@@ -526,6 +527,7 @@ __device__ __forceinline__ void emit_leaf(Dev& D, const NP& np, const double* pr
     atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
     return;
   }
+  atomicAdd(&D.pair_cnt[np.pair], 1);
   ContactRec c;
   c.key = make_key(2 * D.level + 1, 0, np.key, sub);
   c.pair = (int32_t)np.pair, c.na = (int32_t)np.na, c.nb = (int32_t)np.nb, c.kind = kind;
@@ -1023,6 +1025,7 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
       atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
       continue;
     }
+    atomicAdd(&D.pair_cnt[item.pair], 1);
     const NP np = D.soup[item.np];
     ContactRec c;
     c.key = make_key(D.level, 0, np.key, sub);
@@ -1434,21 +1437,6 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
 __device__ __forceinline__ long long n_contacts(const unsigned long long* cnt, long long cap) {
   long long n = (long long)cnt[C_CONTACTS];
   return n < cap ? n : cap;
-}
-// Zeroes the per-pair post-processing counters (a kernel, so that the step
-// can be captured in a graph).
-__global__ void k_post_init(int n_pairs, int* pair_cnt, int* pair_fill, int* kept_cnt, int* n_big) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= n_pairs; p += gridDim.x * blockDim.x) {
-    pair_cnt[p] = 0;
-    kept_cnt[p] = 0;
-    if (p < n_pairs) pair_fill[p] = 0;
-    if (p == 0) n_big[0] = 0, n_big[1] = 0;
-  }
-}
-__global__ void k_count(const unsigned long long* cnt, long long cap, const ContactRec* c, int* pair_cnt) {
-  const long long n = n_contacts(cnt, cap);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
-    atomicAdd(&pair_cnt[c[t].pair], 1);
 }
 __global__ void k_scatter(const unsigned long long* cnt, long long cap, const ContactRec* c, const int* pair_off,
                           int* pair_fill, int* perm) {
@@ -2883,6 +2871,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.np_hit = (level & 1) ? c->np_hit2.p : c->np_hit.p;
   D.np_hit_prev = (level & 1) ? c->np_hit.p : c->np_hit2.p;
   D.exp_desc = c->exp_desc.p;
+  D.pair_cnt = c->pair_cnt.p;
   D.contacts = c->contacts.p;
   D.cnt = c->cnt.p;
   D.lvl = c->lvl.p;
@@ -3035,11 +3024,7 @@ int enqueue_post(d2_ctx* c, int mode) {
   const int sg = post_grid(c, mode);
   const bool filtered = mode == 1;
   {
-    Timer tm(c, K_POST, 0.0);
-    hipLaunchKernelGGL(k_post_init, dim3(grid_for(c, np + 1)), dim3(256), 0, c->stream, np, c->pair_cnt.p,
-                       c->pair_fill.p, c->kept_cnt.p, c->n_big.p);
-    hipLaunchKernelGGL(k_count, dim3(sg), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts, c->contacts.p,
-                       c->pair_cnt.p);
+    Timer tm(c, K_POST, 0.0);  // per-pair counts come from the emitting kernels, zeroed by k_setup
     scan_host_n(c, c->pair_cnt.p, c->pair_off.p, np, filtered ? nullptr : c->cnt.p + C_OUT);
     hipLaunchKernelGGL(k_scatter, dim3(sg), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts, c->contacts.p,
                        c->pair_off.p, c->pair_fill.p, c->perm.p);
@@ -3114,12 +3099,11 @@ int phases_of(int kind, int filter, PhaseSpec* out) {
 void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
   const int np = c->n_pairs;
   Timer tm(c, K_SETUP, np);
-  hipLaunchKernelGGL(k_setup, dim3(grid_for(c, 4ll * std::max(np, 1))), dim3(256), 0, c->stream, np, c->pairs.p,
-                     c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
-                     ph.discrete ? 0 : 1, ph.mt_from_bodies);
   Dev D = make_dev(c, 0, ph.discrete);
-  hipLaunchKernelGGL(k_roots, dim3(grid_for(c, std::max(np, 1))), dim3(256), 0, c->stream, D, c->pair_rec.p,
-                     c->scale.p, ph.keep_ccd);
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(c, 4ll * std::max(np, 1))), dim3(256), 0, c->stream, D, c->pairs.p,
+                     c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
+                     ph.discrete ? 0 : 1, ph.mt_from_bodies, ph.keep_ccd, c->scale.p, c->pair_fill.p, c->kept_cnt.p,
+                     c->n_big.p);
 }
 
 // The learned gate of a phase: proposals (once per run), certificates, and
