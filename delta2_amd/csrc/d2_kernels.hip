@@ -9,14 +9,15 @@
 //     soup_cull    one wavefront per soup node pair, lane = triangle pair
 //                  (8x8): swept-AABB test, ballot/popcount compaction of the
 //                  overlapping triangle pairs into an item list
-//     soup_point   one thread per (item, point-triangle sub-test)  6 per item
-//     soup_edge    one thread per (item, edge-edge sub-test)       9 per item
+//     soup_sub     one thread per (item, point-triangle sub-test)  6 per item,
+//                  and (other blocks) per (item, edge-edge sub-test) 9 per item
 //     soup_reduce  one thread per item: the reference's ordered selection of
 //                  the 15 sub-results, hit test, leaf-pair contact emission
 //     expand       one thread per node pair: children of hit node pairs
 //     leaf_enum    one wavefront per leaf-leaf node pair: enumerates its
 //                  vertex-triangle and edge-edge sub-tests
-//     leaf_point / leaf_edge  one thread per sub-test, hit -> contact
+//     leaf_sub     one thread per sub-test (point-triangle blocks, edge-edge
+//                  blocks), hit -> contact
 //   Discrete level (compareTreesFull, full_tree_comparison.h:13-67)
 //     tri_mask     one wavefront per node pair, lane = triangle pair: FP32
 //                  cull (or keep-all in FP64 mode), kept pairs to the item
@@ -349,14 +350,17 @@ __device__ __forceinline__ void fold_pass(const Dev& D, const double* pair_rec, 
 }
 // Keeps the CCD phase's sizes (and its contact count in the reserved field of
 // level 0) next to the discrete phase's, for sizing the capacities.
-__global__ void k_save_ccd_sizes(Dev D) {
+__device__ __forceinline__ void save_ccd_sizes(Dev& D) {
   const int n = (MAX_LEVELS + 1) * LV_STRIDE;
   for (int i = threadIdx.x; i < n; i += blockDim.x)
     D.lvl[n + i] = i == LV_STRIDE - 1 ? D.cnt[C_CONTACTS] : D.lvl[i];
 }
+// Closing fold of the CCD phase; block 0 also keeps the phase's level sizes
+// and contact count in the second half of the level table.
 __global__ void k_fold_final(Dev D) {
   fold_pass(D, D.pair_rec, D.pass_min_s, D.pass_flag_s);
   fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
+  if (blockIdx.x == 0) save_ccd_sizes(D);
 }
 
 // --------------------------------------------------------- CCD soup level
@@ -423,9 +427,9 @@ __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<
 
 // Point-triangle sub-tests of soup items; slot 5k+dir (k: vertex index,
 // dir 0: vertex of a vs triangle of b, dir 1: vertex of b vs triangle of a).
-__global__ void __launch_bounds__(256) k_soup_point(Dev D) {
+__device__ __forceinline__ void soup_point_body(Dev& D, long long blk, long long nblk) {
   const long long total = D.n_res() * 6;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
     long long it = t / 6;
     int s = (int)(t % 6), k = s >> 1, dir = s & 1;
     Item item = D.items[it];
@@ -446,9 +450,9 @@ __global__ void __launch_bounds__(256) k_soup_point(Dev D) {
 }
 
 // Edge-edge sub-tests of soup items; slot 5k+2+l (edge k of a, edge l of b).
-__global__ void __launch_bounds__(256) k_soup_edge(Dev D) {
+__device__ __forceinline__ void soup_edge_body(Dev& D, long long blk, long long nblk) {
   const long long total = D.n_res() * 9;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
     long long it = t / 9;
     int s = (int)(t % 9), k = s / 3, l = s % 3;
     Item item = D.items[it];
@@ -469,6 +473,15 @@ __global__ void __launch_bounds__(256) k_soup_edge(Dev D) {
                          P, Q, toc);
     store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8, d, P, Q, toc);
   }
+}
+
+// Both kinds of sub-tests of the level's soup items in one launch: blocks
+// [0, g/2) take the point-triangle tests, [g/2, g) the edge-edge tests, so no
+// wavefront mixes the two code paths.
+__global__ void __launch_bounds__(256) k_soup_sub(Dev D) {
+  const int h = (int)(gridDim.x >> 1);
+  if ((int)blockIdx.x < h) soup_point_body(D, blockIdx.x, h);
+  else soup_edge_body(D, blockIdx.x - h, gridDim.x - h);
 }
 
 // triTriCCDLinear's selection over its 15 sub-results in the reference order
@@ -537,9 +550,9 @@ __device__ __forceinline__ void emit_leaf(Dev& D, const NP& np, const double* pr
   D.contacts[slot] = c;
 }
 
-__global__ void __launch_bounds__(256) k_leaf_point(Dev D) {
+__device__ __forceinline__ void leaf_point_body(Dev& D, long long blk, long long nblk) {
   const long long count = D.n_lv(LV_PT);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = blk * blockDim.x + threadIdx.x; t < count; t += nblk * blockDim.x) {
     Item item = D.pt_items[t];
     NP np = D.conn[item.np];
     const int32_t* pi = D.pair_info + 4 * np.pair;
@@ -572,9 +585,9 @@ __global__ void __launch_bounds__(256) k_leaf_point(Dev D) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_leaf_edge(Dev D) {
+__device__ __forceinline__ void leaf_edge_body(Dev& D, long long blk, long long nblk) {
   const long long count = D.n_lv(LV_EE);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
+  for (long long t = blk * blockDim.x + threadIdx.x; t < count; t += nblk * blockDim.x) {
     Item item = D.ee_items[t];
     NP np = D.conn[item.np];
     const int32_t* pi = D.pair_info + 4 * np.pair;
@@ -598,6 +611,13 @@ __global__ void __launch_bounds__(256) k_leaf_edge(Dev D) {
       emit_leaf(D, np, pr, 3, ea, eb, 6 * n * m + s, Q, P, toc);  // continuous_comparison.h:903
     }
   }
+}
+// The leaf-leaf sub-tests of the level in one launch, split by block as in
+// k_soup_sub.
+__global__ void __launch_bounds__(256) k_leaf_sub(Dev D) {
+  const int h = (int)(gridDim.x >> 1);
+  if ((int)blockIdx.x < h) leaf_point_body(D, blockIdx.x, h);
+  else leaf_edge_body(D, blockIdx.x - h, gridDim.x - h);
 }
 
 // ------------------------------------------- ordered enumeration/expansion
@@ -2987,8 +3007,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   } else {
     timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
     timed_launch(c, K_TRI_ENUM, k_mask_enum, dim3(cg), dim3(256), D, (long long)c->cap_res);
-    timed_launch(c, K_SOUP_POINT, k_soup_point, dim3(sg), dim3(256), D);
-    timed_launch(c, K_SOUP_EDGE, k_soup_edge, dim3(sg), dim3(256), D);
+    timed_launch(c, K_SOUP_EDGE, k_soup_sub, dim3(2 * sg), dim3(256), D);  // point and edge sub-tests
     timed_launch(c, K_SOUP_REDUCE, k_soup_reduce, dim3(sg), dim3(256), D);
   }
   if (discrete && c->precision != D2_FP32) {
@@ -3000,8 +3019,8 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     hipLaunchKernelGGL(k_level_write, dim3(eg), dim3(256), 0, c->stream, D);
   }
   if (!discrete) {
-    timed_launch(c, K_LEAF_POINT, k_leaf_point, dim3(sg), dim3(256), D);
-    timed_launch(c, K_LEAF_EDGE, k_leaf_edge, dim3(sg), dim3(256), D);
+    // the connected list of level 0 is empty (roots go to the soup list)
+    if (level > 0) timed_launch(c, K_LEAF_EDGE, k_leaf_sub, dim3(2 * sg), dim3(256), D);
   }
 }
 
@@ -3185,7 +3204,6 @@ int enqueue_end(d2_ctx* c, int kind, const PhaseSpec& ph, int levels, float dt) 
   if (!ph.discrete) {
     Dev D = make_dev(c, levels, 0);
     hipLaunchKernelGGL(k_fold_final, dim3(grid_for(c, std::max(c->n_pairs, 1))), dim3(256), 0, c->stream, D);
-    hipLaunchKernelGGL(k_save_ccd_sizes, dim3(1), dim3(256), 0, c->stream, D);
   }
   RC(enqueue_post(c, ph.post_mode));
   if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true, c->clusters_on_device));
