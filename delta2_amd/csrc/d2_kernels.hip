@@ -417,8 +417,43 @@ __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
       acc += __popcll(m);
     }
   }
+  // The overlapping triangle pairs of the block's range to the level's item
+  // list: one slot range per block (one atomic), in node-pair order within
+  // it; the list's order is free (contacts carry their emission keys).
+  __shared__ int toff[256];
+  __shared__ unsigned long long s_base;
   const long long tot = block_sum(acc);
-  if (threadIdx.x == 0) D.part_a[blockIdx.x] = (int)tot;
+  if (threadIdx.x == 0) {
+    unsigned long long base = 0;
+    if (tot) {
+      base = atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_ITEMS], (unsigned long long)tot);
+      atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)tot);
+      if ((long long)(base + tot) > D.cap_res) atomicOr(&D.cnt[C_OVERFLOW], 1ull);
+    }
+    s_base = base;
+  }
+  __syncthreads();
+  if (tot == 0) return;
+  const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  long long carry = (long long)s_base;
+  for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
+    const long long w = t0 + threadIdx.x;
+    int tile_total;
+    const int ex = block_scan(w < re ? D.cnt_a[w] : 0, tile_total);
+    toff[threadIdx.x] = ex;
+    __syncthreads();
+    const int in_tile = (int)((re - t0) < (long long)blockDim.x ? (re - t0) : (long long)blockDim.x);
+    for (int j = wid; j < in_tile; j += nw) {
+      const long long ww = t0 + j;
+      const unsigned long long m = D.masks[ww];
+      if ((m >> lane) & 1ull) {
+        const long long slot = carry + toff[j] + __popcll(m & lanes_below());
+        if (slot < D.cap_res) D.items[slot] = Item{(uint32_t)ww, (uint32_t)lane};
+      }
+    }
+    carry += tile_total;
+    __syncthreads();
+  }
 }
 
 __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<double> Q, double toc) {
@@ -985,41 +1020,6 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
   }
 }
 
-// Kept triangle pairs of every node pair to the item list at scanned offsets
-// (node-pair order, then triangle-pair order); the level's item count goes to
-// its slot.  Same grid as the mask producer.
-__global__ void __launch_bounds__(256) k_mask_enum(Dev D, long long cap) {
-  __shared__ int toff[256];
-  const int lane = lane_id(), wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  const long long n = D.n_soup();
-  long long rb, re, before, total;
-  chunk_of(n, rb, re);
-  partial_prefix(D.part_a, before, total);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    D.lvl[D.level * LV_STRIDE + LV_ITEMS] = (unsigned long long)total;
-    if (total > cap) atomicOr(&D.cnt[C_OVERFLOW], D.discrete ? 2ull : 1ull);
-    atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)total);
-  }
-  long long carry = before;
-  for (long long t0 = rb; t0 < re; t0 += blockDim.x) {
-    const long long w = t0 + threadIdx.x;
-    int tile_total;
-    const int ex = block_scan(w < re ? D.cnt_a[w] : 0, tile_total);
-    toff[threadIdx.x] = ex;
-    __syncthreads();
-    const int in_tile = (int)((re - t0) < (long long)blockDim.x ? (re - t0) : (long long)blockDim.x);
-    for (int j = wid; j < in_tile; j += nw) {
-      const long long ww = t0 + j;
-      const unsigned long long m = D.masks[ww];
-      if ((m >> lane) & 1ull) {
-        const long long slot = carry + toff[j] + __popcll(m & lanes_below());
-        if (slot < cap) D.items[slot] = Item{(uint32_t)ww, (uint32_t)lane};
-      }
-    }
-    carry += tile_total;
-    __syncthreads();
-  }
-}
 
 // FP64 triangle-pair distances of the level's items (grid-stride).
 __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
@@ -3006,7 +3006,6 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
   } else {
     timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
-    timed_launch(c, K_TRI_ENUM, k_mask_enum, dim3(cg), dim3(256), D, (long long)c->cap_res);
     timed_launch(c, K_SOUP_EDGE, k_soup_sub, dim3(2 * sg), dim3(256), D);  // point and edge sub-tests
     timed_launch(c, K_SOUP_REDUCE, k_soup_reduce, dim3(sg), dim3(256), D);
   }
