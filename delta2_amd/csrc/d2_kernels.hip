@@ -251,6 +251,42 @@ __device__ void partial_prefix(const int* part, long long& before, long long& to
 }
 
 // ----------------------------------------------------------------- set-up
+// Cluster-step accumulators (separate_clusters.cpp:541-598): per cluster the
+// smallest step (as bits) and its first pair / first pair with a non-static
+// body; with body_step, every body's step starts as its clamped last step (a
+// body in no pair is a cluster of its own without interactions,
+// separate_clusters.cpp:594-596, time_step_selection_dynamic_continuous.cpp:19).
+struct ClusterAcc {
+  int n_clusters, n_bodies;
+  float dt;
+  unsigned long long* step_bits;  // null: nothing to do
+  int *first_any, *first_ns;
+  const double* bodies;
+  double* body_step;
+  const int32_t *pairs, *cluster_of_pair;
+};
+__device__ __forceinline__ void cluster_init_range(const ClusterAcc& A, long long t0, long long stride) {
+  const int n = A.body_step && A.n_bodies > A.n_clusters ? A.n_bodies : A.n_clusters;
+  for (long long k = t0; k < n; k += stride) {
+    if (k < A.n_clusters) A.step_bits[k] = ~0ull, A.first_any[k] = 0x7fffffff, A.first_ns[k] = 0x7fffffff;
+    if (A.body_step && k < A.n_bodies) {
+      double s = A.bodies[40 * (size_t)k + 39];
+      const double hi = (double)A.dt;
+      s = (hi < s) ? hi : s;
+      A.body_step[k] = (s < 1e-3) ? 1e-3 : s;
+    }
+  }
+}
+__device__ __forceinline__ void cluster_first_range(const ClusterAcc& A, int n_pairs, long long t0, long long stride) {
+  for (long long p = t0; p < n_pairs; p += stride) {
+    const int k = A.cluster_of_pair[p];
+    atomicMin(&A.first_any[k], (int)p);
+    const bool sa = A.bodies[40 * (size_t)A.pairs[2 * p] + 38] != 0.0;
+    const bool sb = A.bodies[40 * (size_t)A.pairs[2 * p + 1] + 38] != 0.0;
+    if (!(sa && sb)) atomicMin(&A.first_ns[k], (int)p);
+  }
+}
+
 // Per pair: poses, frame changes, kinematics (CCD: future = extrapolate(max_time)).
 // Four lanes per pair split the record: lane q computes the poses and the
 // frame change it writes (q = 0: TA0, inv(TB0) TA0; 1: TA1, inv(TB1) TA1 and
@@ -265,8 +301,10 @@ __device__ void partial_prefix(const int* part, long long& before, long long& to
 __global__ void k_setup(Dev D, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
                         int ccd, int mt_from_bodies, int keep_ccd, double* scale, int* pair_fill, int* kept_cnt,
-                        int* n_big) {
+                        int* n_big, ClusterAcc CA) {
   const int n_pairs = D.n_pairs;
+  if (CA.step_bits)  // a step: its cluster accumulators
+    cluster_init_range(CA, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)
       if (!(keep_ccd && (i == C_OVERFLOW || i == C_GATED_CCD || i == C_CERT_CCD))) D.cnt[i] = 0;
@@ -357,10 +395,15 @@ __device__ __forceinline__ void save_ccd_sizes(Dev& D) {
 }
 // Closing fold of the CCD phase; block 0 also keeps the phase's level sizes
 // and contact count in the second half of the level table.
-__global__ void k_fold_final(Dev D) {
+// In a step it also records each cluster's first pairs (accumulators
+// initialised by k_setup).
+__global__ void k_fold_final(Dev D, ClusterAcc CA) {
   fold_pass(D, D.pair_rec, D.pass_min_s, D.pass_flag_s);
   fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
   if (blockIdx.x == 0) save_ccd_sizes(D);
+  if (CA.step_bits)
+    cluster_first_range(CA, D.n_pairs, (long long)blockIdx.x * blockDim.x + threadIdx.x,
+                        (long long)gridDim.x * blockDim.x);
 }
 
 // --------------------------------------------------------- CCD soup level
@@ -2519,31 +2562,11 @@ __global__ void __launch_bounds__(1024) k_scan_small(const int* in, int* out, lo
 // first pair reading body a (:589-590); a minimum does not depend on the
 // order of the sequential loop.  Steps are non-negative, so they order like
 // their bit patterns.
-// Cluster accumulators; with body_step (a step's projection), every body's
-// step starts as its clamped last step: a body in no pair is a cluster of
-// its own without interactions (separate_clusters.cpp:594-596,
-// time_step_selection_dynamic_continuous.cpp:19).
-__global__ void k_cluster_init(int n_clusters, unsigned long long* step_bits, int* first_any, int* first_ns,
-                               int n_bodies, float dt, const double* bodies, double* body_step) {
-  const int n = body_step && n_bodies > n_clusters ? n_bodies : n_clusters;
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    if (k < n_clusters) step_bits[k] = ~0ull, first_any[k] = 0x7fffffff, first_ns[k] = 0x7fffffff;
-    if (body_step && k < n_bodies) {
-      double s = bodies[40 * (size_t)k + 39];
-      const double hi = (double)dt;
-      s = (hi < s) ? hi : s;
-      body_step[k] = (s < 1e-3) ? 1e-3 : s;
-    }
-  }
+__global__ void k_cluster_init(ClusterAcc CA) {
+  cluster_init_range(CA, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
 }
-__global__ void k_cluster_first(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
-                                int* first_any, int* first_ns) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
-    const int k = cluster_of_pair[p];
-    atomicMin(&first_any[k], p);
-    const bool sa = bodies[40 * (size_t)pairs[2 * p] + 38] != 0.0, sb = bodies[40 * (size_t)pairs[2 * p + 1] + 38] != 0.0;
-    if (!(sa && sb)) atomicMin(&first_ns[k], p);
-  }
+__global__ void k_cluster_first(int n_pairs, ClusterAcc CA) {
+  cluster_first_range(CA, n_pairs, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
 }
 __global__ void k_cluster_min(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
                               const int* pair_cnt, const double* scale, const int* first_ns, unsigned long long* step_bits) {
@@ -3067,17 +3090,27 @@ int enqueue_post(d2_ctx* c, int mode) {
   return D2_OK;
 }
 
+ClusterAcc cluster_acc(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, bool project) {
+  ClusterAcc A{};
+  A.n_clusters = n_clusters, A.n_bodies = c->n_bodies, A.dt = dt;
+  A.step_bits = c->step_bits.p, A.first_any = c->first_any.p, A.first_ns = c->first_ns.p;
+  A.bodies = c->bodies.p, A.body_step = project ? c->body_step.p : nullptr;
+  A.pairs = c->pairs.p, A.cluster_of_pair = cluster_of_pair;
+  return A;
+}
+// started: the accumulators were initialised and the first pairs recorded
+// by the phase's k_setup / k_fold_final.
 int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, float dt, bool project,
-                     bool first_member) {
+                     bool first_member, bool started) {
   const int np = c->n_pairs;
   Timer tm(c, K_CLUSTER, np);
   double* body_step = project ? c->body_step.p : nullptr;
-  hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, project ? std::max(n_clusters, c->n_bodies) : n_clusters)),
-                     dim3(256), 0, c->stream, n_clusters, c->step_bits.p, c->first_any.p, c->first_ns.p, c->n_bodies, dt,
-                     c->bodies.p, body_step);
+  const ClusterAcc A = cluster_acc(c, cluster_of_pair, n_clusters, dt, project);
+  if (!started)
+    hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, project ? std::max(n_clusters, c->n_bodies) : n_clusters)),
+                       dim3(256), 0, c->stream, A);
   if (np > 0) {
-    hipLaunchKernelGGL(k_cluster_first, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
-                       c->bodies.p, c->first_any.p, c->first_ns.p);
+    if (!started) hipLaunchKernelGGL(k_cluster_first, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, A);
     hipLaunchKernelGGL(k_cluster_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
                        c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->step_bits.p);
   }
@@ -3114,14 +3147,21 @@ int phases_of(int kind, int filter, PhaseSpec* out) {
   return 2;
 }
 
-void enqueue_begin(d2_ctx* c, const PhaseSpec& ph) {
+// The cluster-step accumulators of a step run, set up by its CCD phase's
+// k_setup (initialisation) and k_fold_final (first pairs); none otherwise.
+ClusterAcc step_acc(d2_ctx* c, int kind, const PhaseSpec& ph, float dt) {
+  if (kind != RUN_STEP || ph.discrete) return ClusterAcc{};
+  return cluster_acc(c, c->cluster_of_pair.p, c->n_clusters, dt, true);
+}
+
+void enqueue_begin(d2_ctx* c, int kind, const PhaseSpec& ph, float dt) {
   const int np = c->n_pairs;
   Timer tm(c, K_SETUP, np);
   Dev D = make_dev(c, 0, ph.discrete);
   hipLaunchKernelGGL(k_setup, dim3(grid_for(c, 4ll * std::max(np, 1))), dim3(256), 0, c->stream, D, c->pairs.p,
                      c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
                      ph.discrete ? 0 : 1, ph.mt_from_bodies, ph.keep_ccd, c->scale.p, c->pair_fill.p, c->kept_cnt.p,
-                     c->n_big.p);
+                     c->n_big.p, step_acc(c, kind, ph, dt));
 }
 
 // The learned gate of a phase: proposals (once per run), certificates, and
@@ -3135,11 +3175,10 @@ void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
     Dev D = make_dev(c, 0, ph.discrete);
     if (!ph.keep_ccd || !(c->gate_phases & 1)) {  // a step's discrete phase reuses its CCD phase's proposals
       const double* mt = (ph.mt_from_bodies || !c->has_max_time) ? nullptr : c->max_time.p;
-      hipLaunchKernelGGL(k_pair_features, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, c->bodies.p,
-                         c->body_tree.p, c->tree_radius.p, mt, c->feat.p);
-      const int tiles = (np + 15) / 16;
-      hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0, c->stream,
-                         np, c->feat.p, c->mlp_wb.p, c->pred.p);
+      const int tiles = (np + 15) / 16;  // features computed inside the inference kernel
+      hipLaunchKernelGGL(k_mlp_infer<true>, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0,
+                         c->stream, np, (const float*)nullptr, c->mlp_wb.p, c->pred.p,
+                         FeatIn{c->pairs.p, c->bodies.p, c->body_tree.p, c->tree_radius.p, mt});
     }
     hipLaunchKernelGGL(k_certify, dim3(std::max(1, std::min(np, c->cus * 8))), dim3(256), 0, c->stream, np,
                        c->pair_info.p, c->body_tree.p, c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p,
@@ -3202,10 +3241,12 @@ void enqueue_levels(d2_ctx* c, const PhaseSpec& ph, int from, int to) {
 int enqueue_end(d2_ctx* c, int kind, const PhaseSpec& ph, int levels, float dt) {
   if (!ph.discrete) {
     Dev D = make_dev(c, levels, 0);
-    hipLaunchKernelGGL(k_fold_final, dim3(grid_for(c, std::max(c->n_pairs, 1))), dim3(256), 0, c->stream, D);
+    hipLaunchKernelGGL(k_fold_final, dim3(grid_for(c, std::max(c->n_pairs, 1))), dim3(256), 0, c->stream, D,
+                       step_acc(c, kind, ph, dt));
   }
   RC(enqueue_post(c, ph.post_mode));
-  if (kind == RUN_STEP && !ph.discrete) RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true, c->clusters_on_device));
+  if (kind == RUN_STEP && !ph.discrete)
+    RC(enqueue_clusters(c, c->cluster_of_pair.p, c->n_clusters, dt, true, c->clusters_on_device, true));
   HIPCHK(c, hipGetLastError());
   return D2_OK;
 }
@@ -3214,7 +3255,7 @@ int run_eager(d2_ctx* c, int kind, int filter, float dt) {
   PhaseSpec ph[2];
   const int n = phases_of(kind, filter, ph);
   for (int i = 0; i < n; ++i) {
-    enqueue_begin(c, ph[i]);
+    enqueue_begin(c, kind, ph[i], dt);
     enqueue_gate(c, ph[i]);
     HIPCHK(c, hipGetLastError());
     int used = 0;
@@ -3277,8 +3318,8 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
   for (int i = 0; i < n; ++i) {
     const PhaseSpec p = ph[i];
     const int L = sig.levels[i];
-    ops.push_back([c, p] {
-      enqueue_begin(c, p);
+    ops.push_back([c, kind, p, dt] {
+      enqueue_begin(c, kind, p, dt);
       enqueue_gate(c, p);
       return D2_OK;
     });
@@ -3883,7 +3924,7 @@ int d2_cluster_steps(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
       return fail(c, D2_ERR_ARG, "d2_cluster_steps: cluster index out of range");
   RC(ensure_clusters(c, n_clusters)); RC(ensure(c, c->tmp_cluster, std::max(np, 1)));
   if (np) HIPCHK(c, hipMemcpyAsync(c->tmp_cluster.p, cluster_of_pair, sizeof(int32_t) * np, hipMemcpyHostToDevice, c->stream));
-  RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false, false));
+  RC(enqueue_clusters(c, c->tmp_cluster.p, n_clusters, dt, false, false, false));
   HIPCHK(c, hipMemcpyAsync(out, c->cluster_step.p, sizeof(double) * n_clusters, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return D2_OK;
@@ -4152,8 +4193,8 @@ int d2_mlp_infer(d2_ctx* c, const float* X, int n, float* Y) {
   RC(ensure(c, c->tr_x, (size_t)n * MLP_IN)); RC(ensure(c, c->tr_t, (size_t)n * MLP_OUT));
   HIPCHK(c, hipMemcpyAsync(c->tr_x.p, X, sizeof(float) * MLP_IN * (size_t)n, hipMemcpyHostToDevice, c->stream));
   const int tiles = (n + 15) / 16;
-  hipLaunchKernelGGL(k_mlp_infer, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0, c->stream, n,
-                     c->tr_x.p, c->mlp_wb.p, c->tr_t.p);
+  hipLaunchKernelGGL(k_mlp_infer<false>, dim3(std::max(1, std::min((tiles + 3) / 4, c->cus * 4))), dim3(256), 0,
+                     c->stream, n, (const float*)c->tr_x.p, c->mlp_wb.p, c->tr_t.p, FeatIn{});
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(Y, c->tr_t.p, sizeof(float) * MLP_OUT * (size_t)n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -44,9 +44,10 @@ __device__ __forceinline__ float bf2f(short h) { return __uint_as_float(((unsign
 //   10-12 / 13-15 angular velocities of a / b, 16-17 radii, 18 search
 //   distance, 19 centre distance, 20 gap between the bounding spheres,
 //   21 closing speed, 22 rotational sweep (|w_a| r_a + |w_b| r_b), 23 |v_rel|.
-__global__ void k_pair_features(int n_pairs, const int32_t* pairs, const double* bodies, const int32_t* body_tree,
-                                const float* tree_radius, const double* max_time, float* X) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
+__device__ __forceinline__ void pair_features(int p, const int32_t* pairs, const double* bodies,
+                                              const int32_t* body_tree, const float* tree_radius,
+                                              const double* max_time, float* x) {
+  {
     const int a = pairs[2 * p], b = pairs[2 * p + 1];
     const double* ra = bodies + 40 * (size_t)a;
     const double* rb = bodies + 40 * (size_t)b;
@@ -73,7 +74,6 @@ __global__ void k_pair_features(int n_pairs, const int32_t* pairs, const double*
     const v3<double> wa_a = to_a(wa), wb_a = to_a(wb);
     const double dist = vlen(d), vlenr = vlen(vr);
     const double closing = dist > 0.0 ? -dot(d, vr) / dist : 0.0;
-    float* x = X + (size_t)MLP_IN * p;
     const double il = 1.0 / Lc, vt = mt / Lc;
     x[0] = (float)(d.x * il), x[1] = (float)(d.y * il), x[2] = (float)(d.z * il);
     x[3] = (float)qr.w, x[4] = (float)qr.x, x[5] = (float)qr.y, x[6] = (float)qr.z;
@@ -89,13 +89,29 @@ __global__ void k_pair_features(int n_pairs, const int32_t* pairs, const double*
     x[23] = (float)(vlenr * vt);
   }
 }
+__global__ void k_pair_features(int n_pairs, const int32_t* pairs, const double* bodies, const int32_t* body_tree,
+                                const float* tree_radius, const double* max_time, float* X) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x)
+    pair_features(p, pairs, bodies, body_tree, tree_radius, max_time, X + (size_t)MLP_IN * p);
+}
 
 // --------------------------------------------------------- MFMA inference
 // One wavefront per tile of 16 pairs, v_mfma_f32_16x16x16_bf16 (FP32
 // accumulation).  Operand layout: lane l supplies A[l%16][4(l/16)+i] and
 // B[4(l/16)+i][l%16], i = 0..3, and receives D[4(l/16)+i][l%16].
 // Activations go through LDS (bf16) between layers.  wb: bf16 parameters.
-__global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* X, const short* wb, float* Y) {
+// FEAT: the inputs are the pairs' features, computed here (lanes 0-15 of the
+// wavefront, one pair each, into LDS) instead of read from X.
+struct FeatIn {
+  const int32_t* pairs;
+  const double* bodies;
+  const int32_t* body_tree;
+  const float* tree_radius;
+  const double* max_time;
+};
+template <bool FEAT>
+__global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* X, const short* wb, float* Y, FeatIn F) {
+  __shared__ float s_x[FEAT ? 4 : 1][16 * MLP_IN];
   __shared__ short w1[32 * MLP_H];  // K padded to 32
   __shared__ short w2[MLP_H * MLP_H];
   __shared__ short w3[MLP_H * 16];  // N padded to 16
@@ -122,13 +138,20 @@ __global__ void __launch_bounds__(256) k_mlp_infer(int n, const float* X, const 
     const int row = tile * 16 + r;
     const bool valid_row = tile < n_tiles && row < n;
     floatx4 acc[4];
+    if (FEAT) {
+      if (kq == 0 && valid_row)
+        pair_features(row, F.pairs, F.bodies, F.body_tree, F.tree_radius, F.max_time, s_x[wid] + MLP_IN * r);
+      wave_sync_lds();
+    }
     // layer 1
     for (int nt = 0; nt < 4; ++nt) acc[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
     for (int ks = 0; ks < 2; ++ks) {
       bf16x4 a;
       for (int i = 0; i < 4; ++i) {
         const int k = ks * 16 + 4 * kq + i;
-        a[i] = (valid_row && k < MLP_IN) ? f2bf(X[(size_t)row * MLP_IN + k]) : (short)0;
+        const float xv = !(valid_row && k < MLP_IN) ? 0.f : FEAT ? s_x[FEAT ? wid : 0][MLP_IN * r + k]
+                                                                  : X[(size_t)row * MLP_IN + k];
+        a[i] = (valid_row && k < MLP_IN) ? f2bf(xv) : (short)0;
       }
       for (int nt = 0; nt < 4; ++nt) {
         bf16x4 b;
