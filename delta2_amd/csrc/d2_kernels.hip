@@ -297,19 +297,20 @@ __device__ __forceinline__ void cluster_first_range(const ClusterAcc& A, int n_p
 // accumulators, min_time and scale (a step's discrete phase keeps its CCD
 // phase's), the post-processing counters; block 0 clears the counters (a
 // step's discrete phase keeps the CCD phase's overflow flag and gate counts)
-// and sets the level table to one level of n_pairs roots.
+// and sets the level table to one level of n_pairs roots (none when the
+// learned gate's certificate appends the phase's roots).
 __global__ void k_setup(Dev D, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
                         int ccd, int mt_from_bodies, int keep_ccd, double* scale, int* pair_fill, int* kept_cnt,
-                        int* n_big, ClusterAcc CA) {
+                        int* n_big, ClusterAcc CA, int gated) {
   const int n_pairs = D.n_pairs;
   if (CA.step_bits)  // a step: its cluster accumulators
     cluster_init_range(CA, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
   if (blockIdx.x == 0) {
     for (int i = threadIdx.x; i < C_COUNT; i += blockDim.x)
       if (!(keep_ccd && (i == C_OVERFLOW || i == C_GATED_CCD || i == C_CERT_CCD))) D.cnt[i] = 0;
-    for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)
-      D.lvl[i] = i == LV_SOUP ? (unsigned long long)n_pairs : 0ull;
+    for (int i = threadIdx.x; i < (MAX_LEVELS + 1) * LV_STRIDE; i += blockDim.x)  // gated: roots come from k_certify
+      D.lvl[i] = i == LV_SOUP && !gated ? (unsigned long long)n_pairs : 0ull;
     if (threadIdx.x == 0) D.pair_cnt[n_pairs] = 0, kept_cnt[n_pairs] = 0, n_big[0] = 0, n_big[1] = 0;
   }
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < 4ll * n_pairs;
@@ -1735,15 +1736,10 @@ __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off,
     }
   }
 }
-__global__ void k_pair_finish_big(const int* big, const int* n_big, const ContactRec* c, const int* pair_off, int* perm,
-                                  int mode, int* keep, int* kept_cnt, const double* pair_rec, double* scale) {
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < *n_big; t += gridDim.x * blockDim.x)
-    pair_finish_one(big[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
-}
 
 // One wavefront per pair, contacts staged in LDS (pairs with more than the
-// kernel's capacity go on to the next pass; the last pass, k_pair_finish_big,
-// takes one lane per pair).
+// kernel's capacity go on to the next pass; pairs above every capacity take
+// one lane each at the end of k_pair_finish_wave).
 //   1. bitonic sort by emission key -> the reference's contact order
 //   2a. discrete + filter: the order std::sort(compareContacts) produces.  When
 //       no two contacts have equal (eps_a+eps_b, |A-B|) every correct sort
@@ -2005,7 +2001,8 @@ template <int CAP>
 __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int* list, const int* n_list,
                                                          const ContactRec* c, const int* pair_off, int* perm, int mode,
                                                          int* keep, int* kept_cnt, const double* pair_rec,
-                                                         double* scale, int* over, int* n_over) {
+                                                         double* scale, int* over, int* n_over, const int* huge,
+                                                         const int* n_huge) {
   constexpr int POST_CAP = CAP;
   __shared__ unsigned long long skey[POST_CAP];
   __shared__ int sidx[POST_CAP];
@@ -2190,6 +2187,10 @@ __global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int*
     }
     __syncthreads();
   }
+  // pairs above every LDS capacity (listed before this launch): one lane each
+  if (huge)
+    for (int t = blockIdx.x * blockDim.x + lane; t < *n_huge; t += gridDim.x * blockDim.x)
+      pair_finish_one(huge[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
 }
 
 // ---- one block (4 wavefronts) per pair: pairs with at most BLK_CAP contacts
@@ -2297,7 +2298,7 @@ __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* B
 __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const ContactRec* c, const int* pair_off,
                                                            int* perm, int mode, int* keep, int* kept_cnt,
                                                            const double* pair_rec, double* scale, int* over,
-                                                           int* n_over) {
+                                                           int* n_over, int* huge, int* n_huge) {
   __shared__ unsigned long long skey[BLK_CAP];  // emission keys, then the partition lists of the tie replay
   __shared__ int sidx[BLK_CAP];
   __shared__ int eo[BLK_CAP];  // contact at each emission position
@@ -2314,8 +2315,11 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
   for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
-    if (n > BLK_CAP) {
-      if (tid == 0) over[atomicAdd(n_over, 1)] = p;
+    if (n > BLK_CAP) {  // to k_pair_finish_wave: its LDS pass, or (above its capacity) one lane per pair
+      if (tid == 0) {
+        if (n > POST_CAP_LARGE) huge[atomicAdd(n_huge, 1)] = p;
+        else over[atomicAdd(n_over, 1)] = p;
+      }
       continue;
     }
     for (int i = tid; i < n; i += 256) {
@@ -2752,7 +2756,7 @@ struct d2_ctx {
   // learned gate
   DBuf<float> tree_radius, tree_eps_out, hv, feat, pred;
   DBuf<int32_t> hv_off;
-  DBuf<int> cert, cert_off;  // per pair: 1 = traverse (not certified); scanned offsets
+  DBuf<int> cert;  // per pair: 1 = traverse (not certified)
   int gate_phases = 3;       // bit 0: CCD phase, bit 1: discrete phase
   DBuf<float> mlp_w, mlp_m, mlp_v, mlp_wf, mlp_grad;
   DBuf<short> mlp_wb;
@@ -2968,7 +2972,6 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->scan_part, g));
   if (c->learned_on) {
     RC(ensure(c, c->feat, np * MLP_IN)); RC(ensure(c, c->pred, np * MLP_OUT)); RC(ensure(c, c->cert, np));
-    RC(ensure(c, c->cert_off, np + 1));
   }
   return D2_OK;
 }
@@ -3073,14 +3076,16 @@ int enqueue_post(d2_ctx* c, int mode) {
   {
     Timer tm(c, K_POST_PAIRS, (double)np);
     const int wblocks = std::max(1, std::min(np, c->tune.post_blocks));
+    // pairs above the block kernel's capacity: the wavefront kernel's LDS pass
+    // (list big), above that one lane per pair (list huge, filled by the block
+    // kernel too, so the wavefront kernel never adds to it)
     hipLaunchKernelGGL(k_pair_finish_block, dim3(wblocks), dim3(256), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p);
+                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p,
+                       c->huge.p, c->n_big.p + 1);
     hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(64), 0,
                        c->stream, np, c->big.p, c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p,
-                       c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p, c->n_big.p + 1);
-    hipLaunchKernelGGL(k_pair_finish_big, dim3(grid_for(c, std::max(np, 1), 64)), dim3(64), 0, c->stream, c->huge.p,
-                       c->n_big.p + 1, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
-                       c->pair_rec.p, c->scale.p);
+                       c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
+                       (const int*)(c->n_big.p + 1));
   }
   if (filtered) {
     Timer tm(c, K_POST, 0.0);
@@ -3147,6 +3152,11 @@ int phases_of(int kind, int filter, PhaseSpec* out) {
   return 2;
 }
 
+// Whether the learned gate runs in this phase.
+bool gate_on(const d2_ctx* c, const PhaseSpec& ph) {
+  return c->learned_on && c->mlp_ready && c->n_pairs > 0 && (c->gate_phases & (ph.discrete ? 2 : 1));
+}
+
 // The cluster-step accumulators of a step run, set up by its CCD phase's
 // k_setup (initialisation) and k_fold_final (first pairs); none otherwise.
 ClusterAcc step_acc(d2_ctx* c, int kind, const PhaseSpec& ph, float dt) {
@@ -3161,15 +3171,14 @@ void enqueue_begin(d2_ctx* c, int kind, const PhaseSpec& ph, float dt) {
   hipLaunchKernelGGL(k_setup, dim3(grid_for(c, 4ll * std::max(np, 1))), dim3(256), 0, c->stream, D, c->pairs.p,
                      c->max_time.p, c->bodies.p, c->body_tree.p, c->tree_off.p, c->pair_info.p, c->pair_rec.p,
                      ph.discrete ? 0 : 1, ph.mt_from_bodies, ph.keep_ccd, c->scale.p, c->pair_fill.p, c->kept_cnt.p,
-                     c->n_big.p, step_acc(c, kind, ph, dt));
+                     c->n_big.p, step_acc(c, kind, ph, dt), gate_on(c, ph) ? 1 : 0);
 }
 
 // The learned gate of a phase: proposals (once per run), certificates, and
 // the phase's root list without the certified pairs.
 void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
   const int np = c->n_pairs;
-  const int bit = ph.discrete ? 2 : 1;
-  if (!(c->learned_on && c->mlp_ready && np > 0 && (c->gate_phases & bit))) return;
+  if (!gate_on(c, ph)) return;
   {
     Timer tg(c, K_GATE, np);
     Dev D = make_dev(c, 0, ph.discrete);
@@ -3180,13 +3189,12 @@ void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
                          c->stream, np, (const float*)nullptr, c->mlp_wb.p, c->pred.p,
                          FeatIn{c->pairs.p, c->bodies.p, c->body_tree.p, c->tree_radius.p, mt});
     }
-    hipLaunchKernelGGL(k_certify, dim3(std::max(1, std::min(np, c->cus * 8))), dim3(256), 0, c->stream, np,
-                       c->pair_info.p, c->body_tree.p, c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p,
-                       c->hv.p, c->tree_eps_out.p, ph.discrete ? 1 : 0, c->cert.p,
-                       c->cnt.p + (ph.discrete ? C_GATED_DISC : C_GATED_CCD));
-    scan_host_n(c, c->cert.p, c->cert_off.p, np, c->lvl.p + LV_SOUP);  // level-0 size = pairs kept
-    hipLaunchKernelGGL(k_roots_scatter, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->cert.p, c->cert_off.p,
-                       D.soup);
+    // at most CERT_LIST pairs per block; the kept pairs are the phase's roots
+    const int grid = std::max(std::max(1, std::min(np, c->cus * 8)), (np + CERT_LIST - 1) / CERT_LIST);
+    hipLaunchKernelGGL(k_certify, dim3(grid), dim3(256), 0, c->stream, np, c->pair_info.p, c->body_tree.p,
+                       c->pair_rec.p, c->pred.p, c->gate_t, c->gate_s, c->hv_off.p, c->hv.p, c->tree_eps_out.p,
+                       ph.discrete ? 1 : 0, c->cert.p, c->cnt.p + (ph.discrete ? C_GATED_DISC : C_GATED_CCD), D.soup,
+                       c->lvl.p + LV_SOUP);
   }
 }
 
@@ -3508,7 +3516,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster);
   release(c->tree_radius), release(c->tree_eps_out), release(c->hv), release(c->feat), release(c->pred), release(c->hv_off);
-  release(c->cert), release(c->cert_off), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
+  release(c->cert), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
   release(c->mlp_wb), release(c->tr_x), release(c->tr_t), release(c->tr_act), release(c->tr_dlt), release(c->tr_loss);
   release(c->tr_idx);
   release(c->tb_box), release(c->tb_rad), release(c->last_state), release(c->bp_R), release(c->tree_depth_d),

@@ -430,18 +430,26 @@ __device__ float cert_gap_timed(const float* P0, const float* P1, int nP, const 
   return best;
 }
 constexpr int CERT_ITERS = 16;
+constexpr int CERT_LIST = 1024;  // pairs per block at most (the grid is sized for it)
 
 // gate: the MLP outputs (y0 >= s_gate and y1 >= t_gate), or every pair when
 // null.  keep[p] = 0 for a certified pair, 1 otherwise.  counters[0..1]:
-// proposed / certified pairs.
+// proposed / certified pairs.  The kept pairs are the phase's roots: each
+// block appends its kept pairs to soup with one atomic on *n_soup (zeroed by
+// the phase's set-up; the discrete and CCD lists need no order).
 __global__ void __launch_bounds__(256) k_certify(int n_pairs, const int32_t* pair_info, const int32_t* body_tree,
                                                  const double* pair_rec, const float* Y, float t_gate, float s_gate,
                                                  const int32_t* hv_off, const float* hv, const float* tree_eps_out,
-                                                 int mode, int* keep, unsigned long long* counters) {
+                                                 int mode, int* keep, unsigned long long* counters, NP* soup,
+                                                 unsigned long long* n_soup) {
   __shared__ float buf[4][3 * CERT_MAXV];
   __shared__ float red[4 * 16];
   __shared__ float mag_red[16];
+  __shared__ int s_kept[CERT_LIST];
+  __shared__ int s_nk;
+  __shared__ unsigned long long s_base;
   const float EPSF = 5.9604645e-08f;
+  if (threadIdx.x == 0) s_nk = 0;
   for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
     const bool gated = !Y || (Y[2 * p] >= s_gate && Y[2 * p + 1] >= t_gate);
     const int a = pair_info[4 * p], b = pair_info[4 * p + 1];
@@ -450,6 +458,7 @@ __global__ void __launch_bounds__(256) k_certify(int n_pairs, const int32_t* pai
     if (!gated || na > CERT_MAXV || nb > CERT_MAXV) {
       if (threadIdx.x == 0) {
         keep[p] = 1;
+        s_kept[s_nk++] = p;
         if (gated) atomicAdd(&counters[0], 1ull);
       }
       continue;
@@ -524,15 +533,16 @@ __global__ void __launch_bounds__(256) k_certify(int n_pairs, const int32_t* pai
     }
     if (threadIdx.x == 0) {
       keep[p] = ok ? 0 : 1;
+      if (!ok) s_kept[s_nk++] = p;
       atomicAdd(&counters[0], 1ull);
       if (ok) atomicAdd(&counters[1], 1ull);
     }
     __syncthreads();  // LDS reuse by the next pair
   }
+  __syncthreads();
+  if (threadIdx.x == 0) s_base = s_nk ? atomicAdd(n_soup, (unsigned long long)s_nk) : 0ull;
+  __syncthreads();
+  for (int i = threadIdx.x; i < s_nk; i += blockDim.x)
+    soup[s_base + i] = NP{(uint32_t)s_kept[i], 0u, 0u, 0u, 0ull};
 }
 
-// The phase's root list: the kept pairs at their scanned offsets (pair order).
-__global__ void k_roots_scatter(int n_pairs, const int* keep, const int* off, NP* soup) {
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x)
-    if (keep[p]) soup[off[p]] = NP{(uint32_t)p, 0u, 0u, 0u, 0ull};
-}

@@ -95,6 +95,49 @@ def test_narrow_repeat_runs(prec):
         c.close()
 
 
+def _stacked_scene():
+    """Two pairs of bodies sharing their centres (subdivision-3 and -2
+    icospheres): thousands of contacts per pair, beyond the post-processing's
+    256-thread block (512) and its one-wavefront LDS pass (2048)."""
+    from gpu_cases import _seed
+    rng = np.random.default_rng(5)
+    meshes = [scenes.icosphere(3, 1.0), scenes.icosphere(3, 1.0), scenes.icosphere(2, 1.0),
+              scenes.icosphere(2, 1.02)]
+    pos = np.array([[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], [5.0, 0.0, 0.0], [5.0, 0.0, 0.0]])
+    trees, rec, radii = scenes._bodies(meshes, rng, pos, 0.5, _seed())
+    rec[:, 14:28] = rec[:, 0:14]
+    s = scenes.Scene("stacked", meshes, trees, rec, radii)
+    s.pairs = np.array([[0, 1], [2, 3]], dtype=np.int32)
+    return s
+
+
+@pytest.mark.parametrize("prec", [d2.D2_FP64, d2.D2_FP32])
+def test_pairs_with_thousands_of_contacts(prec):
+    s = _stacked_scene()
+    rec = with_future(s.records, scenes.DT)
+    c = d2.Context(0, prec)
+    try:
+        load(c, s, rec)
+        for filt in (False, True):
+            print(f"[stacked] precision={prec} narrow filter={filt}", flush=True)
+            gpu = d2.compare_trees_full(c, s.pairs, filter=filt)
+            o_off, o_ints, o_reals = oracle_scene(s, rec).narrow(s.pairs, filter=filt)
+            if not filt:  # one pair per path: over 2048 contacts, and 512-2048
+                n = np.diff(o_off)
+                assert n[0] > 2048 and 512 < n[1] <= 2048, n
+            assert_same(gpu, o_off, o_ints, o_reals, f"stacked narrow {prec} filter={filt}")
+        load(c, s, s.records)
+        mt = np.full(2, scenes.DT)
+        print(f"[stacked] precision={prec} ccd", flush=True)
+        gpu = d2.compare_trees_full_continuous(c, s.pairs, mt)
+        o_off, o_ints, o_reals, o_pp = oracle_scene(s, s.records).ccd(s.pairs, mt)
+        assert np.all(np.diff(o_off) > 2048)
+        assert_same(gpu, o_off, o_ints, o_reals, f"stacked ccd {prec}")
+        assert close(gpu.per_pair, o_pp).all()
+    finally:
+        c.close()
+
+
 @pytest.mark.parametrize("name,make", SCENES)
 def test_ccd_parity(ctx, name, make):
     s = make()
