@@ -2778,6 +2778,7 @@ struct d2_ctx {
   bool capturing = false;
   Captured graphs[RUN_KINDS][MAX_SEGMENTS];
   int phase_levels[4] = {-1, -1, -1, -1};  // deepest level count seen per phase (narrow, ccd, step ccd, step narrow)
+  int graph_levels[2] = {0, 0};             // level counts of the last graph run's phases
   // level sizes of the last run of each phase (0 discrete, 1 CCD): launch
   // grids of the next runs' levels are sized from them
   long long prev_lv[2][MAX_LEVELS + 1][LV_STRIDE] = {};
@@ -3306,11 +3307,18 @@ int launch_segment(d2_ctx* c, int kind, int seg, const Sig& sig, const std::vect
   return D2_OK;
 }
 
-int run_graph(d2_ctx* c, int kind, int filter, float dt) {
+// Graph runs need every phase's level count observed by an eager run first.
+bool graph_ready(const d2_ctx* c, int kind, int filter) {
   PhaseSpec ph[2];
   const int n = phases_of(kind, filter, ph);
   for (int i = 0; i < n; ++i)
-    if (c->phase_levels[ph[i].slot] < 0) return run_eager(c, kind, filter, dt);  // observe first
+    if (c->phase_levels[ph[i].slot] < 0) return false;
+  return true;
+}
+
+int run_graph(d2_ctx* c, int kind, int filter, float dt) {
+  PhaseSpec ph[2];
+  const int n = phases_of(kind, filter, ph);
   const int budget = level_budget(c);
   Sig sig;
   sig.n_pairs = c->n_pairs, sig.n_bodies = c->n_bodies, sig.n_clusters = kind == RUN_STEP ? c->n_clusters : 0;
@@ -3332,22 +3340,30 @@ int run_graph(d2_ctx* c, int kind, int filter, float dt) {
       return D2_OK;
     });
     ops.push_back([c, p, L] { enqueue_levels(c, p, 0, L); return D2_OK; });
-    int used = L;
-    if (L < budget) {  // end of a segment: check the next level on the host
-      RC(launch_segment(c, kind, seg++, sig, ops));
-      ops.clear();
-      RC(read_slots(c, L, 1));
-      const long long ns = (long long)c->host_lvl[L * LV_STRIDE + LV_SOUP];
-      const long long nc = (long long)c->host_lvl[L * LV_STRIDE + LV_CONN];
-      if (ns + nc > 0) {
-        RC(eager_levels(c, p, L, ns, nc, &used));
-        c->phase_levels[p.slot] = std::max(c->phase_levels[p.slot], used);
-      }
-    }
-    ops.push_back([c, kind, p, used, dt] { return enqueue_end(c, kind, p, used, dt); });
+    ops.push_back([c, kind, p, L, dt] { return enqueue_end(c, kind, p, L, dt); });
+    c->graph_levels[i] = L;
   }
   RC(launch_segment(c, kind, seg, sig, ops));
   return D2_OK;
+}
+
+// After a graph run: whether a phase's traversal went on past the levels in
+// the graph (its next level is not empty).  The graph runs every phase to the
+// deepest level count seen so far without stopping for the host; when a run
+// needs more, it is repeated eagerly (the inputs are unchanged by a run) and
+// the next capture holds more levels.
+bool graph_cut_short(const d2_ctx* c, int kind, int filter) {
+  PhaseSpec ph[2];
+  const int n = phases_of(kind, filter, ph);
+  const int half = (MAX_LEVELS + 1) * LV_STRIDE;
+  for (int i = 0; i < n; ++i) {
+    const int L = c->graph_levels[i];
+    if (L >= level_budget(c)) continue;  // the budget check covers it
+    // a step's CCD phase: its sizes as saved by k_fold_final
+    const unsigned long long* t = c->host_lvl + (kind == RUN_STEP && !ph[i].discrete ? half : 0) + L * LV_STRIDE;
+    if (t[LV_SOUP] || t[LV_CONN]) return true;
+  }
+  return false;
 }
 
 int ensure_clusters(d2_ctx* c, int n_clusters) {
@@ -3452,9 +3468,14 @@ int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out)
   if (kind == RUN_STEP) RC(ensure_clusters(c, c->n_clusters));
   c->ccd_valid = false;
   c->last_mode = -1;
-  if (c->graph_on) RC(run_graph(c, kind, filter, dt));
+  const bool graph = c->graph_on && graph_ready(c, kind, filter);
+  if (graph) RC(run_graph(c, kind, filter, dt));
   else RC(run_eager(c, kind, filter, dt));
   RC(finish_run(c, kind, filter));
+  if (graph && graph_cut_short(c, kind, filter)) {
+    RC(run_eager(c, kind, filter, dt));
+    RC(finish_run(c, kind, filter));
+  }
   if (n_contacts_out) *n_contacts_out = c->n_out;
   return D2_OK;
 }
