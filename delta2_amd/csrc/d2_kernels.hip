@@ -94,7 +94,10 @@ enum Counter : int {
   C_GATED_CCD = 4, C_CERT_CCD = 5, C_GATED_DISC = 6, C_CERT_DISC = 7,  // learned gate: proposed / certified pairs
   C_COUNT = 8
 };
-enum LevelSlot : int { LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_STRIDE = 8 };
+// LV_PACK (discrete levels): the level's item count (low 32 bits) and its
+// children count (high 32 bits), taken together by k_tri_mask's allocation
+// atomic and unpacked into LV_ITEMS and the next level's LV_SOUP by k_tri_dist.
+enum LevelSlot : int { LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_PACK = 5, LV_STRIDE = 8 };
 constexpr int MAX_LEVELS = 16;  // contact keys hold 16 base-8 path digits
 // Key layout: level(5) | list(1) | path(48) | sub(10)
 __device__ __forceinline__ unsigned long long make_key(int level, int list, unsigned long long path, int sub) {
@@ -1029,17 +1032,16 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
     const int par = it & 1;
     if (lane == 63) s_tot[par][wid] = incl, s_ch[par][wid] = ich;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0) {  // one atomic for both ranges (same-address atomics from every block are costly)
       int sum = 0, sch = 0;
       for (int v = 0; v < nw; ++v) sum += s_tot[par][v], sch += s_ch[par][v];
       unsigned long long base = 0, chb = 0;
-      if (sum) {
-        base = atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_ITEMS], (unsigned long long)sum);
-        if ((long long)(base + sum) > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
-      }
-      if (sch) {
-        chb = atomicAdd(&D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP], (unsigned long long)sch);
-        if ((long long)(chb + sch) > D.cap_np) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+      if (sum || sch) {
+        const unsigned long long old =
+            atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_PACK], ((unsigned long long)sch << 32) | (unsigned)sum);
+        base = old & 0xffffffffull, chb = old >> 32;
+        if ((long long)(base + sum) > D.cap_items || (long long)(chb + sch) > D.cap_np)
+          atomicOr(&D.cnt[C_OVERFLOW], 2ull);
       }
       s_base[par] = base, s_chb[par] = chb;
     }
@@ -1067,8 +1069,14 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
 
 // FP64 triangle-pair distances of the level's items (grid-stride).
 __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
-  const long long count = D.n_lv(LV_ITEMS);
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)D.lv(D.level, LV_ITEMS));
+  const unsigned long long pack = D.lv(D.level, LV_PACK);
+  const long long n_items = (long long)(pack & 0xffffffffull);
+  const long long count = n_items < D.cap_items ? n_items : D.cap_items;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {  // the plain sizes, for the next level and the host
+    D.lvl[D.level * LV_STRIDE + LV_ITEMS] = (unsigned long long)n_items;
+    D.lvl[(D.level + 1) * LV_STRIDE + LV_SOUP] = pack >> 32;
+    atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)n_items);
+  }
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
     const DItem item = D.ditems[t];
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * item.pair;
