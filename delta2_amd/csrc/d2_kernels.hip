@@ -2330,6 +2330,13 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
       }
       continue;
     }
+    if (n == 0) {  // what the general path writes for a pair without contacts, without its barriers
+      if (tid == 0) {
+        if (mode == 2) scale[p] = 1.0;
+        if (mode == 1) kept_cnt[p] = 0;
+      }
+      continue;
+    }
     for (int i = tid; i < n; i += 256) {
       const int idx = perm[b + i];
       sidx[i] = idx;
