@@ -103,9 +103,12 @@ int d2_get_tune(const d2_ctx* ctx, d2_tune* tune);
 int d2_autotune(d2_ctx* ctx, int kind, float dt, int filter, int reps, d2_tune* best, double* best_ms);
 
 /* Graph mode (no reference counterpart).  When on, each kind of run (narrow,
- * CCD, step) is captured once into a HIP graph with the full level budget of
- * the uploaded trees and replayed on later calls; uploads that change sizes
- * or launch parameters recapture.  Results are identical to eager runs. */
+ * CCD, step) is captured into a HIP graph and replayed on later calls as one
+ * launch: every phase runs the deepest level count seen so far (the first run
+ * of a kind is eager, to see it); a run whose traversal goes deeper is
+ * repeated eagerly and the next capture holds more levels.  Uploads that
+ * change sizes or launch parameters recapture.  Results are identical to
+ * eager runs. */
 int d2_set_graph(d2_ctx* ctx, int on);
 
 /* ---- surrogate trees ---------------------------------------------------- */
