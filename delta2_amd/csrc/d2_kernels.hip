@@ -1745,19 +1745,10 @@ __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off,
   }
 }
 
-// One wavefront per pair, contacts staged in LDS (pairs with more than the
-// kernel's capacity go on to the next pass; pairs above every capacity take
-// one lane each at the end of k_pair_finish_wave).
-//   1. bitonic sort by emission key -> the reference's contact order
-//   2a. discrete + filter: the order std::sort(compareContacts) produces.  When
-//       no two contacts have equal (eps_a+eps_b, |A-B|) every correct sort
-//       gives that order, so a parallel sort is used; with exact ties lane 0
-//       replays libstdc++'s introsort (Sorter) on the LDS copy.  Then the
-//       greedy pass of filterContacts, each candidate tested against the kept
-//       set across the lanes.
-//   2b. CCD: the step rule, lanes over contacts, min-reduced.
-constexpr int POST_CAP_WAVE = 512;   // first pass: 30 KB of LDS per one-wavefront block
-constexpr int POST_CAP_LARGE = 2048; // second pass: 120 KB of LDS, one block per CU
+// Contacts a pair may have for the LDS passes of the post-processing: the
+// second pass (one 256-thread block per CU, 142 KB of LDS) takes pairs up to
+// POST_CAP_LARGE; above it a pair takes one lane (pair_finish_one).
+constexpr int POST_CAP_LARGE = 2048;
 struct LdsKey {
   double eps, dist;
 };
@@ -1931,287 +1922,21 @@ __device__ int wave_partition(int lo, int hi, int* ord, const int* rk, int* Apos
   return ak < bk1 ? ak : bk1;
 }
 
-// ord[0, n) := the order std::sort gives.  mark, Apos, Bpos hold n ints each,
-// stk 3 * 64 ints (the segment stack is at most 2*log2(n)+1 deep).
-__device__ void wave_std_sort(int n, int* ord, const int* rk, int* Apos, int* Bpos, int* mark, int* stk) {
-  const int lane = lane_id();
-  for (int i = lane; i < n; i += 64) mark[i] = 0;
-  if (lane == 0) stk[0] = 0, stk[1] = n, stk[2] = 2 * (63 - __clzll((long long)(n > 0 ? n : 1)));
-  __syncthreads();
-  if (n <= 1) return;
-  int sp = 1;
-  while (sp > 0) {
-    --sp;
-    int f = stk[3 * sp], l = stk[3 * sp + 1], d = stk[3 * sp + 2];
-    __syncthreads();
-    bool heaped = false;
-    while (l - f > 16) {
-      if (d == 0) {
-        if (lane == 0) {
-          LdsSorter srt{ord, rk};
-          srt.heap_sort(f, l);
-        }
-        heaped = true;
-        __syncthreads();
-        break;
-      }
-      --d;
-      {  // median of three to the front (uniform decision, lane 0 swaps)
-        const int x = f + 1, y = f + (l - f) / 2, z = l - 1;
-        const int kx = rk[ord[x]], ky = rk[ord[y]], kz = rk[ord[z]];
-        int s;
-        if (kx < ky) s = ky < kz ? y : (kx < kz ? z : x);
-        else s = kx < kz ? x : (ky < kz ? z : y);
-        __syncthreads();
-        if (lane == 0) {
-          const int t = ord[f];
-          ord[f] = ord[s];
-          ord[s] = t;
-        }
-        __syncthreads();
-      }
-      const int cut = wave_partition(f + 1, l, ord, rk, Apos, Bpos);
-      if (lane == 0) stk[3 * sp] = cut, stk[3 * sp + 1] = l, stk[3 * sp + 2] = d;
-      ++sp;
-      l = cut;
-      __syncthreads();
-    }
-    if (lane == 0 && l > f) mark[f] = heaped ? 2 : 1;
-    __syncthreads();
-  }
-  // final segments tile [0, n); stable-sort the leaves (<= 16 elements each)
-  for (int j = lane; j < n; j += 64) {
-    int f = j;
-    while (mark[f] == 0) --f;
-    int dst = j;
-    if (mark[f] == 1) {
-      int l = j + 1;
-      while (l < n && mark[l] == 0) ++l;
-      const int kj = rk[ord[j]];
-      int r = 0;
-      for (int i = f; i < l; ++i) {
-        const int ki = rk[ord[i]];
-        r += ki < kj || (i < j && !(kj < ki));
-      }
-      dst = f + r;
-    }
-    Bpos[dst] = ord[j];
-  }
-  __syncthreads();
-  for (int j = lane; j < n; j += 64) ord[j] = Bpos[j];
-  __syncthreads();
-}
 
-// CAP: contacts a pair may have to be handled here (LDS 60 B each); pairs
-// with more go to the `over` list.  `list` (device count *n_list) selects
-// the pairs, or all n_pairs when it is null.
-template <int CAP>
-__global__ void __launch_bounds__(64) k_pair_finish_wave(int n_pairs, const int* list, const int* n_list,
-                                                         const ContactRec* c, const int* pair_off, int* perm, int mode,
-                                                         int* keep, int* kept_cnt, const double* pair_rec,
-                                                         double* scale, int* over, int* n_over, const int* huge,
-                                                         const int* n_huge) {
-  constexpr int POST_CAP = CAP;
-  __shared__ unsigned long long skey[POST_CAP];
-  __shared__ int sidx[POST_CAP];
-  __shared__ LdsKey sk[POST_CAP];
-  __shared__ int ord[POST_CAP];
-  __shared__ double smid[POST_CAP * 3];
-  __shared__ int kept_list[POST_CAP];
-  __shared__ int rk[POST_CAP];
-  __shared__ int stk[3 * 64];
-  const int lane = threadIdx.x;
-  const int n_items = list ? *n_list : n_pairs;
-  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
-    const int p = list ? list[it] : it;
-    const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
-    if (n > POST_CAP) {
-      if (lane == 0) over[atomicAdd(n_over, 1)] = p;
-      continue;
-    }
-    int n2 = 1;
-    while (n2 < n) n2 <<= 1;
-    for (int i = lane; i < n2; i += 64) {
-      int idx = i < n ? perm[b + i] : -1;
-      skey[i] = i < n ? c[idx].key : ~0ull;
-      sidx[i] = idx;
-    }
-    __syncthreads();
-    // bitonic sort of (skey, sidx) ascending (keys are unique)
-    for (int kk = 2; kk <= n2; kk <<= 1)
-      for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-        for (int i = lane; i < n2; i += 64) {
-          int l = i ^ jj;
-          if (l > i) {
-            bool up = (i & kk) == 0;
-            unsigned long long ki = skey[i], kl = skey[l];
-            if ((ki > kl) == up) {
-              skey[i] = kl, skey[l] = ki;
-              int t = sidx[i];
-              sidx[i] = sidx[l], sidx[l] = t;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    if (mode == 2) {  // step rule, separate_clusters.cpp:496-564
-      const double* pr = pair_rec + (size_t)PR_STRIDE * p;
-      double mt = pr[PR_MAXT], ms = 1.0;
-      v3<double> va{pr[PR_VA], pr[PR_VA + 1], pr[PR_VA + 2]}, wa{pr[PR_WA], pr[PR_WA + 1], pr[PR_WA + 2]};
-      v3<double> xa{pr[PR_XA], pr[PR_XA + 1], pr[PR_XA + 2]};
-      v3<double> vb{pr[PR_VB], pr[PR_VB + 1], pr[PR_VB + 2]}, wb{pr[PR_WB], pr[PR_WB + 1], pr[PR_WB + 2]};
-      v3<double> xb{pr[PR_XB], pr[PR_XB + 1], pr[PR_XB + 2]};
-      for (int i = lane; i < n; i += 64) {
-        const ContactRec& ct = c[sidx[i]];
-        v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
-        v3<double> nrm = sub(A, B);
-        double inter = ct.r[6] + ct.r[7], toc = ct.r[10];
-        v3<double> rel = sub(add(va, cross(wa, sub(A, xa))), add(vb, cross(wb, sub(B, xb))));
-        double len = vlen(nrm);
-        if (!(dot(unit(nrm), unit(rel)) <= 0.0 || len < 1e-4)) continue;
-        double s;
-        if (len < 1e-4) {
-          if (toc == 0.0) continue;
-          double st = toc - (vlen(rel) * mt) / inter;
-          st = st > 0.0 ? st : 0.0;
-          s = lerp(st, toc, 0.75);
-        } else {
-          if (!(toc < 1.0)) continue;
-          double pv = -(dot(unit(nrm), rel));
-          double depth = inter - len;
-          double pd = (pv * toc) * mt;
-          if (!(pd > 1e-6)) continue;
-          if (depth < pd) {
-            s = lerp(toc * (1.0 - depth / pd), toc, 0.5);
-          } else {
-            double fp = toc + ((1.0 - toc) * inter) / (pv * mt);
-            s = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);
-          }
-        }
-        if (s < ms) ms = s;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        double o = __shfl_xor(ms, off);
-        if (o < ms) ms = o;
-      }
-      if (lane == 0) scale[p] = ms;
-      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[i];
-    } else if (mode == 1) {  // filterContacts(.., 0.0)
-      for (int i = lane; i < n; i += 64) {
-        const ContactRec& ct = c[sidx[i]];
-        sk[i].eps = ct.r[6] + ct.r[7];
-        sk[i].dist = vlen(sub(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}));
-        v3<double> m = dvs(add(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}), 2.0);
-        smid[3 * i] = m.x, smid[3 * i + 1] = m.y, smid[3 * i + 2] = m.z;
-      }
-      for (int i = lane; i < n2; i += 64) ord[i] = i;
-      __syncthreads();
-      // parallel sort by (eps, dist, position); padding sorts last
-      for (int kk = 2; kk <= n2; kk <<= 1)
-        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
-          for (int i = lane; i < n2; i += 64) {
-            int l = i ^ jj;
-            if (l > i) {
-              bool up = (i & kk) == 0;
-              int oi = ord[i], ol = ord[l];
-              bool gt;  // ord[i] sorts after ord[l]
-              if (oi >= n || ol >= n) gt = oi > ol;
-              else if (lds_less(sk[ol], sk[oi])) gt = true;
-              else if (lds_less(sk[oi], sk[ol])) gt = false;
-              else gt = oi > ol;
-              if (gt == up) ord[i] = ol, ord[l] = oi;
-            }
-          }
-          __syncthreads();
-        }
-      // dense integer rank of every key (equal keys share a rank) from the
-      // sorted order; a repeated key means std::sort's order among the tied
-      // contacts must be replayed
-      bool tie = false;
-      int run = 0;
-      for (int t0 = 0; t0 < n; t0 += 64) {
-        const int t = t0 + lane;
-        bool step = false;
-        if (t < n && t > 0) {
-          const LdsKey x = sk[ord[t - 1]], y = sk[ord[t]];
-          step = !(x.eps == y.eps && x.dist == y.dist);
-          tie = tie || !step;
-        }
-        const unsigned long long m = __ballot(step);
-        if (t < n) rk[ord[t]] = run + __popcll(m & (lanes_below() | (1ull << lane)));
-        run += __popcll(m);
-      }
-      __syncthreads();
-      if (__ballot(tie)) {
-        for (int i = lane; i < n; i += 64) ord[i] = i;
-        __syncthreads();
-        // skey (no longer needed) holds the two rank lists
-        wave_std_sort(n, ord, rk, reinterpret_cast<int*>(skey), reinterpret_cast<int*>(skey) + POST_CAP, kept_list,
-                      stk);
-      }
-      // greedy pass in sorted order, 64 candidates at a time: each lane tests
-      // its candidate against the contacts kept before the block and against
-      // the earlier candidates of the block; the block's decisions are then
-      // made in order on the wave's scalar unit
-      int nk = 0;
-      for (int t0 = 0; t0 < n; t0 += 64) {
-        const int t = t0 + lane;
-        const bool valid = t < n;
-        const int ci = valid ? ord[t] : 0;
-        const double mx = smid[3 * ci], my = smid[3 * ci + 1], mz = smid[3 * ci + 2];
-        const double search = sk[ci].eps + 0.0;
-        bool near = false;
-        for (int q0 = 0; q0 < nk; q0 += 64) {
-          const int h = q0 + lane < nk ? kept_list[q0 + lane] : 0;
-          const double hx = smid[3 * h], hy = smid[3 * h + 1], hz = smid[3 * h + 2];
-          const int m = nk - q0 < 64 ? nk - q0 : 64;
-          for (int u = 0; u < m; ++u) {
-            const v3<double> dd{mx - __shfl(hx, u), my - __shfl(hy, u), mz - __shfl(hz, u)};
-            near = near || near_mid(dd, search);
-          }
-        }
-        unsigned long long conf = 0;  // bit u: an earlier candidate u of the block is near this one
-        const int m = n - t0 < 64 ? n - t0 : 64;
-        for (int u = 0; u < m; ++u) {
-          const v3<double> dd{mx - __shfl(mx, u), my - __shfl(my, u), mz - __shfl(mz, u)};
-          if (u < lane && near_mid(dd, search)) conf |= 1ull << u;
-        }
-        const unsigned long long blocked = __ballot(near);
-        unsigned long long kept = 0;
-        for (int i = 0; i < m; ++i) {
-          const unsigned long long ci_conf = __shfl(conf, i);
-          if (!((blocked >> i) & 1ull) && !(ci_conf & kept)) kept |= 1ull << i;
-        }
-        const bool fresh = (kept >> lane) & 1ull;
-        if (valid) keep[b + t] = fresh;
-        if (fresh) kept_list[nk + __popcll(kept & lanes_below())] = ci;
-        nk += __popcll(kept);
-        __syncthreads();
-      }
-      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[ord[i]];
-      if (lane == 0) kept_cnt[p] = nk;
-    } else {
-      for (int i = lane; i < n; i += 64) perm[b + i] = sidx[i];
-    }
-    __syncthreads();
-  }
-  // pairs above every LDS capacity (listed before this launch): one lane each
-  if (huge)
-    for (int t = blockIdx.x * blockDim.x + lane; t < *n_huge; t += gridDim.x * blockDim.x)
-      pair_finish_one(huge[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
-}
 
-// ---- one block (4 wavefronts) per pair: pairs with at most BLK_CAP contacts
+// ---- one block (4 wavefronts) per pair: pairs with at most CAP contacts
+// (BLK_CAP for every pair, then POST_CAP_LARGE for the larger ones)
 constexpr int BLK_CAP = 512;
-constexpr int BLK_SEGS = 3 * 64;  // queued segments per round (> 16 elements each: at most BLK_CAP / 17)
+// queued segments per round (> 16 elements each: at most CAP / 17), 3 ints each
+constexpr int blk_segs(int cap) { return 3 * (cap / 17 + 1); }
 
-// ord[0, n) := the order std::sort gives (as wave_std_sort), with the
+// ord[0, n) := the order std::sort gives, with the
 // partitions of one round's segments spread over the block's wavefronts:
 // partitions of disjoint segments commute, and every segment keeps its own
 // depth budget, so the order equals the sequential introsort's.  Apos/Bpos
 // are indexed by position (a segment's lists stay inside its range).
 __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* Bpos, int* mark, int* segs,
-                               int* s_cnt) {
+                               int* s_cnt, int seg_stride) {
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
   for (int i = tid; i < n; i += blockDim.x) mark[i] = 0;
   if (tid == 0) {  // thread 0 also zeroed mark[0] above
@@ -2225,8 +1950,8 @@ __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* B
   for (;;) {
     const int m = s_cnt[cur];
     if (m == 0) break;
-    const int* L = segs + cur * BLK_SEGS;
-    int* N = segs + (cur ^ 1) * BLK_SEGS;
+    const int* L = segs + cur * seg_stride;
+    int* N = segs + (cur ^ 1) * seg_stride;
     for (int q = w; q < m; q += (int)(blockDim.x >> 6)) {
       const int f = L[3 * q], l = L[3 * q + 1];
       int d = L[3 * q + 2];
@@ -2297,33 +2022,41 @@ __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* B
   __syncthreads();
 }
 
-// The first post-processing pass (as k_pair_finish_wave, same results) with a
-// 256-thread block per pair: orders by rank counting instead of sorting
+// Post-processing of one pair per 256-thread block: orders by rank counting instead of sorting
 // networks (emission keys are unique; (eps, dist) ranks equal exactly for tied
 // keys), the tie replay over the block's wavefronts, and the greedy pass of
 // filterContacts 64 candidates at a time with the distance tests spread over
 // the wavefronts and the decisions taken in order by the first one.
-__global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const ContactRec* c, const int* pair_off,
-                                                           int* perm, int mode, int* keep, int* kept_cnt,
+// CAP = BLK_CAP: every pair (list null); pairs above CAP go to `over` (up to
+// POST_CAP_LARGE contacts) or `huge`.  CAP = POST_CAP_LARGE: the pairs of
+// `list`, then the pairs of `tail` one lane each (above every LDS capacity).
+template <int CAP>
+__global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const int* list, const int* n_list,
+                                                           const ContactRec* c, const int* pair_off, int* perm,
+                                                           int mode, int* keep, int* kept_cnt,
                                                            const double* pair_rec, double* scale, int* over,
-                                                           int* n_over, int* huge, int* n_huge) {
-  __shared__ unsigned long long skey[BLK_CAP];  // emission keys, then the partition lists of the tie replay
-  __shared__ int sidx[BLK_CAP];
-  __shared__ int eo[BLK_CAP];  // contact at each emission position
-  __shared__ LdsKey sk[BLK_CAP];
-  __shared__ double smid[3 * BLK_CAP];
-  __shared__ int rk[BLK_CAP];
-  __shared__ int ord[BLK_CAP];
-  __shared__ int mark[BLK_CAP];  // leaf marks of the tie replay, then the kept list
-  __shared__ int segs[2 * BLK_SEGS];
+                                                           int* n_over, int* huge, int* n_huge, const int* tail,
+                                                           const int* n_tail) {
+  constexpr int SEGS = blk_segs(CAP);
+  __shared__ unsigned long long skey[CAP];  // emission keys, then the partition lists of the tie replay
+  __shared__ int sidx[CAP];
+  __shared__ int eo[CAP];  // contact at each emission position
+  __shared__ LdsKey sk[CAP];
+  __shared__ double smid[3 * CAP];
+  __shared__ int rk[CAP];
+  __shared__ int ord[CAP];
+  __shared__ int mark[CAP];  // leaf marks of the tie replay, then the kept list
+  __shared__ int segs[2 * SEGS];
   __shared__ unsigned long long conf_w[4][64];
   __shared__ int near_w[4][64];
   __shared__ double s_min[4];
   __shared__ int s_misc[4];
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
-  for (int p = blockIdx.x; p < n_pairs; p += gridDim.x) {
+  const int n_items = list ? *n_list : n_pairs;
+  for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
+    const int p = list ? list[it] : it;
     const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
-    if (n > BLK_CAP) {  // to k_pair_finish_wave: its LDS pass, or (above its capacity) one lane per pair
+    if (n > CAP) {  // to the second launch: its LDS pass, or (above its capacity) one lane per pair
       if (tid == 0) {
         if (n > POST_CAP_LARGE) huge[atomicAdd(n_huge, 1)] = p;
         else over[atomicAdd(n_over, 1)] = p;
@@ -2441,7 +2174,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
         for (int i = tid; i < n; i += 256) ord[i] = i;
         __syncthreads();
         int* lists = reinterpret_cast<int*>(skey);
-        block_std_sort(n, ord, rk, lists, lists + BLK_CAP, mark, segs, s_misc + 2);
+        block_std_sort(n, ord, rk, lists, lists + CAP, mark, segs, s_misc + 2, SEGS);
       }
       // greedy pass in sorted order
       int nk = 0;
@@ -2493,6 +2226,11 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const Co
       for (int i = tid; i < n; i += 256) perm[b + i] = eo[i];
     }
     __syncthreads();
+  }
+  if constexpr (CAP == POST_CAP_LARGE) {  // (kept out of the first launch's code: registers, scratch)
+    if (tail)  // pairs above every LDS capacity (listed by the first launch): one lane each
+      for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x)
+        pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
   }
 }
 
@@ -3095,12 +2833,14 @@ int enqueue_post(d2_ctx* c, int mode) {
     // pairs above the block kernel's capacity: the wavefront kernel's LDS pass
     // (list big), above that one lane per pair (list huge, filled by the block
     // kernel too, so the wavefront kernel never adds to it)
-    hipLaunchKernelGGL(k_pair_finish_block, dim3(wblocks), dim3(256), 0, c->stream, np, c->contacts.p, c->pair_off.p,
-                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p,
-                       c->huge.p, c->n_big.p + 1);
-    hipLaunchKernelGGL(k_pair_finish_wave<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(64), 0,
-                       c->stream, np, c->big.p, c->n_big.p, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p,
-                       c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
+    hipLaunchKernelGGL(k_pair_finish_block<BLK_CAP>, dim3(wblocks), dim3(256), 0, c->stream, np, (const int*)nullptr,
+                       (const int*)nullptr, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
+                       c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p, c->huge.p, c->n_big.p + 1,
+                       (const int*)nullptr, (const int*)nullptr);
+    hipLaunchKernelGGL(k_pair_finish_block<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(256), 0,
+                       c->stream, np, (const int*)c->big.p, (const int*)c->n_big.p, c->contacts.p, c->pair_off.p,
+                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p,
+                       c->n_big.p + 1, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
                        (const int*)(c->n_big.p + 1));
   }
   if (filtered) {
