@@ -1656,6 +1656,50 @@ struct Sorter {
 
 // Per pair: restore the breadth-first order (insertion sort on the key), then
 // either the contact filter (discrete) or the step rule (CCD).
+// The step rule's scale for one contact (separate_clusters.cpp:496-564):
+// false when the contact sets none.  k: the pair's future-state kinematics.
+struct StepKin {
+  double mt;
+  v3<double> va, wa, xa, vb, wb, xb;
+};
+__device__ __forceinline__ StepKin step_kin(const double* pr) {
+  StepKin k;
+  k.mt = pr[PR_MAXT];
+  k.va = {pr[PR_VA], pr[PR_VA + 1], pr[PR_VA + 2]}, k.wa = {pr[PR_WA], pr[PR_WA + 1], pr[PR_WA + 2]};
+  k.xa = {pr[PR_XA], pr[PR_XA + 1], pr[PR_XA + 2]};
+  k.vb = {pr[PR_VB], pr[PR_VB + 1], pr[PR_VB + 2]}, k.wb = {pr[PR_WB], pr[PR_WB + 1], pr[PR_WB + 2]};
+  k.xb = {pr[PR_XB], pr[PR_XB + 1], pr[PR_XB + 2]};
+  return k;
+}
+__device__ __forceinline__ bool step_rule_scale(const ContactRec& ct, const StepKin& k, double& s) {
+  const double mt = k.mt;
+  v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
+  v3<double> nrm = sub(A, B);
+  double inter = ct.r[6] + ct.r[7], toc = ct.r[10];
+  v3<double> rel = sub(add(k.va, cross(k.wa, sub(A, k.xa))), add(k.vb, cross(k.wb, sub(B, k.xb))));
+  double len = vlen(nrm);
+  if (!(dot(unit(nrm), unit(rel)) <= 0.0 || len < 1e-4)) return false;
+  if (len < 1e-4) {
+    if (toc == 0.0) return false;
+    double st = toc - (vlen(rel) * mt) / inter;
+    st = st > 0.0 ? st : 0.0;  // std::max(0.0, x): x when x > 0 (NaN keeps 0)
+    s = lerp(st, toc, 0.75);
+    return true;
+  }
+  if (!(toc < 1.0)) return false;
+  double pv = -(dot(unit(nrm), rel));
+  double depth = inter - len;
+  double pd = (pv * toc) * mt;
+  if (!(pd > 1e-6)) return false;
+  if (depth < pd) {
+    s = lerp(toc * (1.0 - depth / pd), toc, 0.5);
+  } else {
+    double fp = toc + ((1.0 - toc) * inter) / (pv * mt);
+    s = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);  // std::min(fp, 1.0)
+  }
+  return true;
+}
+
 __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
                                 int* kept_cnt, const double* pair_rec, double* scale) {
   {
@@ -1703,42 +1747,11 @@ __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off,
       }
       kept_cnt[p] = kept;
     } else if (mode == 2) {  // step rule, separate_clusters.cpp:496-564
-      const double* pr = pair_rec + (size_t)PR_STRIDE * p;
-      double mt = pr[PR_MAXT], ms = 1.0;
-      v3<double> va{pr[PR_VA], pr[PR_VA + 1], pr[PR_VA + 2]}, wa{pr[PR_WA], pr[PR_WA + 1], pr[PR_WA + 2]};
-      v3<double> xa{pr[PR_XA], pr[PR_XA + 1], pr[PR_XA + 2]};
-      v3<double> vb{pr[PR_VB], pr[PR_VB + 1], pr[PR_VB + 2]}, wb{pr[PR_WB], pr[PR_WB + 1], pr[PR_WB + 2]};
-      v3<double> xb{pr[PR_XB], pr[PR_XB + 1], pr[PR_XB + 2]};
+      const StepKin k = step_kin(pair_rec + (size_t)PR_STRIDE * p);
+      double ms = 1.0;
       for (int i = b; i < e; ++i) {
-        const ContactRec& ct = c[perm[i]];
-        v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
-        v3<double> nrm = sub(A, B);
-        double inter = ct.r[6] + ct.r[7], toc = ct.r[10];
-        v3<double> rel = sub(add(va, cross(wa, sub(A, xa))), add(vb, cross(wb, sub(B, xb))));
-        double len = vlen(nrm);
-        if (dot(unit(nrm), unit(rel)) <= 0.0 || len < 1e-4) {
-          double s;
-          if (len < 1e-4) {
-            if (toc == 0.0) continue;
-            double st = toc - (vlen(rel) * mt) / inter;
-            st = st > 0.0 ? st : 0.0;  // std::max(0.0, x): x when x > 0 (NaN keeps 0)
-            s = lerp(st, toc, 0.75);
-          } else if (toc < 1.0) {
-            double pv = -(dot(unit(nrm), rel));
-            double depth = inter - len;
-            double pd = (pv * toc) * mt;
-            if (!(pd > 1e-6)) continue;
-            if (depth < pd) {
-              s = lerp(toc * (1.0 - depth / pd), toc, 0.5);
-            } else {
-              double fp = toc + ((1.0 - toc) * inter) / (pv * mt);
-              s = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);  // std::min(fp, 1.0)
-            }
-          } else {
-            continue;
-          }
-          if (s < ms) ms = s;  // std::min(ms, s): NaN never wins
-        }
+        double s;
+        if (step_rule_scale(c[perm[i]], k, s) && s < ms) ms = s;  // std::min(ms, s): NaN never wins
       }
       scale[p] = ms;
     }
@@ -2027,6 +2040,66 @@ __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* B
 // keys), the tie replay over the block's wavefronts, and the greedy pass of
 // filterContacts 64 candidates at a time with the distance tests spread over
 // the wavefronts and the decisions taken in order by the first one.
+// A pair above every LDS capacity without the filter (mode 0 or 2), one block:
+// the emission order by rank counting (the keys are unique; each thread ranks
+// up to 8 x 256 contacts at a time against key tiles of CAP staged in LDS),
+// written to perm from a copy of the pair's contact indices in keep (unused
+// by these modes); in mode 2 the step rule's minimum over the block.
+template <int CAP>
+__device__ void pair_finish_huge(int p, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
+                                 const double* pair_rec, double* scale, unsigned long long* skey, double* s_min) {
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const int b = pair_off[p], n = pair_off[p + 1] - b;
+  for (int i = tid; i < n; i += blockDim.x) keep[b + i] = perm[b + i];
+  __syncthreads();
+  constexpr int K = 8;
+  for (int q0 = 0; q0 < n; q0 += K * (int)blockDim.x) {
+    unsigned long long mk[K];
+    int rk[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = q0 + k * (int)blockDim.x + tid;
+      mk[k] = i < n ? c[keep[b + i]].key : ~0ull;
+      rk[k] = 0;
+    }
+    for (int t0 = 0; t0 < n; t0 += CAP) {
+      const int m = n - t0 < CAP ? n - t0 : CAP;
+      for (int j = tid; j < m; j += blockDim.x) skey[j] = c[keep[b + t0 + j]].key;
+      __syncthreads();
+      for (int j = 0; j < m; ++j) {
+        const unsigned long long kj = skey[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) rk[k] += kj < mk[k] ? 1 : 0;
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = q0 + k * (int)blockDim.x + tid;
+      if (i < n) perm[b + rk[k]] = keep[b + i];
+    }
+  }
+  if (mode == 2) {  // step rule, separate_clusters.cpp:496-564: the minimum over the pair's contacts
+    const StepKin kin = step_kin(pair_rec + (size_t)PR_STRIDE * p);
+    double ms = 1.0;
+    for (int i = tid; i < n; i += blockDim.x) {
+      double sc;
+      if (step_rule_scale(c[keep[b + i]], kin, sc) && sc < ms) ms = sc;  // NaN never wins
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double o = __shfl_xor(ms, off);
+      if (o < ms) ms = o;
+    }
+    if (lane == 0) s_min[w] = ms;
+    __syncthreads();
+    if (tid == 0) {
+      double m = s_min[0];
+      for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = (s_min[k] < m) ? s_min[k] : m;
+      scale[p] = m;
+    }
+  }
+}
+
 // CAP = BLK_CAP: every pair (list null); pairs above CAP go to `over` (up to
 // POST_CAP_LARGE contacts) or `huge`.  CAP = POST_CAP_LARGE: the pairs of
 // `list`, then the pairs of `tail` one lane each (above every LDS capacity).
@@ -2228,9 +2301,17 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
     __syncthreads();
   }
   if constexpr (CAP == POST_CAP_LARGE) {  // (kept out of the first launch's code: registers, scratch)
-    if (tail)  // pairs above every LDS capacity (listed by the first launch): one lane each
-      for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x)
-        pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
+    if (tail) {  // pairs above every LDS capacity (listed by the first launch)
+      if (mode == 1) {  // the filter's std::sort replay: one lane per pair
+        for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x)
+          pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
+      } else {  // one block per pair
+        for (int t = blockIdx.x; t < *n_tail; t += gridDim.x) {
+          pair_finish_huge<CAP>(tail[t], c, pair_off, perm, mode, keep, pair_rec, scale, skey, s_min);
+          __syncthreads();
+        }
+      }
+    }
   }
 }
 
