@@ -503,6 +503,66 @@ __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
   }
 }
 
+// FP32 culls only decide which FP64 tests run, so their arithmetic may fuse:
+// an FMA rounds once where a multiply-add rounds twice, so the error bounds
+// the margins cover (gamma_n |a||b| for an n-term dot product) still hold.
+__device__ __forceinline__ float f3dot(const float* a, const float* b) {
+  return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
+// FP32 bounds of the CCD edge-edge sub-tests (soup and leaf levels): see
+// the FP32 culls of CCD leaf sub-tests below for the argument and margin.
+struct M34f {
+  float m[12];
+};
+__device__ __forceinline__ M34f to_m34f(const double* m, const double* org) {
+  M34f o;
+  for (int r = 0; r < 3; ++r) {
+    o.m[4 * r] = (float)m[4 * r], o.m[4 * r + 1] = (float)m[4 * r + 1], o.m[4 * r + 2] = (float)m[4 * r + 2];
+    o.m[4 * r + 3] = (float)(m[4 * r + 3] - (org ? org[4 * r + 3] : 0.0));
+  }
+  return o;
+}
+__device__ __forceinline__ void xf(const M34f& M, const v3<double>& p, float* o) {
+  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
+  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
+  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
+  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
+}
+__device__ __forceinline__ float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
+// Separation of segment Q above segment P along u at one instant (negative
+// when none), scaled by 1/|u|.
+__device__ __forceinline__ float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0,
+                                         const float* q1) {
+  const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
+  return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
+}
+__device__ bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
+                               const float* bs1, const float* be0, const float* be1, float thr) {
+  const float EPSF = 5.9604645e-08f;
+  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
+                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
+  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
+  float ax[4][3];
+  for (int r = 0; r < 3; ++r) {
+    ax[0][r] = 0.5f * ((bs0[r] + bs1[r]) - (as0[r] + as1[r]));
+    ax[1][r] = 0.5f * ((be0[r] + be1[r]) - (ae0[r] + ae1[r]));
+  }
+  const float da[3] = {as1[0] - as0[0], as1[1] - as0[1], as1[2] - as0[2]};
+  const float db[3] = {bs1[0] - bs0[0], bs1[1] - bs0[1], bs1[2] - bs0[2]};
+  const float ea[3] = {ae1[0] - ae0[0], ae1[1] - ae0[1], ae1[2] - ae0[2]};
+  const float eb[3] = {be1[0] - be0[0], be1[1] - be0[1], be1[2] - be0[2]};
+  ax[2][0] = da[1] * db[2] - da[2] * db[1], ax[2][1] = da[2] * db[0] - da[0] * db[2], ax[2][2] = da[0] * db[1] - da[1] * db[0];
+  ax[3][0] = ea[1] * eb[2] - ea[2] * eb[1], ax[3][1] = ea[2] * eb[0] - ea[0] * eb[2], ax[3][2] = ea[0] * eb[1] - ea[1] * eb[0];
+  for (int k = 0; k < 4; ++k) {
+    const float nu = sqrtf(f3dot(ax[k], ax[k]));
+    if (!(nu > 1e-20f)) continue;
+    const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
+    const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
+    if (fmaxf(up, dn) > lim) return true;
+  }
+  return false;
+}
+
 __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<double> Q, double toc) {
   r[0] = d, r[1] = toc, r[2] = P.x, r[3] = P.y, r[4] = P.z, r[5] = Q.x, r[6] = Q.y, r[7] = Q.z;
 }
@@ -548,12 +608,29 @@ __device__ __forceinline__ void soup_edge_body(Dev& D, long long blk, long long 
     tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), j);
     v3<double> a0 = corner(ta, k), a1 = corner(ta, (k + 1) % 3);
     v3<double> b0 = corner(tb, l), b1 = corner(tb, (l + 1) % 3);
+    double* slot = D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8;
+    if (D.fp32) {
+      // FP32 form: the separating-axis bound of the leaf level's edge-edge
+      // cull (cull_edge_edge, same lerped world-space edges).  A culled
+      // sub-test reports no distance below the search distance, so it cannot
+      // be the winner of a hit; its slot stays at +inf, toc 1, which the
+      // selection never takes.
+      const M34f A0 = to_m34f(pr + PR_TA0, pr + PR_TB0), A1 = to_m34f(pr + PR_TA1, pr + PR_TB0);
+      const M34f B0 = to_m34f(pr + PR_TB0, pr + PR_TB0), B1 = to_m34f(pr + PR_TB1, pr + PR_TB0);
+      float as0[3], as1[3], ae0[3], ae1[3], bs0[3], bs1[3], be0[3], be1[3];
+      xf(A0, a0, as0), xf(A0, a1, as1), xf(A1, a0, ae0), xf(A1, a1, ae1);
+      xf(B0, b0, bs0), xf(B0, b1, bs1), xf(B1, b0, be0), xf(B1, b1, be1);
+      if (cull_edge_edge(as0, as1, ae0, ae1, bs0, bs1, be0, be1, (float)pr[PR_SEARCH])) {
+        store_res(slot, 1.0 / 0.0, v3<double>{0, 0, 0}, v3<double>{0, 0, 0}, 1.0);
+        continue;
+      }
+    }
     v3<double> P, Q;
     double toc = 1.0;
     double d = edge_edge(xform(pr + PR_TA0, a0), xform(pr + PR_TA0, a1), xform(pr + PR_TA1, a0), xform(pr + PR_TA1, a1),
                          xform(pr + PR_TB0, b0), xform(pr + PR_TB0, b1), xform(pr + PR_TB1, b0), xform(pr + PR_TB1, b1),
                          P, Q, toc);
-    store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8, d, P, Q, toc);
+    store_res(slot, d, P, Q, toc);
   }
 }
 
@@ -719,12 +796,6 @@ __global__ void __launch_bounds__(256) k_leaf_sub(Dev D) {
 // dist_ref <= the smallest such bound over the 9 edge pairs.
 // margin = 64*eps_f*kappa*L (4x the barycentric bound) + 16*eps_f*|coords|
 // (FP32 rounding of the relative coordinates and projections) + 1e-6 m.
-// FP32 culls only decide which FP64 tests run, so their arithmetic may fuse:
-// an FMA rounds once where a multiply-add rounds twice, so the error bounds
-// the margins cover (gamma_n |a||b| for an n-term dot product) still hold.
-__device__ __forceinline__ float f3dot(const float* a, const float* b) {
-  return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
-}
 __device__ int cull_class(const double* pa, const double* pb, const tri<double>& la, const tri<double>& lt, double thr_d) {
   const float EPSF = 5.9604645e-08f;
   float ra[9], rb[9], ta[3];
@@ -1130,57 +1201,6 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
 //     same side at t = 0 and at t = 1 (point offsets are linear in t).  Axes:
 //     the midpoint offsets and the edge-direction cross products at both ends.
 // margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products).
-struct M34f {
-  float m[12];
-};
-__device__ __forceinline__ M34f to_m34f(const double* m, const double* org) {
-  M34f o;
-  for (int r = 0; r < 3; ++r) {
-    o.m[4 * r] = (float)m[4 * r], o.m[4 * r + 1] = (float)m[4 * r + 1], o.m[4 * r + 2] = (float)m[4 * r + 2];
-    o.m[4 * r + 3] = (float)(m[4 * r + 3] - (org ? org[4 * r + 3] : 0.0));
-  }
-  return o;
-}
-__device__ __forceinline__ void xf(const M34f& M, const v3<double>& p, float* o) {
-  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
-  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
-  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
-  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
-}
-__device__ __forceinline__ float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
-// Separation of segment Q above segment P along u at one instant (negative
-// when none), scaled by 1/|u|.
-__device__ __forceinline__ float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0,
-                                         const float* q1) {
-  const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
-  return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
-}
-__device__ bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
-                               const float* bs1, const float* be0, const float* be1, float thr) {
-  const float EPSF = 5.9604645e-08f;
-  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
-                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
-  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
-  float ax[4][3];
-  for (int r = 0; r < 3; ++r) {
-    ax[0][r] = 0.5f * ((bs0[r] + bs1[r]) - (as0[r] + as1[r]));
-    ax[1][r] = 0.5f * ((be0[r] + be1[r]) - (ae0[r] + ae1[r]));
-  }
-  const float da[3] = {as1[0] - as0[0], as1[1] - as0[1], as1[2] - as0[2]};
-  const float db[3] = {bs1[0] - bs0[0], bs1[1] - bs0[1], bs1[2] - bs0[2]};
-  const float ea[3] = {ae1[0] - ae0[0], ae1[1] - ae0[1], ae1[2] - ae0[2]};
-  const float eb[3] = {be1[0] - be0[0], be1[1] - be0[1], be1[2] - be0[2]};
-  ax[2][0] = da[1] * db[2] - da[2] * db[1], ax[2][1] = da[2] * db[0] - da[0] * db[2], ax[2][2] = da[0] * db[1] - da[1] * db[0];
-  ax[3][0] = ea[1] * eb[2] - ea[2] * eb[1], ax[3][1] = ea[2] * eb[0] - ea[0] * eb[2], ax[3][2] = ea[0] * eb[1] - ea[1] * eb[0];
-  for (int k = 0; k < 4; ++k) {
-    const float nu = sqrtf(f3dot(ax[k], ax[k]));
-    if (!(nu > 1e-20f)) continue;
-    const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
-    const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
-    if (fmaxf(up, dn) > lim) return true;
-  }
-  return false;
-}
 // Per connected node pair, shared across its wavefront: lane v < 24 holds
 // vertex v of a's leaf (triangle v/3, corner v%3), lanes 24-47 those of b:
 // the vertex at both ends in world coordinates (relative to b's start
