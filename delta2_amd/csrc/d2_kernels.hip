@@ -879,7 +879,8 @@ __device__ int cull_class(const double* pa, const double* pb, const tri<double>&
 // edge, own-normal extent, vertex sum, largest coordinate, and whether its
 // barycentric system is well conditioned.
 struct __align__(16) CullTri {
-  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok, pad[2];
+  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok;
+  double eps;  // the triangle's eps from its bucket (read with the triangle)
 };
 __device__ __forceinline__ void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
   const v3<double> vv[3] = {t.a, t.b, t.c};
@@ -905,12 +906,6 @@ __device__ __forceinline__ void cull_prep(const float* rot, const float* tr, con
   o.pmin = fminf(p0, fminf(p1, p2)), o.pmax = fmaxf(p0, fmaxf(p1, p2));
   for (int r = 0; r < 3; ++r) o.cen[r] = (o.v[r] + o.v[3 + r]) + o.v[6 + r];
   o.crd = crd;
-}
-__device__ __forceinline__ void cull_fetch(const CullTri& mine, int src, CullTri& o) {
-  const float* m = reinterpret_cast<const float*>(&mine);
-  float* d = reinterpret_cast<float*>(&o);
-#pragma unroll
-  for (int k = 0; k < (int)(sizeof(CullTri) / sizeof(float)); ++k) d[k] = __shfl(m[k], src);
 }
 // 0 certain miss, 1 needs the FP64 test, 2 certain hit (the bounds of cull_class).
 // The lower bound compares g^2 > T'^2 |n|^2 per axis instead of g / |n| > T,
@@ -1022,6 +1017,7 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
     const long long w0 = b0 + wid;
     const long long wl = w0 + (long long)nw * lane;
     int l_pair = 0, l_offa = 0, l_offb = 0, l_bka = 0, l_bkb = 0, l_nta = 0, l_ntb = 0, l_inner = 0, l_desc = 0;
+    double l_search = 0.0;
     NP l_np{};
     if (wl < re) {
       const NP np = D.soup[wl];
@@ -1035,6 +1031,7 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
         l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
         l_inner = (na.inner || nb.inner) ? 1 : 0;
         l_desc = expansion_desc(na, nb);
+        if (fp32) l_search = D.pair_rec[(size_t)PR_STRIDE * np.pair + PR_SEARCH];
       }
       if (!fp32) D.exp_desc[wl] = l_desc;
     }
@@ -1065,23 +1062,26 @@ __global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
             tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
           }
           cull_prep(rot, tr, load_tri<double>(D.buckets + (size_t)BUCKET * gbk, kk), mine);
+          mine.eps = D.buckets[(size_t)BUCKET * gbk + 72 + kk];
         }
         s_tri[wid][lane >> 4][slot] = mine;
         wave_sync_lds();
       }
       const int kend = k0 + 4 < batch ? k0 + 4 : batch;
       for (int k = k0; k < kend; ++k) {
-        const int pair = rl(l_pair, k), offa = rl(l_offa, k), offb = rl(l_offb, k);
         const int nta = rl(l_nta, k), ntb = rl(l_ntb, k);
         const bool inner = rl(l_inner, k) != 0;
         const bool valid = i < nta && j < ntb;
         int cls = valid ? 1 : 0;
         if (fp32 && valid) {  // operands read from LDS where they are used (fewer live registers)
-          const double* pr = D.pair_rec + (size_t)PR_STRIDE * pair;
-          const double* ba = D.buckets + (size_t)BUCKET * (offa + rl(l_bka, k));
-          const double* bb = D.buckets + (size_t)BUCKET * (offb + rl(l_bkb, k));
-          const float thr = (float)((pr[PR_SEARCH] + ba[72 + i]) + bb[72 + j]);
-          cls = cull_pair(s_tri[wid][k - k0][i], s_tri[wid][k - k0][8 + j], thr, inner);
+          // no global load on the node pair's chain: the search distance came
+          // with the node pair, the eps values with the prepared triangles
+          const CullTri& A = s_tri[wid][k - k0][i];
+          const CullTri& B = s_tri[wid][k - k0][8 + j];
+          const double search = __longlong_as_double((long long)readlane_u64(
+              (unsigned long long)__double_as_longlong(l_search), k));
+          const float thr = (float)((search + A.eps) + B.eps);
+          cls = cull_pair(A, B, thr, inner);
         }
         unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
         const bool settled = inner && __ballot(cls == 2) != 0ull;
