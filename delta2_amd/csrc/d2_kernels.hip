@@ -150,6 +150,12 @@ template <class R> __device__ __forceinline__ tri<R> load_tri(const double* buck
                 {(R)bucket[3 * 8 + k], (R)bucket[4 * 8 + k], (R)bucket[5 * 8 + k]},
                 {(R)bucket[6 * 8 + k], (R)bucket[7 * 8 + k], (R)bucket[8 * 8 + k]}};
 }
+// Corner v (0-2) of triangle k, read from the bucket: with a runtime v this
+// is an indexed global load, where selecting from a loaded triangle puts the
+// triangle in scratch memory.
+__device__ __forceinline__ v3<double> load_corner(const double* bucket, int k, int v) {
+  return v3<double>{bucket[(3 * v) * 8 + k], bucket[(3 * v + 1) * 8 + k], bucket[(3 * v + 2) * 8 + k]};
+}
 
 struct Dev {
   // trees
@@ -302,7 +308,7 @@ __device__ __forceinline__ void cluster_first_range(const ClusterAcc& A, int n_p
 // step's discrete phase keeps the CCD phase's overflow flag and gate counts)
 // and sets the level table to one level of n_pairs roots (none when the
 // learned gate's certificate appends the phase's roots).
-__global__ void k_setup(Dev D, const int32_t* pairs, const double* max_time, const double* bodies,
+__global__ void __launch_bounds__(256) k_setup(Dev D, const int32_t* pairs, const double* max_time, const double* bodies,
                         const int32_t* body_tree, const int32_t* tree_off, int32_t* pair_info, double* pair_rec,
                         int ccd, int mt_from_bodies, int keep_ccd, double* scale, int* pair_fill, int* kept_cnt,
                         int* n_big, ClusterAcc CA, int gated) {
@@ -325,37 +331,39 @@ __global__ void k_setup(Dev D, const int32_t* pairs, const double* max_time, con
     // max_time: given per pair, or min(last steps of a, b) (separate_clusters.cpp:490)
     const double mt = !ccd ? 0.0 : mt_from_bodies ? ((rb[39] < ra[39]) ? rb[39] : ra[39]) : max_time[p];
     const bool end = q & 1;  // lanes 1, 3 work on the end poses
-    body_state sa, sb;
+    const bool of_a = q < 2;  // lanes 0, 1 own a's pose, lanes 2, 3 b's
+    // the own body's record and the other's (selecting pointers, not private
+    // arrays, keeps the poses in registers)
+    const double* ro = of_a ? ra : rb;
+    const double* rx = of_a ? rb : ra;
+    body_state so, sx;
     if (!end) {
-      sa = load_state(ra), sb = load_state(rb);
+      so = load_state(ro), sx = load_state(rx);
     } else if (ccd) {
-      sa = extrapolate(load_state(ra), mt, ra + 28);
-      sb = extrapolate(load_state(rb), mt, rb + 28);
+      so = extrapolate(load_state(ro), mt, ro + 28);
+      sx = extrapolate(load_state(rx), mt, rx + 28);
     } else {
-      sa = load_state(ra + 14), sb = load_state(rb + 14);
+      so = load_state(ro + 14), sx = load_state(rx + 14);
     }
     double* pr = pair_rec + (size_t)PR_STRIDE * p;
-    double Ta[16], Tb[16];
-    pose(sa.q, sa.t, Ta);
-    pose(sb.q, sb.t, Tb);
-    const bool of_a = q < 2;  // lanes 0, 1 own a's pose, lanes 2, 3 b's
-    const double* own = of_a ? Ta : Tb;
+    double To[16];
+    pose(so.q, so.t, To);
     double* dst = pr + (of_a ? (end ? PR_TA1 : PR_TA0) : (end ? PR_TB1 : PR_TB0));
 #pragma unroll
-    for (int k = 0; k < 16; ++k) dst[k] = own[k];
+    for (int k = 0; k < 16; ++k) dst[k] = To[k];
     if (ccd) {
-      double inv[16];
-      inv44(of_a ? Tb : Ta, inv);
-      mul44(inv, own, pr + (of_a ? (end ? PR_MAB1 : PR_MAB0) : (end ? PR_MBA1 : PR_MBA0)));
+      double Tx[16], inv[16];
+      pose(sx.q, sx.t, Tx);
+      inv44(Tx, inv);
+      mul44(inv, To, pr + (of_a ? (end ? PR_MAB1 : PR_MAB0) : (end ? PR_MBA1 : PR_MBA0)));
       if (end) {
-        const body_state& s = of_a ? sa : sb;
         double Iw[9];
-        world_inv_inertia(s.q, (of_a ? ra : rb) + 28, Iw);
-        const v3<double> w = mul33v(Iw, s.L);
+        world_inv_inertia(so.q, ro + 28, Iw);
+        const v3<double> w = mul33v(Iw, so.L);
         double* k = pr + (of_a ? PR_VA : PR_VB);
-        k[0] = s.v.x, k[1] = s.v.y, k[2] = s.v.z;
+        k[0] = so.v.x, k[1] = so.v.y, k[2] = so.v.z;
         k[3] = w.x, k[4] = w.y, k[5] = w.z;  // PR_WA / PR_WB
-        k[6] = s.t.x, k[7] = s.t.y, k[8] = s.t.z;  // PR_XA / PR_XB
+        k[6] = so.t.x, k[7] = so.t.y, k[8] = so.t.z;  // PR_XA / PR_XB
       }
     }
     if (q == 0) {
@@ -581,12 +589,16 @@ __device__ __forceinline__ void soup_point_body(Dev& D, long long blk, long long
     const NodeRec na = D.nodes[pi[2] + np.na];
     const NodeRec nb = D.nodes[pi[3] + np.nb];
     int i = item.sub >> 3, j = item.sub & 7;
-    tri<double> ta = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[2] + na.bucket), i);
-    tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), j);
+    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
     v3<double> P, Q;
     double toc = 1.0, d;
-    if (dir == 0) d = point_tri(corner(ta, k), pr + PR_MAB0, pr + PR_MAB1, tb, pr + PR_TB0, pr + PR_TB1, P, Q, toc);
-    else d = point_tri(corner(tb, k), pr + PR_MBA0, pr + PR_MBA1, ta, pr + PR_TA0, pr + PR_TA1, P, Q, toc);
+    if (dir == 0)
+      d = point_tri(load_corner(ba, i, k), pr + PR_MAB0, pr + PR_MAB1, load_tri<double>(bb, j), pr + PR_TB0, pr + PR_TB1,
+                    P, Q, toc);
+    else
+      d = point_tri(load_corner(bb, j, k), pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(ba, i), pr + PR_TA0, pr + PR_TA1,
+                    P, Q, toc);
     store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + dir) * 8, d, P, Q, toc);
   }
 }
@@ -604,10 +616,10 @@ __device__ __forceinline__ void soup_edge_body(Dev& D, long long blk, long long 
     const NodeRec na = D.nodes[pi[2] + np.na];
     const NodeRec nb = D.nodes[pi[3] + np.nb];
     int i = item.sub >> 3, j = item.sub & 7;
-    tri<double> ta = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[2] + na.bucket), i);
-    tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), j);
-    v3<double> a0 = corner(ta, k), a1 = corner(ta, (k + 1) % 3);
-    v3<double> b0 = corner(tb, l), b1 = corner(tb, (l + 1) % 3);
+    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+    v3<double> a0 = load_corner(ba, i, k), a1 = load_corner(ba, i, (k + 1) % 3);
+    v3<double> b0 = load_corner(bb, j, l), b1 = load_corner(bb, j, (l + 1) % 3);
     double* slot = D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8;
     if (D.fp32) {
       // FP32 form: the separating-axis bound of the leaf level's edge-edge
@@ -726,13 +738,13 @@ __device__ __forceinline__ void leaf_point_body(Dev& D, long long blk, long long
     int kind, ia, ib;
     if (s < 3 * n * m) {
       int v = s / m, j = s % m;
-      tri<double> ta = load_tri<double>(ba, v / 3), tb = load_tri<double>(bb, j);
-      d = point_tri(corner(ta, v % 3), pr + PR_MAB0, pr + PR_MAB1, tb, pr + PR_TB0, pr + PR_TB1, P, Q, toc);
+      d = point_tri(load_corner(ba, v / 3, v % 3), pr + PR_MAB0, pr + PR_MAB1, load_tri<double>(bb, j), pr + PR_TB0,
+                    pr + PR_TB1, P, Q, toc);
       kind = 1, ia = v, ib = j;
     } else {
       int s2 = s - 3 * n * m, v = s2 / n, i = s2 % n;
-      tri<double> tb = load_tri<double>(bb, v / 3), ta = load_tri<double>(ba, i);
-      d = point_tri(corner(tb, v % 3), pr + PR_MBA0, pr + PR_MBA1, ta, pr + PR_TA0, pr + PR_TA1, P, Q, toc);
+      d = point_tri(load_corner(bb, v / 3, v % 3), pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(ba, i), pr + PR_TA0,
+                    pr + PR_TA1, P, Q, toc);
       kind = 2, ia = i, ib = v;
     }
     if (d < pr[PR_SEARCH]) {
@@ -755,10 +767,10 @@ __device__ __forceinline__ void leaf_edge_body(Dev& D, long long blk, long long 
     const NodeRec nb = D.nodes[pi[3] + np.nb];
     int s = (int)item.sub, n = na.ntris, m = nb.ntris;
     int ea = s / (3 * m), eb = s % (3 * m);
-    tri<double> ta = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[2] + na.bucket), ea / 3);
-    tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), eb / 3);
-    v3<double> a0 = corner(ta, ea % 3), a1 = corner(ta, (ea % 3 + 1) % 3);
-    v3<double> b0 = corner(tb, eb % 3), b1 = corner(tb, (eb % 3 + 1) % 3);
+    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+    v3<double> a0 = load_corner(ba, ea / 3, ea % 3), a1 = load_corner(ba, ea / 3, (ea % 3 + 1) % 3);
+    v3<double> b0 = load_corner(bb, eb / 3, eb % 3), b1 = load_corner(bb, eb / 3, (eb % 3 + 1) % 3);
     v3<double> P, Q;
     double toc = 1.0;
     double d = edge_edge(xform(pr + PR_TA0, a0), xform(pr + PR_TA0, a1), xform(pr + PR_TA1, a0), xform(pr + PR_TA1, a1),
@@ -1000,7 +1012,10 @@ __device__ __forceinline__ void write_children(const Dev& D, long long slot, con
     if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, parent, np.key * 8ull + (unsigned long long)c};
   }
 }
-__global__ void __launch_bounds__(256) k_tri_mask(Dev D, int fp32) {
+// Five wavefronts per SIMD (at most 96 VGPRs, no spills) instead of the four
+// its natural 101 VGPRs allow: the kernel waits on dependent loads, and more
+// resident wavefronts hide more of them.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_tri_mask(Dev D, int fp32) {
   __shared__ int s_tot[2][4], s_ch[2][4];
   __shared__ CullTri s_tri[4][4][16];  // per wavefront: 4 node pairs' 8 + 8 prepared triangles
   __shared__ unsigned long long s_base[2], s_chb[2];
@@ -1236,7 +1251,7 @@ __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
     const bool of_b = lane >= 24;
     const int v = of_b ? lane - 24 : lane;
     if (v / 3 < (of_b ? I.m : I.n)) {
-      const v3<double> p = corner(load_tri<double>(of_b ? bb : ba, v / 3), v % 3);
+      const v3<double> p = load_corner(of_b ? bb : ba, v / 3, v % 3);
       const M34f W0 = to_m34f(of_b ? pr + PR_TB0 : pr + PR_TA0, pr + PR_TB0);
       const M34f W1 = to_m34f(of_b ? pr + PR_TB1 : pr + PR_TA1, pr + PR_TB0);
       const M34f F0 = to_m34f(of_b ? pr + PR_MBA0 : pr + PR_MAB0, nullptr);
