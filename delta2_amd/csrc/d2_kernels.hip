@@ -419,51 +419,67 @@ __global__ void k_fold_final(Dev D, ClusterAcc CA) {
 }
 
 // --------------------------------------------------------- CCD soup level
+// Swept box of a triangle: bbox at the start pose expanded by bbox at the end
+// pose, then by geo_eps (continuous_comparison.h:804-812); lo xyz, hi xyz.
+__device__ __forceinline__ void swept_box(const tri<double>& t, const double* m0, const double* m1, double g,
+                                          double* o) {
+  const tri<double> u = xform(m0, t), v = xform(m1, t);
+  const double ux[3] = {u.a.x, u.b.x, u.c.x}, uy[3] = {u.a.y, u.b.y, u.c.y}, uz[3] = {u.a.z, u.b.z, u.c.z};
+  const double vx[3] = {v.a.x, v.b.x, v.c.x}, vy[3] = {v.a.y, v.b.y, v.c.y}, vz[3] = {v.a.z, v.b.z, v.c.z};
+  const double* U[3] = {ux, uy, uz};
+  const double* V[3] = {vx, vy, vz};
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double lo = smin(smin(U[k][0], U[k][1]), U[k][2]), hi = smax(smax(U[k][0], U[k][1]), U[k][2]);
+    const double lo2 = smin(smin(V[k][0], V[k][1]), V[k][2]), hi2 = smax(smax(V[k][0], V[k][1]), V[k][2]);
+    o[k] = smin(lo, lo2) - g, o[3 + k] = smax(hi, hi2) + g;
+  }
+}
 // One wavefront per soup node pair (chunked ranges), lane = triangle pair:
 // swept-AABB overlap mask, its popcount and the block partial; also folds the
-// connected pass of the previous level.
+// connected pass of the previous level.  Lanes 0-7 build the swept boxes of
+// bucket a's triangles, lanes 8-15 bucket b's, into LDS; each triangle pair
+// then compares two of them.
 __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
+  __shared__ double s_box[4][16][6];
   fold_pass(D, D.pair_rec, D.pass_min_c, D.pass_flag_c);
   const long long n = D.n_soup();
   const int lane = lane_id();
   long long rb, re;
   chunk_of(n, rb, re);
   long long acc = 0;
-  for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {
+  // the wavefront index made visibly uniform: the node pair's record, nodes
+  // and poses then come through scalar loads into scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  for (long long w = rb + wv; w < re; w += (blockDim.x >> 6)) {
     NP np = D.soup[w];
     const int32_t* pi = D.pair_info + 4 * np.pair;
     const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
     const NodeRec na = D.nodes[pi[2] + np.na];
     const NodeRec nb = D.nodes[pi[3] + np.nb];
+    if (lane < 16) {
+      const bool side = lane >= 8;
+      const int k = lane & 7;
+      if (k < (side ? nb.ntris : na.ntris)) {
+        const double* bk = D.buckets + (size_t)BUCKET * (side ? pi[3] + nb.bucket : pi[2] + na.bucket);
+        swept_box(load_tri<double>(bk, k), side ? pr + PR_TB0 : pr + PR_TA0, side ? pr + PR_TB1 : pr + PR_TA1,
+                  side ? pr[PR_GEOB] : pr[PR_GEOA], s_box[wv][lane]);
+      }
+    }
+    wave_sync_lds();
     int i = lane >> 3, j = lane & 7;
     bool ok = i < na.ntris && j < nb.ntris;
     bool hit = false;
     if (ok) {
-      const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-      const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-      tri<double> ta = load_tri<double>(ba, i), tb = load_tri<double>(bb, j);
-      // bbox(start) expanded by bbox(end), then by geo_eps (continuous_comparison.h:804-812)
-      double lo[3], hi[3], lo2[3], hi2[3];
-      auto box = [](const tri<double>& t, const double* m, double* l, double* h) {
-        tri<double> w = xform(m, t);
-        l[0] = smin(smin(w.a.x, w.b.x), w.c.x), h[0] = smax(smax(w.a.x, w.b.x), w.c.x);
-        l[1] = smin(smin(w.a.y, w.b.y), w.c.y), h[1] = smax(smax(w.a.y, w.b.y), w.c.y);
-        l[2] = smin(smin(w.a.z, w.b.z), w.c.z), h[2] = smax(smax(w.a.z, w.b.z), w.c.z);
-      };
-      double ga = pr[PR_GEOA], gb = pr[PR_GEOB];
-      box(ta, pr + PR_TA0, lo, hi);
-      box(ta, pr + PR_TA1, lo2, hi2);
-      double al[3], ah[3], bl[3], bh[3];
-      for (int k = 0; k < 3; ++k) al[k] = smin(lo[k], lo2[k]) - ga, ah[k] = smax(hi[k], hi2[k]) + ga;
-      box(tb, pr + PR_TB0, lo, hi);
-      box(tb, pr + PR_TB1, lo2, hi2);
-      for (int k = 0; k < 3; ++k) bl[k] = smin(lo[k], lo2[k]) - gb, bh[k] = smax(hi[k], hi2[k]) + gb;
+      const double* A = s_box[wv][i];
+      const double* B = s_box[wv][8 + j];
       hit = true;
       for (int k = 0; k < 3; ++k) {  // bbox.h:8-23, a is "this"
-        bool o = (bl[k] < al[k]) ? (bh[k] > al[k]) : (bl[k] < ah[k]);
+        bool o = (B[k] < A[k]) ? (B[3 + k] > A[k]) : (B[k] < A[3 + k]);
         hit = hit && o;
       }
     }
+    wave_sync_lds();  // read before the next node pair overwrites
     const unsigned long long m = __ballot(hit);
     if (lane == 0) {
       D.masks[w] = m;
@@ -1380,7 +1396,8 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
       chunk_of(n, rb, re);
       long long ap = 0, ae = 0;  // lane 0 of each wavefront
       const int lane = lane_id();
-      for (long long w = rb + (threadIdx.x >> 6); w < re; w += (blockDim.x >> 6)) {  // one wavefront per node pair
+      const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // uniform: scalar loads
+      for (long long w = rb + wv; w < re; w += (blockDim.x >> 6)) {  // one wavefront per node pair
         const NP np = D.conn[w];
         LeafShare sh;
         const LeafInfo L = leaf_share(D, np, sh);
