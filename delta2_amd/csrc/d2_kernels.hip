@@ -1562,13 +1562,45 @@ __device__ __forceinline__ long long n_contacts(const unsigned long long* cnt, l
   long long n = (long long)cnt[C_CONTACTS];
   return n < cap ? n : cap;
 }
-__global__ void k_scatter(const unsigned long long* cnt, long long cap, const ContactRec* c, const int* pair_off,
-                          int* pair_fill, int* perm) {
+// Pair lists up to this size are scanned inside k_scatter.
+constexpr int SCAN_FUSE = 4096;
+// Contacts to their pair's segment of perm.  With pair_cnt (n_pairs <=
+// SCAN_FUSE) every block first scans the per-pair counts into LDS itself and
+// block 0 stores the offsets and their total, so no scan launch precedes it;
+// without, pair_off comes from a scan launch.
+__global__ void __launch_bounds__(256) k_scatter(const unsigned long long* cnt, long long cap, const ContactRec* c,
+                                                 int* pair_off, int* pair_fill, int* perm, const int* pair_cnt,
+                                                 int n_pairs, unsigned long long* total_dst) {
+  __shared__ int s_off[SCAN_FUSE + 1];
+  const int* off = pair_off;
+  if (pair_cnt) {
+    const int per = (n_pairs + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int b = (int)threadIdx.x * per, e = b + per < n_pairs ? b + per : n_pairs;
+    int acc = 0;
+    for (int i = b; i < e; ++i) acc += pair_cnt[i];
+    int total;
+    int run = block_scan(acc, total);
+    for (int i = b; i < e; ++i) {
+      const int v = pair_cnt[i];
+      s_off[i] = run;
+      if (blockIdx.x == 0) pair_off[i] = run;
+      run += v;
+    }
+    if (threadIdx.x == 0) {
+      s_off[n_pairs] = total;
+      if (blockIdx.x == 0) {
+        pair_off[n_pairs] = total;
+        if (total_dst) *total_dst = (unsigned long long)total;
+      }
+    }
+    __syncthreads();
+    off = s_off;
+  }
   const long long n = n_contacts(cnt, cap);
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
     int p = c[t].pair;
     int k = atomicAdd(&pair_fill[p], 1);
-    perm[pair_off[p] + k] = (int)t;
+    perm[off[p] + k] = (int)t;
   }
 }
 
@@ -2161,7 +2193,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
                                                            int mode, int* keep, int* kept_cnt,
                                                            const double* pair_rec, double* scale, int* over,
                                                            int* n_over, int* huge, int* n_huge, const int* tail,
-                                                           const int* n_tail) {
+                                                           const int* n_tail, unsigned long long* out_total) {
   constexpr int SEGS = blk_segs(CAP);
   __shared__ unsigned long long skey[CAP];  // emission keys, then the partition lists of the tie replay
   __shared__ int sidx[CAP];
@@ -2178,6 +2210,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
   __shared__ int s_misc[4];
   const int tid = threadIdx.x, w = tid >> 6, lane = lane_id();
   const int n_items = list ? *n_list : n_pairs;
+  unsigned long long kept_sum = 0;  // (thread 0) the filter's kept contacts of this block's pairs
   for (int it = blockIdx.x; it < n_items; it += gridDim.x) {
     const int p = list ? list[it] : it;
     const int b = pair_off[p], e = pair_off[p + 1], n = e - b;
@@ -2346,7 +2379,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
         nk = s_misc[1];
       }
       for (int t = tid; t < n; t += 256) perm[b + t] = eo[ord[t]];
-      if (tid == 0) kept_cnt[p] = nk;
+      if (tid == 0) kept_cnt[p] = nk, kept_sum += (unsigned long long)nk;
     } else {
       for (int i = tid; i < n; i += 256) perm[b + i] = eo[i];
     }
@@ -2355,8 +2388,10 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
   if constexpr (CAP == POST_CAP_LARGE) {  // (kept out of the first launch's code: registers, scratch)
     if (tail) {  // pairs above every LDS capacity (listed by the first launch)
       if (mode == 1) {  // the filter's std::sort replay: one lane per pair
-        for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x)
+        for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x) {
           pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
+          if (out_total) atomicAdd(out_total, (unsigned long long)kept_cnt[tail[t]]);
+        }
       } else {  // one block per pair
         for (int t = blockIdx.x; t < *n_tail; t += gridDim.x) {
           pair_finish_huge<CAP>(tail[t], c, pair_off, perm, mode, keep, pair_rec, scale, skey, s_min);
@@ -2365,6 +2400,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
       }
     }
   }
+  if (tid == 0 && out_total && kept_sum) atomicAdd(out_total, kept_sum);  // one atomic per block
 }
 
 // Gathers one pair per wavefront into the output arrays (CSR order).
@@ -2956,9 +2992,12 @@ int enqueue_post(d2_ctx* c, int mode) {
   const bool filtered = mode == 1;
   {
     Timer tm(c, K_POST, 0.0);  // per-pair counts come from the emitting kernels, zeroed by k_setup
-    scan_host_n(c, c->pair_cnt.p, c->pair_off.p, np, filtered ? nullptr : c->cnt.p + C_OUT);
+    unsigned long long* total = filtered ? nullptr : c->cnt.p + C_OUT;
+    const bool fuse = np <= SCAN_FUSE;  // the scan inside k_scatter
+    if (!fuse) scan_host_n(c, c->pair_cnt.p, c->pair_off.p, np, total);
     hipLaunchKernelGGL(k_scatter, dim3(sg), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts, c->contacts.p,
-                       c->pair_off.p, c->pair_fill.p, c->perm.p);
+                       c->pair_off.p, c->pair_fill.p, c->perm.p, fuse ? (const int*)c->pair_cnt.p : nullptr, np,
+                       fuse ? total : nullptr);
   }
   {
     Timer tm(c, K_POST_PAIRS, (double)np);
@@ -2966,19 +3005,18 @@ int enqueue_post(d2_ctx* c, int mode) {
     // pairs above the block kernel's capacity: the wavefront kernel's LDS pass
     // (list big), above that one lane per pair (list huge, filled by the block
     // kernel too, so the wavefront kernel never adds to it)
+    // the filter's kept total accumulates into C_OUT (zeroed by k_setup); the
+    // output offsets are scanned when the contacts are read (d2_result_contacts)
+    unsigned long long* kept_total = filtered ? c->cnt.p + C_OUT : nullptr;
     hipLaunchKernelGGL(k_pair_finish_block<BLK_CAP>, dim3(wblocks), dim3(256), 0, c->stream, np, (const int*)nullptr,
                        (const int*)nullptr, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
                        c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p, c->huge.p, c->n_big.p + 1,
-                       (const int*)nullptr, (const int*)nullptr);
+                       (const int*)nullptr, (const int*)nullptr, kept_total);
     hipLaunchKernelGGL(k_pair_finish_block<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(256), 0,
                        c->stream, np, (const int*)c->big.p, (const int*)c->n_big.p, c->contacts.p, c->pair_off.p,
                        c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p,
                        c->n_big.p + 1, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
-                       (const int*)(c->n_big.p + 1));
-  }
-  if (filtered) {
-    Timer tm(c, K_POST, 0.0);
-    scan_host_n(c, c->kept_cnt.p, c->out_off.p, np, c->cnt.p + C_OUT);
+                       (const int*)(c->n_big.p + 1), kept_total);
   }
   HIPCHK(c, hipGetLastError());
   return D2_OK;
@@ -3798,6 +3836,10 @@ int d2_result_contacts(d2_ctx* c, int32_t* offsets, int32_t* ints, double* reals
   if (!c) return D2_ERR_ARG;
   if (c->last_mode < 0) return fail(c, D2_ERR_STATE, "no result: run the path first");
   const int np = c->n_pairs;
+  if (c->out_filtered) {  // the filtered output's offsets (the run only counted its total)
+    scan_host_n(c, c->kept_cnt.p, c->out_off.p, np, nullptr);
+    HIPCHK(c, hipGetLastError());
+  }
   const int* outoff = c->out_filtered ? c->out_off.p : c->pair_off.p;
   if ((ints || reals) && c->n_out > 0) {  // the CSR total and the record count must agree
     int total = 0;
