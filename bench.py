@@ -34,8 +34,8 @@ METRIC = "narrow+CCD pair queries/s, 1 GPU; learned vs plain speedup at equal TO
 #                the pair's body/tree ids (16 B), the triangles (2 x 576 B)
 #                and eps (2 x 64 B) of both buckets; hit flag and expansion
 #                descriptor out (8 B); its kept items out (counted under tri_dist)
-#   tri_enum   : one CCD soup node pair: mask and count in (12 B), its kept items out
-#                (counted under tri_dist)
+#   tri_enum   : no launch of its own (the enumeration is part of tri_mask);
+#                its item count is the discrete node pairs received
 #   tri_dist   : one triangle pair: item (20 B: node pair, pair, buckets),
 #                two triangles (144 B), two eps (16 B)
 #   soup_cull  : one soup node pair: as tri_mask (swept boxes of both buckets)

@@ -2570,6 +2570,8 @@ namespace {
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
                               "leaf_point", "leaf_edge", "tri_mask", "tri_enum", "tri_dist", "post", "post_pairs",
                               "cluster", "gate"};
+// K_TRI_ENUM launches nothing (the enumeration is part of k_tri_mask); its
+// item count is the discrete node pairs received, kept for instrumentation.
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
            K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_NUM };
 
@@ -3002,9 +3004,10 @@ int enqueue_post(d2_ctx* c, int mode) {
   {
     Timer tm(c, K_POST_PAIRS, (double)np);
     const int wblocks = std::max(1, std::min(np, c->tune.post_blocks));
-    // pairs above the block kernel's capacity: the wavefront kernel's LDS pass
-    // (list big), above that one lane per pair (list huge, filled by the block
-    // kernel too, so the wavefront kernel never adds to it)
+    // pairs above the first launch's capacity (512 contacts): the second
+    // launch's 2048-contact LDS pass (list big); above that, one block per
+    // pair, or one lane per pair with the filter (list huge, filled by the
+    // first launch only)
     // the filter's kept total accumulates into C_OUT (zeroed by k_setup); the
     // output offsets are scanned when the contacts are read (d2_result_contacts)
     unsigned long long* kept_total = filtered ? c->cnt.p + C_OUT : nullptr;
