@@ -278,7 +278,8 @@ int d2_set_learned(d2_ctx* ctx, int on, float t_gate, float s_gate);
 
 /* Results of the last run: CSR offsets (n_pairs+1), contact records, and
  * per-pair values (CCD: scale, min_time_out; discrete: unused).
- * Any pointer may be NULL. */
+ * Any pointer may be NULL.  For a filtered run the call computes the
+ * filtered CSR offsets on the context's stream before copying. */
 int d2_result_contacts(d2_ctx* ctx, int32_t* offsets, int32_t* ints, double* reals);
 int d2_result_pairs(d2_ctx* ctx, double* per_pair);
 
