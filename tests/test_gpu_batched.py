@@ -4,7 +4,10 @@
 one d2_step_run.  Checked: the batched result restricted to a scene equals
 that scene run alone; cluster steps and contacts equal the oracle; the FP32
 cull with graph replay gives exactly the FP64 result; the autotuner leaves the
-shape it reports set, and the result does not change under it.
+shape it reports set, and the result does not change under it.  A larger
+batch (12 scenes of 256-512 bodies, 7.8 k pairs) takes the post-processing's
+separate scan launch; its scenes alone (below 4096 pairs) take the scan inside
+the scatter kernel, and both give the same contacts and steps.
 """
 import functools
 
@@ -134,3 +137,49 @@ def test_autotune_shape_and_result():
         assert np.array_equal(r0.ints, r1.ints) and np.array_equal(r0.reals, r1.reals)
     finally:
         ctx.close()
+
+
+@functools.lru_cache(maxsize=None)
+def large_batch():
+    return scenes.scene_c3(seed=1107, n_scenes=12, lo=256, hi=512, threads=4)
+
+
+def test_large_pair_list_equals_scenes_alone():
+    """More pairs than k_scatter scans in-kernel (4096): the batched step (scan
+    launch before the scatter; the filtered offsets scanned at result time)
+    equals its scenes run alone (each below 4096 pairs: the in-kernel scan),
+    in FP32 with graph replay and in FP64."""
+    s = large_batch()
+    print(f"[batched-large] bodies={s.n_bodies} pairs={s.pairs.shape[0]} scenes={len(s.counts)}", flush=True)
+    assert s.pairs.shape[0] > 4096
+    sob = s.scene_of_body[s.pairs[:, 0]]
+    for prec, graph in ((d2.D2_FP32, True), (d2.D2_FP64, False)):
+        print(f"[batched-large] precision={prec} graph={graph}", flush=True)
+        ctx = d2.Context(0, prec)
+        try:
+            cl, k = setup(ctx, s)
+            ctx.set_graph(graph)
+            for _ in range(2):  # FP32: capture, then replay
+                ctx.upload_bodies(s.tree_of_body, s.records)
+                n, steps = ctx.step_run(scenes.DT, True, want_steps=True)
+            batched = ctx.result(with_pairs=True)
+            assert n == batched.offsets[-1] and n > 0
+            ctx.set_graph(False)
+            for sc in range(2):  # two scenes alone (each below 4096 pairs)
+                idx = np.nonzero(sob == sc)[0]
+                assert 0 < idx.size <= 4096
+                used, lcl = np.unique(cl[idx], return_inverse=True)
+                ctx.upload_bodies(s.tree_of_body, s.records)
+                ctx.upload_pairs(s.pairs[idx])
+                ctx.upload_clusters(lcl.astype(np.int32), used.size)
+                _, st = ctx.step_run(scenes.DT, True, want_steps=True)
+                alone = ctx.result(with_pairs=True)
+                bi, br, bc = subset(batched, idx)
+                assert np.array_equal(np.diff(alone.offsets), bc), f"scene {sc}: counts"
+                ai = alone.ints.copy()
+                ai[:, 0] = idx[ai[:, 0]]
+                assert np.array_equal(ai, bi), f"scene {sc}: index tuples"
+                assert np.array_equal(alone.reals, br), f"scene {sc}: reals"
+                assert np.array_equal(st, steps[used]), f"scene {sc}: cluster steps"
+        finally:
+            ctx.close()
