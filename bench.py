@@ -14,7 +14,7 @@ The workload is the 4096-body scene (2048 subdivision-3 icospheres scaled
 U(0.5,1.5) and 2048 convex hulls of 64 ellipsoid points, Poisson-disk placed
 at 30% packing of the bounding spheres, velocities N(0,2), dt = 1/60 s).
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the path runs on one GPU.
 """
 import argparse
 import json
