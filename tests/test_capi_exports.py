@@ -1,5 +1,6 @@
 """The C-ABI library loads without a GPU and exports every symbol of include/d2hip.h."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -43,3 +44,22 @@ def test_tree_build_small_output_keeps_rng():
                                 F.shape[0], rng.ctypes.data_as(d2._u64p), 1, None, None, ctypes.byref(n))
     assert rc == d2.D2_ERR_CAPACITY and n.value > 1
     assert (rng == before).all()
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU fallback: without the built library the package refuses to run."""
+    saved_lib, saved_path = d2._lib, d2.LIB_PATH
+    try:
+        d2._lib = None
+        d2.LIB_PATH = str(tmp_path / "libd2hip.so")
+        with pytest.raises(ImportError):
+            d2.lib()
+    finally:
+        d2._lib, d2.LIB_PATH = saved_lib, saved_path
+
+
+@pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present: contexts can be created")
+def test_context_without_gpu_fails_loudly():
+    """Without a GPU a context cannot be created, so no compute call can run."""
+    with pytest.raises(d2.D2Error):
+        d2.Context(0, d2.D2_FP64)
