@@ -67,6 +67,10 @@ enum d2_precision {
 typedef struct d2_ctx d2_ctx;
 
 /* ---- context ----------------------------------------------------------- */
+/* No reference counterpart (the reference path works on host objects): a
+ * context owns one HIP stream at default priority, the resident inputs and
+ * the work lists of one device.  d2_ctx_create fails (D2_ERR_HIP) without a
+ * visible GPU; there is no CPU fallback. */
 int d2_ctx_create(int device, int precision, d2_ctx** out);
 int d2_ctx_destroy(d2_ctx* ctx);
 const char* d2_last_error(const d2_ctx* ctx);
@@ -84,7 +88,7 @@ int d2_set_auto_grow(d2_ctx* ctx, int on);
 /* Launch shapes (no reference counterpart: the reference runs on the CPU).
  * chunk_blocks: blocks of the chunked list kernels (1-4096; a list is split
  * into that many contiguous ranges); stream_blocks: blocks of the
- * grid-stride kernels (1-16384); post_blocks: one-wavefront blocks of the
+ * grid-stride kernels (1-16384); post_blocks: 256-thread blocks of the
  * per-pair post-processing (1-16384).  Defaults: 4, 8 and 16 per CU.
  * Results do not depend on them; d2_autotune picks them by timing. */
 typedef struct d2_tune {
@@ -276,8 +280,10 @@ int d2_pair_features(d2_ctx* ctx, float* X);
  * phases it applies to (1: CCD, 2: discrete, 3: both). */
 int d2_set_learned(d2_ctx* ctx, int on, float t_gate, float s_gate);
 
-/* Results of the last run: CSR offsets (n_pairs+1), contact records, and
- * per-pair values (CCD: scale, min_time_out; discrete: unused).
+/* Results of the last run: CSR offsets (n_pairs+1), contact records (the
+ * Contact<double> vectors the reference fills, collision_detection/contact.h:
+ * 10-28: A, B, eps_a, eps_b, plus pair, node and sub-test ids), and per-pair
+ * values (CCD: scale, min_time_out; discrete: unused).
  * Any pointer may be NULL.  For a filtered run the call computes the
  * filtered CSR offsets on the context's stream before copying. */
 int d2_result_contacts(d2_ctx* ctx, int32_t* offsets, int32_t* ints, double* reals);

@@ -61,5 +61,6 @@ def test_missing_library_fails_loudly(tmp_path):
 @pytest.mark.skipif(os.path.exists("/dev/kfd"), reason="a GPU driver is present: contexts can be created")
 def test_context_without_gpu_fails_loudly():
     """Without a GPU a context cannot be created, so no compute call can run."""
-    with pytest.raises(d2.D2Error):
+    with pytest.raises(d2.D2Error) as e:
         d2.Context(0, d2.D2_FP64)
+    assert e.value.code == d2.D2_ERR_HIP
