@@ -129,7 +129,7 @@ D2_HD v3<double> mul33v(const double* a, v3<double> v) {
   return {(a[0] * v.x + a[1] * v.y) + a[2] * v.z, (a[3] * v.x + a[4] * v.y) + a[5] * v.z,
           (a[6] * v.x + a[7] * v.y) + a[8] * v.z};
 }
-// World inverse inertia R * I^-1 * R^T  (state.cpp:132-133, particle.cpp:73).
+// World inverse inertia R * I^-1 * R^T  (state.cpp:95-97, particle.cpp:73-75).
 D2_HD void world_inv_inertia(quat q, const double* ib, double* out) {
   double R[9], Rt[9], t[9];
   qrot(q, R);
@@ -138,7 +138,7 @@ D2_HD void world_inv_inertia(quat q, const double* ib, double* out) {
   mul33(R, ib, t);
   mul33(t, Rt, out);
 }
-// Pose matrix (basic_utils.h:207-213).
+// Pose matrix (basic_utils.h:79-86).
 D2_HD void pose(quat q, v3<double> t, double* m) {
   double r[9];
   qrot(q, r);
@@ -147,7 +147,7 @@ D2_HD void pose(quat q, v3<double> t, double* m) {
   m[8] = r[6], m[9] = r[7], m[10] = r[8], m[11] = t.z;
   m[12] = 0.0, m[13] = 0.0, m[14] = 0.0, m[15] = 1.0;
 }
-// State::extrapolate (state.cpp:127-139): translation t*v, rotation times
+// State::extrapolate (state.cpp:90-102): translation t*v, rotation times
 // exp(0.5*omega*t) on the right; basic_utils.cpp:36-44 for exp.
 struct body_state {
   quat q;

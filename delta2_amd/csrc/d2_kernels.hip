@@ -2636,6 +2636,9 @@ struct d2_ctx {
   DBuf<int32_t> pairs, pair_info, cluster_of_pair;
   DBuf<double> max_time, pair_rec;
   int n_pairs = 0, n_clusters = 0;
+  // a pair list describes the uploaded bodies: a tree upload or a body upload
+  // of another count invalidates it (and its clusters) until pairs come again
+  bool pairs_valid = false;
   // work
   DBuf<NP> soup_list[2], conn_list[2];  // level l reads [l & 1] and writes [(l + 1) & 1]
   DBuf<Item> items, pt_items, ee_items;
@@ -2740,6 +2743,18 @@ void drop_graphs(d2_ctx* c) {
       if (g.exec) (void)hipGraphExecDestroy(g.exec);
       g = Captured{};
     }
+}
+
+// The pair list, its clusters and every result refer to the uploaded bodies:
+// forget them when the bodies they index change (runs then fail with
+// D2_ERR_STATE until a new pair list is uploaded or built).
+void invalidate_pairs(d2_ctx* c) {
+  c->pairs_valid = false;
+  c->n_pairs = 0;
+  c->n_clusters = 0;
+  c->clusters_on_device = false;
+  c->last_mode = -1;
+  c->ccd_valid = false;
 }
 
 // (Re)allocation invalidates captured graphs: they hold buffer addresses.
@@ -3386,7 +3401,7 @@ int run(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
 }
 
 int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
-  if (c->n_pairs < 0 || c->n_bodies <= 0 || c->n_trees <= 0)
+  if (!c->pairs_valid || c->n_bodies <= 0 || c->n_trees <= 0)
     return fail(c, D2_ERR_STATE, "upload trees, bodies and pairs first");
   if (c->pair_depth + 1 > MAX_LEVELS)
     return fail(c, D2_ERR_ARG, "a pair's trees are too deep: depth(a) + depth(b) must stay below 16");
@@ -3609,20 +3624,23 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   HIPCHK(c, hipMemcpyAsync(c->buckets.p, buckets, sizeof(double) * BUCKET * (size_t)total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->tree_off.p, tree_node_offset, sizeof(int32_t) * (n_trees + 1), hipMemcpyHostToDevice, c->stream));
   // Per tree, for the learned gate's certificate: the distinct vertices of the
-  // leaf triangles (body coordinates), the bounding radius, and the largest
-  // distance a float-barycentric "inside" test can accept a point outside a
-  // triangle: 64 eps_f kappa L (kappa = d00 d11 / (d00 d11 - d01^2), L the
-  // longest edge; 4x the 16 eps_f kappa L rounding bound).
+  // leaf triangles (body coordinates), the bounding radius, and how far beyond
+  // the search distance a leaf contact can be accepted: the largest distance a
+  // float-barycentric "inside" test can accept a point outside a triangle,
+  // 64 eps_f kappa L (kappa = d00 d11 / (d00 d11 - d01^2), L the longest edge;
+  // 4x the 16 eps_f kappa L rounding bound), plus the largest eps of the leaf
+  // buckets (the discrete hit test adds them, comparison_dist.h:233).
   std::vector<float> hv, radius(n_trees), eps_out(n_trees);
   std::vector<int32_t> hv_off(n_trees + 1, 0);
   for (int t = 0; t < n_trees; ++t) {
     std::vector<std::array<double, 3>> vs;
-    double rmax = 0.0, eo = 0.0;
+    double rmax = 0.0, eo = 0.0, leaf_eps = 0.0;
     for (int i = tree_node_offset[t]; i < tree_node_offset[t + 1]; ++i) {
       const int32_t* r = nodes + 8 * (size_t)i;
       if (r[2]) continue;
       const double* bk = buckets + (size_t)BUCKET * (tree_node_offset[t] + r[1]);
       for (int k = 0; k < r[7]; ++k) {
+        leaf_eps = std::max(leaf_eps, bk[72 + k]);
         std::array<double, 3> v[3];
         for (int q = 0; q < 3; ++q) v[q] = {bk[(3 * q) * 8 + k], bk[(3 * q + 1) * 8 + k], bk[(3 * q + 2) * 8 + k]};
         double e0[3], e1[3], e2[3];
@@ -3645,9 +3663,10 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
     for (auto& v : vs) hv.push_back((float)v[0]), hv.push_back((float)v[1]), hv.push_back((float)v[2]);
     hv_off[t + 1] = hv_off[t] + (int32_t)vs.size();
     radius[t] = (float)rmax * (1.0f + 1e-6f);
-    eps_out[t] = (float)std::min(eo, 1e30);
+    // rounded up so the float margin is never below the double one
+    eps_out[t] = (float)std::min((eo + leaf_eps) * (1.0 + 1e-6), 1e30);
   }
-  // Per tree for the broad phase, as MeshData::initBoudingData (model/mesh.cpp:43-50)
+  // Per tree for the broad phase, as MeshData::initBoudingData (model/mesh.cpp:44-51)
   // computes it over the mesh vertices (here: the leaf triangles' vertices,
   // which are every vertex of a mesh whose vertices all belong to a face):
   // the float AABB and the largest vertex norm.
@@ -3683,7 +3702,11 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   c->n_trees = n_trees;
   c->tree_depth = depth;
   c->pair_depth = std::min(MAX_LEVELS - 1, 2 * *std::max_element(depth.begin(), depth.end()));  // until pairs come
+  // body records name trees by index: bodies, pairs and clusters must come again
   c->host_body_tree.clear();
+  c->n_bodies = 0;
+  c->has_groups = false;
+  invalidate_pairs(c);
   return D2_OK;
 }
 
@@ -3724,8 +3747,16 @@ int d2_bodies_upload(d2_ctx* c, int n_bodies, const int32_t* tree_of_body, const
     HIPCHK(c, hipMemcpyAsync(c->bp_statics.p, c->host_static.data(), sizeof(int) * c->host_static.size(),
                              hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (n_bodies != c->n_bodies) {
+    // pairs, clusters and scene groups index bodies: a new count voids them
+    invalidate_pairs(c);
+    c->has_groups = false;
+  } else if (c->pairs_valid && !std::equal(tree_of_body, tree_of_body + n_bodies, c->host_body_tree.begin())) {
+    // same bodies, other trees: the kept pairs' level budget becomes the
+    // deepest any pair of these trees can need
+    c->pair_depth = std::min(MAX_LEVELS - 1, 2 * *std::max_element(c->tree_depth.begin(), c->tree_depth.end()));
+  }
   c->n_bodies = n_bodies;
-  c->has_groups = false;
   c->host_body_tree.assign(tree_of_body, tree_of_body + n_bodies);
   return D2_OK;
 }
@@ -3782,11 +3813,13 @@ int d2_pairs_upload(d2_ctx* c, int n_pairs, const int32_t* pairs, const double* 
   c->has_max_time = max_time != nullptr;
   c->last_mode = -1;
   c->ccd_valid = false;
+  c->pairs_valid = true;
   return D2_OK;
 }
 
 int d2_clusters_upload(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters) {
   if (!c) return D2_ERR_ARG;
+  if (!c->pairs_valid) return fail(c, D2_ERR_STATE, "d2_clusters_upload: upload or build the pair list first");
   if (n_clusters <= 0 || (c->n_pairs > 0 && !cluster_of_pair)) return fail(c, D2_ERR_ARG, "d2_clusters_upload: bad arguments");
   for (int p = 0; p < c->n_pairs; ++p)
     if (cluster_of_pair[p] < 0 || cluster_of_pair[p] >= n_clusters)
@@ -3946,11 +3979,7 @@ int d2_broad_phase(d2_ctx* c, int mode, double t, int64_t* n_pairs_out) {
   if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_broad_phase: upload bodies first");
   (void)hipSetDevice(c->device);
   // the resident pair list is replaced (and invalid until this call succeeds)
-  c->n_pairs = 0;
-  c->n_clusters = 0;
-  c->clusters_on_device = false;
-  c->last_mode = -1;
-  c->ccd_valid = false;
+  invalidate_pairs(c);
   const int n = c->n_bodies;
   long long T = 64;
   while (T < 2ll * n) T <<= 1;
@@ -4005,11 +4034,8 @@ int d2_broad_phase(d2_ctx* c, int mode, double t, int64_t* n_pairs_out) {
     RC(ensure(c, c->pair_rec, (size_t)std::max(np, 1) * PR_STRIDE)); RC(ensure(c, c->pair_info, 4 * (size_t)std::max(np, 1)));
     c->n_pairs = np;
     c->pair_depth = c->host_red[BR_DEPTH];
-    c->n_clusters = 0;
-    c->clusters_on_device = false;
     c->has_max_time = true;
-    c->last_mode = -1;
-    c->ccd_valid = false;
+    c->pairs_valid = true;
     if (n_pairs_out) *n_pairs_out = np;
     return D2_OK;
   }
@@ -4029,6 +4055,7 @@ int d2_pairs_download(d2_ctx* c, int32_t* pairs, double* max_time) {
 int d2_clusters_build(d2_ctx* c, int* n_clusters_out) {
   if (!c) return D2_ERR_ARG;
   if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "d2_clusters_build: upload bodies first");
+  if (!c->pairs_valid) return fail(c, D2_ERR_STATE, "d2_clusters_build: upload or build the pair list first");
   (void)hipSetDevice(c->device);
   const int n = c->n_bodies, np = c->n_pairs;
   c->n_clusters = 0;  // invalid until this call succeeds

@@ -14,7 +14,8 @@ Conventions (stated once, used everywhere):
     time step dt = float(1/60) (Options::time_step_size is a float);
   * velocities "N(0, s)" are normal with standard deviation s per axis;
   * trees and bounding radii are built from the un-centred vertices, the
-    inertia about the centre of mass (model/mesh.cpp:63-73,87-94).
+    inertia about the centre of mass (model/mesh.cpp:20-30 builds the tree
+    at :21; initInertiaDataVolume re-centres the vertices at :75).
 """
 import math
 
@@ -80,7 +81,7 @@ def ellipsoid_hull(rng, n_points=64, lo=0.3, hi=1.2):
 def mass_properties(V, F):
     """Volume, centre of mass and unit-density inertia about it (divergence theorem
     over the tetrahedra (0, a, b, c)); replaces the reference's tetgen route
-    (model/mesh.cpp:96-134), which is absent here."""
+    (model/mesh.cpp:53-91), which is absent here."""
     a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
     det = np.einsum("ij,ij->i", a, np.cross(b, c))
     vol = det.sum() / 6.0

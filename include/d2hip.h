@@ -129,17 +129,24 @@ int d2_pcg_seed(uint64_t initstate, uint64_t initseq, uint64_t* rng_out);
 
 /* Uploads n_trees concatenated trees; tree t owns node records
  * [tree_node_offset[t], tree_node_offset[t+1]).  Replaces MeshData storage
- * read through SurrogateTree::getNode/getBucket (model/surrogate_tree.cpp:20-27). */
+ * read through SurrogateTree::getNode/getBucket (model/surrogate_tree.cpp:20-27).
+ * Body records name trees by index, so an upload forgets the bodies, scene
+ * groups, pair list and clusters: runs fail with D2_ERR_STATE until they are
+ * uploaded again. */
 int d2_trees_upload(d2_ctx* ctx, int n_trees, const int32_t* tree_node_offset, const int32_t* nodes,
                     const double* buckets);
 
 /* ---- bodies ------------------------------------------------------------- */
 /* Uploads n body records; tree_of_body[i] selects the tree of body i.
- * Replaces the Particle fields the path reads (model/particle.h:9-64). */
+ * Replaces the Particle fields the path reads (model/particle.h:9-64).
+ * Last states are reset to the current states (d2_bodies_last_upload sets
+ * them).  With the same body count as before, the pair list, its clusters and
+ * the scene groups stay; with another count they are forgotten (runs fail
+ * with D2_ERR_STATE until a pair list is uploaded or built again). */
 int d2_bodies_upload(d2_ctx* ctx, int n_bodies, const int32_t* tree_of_body, const double* records);
 int d2_bodies_download(d2_ctx* ctx, double* records);
 /* future = current.extrapolate(step[i]) for every non-null body
- * (Particle::projectFutureState, model/particle.cpp:64-66; state.cpp:127-139). */
+ * (Particle::projectFutureState, model/particle.cpp:64-66; state.cpp:90-102). */
 int d2_project_future(d2_ctx* ctx, const double* step_per_body);
 
 /* ---- pair lists --------------------------------------------------------- */
@@ -201,7 +208,7 @@ int d2_step_run(d2_ctx* ctx, float dt, int filter, double* cluster_step_out, int
 
 /* ---- broad phase and clusters -------------------------------------------- */
 /* Per-tree mesh bounds the broad phase reads: the float AABB (lo xyz, hi xyz)
- * and the bounding radius of MeshData::initBoudingData (model/mesh.cpp:43-50,
+ * and the bounding radius of MeshData::initBoudingData (model/mesh.cpp:44-51,
  * getAABB :132-153, getSphereBounds :155-158, getBoundingRadius :160-162).
  * d2_trees_upload computes them from the leaf triangles (every vertex of a
  * mesh whose vertices all belong to a face); the upload overrides them. */

@@ -58,6 +58,7 @@ def lib():
             f.restype = ctypes.c_double
         L.d2o_extrapolate.argtypes = [_dp, _dp, ctypes.c_double, _dp]
         L.d2o_state_transformation.argtypes = [_dp, _dp]
+        L.d2o_future_point_velocity.argtypes = [_dp, _dp, _dp]
         L.d2o_pcg_draws.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int, _u32p]
         L.d2o_pcg_seed.argtypes = [ctypes.c_uint64, ctypes.c_uint64, _u64p]
         L.d2o_tree_build.argtypes = [_dp, ctypes.c_int, _ip, ctypes.c_int, _u64p]
@@ -77,6 +78,8 @@ def lib():
         L.d2o_ccd.argtypes = [_vp, ctypes.c_int, _ip, _dp]
         L.d2o_ccd.restype = ctypes.c_int
         L.d2o_scene_result.argtypes = [_vp, _ip, _ip, _dp, _dp]
+        L.d2o_exhaustive_leaves.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.d2o_exhaustive_leaves.restype = ctypes.c_int
         L.d2o_cluster_step.argtypes = [_vp, ctypes.c_int, _ip, ctypes.c_float]
         L.d2o_cluster_step.restype = ctypes.c_double
         L.d2o_time_pass.argtypes = [_vp, ctypes.c_int, _ip, _dp, ctypes.c_int, ctypes.c_int, _dp]
@@ -184,6 +187,13 @@ def state_transformation(state14):
     return o.reshape(4, 4)
 
 
+def future_point_velocity(body40, pt):
+    """Particle::futurePointVelocity of a body record at world point pt."""
+    o = np.zeros(3)
+    lib().d2o_future_point_velocity(_p(_d(body40, 40)), _p(_d(pt, 3)), _p(o))
+    return o
+
+
 def pcg_draws(s0, s1, bound, n):
     o = np.zeros(n, dtype=np.uint32)
     lib().d2o_pcg_draws(s0, s1, bound, n, o.ctypes.data_as(_u32p))
@@ -259,6 +269,13 @@ class Scene:
         n = lib().d2o_ccd(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), _p(mt))
         off, ints, reals, pp = self._result(pairs.shape[0], n, 2 * pairs.shape[0])
         return off, ints, reals, pp.reshape(-1, 2)
+
+    def exhaustive_leaves(self, a, b, continuous=False, max_time=0.0):
+        """Every leaf pair of bodies a, b through the discrete (or, with
+        continuous, the connected CCD) bucket comparison: (ints, reals)."""
+        n = lib().d2o_exhaustive_leaves(self.h, int(a), int(b), 1 if continuous else 0, float(max_time))
+        _, ints, reals, _ = self._result(1, n, 2)
+        return ints, reals
 
     def filter_last(self, continuous):
         """filterContacts (discrete) / filterContinuousContacts on the last result."""

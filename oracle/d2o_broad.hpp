@@ -64,7 +64,7 @@ inline bool overlap(const Box3f& a, const Box3f& b) {
   return true;
 }
 
-// MeshData::initBoudingData (model/mesh.cpp:43-50): float AABB of the
+// MeshData::initBoudingData (model/mesh.cpp:44-51): float AABB of the
 // vertices and the largest vertex norm (double).
 struct MeshBounds {
   Box3f aabb;

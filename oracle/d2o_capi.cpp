@@ -238,6 +238,14 @@ extern "C" void d2o_state_transformation(const double* state14, double* out16) {
   M m = stateFrom(state14).transformation();
   for (int i = 0; i < 16; ++i) out16[i] = m.m[i];
 }
+// Particle::futurePointVelocity (model/particle.cpp:72-78) of a body record
+// (its future state and body-frame inverse inertia) at world point pt.
+extern "C" void d2o_future_point_velocity(const double* body40, const double* pt, double* out3) {
+  Body b;
+  b.future = stateFrom(body40 + 14);
+  for (int k = 0; k < 9; ++k) b.inv_inertia.m[k] = body40[28 + k];
+  vTo(b.futurePointVelocity(vFrom<double>(pt)), out3);
+}
 
 // PCG32 (common/pcg_basic.cpp) for seeding checks: writes n bounded draws.
 extern "C" void d2o_pcg_draws(uint64_t s0, uint64_t s1, uint32_t bound, int n, uint32_t* out) {
@@ -399,6 +407,39 @@ extern "C" int d2o_ccd(void* h, int n_pairs, const int32_t* pairs, const double*
     exportContacts(*sc, p, cs);
     sc->out_off.push_back((int32_t)(sc->out_i.size() / 8));
   }
+  return (int)(sc->out_i.size() / 8);
+}
+
+// Every leaf of body a against every leaf of body b, without the traversal:
+// mode 0 runs the discrete bucket comparison (comparison_dist.h:205-240) at the
+// future states; mode 1 projects both future states at max_time as
+// compareTreesFullContinuous does (full_tree_comparison.h:78-79) and runs the
+// connected CCD bucket comparison (continuous_comparison.h:830-910).  Leaf
+// pairs in (leaf of a, leaf of b) node order.  The result replaces the
+// scene's last result (one pair); returns the number of hits.
+extern "C" int d2o_exhaustive_leaves(void* h, int ia, int ib, int mode, double max_time) {
+  Scene* sc = static_cast<Scene*>(h);
+  Body& a = sc->bodies[ia];
+  Body& b = sc->bodies[ib];
+  if (mode == 1) {
+    a.projectFutureState(max_time);
+    b.projectFutureState(max_time);
+  }
+  std::vector<Contact> hits;
+  for (const Node& na : a.tree->nodes) {
+    if (na.is_inner) continue;
+    for (const Node& nb : b.tree->nodes) {
+      if (nb.is_inner) continue;
+      if (mode == 0) bucketCompare(a, b, na, nb, hits);
+      else bucketCompareContinuousConnected(a, b, na, nb, hits);
+    }
+  }
+  sc->out_i.clear();
+  sc->out_r.clear();
+  sc->out_off.assign(1, 0);
+  sc->out_pair.assign(2, 0.0);
+  exportContacts(*sc, 0, hits);
+  sc->out_off.push_back((int32_t)(sc->out_i.size() / 8));
   return (int)(sc->out_i.size() / 8);
 }
 
@@ -591,7 +632,7 @@ std::vector<BpBody> bpBodies(int n, const double* records, const double* last14,
 }
 }  // namespace
 
-// MeshData::initBoudingData (model/mesh.cpp:43-50) of one vertex list.
+// MeshData::initBoudingData (model/mesh.cpp:44-51) of one vertex list.
 extern "C" void d2o_mesh_bounds(const double* Vp, int nv, float* aabb6, double* radius) {
   std::vector<V> vs(nv);
   for (int i = 0; i < nv; ++i) vs[i] = V(Vp[3 * i], Vp[3 * i + 1], Vp[3 * i + 2]);

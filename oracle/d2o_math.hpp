@@ -11,7 +11,7 @@
 //   * M*v and M*M sum their four products left to right
 //   * the 4x4 inverse is the cofactor (Laplace expansion) inverse
 //   * quaternion product is the Hamilton product, toRotationMatrix is the
-//     standard unit-quaternion formula (common/basic_utils.h:207-213 uses it)
+//     standard unit-quaternion formula (common/basic_utils.h:79-86 uses it)
 // Eigen's own operation order is not pinned by anything available here; the
 // product's GPU code uses these same orders so the two agree bit for bit.
 #pragma once
@@ -190,7 +190,7 @@ template <class R> inline M3<R> toRotation(const Quat<R>& q) {
   r(2, 2) = R(1) - (txx + tyy);
   return r;
 }
-// common/basic_utils.h:207-213
+// common/basic_utils.h:79-86
 template <class R> inline M4<R> transformationMatrix(const Quat<R>& q, const V3<R>& t) {
   M3<R> r = toRotation(q);
   M4<R> o = M4<R>::identity();
@@ -201,7 +201,7 @@ template <class R> inline M4<R> transformationMatrix(const Quat<R>& q, const V3<
   o(2, 3) = t.z;
   return o;
 }
-// common/basic_utils.h:198-204: AngleAxis(x,X) * AngleAxis(y,Y) * AngleAxis(z,Z)
+// common/basic_utils.h:70-77: AngleAxis(x,X) * AngleAxis(y,Y) * AngleAxis(z,Z)
 template <class R> inline Quat<R> angleAxis(R ang, int axis) {
   R h = ang / R(2);
   Quat<R> q{std::cos(h), R(0), R(0), R(0)};

@@ -228,8 +228,8 @@ struct State {
   double time = 0;
   Quat<double> rot = Quat<double>::identity();
   V L, trans, vel;
-  M transformation() const { return transformationMatrix(rot, trans); }  // state.cpp:123-125
-  // state.cpp:127-139 (rotation delta multiplied on the right)
+  M transformation() const { return transformationMatrix(rot, trans); }  // state.cpp:86-88
+  // state.cpp:90-102 (rotation delta multiplied on the right)
   State extrapolate(double t, const M3<double>& inv_inertia) const {
     State r;
     r.time = time + t;

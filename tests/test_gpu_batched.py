@@ -28,8 +28,9 @@ def small_batch():
 
 
 def close(a, b, tol=TOL):
+    """Relative bound |a - b| <= tol |b| (exact where b is zero)."""
     a, b = np.asarray(a), np.asarray(b)
-    return np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0)
+    return np.abs(a - b) <= tol * np.abs(b)
 
 
 def subset(res, idx):

@@ -1,8 +1,14 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
 FP64 is required to reproduce the oracle: identical contact index tuples in
-identical order, reals within 1e-9 relative (|x - y| <= 1e-9 * max(|y|, 1)),
-identical per-pair step scales and min_time_out to the same tolerance.
+identical order, and reals
+  * bit for bit where the device works on the same inputs as the oracle (the
+    discrete phase over uploaded future states): the kernels evaluate the
+    reference's arithmetic in the oracle's order without contraction;
+  * within 1e-9 relative (|x - y| <= 1e-9 |y|) where the device first computes
+    poses itself (the CCD projects the future states with the device's sin
+    and cos, which may differ from glibc's in the last bit).
+Per-pair step scales and min_time_out are held to the same relative bound.
 """
 import numpy as np
 import pytest
@@ -18,14 +24,22 @@ TOL = 1e-9
 
 def close(a, b):
     a, b = np.asarray(a), np.asarray(b)
-    return np.abs(a - b) <= TOL * np.maximum(np.abs(b), 1.0)
+    return np.abs(a - b) <= TOL * np.abs(b)
 
 
-def assert_same(gpu, ora_off, ora_ints, ora_reals, label):
+def assert_same(gpu, ora_off, ora_ints, ora_reals, label, exact=True):
     assert np.array_equal(gpu.offsets, ora_off), f"{label}: per-pair contact counts differ"
     assert np.array_equal(gpu.ints, ora_ints), f"{label}: contact index tuples differ"
+    if exact:
+        bad = gpu.reals != ora_reals
+        assert not bad.any(), f"{label}: {np.count_nonzero(bad)} reals differ in their bits " \
+                              f"(max abs diff {np.abs(gpu.reals - ora_reals).max()})"
+        return
     ok = close(gpu.reals, ora_reals)
-    assert ok.all(), f"{label}: {np.count_nonzero(~ok)} reals beyond 1e-9 (max abs diff {np.abs(gpu.reals - ora_reals).max()})"
+    same = np.count_nonzero(gpu.reals == ora_reals)
+    print(f"[{label}] {same} of {ora_reals.size} reals bit-identical", flush=True)
+    assert ok.all(), f"{label}: {np.count_nonzero(~ok)} reals beyond 1e-9 relative " \
+                     f"(max abs diff {np.abs(gpu.reals - ora_reals).max()})"
 
 
 def _fingerprint():
@@ -132,7 +146,7 @@ def test_pairs_with_thousands_of_contacts(prec):
         gpu = d2.compare_trees_full_continuous(c, s.pairs, mt)
         o_off, o_ints, o_reals, o_pp = oracle_scene(s, s.records).ccd(s.pairs, mt)
         assert np.all(np.diff(o_off) > 2048)
-        assert_same(gpu, o_off, o_ints, o_reals, f"stacked ccd {prec}")
+        assert_same(gpu, o_off, o_ints, o_reals, f"stacked ccd {prec}", exact=False)
         assert close(gpu.per_pair, o_pp).all()
     finally:
         c.close()
@@ -147,7 +161,7 @@ def test_ccd_parity(ctx, name, make):
     mt = np.minimum(rec[s.pairs[:, 0], 39], rec[s.pairs[:, 1], 39]) if s.pairs.size else np.zeros(0)
     gpu = d2.compare_trees_full_continuous(ctx, s.pairs, mt)
     o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
-    assert_same(gpu, o_off, o_ints, o_reals, f"ccd {name}")
+    assert_same(gpu, o_off, o_ints, o_reals, f"ccd {name}", exact=False)
     ok = close(gpu.per_pair, o_pp)
     assert ok.all(), f"ccd {name}: per-pair scale/min_time differ at {np.argwhere(~ok)[:5]}"
 
@@ -160,7 +174,7 @@ def test_ccd_long_interval(ctx):
     mt = np.full(s.pairs.shape[0], 4 * scenes.DT)
     gpu = d2.compare_trees_full_continuous(ctx, s.pairs, mt)
     o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
-    assert_same(gpu, o_off, o_ints, o_reals, "ccd dense 4dt")
+    assert_same(gpu, o_off, o_ints, o_reals, "ccd dense 4dt", exact=False)
     assert close(gpu.per_pair, o_pp).all()
     assert (o_ints[:, 3] == 3).any() and (o_ints[:, 3] == 1).any()  # edge and vertex contacts present
 
@@ -316,7 +330,7 @@ def test_ccd_parity_fp32_culls(ctx32, name, make):
     gpu = d2.compare_trees_full_continuous(ctx32, s.pairs, mt)
     ctx32.set_profiling(False)
     o_off, o_ints, o_reals, o_pp = oracle_scene(s, rec).ccd(s.pairs, mt)
-    assert_same(gpu, o_off, o_ints, o_reals, f"ccd fp32 {name}")
+    assert_same(gpu, o_off, o_ints, o_reals, f"ccd fp32 {name}", exact=False)
     assert close(gpu.per_pair, o_pp).all()
     st = ctx32.kernel_stats()
     leaf_pairs = int(sum(lv[:, 1].sum() for lv in [ctx32.run_info()[1][1]]))

@@ -22,8 +22,9 @@ TOL = 1e-9
 
 
 def close(a, b, tol=TOL):
+    """Relative bound |a - b| <= tol |b| (exact where b is zero)."""
     a, b = np.asarray(a), np.asarray(b)
-    return np.abs(a - b) <= tol * np.maximum(np.abs(b), 1.0)
+    return np.abs(a - b) <= tol * np.abs(b)
 
 
 def same_result(a, b, label):

@@ -6,8 +6,8 @@ where the reference runs both (TEMPLATE_TEST_CASE ... float, double).
 
 Euler-angle fixtures: the reference's expected values for poses built with
 ``transformationMatrix(euler, pos)`` were produced in the R = Rz*Ry*Rx
-convention (test_utils.cpp:245-256 expects Ry*Rx for euler (x, y, 0)), while
-the code (basic_utils.h:198-204, Eigen AngleAxis X*Y*Z) yields Rx*Ry*Rz.  The
+convention (test_utils.cpp:68-79 expects Ry*Rx for euler (x, y, 0)), while
+the code (basic_utils.h:70-77, Eigen AngleAxis X*Y*Z) yields Rx*Ry*Rz.  The
 oracle follows the code; fixtures are rebuilt here in the convention their
 expected values were produced with (``fixture_pose``), and a separate test
 pins the code's convention against its closed form.
@@ -49,13 +49,13 @@ def transformed(tri, M, prec):
 
 
 def deg(x):
-    return 2.0 * 3.14159265358979323846 * x / 360.0  # basic_utils.h:228-232
+    return 2.0 * 3.14159265358979323846 * x / 360.0  # basic_utils.h:101-105
 
 
 # --- common/test_utils.cpp -------------------------------------------------
 @pytest.mark.parametrize("prec", PRECS)
 def test_transformation_matrix_rotation_z(prec):
-    # test_utils.cpp:207-224 (single-axis: convention independent)
+    # test_utils.cpp:30-47 (single-axis: convention independent)
     m = d2o.transformation_matrix_euler([0, 0, 3.14159 / 2.0], [0, 0, 0], prec)
     want = np.array([[0, -1, 0, 0], [1, 0, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1.0]])
     assert np.all(np.abs(m - want) <= 1e-3)
@@ -71,7 +71,7 @@ def test_transformation_matrix_code_convention(prec):
     tol = 1e-12 if prec == "f64" else 1e-6
     assert np.allclose(m[:3, :3], want, atol=tol)
     assert np.allclose(m[:3, 3], [0, 1, 1.2651], atol=tol)
-    # test_utils.cpp:245-256 expected matrix is the fixture convention Ry*Rx:
+    # test_utils.cpp:68-79 expected matrix is the fixture convention Ry*Rx:
     ref_expected = np.array([[0.9396926164627075, 0.1710100769996643, 0.29619812965393066],
                              [0.0, 0.8660253882408142, -0.5],
                              [-0.3420201241970062, 0.46984633803367615, 0.813797652721405]])
