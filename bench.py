@@ -1,18 +1,28 @@
 """Benchmark: narrow+CCD pair queries/s on one MI355X.
 
-A step is one pass of the contact-detection path over the workload's
-broad-phase pair list with inputs resident in HBM (d2_step_run):
+A step is one pass of the contact-detection path over the step's broad-phase
+pair list with its inputs resident in HBM (d2_step_run):
   CCD (compareTreesFullContinuous over [0, min(last steps)]) + the TOI -> step
   rule, the clamped cluster steps, the future states at those steps, then the
   discrete narrow phase (compareTreesFull) + filterContacts,
-for every pair.  value = pairs * steps / wall time of the timed steps.  The
-timed steps replay one captured HIP graph per step (--graph off: eager
-launches); a separate eager pass with HIP events brackets every kernel for the
-roofline figures.
+for every pair.  value = pairs summed over the timed steps / the summed wall
+time of their d2_step_run calls (each bracketed by stream synchronisation).
 
-The workload is the 4096-body scene (2048 subdivision-3 icospheres scaled
-U(0.5,1.5) and 2048 convex hulls of 64 ellipsoid points, Poisson-disk placed
-at 30% packing of the bounding spheres, velocities N(0,2), dt = 1/60 s).
+Default workload (the largest single-GPU configuration): the large pile --
+65 536 subdivision-3 icospheres (83.9 M triangles) in and above a static 40 m
+funnel, falling; each step's pair list and clusters come from the device
+broad phase and cluster separation over that step's body states, as the
+reference's cluster stepper forms them (between steps the bodies move
+ballistically: contact response is the solver's, outside this path).  The
+learned + FP32 form and the plain FP64 path run the same steps; their
+results are compared step by step.  The line also carries:
+  roofline      the dominant kernel's algorithmic bytes per launch (the
+                (body, node) records its levels test, counted on the device)
+                over its GPU-stamped launch time, against the HBM peak; and the
+                whole step's algorithmic bytes over the step time;
+  cpu_baseline  the oracle (the reference path restated in C++) on the host
+                cores over a sample of the same step's pairs;
+  c2            the 4096-body workload's figures (plain FP64 vs this form).
 
 Prints one JSON line.  ``--gpus N`` with N > 1 exits: the path runs on one GPU.
 """
@@ -26,59 +36,100 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "narrow+CCD pair queries/s, 1 GPU; learned vs plain speedup at equal TOI error"
-
-# Algorithmic HBM bytes per unit of work of each kernel (unit = the item the
-# kernel's launch counts): what one unit must read or write once, per-pair
-# records (poses, shared by all of a pair's units) not counted.
-#   tri_mask   : one node pair: its record (24 B), two node records (64 B),
-#                the pair's body/tree ids (16 B), the triangles (2 x 576 B)
-#                and eps (2 x 64 B) of both buckets; hit flag and expansion
-#                descriptor out (8 B); its kept items out (counted under tri_dist)
-#   tri_enum   : no launch of its own (the enumeration is part of tri_mask);
-#                its item count is the discrete node pairs received
-#   tri_dist   : one triangle pair: item (20 B: node pair, pair, buckets),
-#                two triangles (144 B), two eps (16 B)
-#   soup_cull  : one soup node pair: as tri_mask (swept boxes of both buckets)
-#   soup_point / soup_edge : one sub-test: item (8 B), two triangles (144 B), one 64-B result
-#   leaf_point / leaf_edge : one sub-test: item (8 B), two triangles (144 B)
-#   soup_reduce: one triangle pair: 15 results (960 B)
-ALG_BYTES = {"tri_mask": 1392.0, "tri_enum": 12.0, "tri_dist": 180.0, "soup_cull": 1400.0, "soup_point": 216.0,
-             "soup_edge": 216.0, "leaf_point": 152.0, "leaf_edge": 152.0, "soup_reduce": 960.0}
-# Estimated FP32 operations per unit of the FP32 cull (tri_mask): per node
-# pair 16 triangle preparations (~80 ops) and 64 pair classifications (~130 ops).
-ALU_OPS = {"tri_mask": 16 * 80 + 64 * 130}
-FP32_PEAK_TFLOPS = 157.3
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
+P64 = 8  # byte width of the reals the traversal kernels read (FP64 buckets and poses)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(scene, max_time, budget_s=12.0):
+# --- algorithmic bytes (per node record tested, per body, per pair, per hit) ----
+def node_bytes(c, P=P64):
+    """Bytes of the records behind touch counts c = (inner nodes, leaf nodes,
+    leaf triangles, bodies): a node's 16 B record, an inner bucket's 8 x 11
+    reals, a leaf's n x 9 reals, a body's 40 reals."""
+    return int(c[0]) * (16 + 88 * P) + int(c[1]) * 16 + int(c[2]) * 9 * P + int(c[3]) * 40 * P
+
+
+def level_bytes(per_level, phase, P=P64):
+    """Summed over a phase's levels: what its per-level kernels must read."""
+    return sum(node_bytes(per_level[phase, lv], P) for lv in range(per_level.shape[1]))
+
+
+def step_alg_bytes(run_counts, n_pairs, raw_hits, P=P64):
+    """One step's algorithmic bytes: the distinct (body, node) records of both
+    traversals, 16 B per pair (ids in, step scale out), 11 reals + 16 B per
+    raw hit."""
+    return node_bytes(run_counts, P) + 16 * n_pairs + raw_hits * (11 * P + 16)
+
+
+# phase whose levels each traversal-cull kernel runs (0 discrete, 1 CCD)
+KERNEL_PHASE = {"tri_mask": 0, "soup_cull": 1}
+
+
+def roofline_of(stats, per_level, traffic_key):
+    """The dominant kernel among the per-level culls: its bytes per launch
+    over its average GPU-stamped launch time."""
+    dom = max((k for k in KERNEL_PHASE if stats.get(k, (0, 0.0, 0))[0]), key=lambda k: stats[k][1])
+    n, ms, _ = stats[dom]
+    avg_s = ms / n / 1e3
+    per_launch = level_bytes(per_level, KERNEL_PHASE[dom]) / n
+    achieved = per_launch / avg_s / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f).get(traffic_key, {}).get(dom)
+        traffic = t.get("bytes_per_dispatch") if t else None
+    return {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "avg_launch_us": avg_s * 1e6,
+            "alg_bytes_per_launch": per_launch,
+            "bytes_model": "per level: distinct (body, node) records the level tests (device count) x "
+                           "[16 + 88P inner | 16 + 9nP leaf] + 40P per body, P = 8"}
+
+
+def kernel_table(ctx, tag):
+    stats = ctx.kernel_stats()
+    for k, (n, ms, items) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
+        if n:
+            log(f"[{tag}]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")
+    return stats
+
+
+# --- the CPU baseline -------------------------------------------------------------
+def cpu_baseline(trees, tree_of_body, records, pairs, max_time, budget_s=12.0):
     """The oracle (C++ restatement of the reference path) on the host cores,
-    on a bounded random sample of the same pairs."""
+    on a bounded random sample of the pairs: CCD + step rule, then discrete +
+    filter per pair (the discrete phase at the records' future states).
+    Only the sampled pairs' bodies and trees are handed to the oracle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import d2o
     d2o.lib()
-    osc = d2o.Scene(scene.trees, scene.tree_of_body, scene.records)
-    pairs = scene.pairs
     rng = np.random.default_rng(0)
     order = rng.permutation(pairs.shape[0])
-    # calibrate on a small sample, then size the sample to the budget
+
+    def sub(sel):
+        bodies, inv = np.unique(pairs[sel], return_inverse=True)
+        tids = np.unique(tree_of_body[bodies])
+        tmap = {int(t): i for i, t in enumerate(tids)}
+        sc = d2o.Scene([trees[int(t)] for t in tids], [tmap[int(tree_of_body[b])] for b in bodies], records[bodies])
+        return sc, inv.reshape(-1, 2).astype(np.int32)
+
     n = min(16, pairs.shape[0])
-    t1, _ = osc.time_pass(pairs[order[:n]], max_time[order[:n]], threads=1, repeats=1)
+    sc, p = sub(order[:n])
+    t1, _ = sc.time_pass(p, max_time[order[:n]], threads=1, repeats=1)
     per_pair = max(t1 / n, 1e-6)
     n = int(min(pairs.shape[0], max(n, budget_s / 3.0 / per_pair)))
     sel = order[:n]
-    t1, _ = osc.time_pass(pairs[sel], max_time[sel], threads=1, repeats=3)
+    sc, p = sub(sel)
+    t1, _ = sc.time_pass(p, max_time[sel], threads=1, repeats=3)
     cores = len(os.sched_getaffinity(0))
-    tn, _ = osc.time_pass(pairs[sel], max_time[sel], threads=cores, repeats=3)
-    return {"value": n / tn, "unit": "pair queries/s", "cores": cores, "kind": "port",
-            "value_1_thread": n / t1,
-            "sample": f"{n} of {pairs.shape[0]} pairs drawn uniformly (seed 0), CCD+step rule+discrete+filter, "
-                      f"median of 3 passes; oracle built with g++ -O2 -ffp-contract=off"}
+    tn, _ = sc.time_pass(p, max_time[sel], threads=cores, repeats=3)
+    return {"value": n / tn, "unit": "pair queries/s", "cores": cores, "kind": "port", "value_1_thread": n / t1,
+            "sample": f"{n} of {pairs.shape[0]} pairs of the measured step drawn uniformly (seed 0): CCD + step rule, "
+                      f"then discrete + filter at the step's future states; median of 3 passes; oracle built with "
+                      f"g++ -O3 -ffp-contract=off"}
 
 
 def copy_calibration(mib):
@@ -122,6 +173,205 @@ def same_results(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+# --- the large pile (default) ------------------------------------------------------
+class PileStepper:
+    """One context stepping the pile: per step, the body states are uploaded,
+    projected as the cluster stepper does before its broad phase
+    (broad_phase_embree_cluster.cpp:126-132), then broad phase and cluster
+    separation run on the device, then the timed d2_step_run."""
+
+    def __init__(self, sc, prec, weights=None):
+        import delta2_amd as d2
+        self.sc = sc
+        self.ctx = d2.Context(0, prec)
+        t0 = time.time()
+        self.ctx.upload_trees(sc.trees)
+        self.upload_s = time.time() - t0
+        if weights is not None:
+            self.ctx.mlp_set_params(weights)
+            self.ctx.set_learned(True)
+        self.prep_ms = []
+
+    def prepare(self, rec):
+        from delta2_amd import scenes
+        c = self.ctx
+        c.upload_bodies(self.sc.tree_of_body, rec)
+        c.project_future(np.minimum(2.0 * rec[:, 39], scenes.DT))
+        c.sync()
+        t0 = time.perf_counter()
+        n = c.broad_phase(mode=1)
+        t1 = time.perf_counter()
+        c.build_clusters()
+        c.sync()
+        self.prep_ms.append((1e3 * (t1 - t0), 1e3 * (time.perf_counter() - t1)))
+        return n
+
+    def step(self, rec):
+        """(pairs, seconds of d2_step_run)."""
+        from delta2_amd import scenes
+        n = self.prepare(rec)
+        self.ctx.sync()
+        t0 = time.perf_counter()
+        self.ctx.step_run(scenes.DT, True)
+        self.ctx.sync()
+        return n, time.perf_counter() - t0
+
+    def results(self):
+        return step_results(self.ctx) + (self.ctx.download_bodies()[:, 14:28].copy(),)
+
+    def close(self):
+        self.ctx.close()
+
+
+def pile_states(sc, n):
+    """The pile's body states for n steps: each step advances the previous one
+    ballistically under gravity."""
+    from delta2_amd import scenes
+    out = [sc.records]
+    for _ in range(n - 1):
+        out.append(scenes.advance_gravity(out[-1]))
+    return out
+
+
+def run_pile(args):
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    t_start = time.time()
+    name = f"c5_s{args.subdiv}_{args.bodies}"
+    sc = scenes.cached(name, lambda: scenes.scene_c5(n_bodies=args.bodies, subdiv=args.subdiv,
+                                                       progress=lambda m: log(f"[c5] {m}")))
+    build_s = time.time() - t_start
+    log(f"[c5] {sc.n_bodies} bodies, {sc.n_triangles()} triangles, {int(sc.trees.offs[-1])} tree nodes, "
+        f"built in {build_s:.1f}s")
+    weights, wsrc = gate_weights() if args.learned == "on" else (None, None)
+    prec = d2.D2_FP32 if args.precision == "fp32" else d2.D2_FP64
+    W, K = args.warmup, args.steps
+    n_plain = K if args.plain_steps <= 0 else min(K, args.plain_steps)
+    states = pile_states(sc, W + K)
+
+    # this form: warm-up steps (the first sizes buffers, then the autotuner
+    # picks launch shapes on its resident inputs), then the K timed steps
+    fast = PileStepper(sc, prec, weights)
+    log(f"[c5] trees uploaded in {fast.upload_s:.1f}s")
+    fast.step(states[0])
+    if not args.no_autotune:
+        shape, ms = fast.ctx.autotune("step", reps=1)
+        log(f"[c5] autotuned {shape}: {ms:.1f} ms")
+    for k in range(1, W):
+        fast.step(states[k])
+    pairs_t, secs, ref = [], [], []
+    for k in range(W, W + K):
+        n, s = fast.step(states[k])
+        pairs_t.append(n)
+        secs.append(s)
+        if k - W < n_plain and not args.no_plain:
+            ref.append(fast.results())
+        log(f"[c5] step {k - W}: {n} pairs, {1e3 * s:.1f} ms")
+    value = sum(pairs_t) / sum(secs)
+    ms_per_step = 1e3 * sum(secs) / K
+    gate = fast.ctx.gate_counts() if weights is not None else None
+
+    # per-kernel times (one eager step with HIP events around every kernel)
+    # and the (body, node) records its levels test, on the last timed state
+    last = states[W + K - 1]
+    fast.prepare(last)
+    fast.ctx.reset_stats()
+    fast.ctx.set_profiling(True)
+    fast.ctx.step_run(scenes.DT, True)
+    fast.ctx.sync()
+    fast.ctx.set_profiling(False)
+    stats = kernel_table(fast.ctx, "c5")
+    fast.ctx.set_touch_stats(True)
+    fast.ctx.step_run(scenes.DT, True)
+    fast.ctx.set_touch_stats(False)
+    per_level, _ = fast.ctx.touch_stats()
+    tag = f"c5/{args.precision}/{'learned' if weights is not None else 'plain'}"
+    roofline = roofline_of(stats, per_level, tag)
+    log(f"[c5] roofline {roofline['kernel']}: {roofline['achieved']:.0f} GB/s ({100 * roofline['frac']:.1f}% of HBM)")
+    # end to end: host body states in, contacts and per-pair CCD values out
+    t0 = time.perf_counter()
+    fast.prepare(last)
+    fast.ctx.step_run(scenes.DT, True)
+    fast.ctx.result(with_pairs=True)
+    e2e_ms = 1e3 * (time.perf_counter() - t0)
+    # the CPU baseline's inputs: the first timed step's pairs and future states
+    fast.prepare(states[W])
+    cpu_pairs, cpu_mt = fast.ctx.download_pairs()
+    fast.ctx.step_run(scenes.DT, True)
+    cpu_rec = fast.ctx.download_bodies()
+    fast_prep = np.array(fast.prep_ms)
+    fast.close()
+
+    # the plain FP64 path on the same states
+    plain_out, identical = None, None
+    step_bytes = None
+    if not args.no_plain:
+        plain = PileStepper(sc, d2.D2_FP64)
+        plain.step(states[0])
+        psecs, ppairs, identical = [], [], True
+        for k in range(W, W + n_plain):
+            n, s = plain.step(states[k])
+            psecs.append(s)
+            ppairs.append(n)
+            same = same_results(plain.results(), ref[k - W])
+            identical = identical and same
+            log(f"[c5] plain fp64 step {k - W}: {1e3 * s:.1f} ms, identical {same}")
+        plain_out = {"value": sum(ppairs) / sum(psecs), "ms_per_step": 1e3 * sum(psecs) / n_plain, "steps": n_plain}
+        # the reference path's algorithmic bytes of one step (no gate: every
+        # pair traversed), at the last timed state
+        plain.prepare(last)
+        plain.ctx.set_touch_stats(True)
+        plain.ctx.step_run(scenes.DT, True)
+        plain.ctx.set_touch_stats(False)
+        _, run_counts = plain.ctx.touch_stats()
+        g, lv = plain.ctx.run_info()
+        raw_hits = int(g[0]) + int(lv[1][0][7])  # discrete contacts before the filter + CCD contacts
+        step_bytes = step_alg_bytes(run_counts, pairs_t[-1], raw_hits)
+        plain.close()
+    if step_bytes is not None:
+        ach = step_bytes / (ms_per_step / 1e3) / 1e9
+        roofline["step"] = {"alg_bytes": step_bytes, "ms": ms_per_step, "achieved": ach, "frac": ach / HBM_PEAK_GBS,
+                            "bytes_model": "distinct (body, node) records of both traversals of the plain path "
+                                           "+ 40P per body + 16 B per pair + (11P + 16) B per raw hit, P = 8"}
+    cpu = None
+    if not args.no_cpu_baseline:
+        t0 = time.time()
+        cpu = cpu_baseline(sc.trees, sc.tree_of_body, cpu_rec, cpu_pairs, cpu_mt)
+        log(f"[c5] cpu baseline in {time.time() - t0:.1f}s: {cpu['value']:.1f} pairs/s on {cpu['cores']} cores, "
+            f"{cpu['value_1_thread']:.1f} on 1")
+    del sc, states
+    c2 = None
+    if args.c2 and time.time() - t_start < args.c2_deadline:
+        c2 = run_small_figures("c2", args, weights)
+    out = {
+        "metric": METRIC, "value": value, "unit": "pair queries/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == d2.D2_FP32 else "f64", "data": "synthetic",
+        "form": {"precision": args.precision, "learned_gate": weights is not None, "graph": False, "weights": wsrc},
+        "plain_fp64": plain_out, "speedup_vs_plain_fp64": plain_out["ms_per_step"] / ms_per_step if plain_out else None,
+        "identical_to_plain_fp64": identical, "gate": gate,
+        "config": {"workload": f"{args.bodies} subdivision-{args.subdiv} icospheres U(0.5,1.5) ({sc_tris(args)} "
+                               f"triangles) in and above a static 40 m funnel (32768 triangles) at 0.1 bodies/m^3, "
+                               f"v ~ N(0,1) - 3z m/s, dt = 1/60; per step: device broad phase + clusters over the "
+                               f"step's states, then CCD + step rule + cluster steps + discrete + filter; ballistic "
+                               f"motion between steps",
+                   "bodies": args.bodies + 1, "triangles": sc_tris(args), "pairs_per_step": pairs_t},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "end_to_end": {"ms_per_step": e2e_ms, "value": pairs_t[-1] / (e2e_ms / 1e3), "unit": "pair queries/s",
+                       "includes": "H2D of body states, device broad phase + clusters, the step, D2H of contacts and "
+                                   "per-pair CCD values"},
+        "device_pipeline": {"broad_phase_ms": float(fast_prep[:, 0].mean()), "clusters_ms": float(fast_prep[:, 1].mean())},
+        "setup_s": {"scene": build_s, "tree_upload": fast.upload_s},
+        "c2": c2,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def sc_tris(args):
+    return 32768 + args.bodies * 20 * 4 ** args.subdiv
+
+
+# --- the 4096-body and 100-body workloads --------------------------------------------
 def make_ctx(scene, prec, graph, weights=None, clusters=None, tune=True):
     """A context holding the scene; launch shapes (and the gate's phase mask)
     picked by the library's autotuner on this step."""
@@ -144,12 +394,99 @@ def make_ctx(scene, prec, graph, weights=None, clusters=None, tune=True):
     return ctx
 
 
+def timed_steps(ctx, dt, steps, warmup):
+    for _ in range(warmup):
+        ctx.step_run(dt, True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step_run(dt, True)
+    ctx.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def run_small_figures(workload, args, weights, steps=None, tune=True):
+    """One scene's step replayed: the plain FP64 path and this form (graph
+    replay, autotuned shapes), their results compared."""
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    t0 = time.time()
+    scene = scenes.cached(workload, scenes.scene_c2 if workload == "c2" else scenes.scene_c1)
+    log(f"[{workload}] {scene.n_bodies} bodies, {scene.n_triangles()} triangles, {scene.pairs.shape[0]} pairs, "
+        f"built in {time.time() - t0:.1f}s")
+    steps = steps or min(args.steps, 10)
+    clusters = scenes.clusters(scene.pairs, scene.n_bodies, scene.records)
+    prec = d2.D2_FP64 if args.precision == "fp64" else d2.D2_FP32
+    plain = make_ctx(scene, d2.D2_FP64, True, None, clusters, tune=tune)
+    plain_sec = timed_steps(plain, scenes.DT, steps, 2)
+    plain_res = step_results(plain)
+    plain.close()
+    ctx = make_ctx(scene, prec, args.graph == "on", weights, clusters, tune=tune)
+    sec = timed_steps(ctx, scenes.DT, steps, 2)
+    identical = same_results(step_results(ctx), plain_res)
+    gate = ctx.gate_counts() if weights is not None else None
+    ctx.close()
+    n = int(scene.pairs.shape[0])
+    log(f"[{workload}] plain fp64 {1e3 * plain_sec:.3f} ms/step, this form {1e3 * sec:.3f} ms/step, identical {identical}")
+    return {"value": n / sec, "ms_per_step": 1e3 * sec, "steps": steps, "pairs": n,
+            "plain_fp64": {"value": n / plain_sec, "ms_per_step": 1e3 * plain_sec},
+            "speedup_vs_plain_fp64": plain_sec / sec, "identical_to_plain_fp64": identical, "gate": gate,
+            "graph": args.graph == "on",
+            "workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
+                        "Poisson-disk 30% packing; v~N(0,2); dt=1/60; one step replayed"
+            if workload == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60"}
+
+
+def run_small(args):
+    """--workload c1 / c2: the one-step figures plus roofline and CPU baseline."""
+    import delta2_amd as d2
+    from delta2_amd import scenes
+    weights, wsrc = gate_weights() if args.learned == "on" else (None, None)
+    fig = run_small_figures(args.workload, args, weights, steps=args.steps, tune=not args.no_autotune)
+    scene = scenes.cached(args.workload, scenes.scene_c2 if args.workload == "c2" else scenes.scene_c1)
+    prec = d2.D2_FP64 if args.precision == "fp64" else d2.D2_FP32
+    clusters = scenes.clusters(scene.pairs, scene.n_bodies, scene.records)
+    ctx = make_ctx(scene, prec, False, weights, clusters, tune=not args.no_autotune)
+    ctx.reset_stats()
+    ctx.set_profiling(True)
+    for _ in range(args.profile_steps):
+        ctx.step_run(scenes.DT, True)
+    ctx.sync()
+    ctx.set_profiling(False)
+    stats = kernel_table(ctx, args.workload)
+    ctx.set_touch_stats(True)
+    ctx.step_run(scenes.DT, True)
+    ctx.set_touch_stats(False)
+    per_level, _ = ctx.touch_stats()
+    per_level = per_level * args.profile_steps  # the profiled launches ran that many steps
+    roofline = roofline_of(stats, per_level, f"{args.workload}/{args.precision}/"
+                                             f"{'learned' if weights is not None else 'plain'}")
+    rec = ctx.download_bodies()
+    ctx.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        mt = np.minimum(scene.records[scene.pairs[:, 0], 39], scene.records[scene.pairs[:, 1], 39])
+        cpu = cpu_baseline(scene.trees, scene.tree_of_body, rec, scene.pairs, mt)
+    out = {"metric": METRIC, "value": fig["value"], "unit": "pair queries/s", "n_gpus": 1, "steps": fig["steps"],
+           "warmup": 2, "ms_per_step": fig["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32+f64" if prec == d2.D2_FP32 else "f64", "data": "synthetic",
+           "form": {"precision": args.precision, "learned_gate": weights is not None, "graph": args.graph == "on",
+                    "weights": wsrc},
+           "plain_fp64": fig["plain_fp64"], "speedup_vs_plain_fp64": fig["speedup_vs_plain_fp64"],
+           "identical_to_plain_fp64": fig["identical_to_plain_fp64"], "gate": fig["gate"],
+           "config": {"workload": fig["workload"], "bodies": scene.n_bodies, "triangles": scene.n_triangles(),
+                      "pairs": fig["pairs"]},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+# --- the held-out learned evaluation and the batched scenes --------------------------
 def run_c4(args):
     """Learned vs plain at equal TOI error on the held-out 4096-body C1-density
     scene: identical step results (contacts, per-pair TOI/scale, cluster
     steps) are checked, so the speedup is at zero TOI difference."""
     import delta2_amd as d2
-    from delta2_amd import learned, scenes
+    from delta2_amd import scenes
     t0 = time.time()
     sc = scenes.scene_c4()
     cl = scenes.clusters(sc.pairs, sc.n_bodies, sc.records)
@@ -180,169 +517,6 @@ def run_c4(args):
            "learned_vs_plain_same_precision": {"fp64": out["learned_fp64"]["value"] / out["plain_fp64"]["value"],
                                                "fp32": best["value"] / out["plain_fp32"]["value"]}}
     print(json.dumps(res), flush=True)
-
-
-def run_c5(args):
-    """Large dense pile: 65536 C1-mesh bodies at the C1 density in and above a
-    static 40 m funnel, falling.  Each step: ballistic advance under gravity
-    (no contact response: the solver is outside this path), broad phase and
-    clusters on the host, then the timed d2_step_run (inputs resident).  The
-    plain FP64 path and the learned FP32 form run the same steps; their step
-    results are compared every step."""
-    import delta2_amd as d2
-    from delta2_amd import scenes
-    t0 = time.time()
-    sc = scenes.scene_c5()
-    log(f"[c5] {sc.n_bodies} bodies ({sc.n_triangles()} triangles), built in {time.time() - t0:.1f}s")
-    w, src = gate_weights()
-    steps = []
-    rec = sc.records
-    for k in range(args.steps):
-        pairs = scenes.c5_pairs(rec, sc.radii) if k else sc.pairs
-        steps.append((rec, pairs, scenes.clusters(pairs, sc.n_bodies, rec)))
-        rec = scenes.advance_gravity(rec)
-    out, ref = {}, {}
-    for name, prec, wt in (("plain_fp64", d2.D2_FP64, None), ("learned_fp32", d2.D2_FP32, w)):
-        ctx = d2.Context(0, prec)
-        ctx.upload_trees(sc.trees)
-        if wt is not None:
-            ctx.mlp_set_params(wt)
-            ctx.set_learned(True)
-        total, n_total, same = 0.0, 0, True
-        cl_steps = []
-        for k, (r, pairs, cl) in enumerate(steps):
-            ctx.upload_bodies(sc.tree_of_body, r)
-            ctx.upload_pairs(pairs)
-            ctx.upload_clusters(*cl)
-            if k == 0:
-                ctx.step_run(scenes.DT, True)  # sizes buffers and capacities
-                if wt is not None:
-                    shape, ms = ctx.autotune("step", reps=1)
-                    log(f"[c5] autotuned {shape}: {ms:.1f} ms")
-                ctx.upload_bodies(sc.tree_of_body, r)
-            ctx.sync()
-            t1 = time.perf_counter()
-            _, st = ctx.step_run(scenes.DT, True, want_steps=True)
-            ctx.sync()
-            total += time.perf_counter() - t1
-            n_total += pairs.shape[0]
-            res = step_results(ctx)
-            if name == "plain_fp64":
-                ref[k] = (res, st)
-            else:
-                same = same and same_results(res, ref[k][0]) and np.array_equal(st, ref[k][1])
-            cl_steps.append(st)
-        allst = np.concatenate(cl_steps)
-        out[name] = {"ms_per_step": 1e3 * total / len(steps), "value": n_total / total,
-                     "identical_to_plain_fp64": same, "gate": ctx.gate_counts() if wt is not None else None,
-                     "cluster_step_range_s": [float(allst.min()), float(allst.max())]}
-        log(f"[c5] {name}: {out[name]['ms_per_step']:.1f} ms/step, identical {same}")
-        ctx.close()
-    # the same steps with broad phase and clusters on the device, as the
-    # reference's cluster stepper forms them (broadPhaseEmbree(particles): the
-    # static funnel pairs with every body inside its box)
-    ctx = d2.Context(0, d2.D2_FP32)
-    ctx.upload_trees(sc.trees)
-    ctx.mlp_set_params(w)
-    ctx.set_learned(True)
-    dev = {"broad_phase_ms": [], "clusters_ms": [], "step_ms": [], "pairs": [], "clusters": []}
-    for k, (r, _, _) in enumerate(steps):
-        for rep in range(2 if k == 0 else 1):  # the first pass sizes buffers
-            ctx.upload_bodies(sc.tree_of_body, r)
-            ctx.project_future(np.minimum(2.0 * r[:, 39], scenes.DT))  # broad_phase_embree_cluster.cpp:121-127
-            ctx.sync()
-            t1 = time.perf_counter()
-            n_pairs = ctx.broad_phase(mode=1)
-            t2 = time.perf_counter()
-            n_cl = ctx.build_clusters()
-            t3 = time.perf_counter()
-            ctx.step_run(scenes.DT, True)
-            ctx.sync()
-            t4 = time.perf_counter()
-        dev["broad_phase_ms"].append(1e3 * (t2 - t1))
-        dev["clusters_ms"].append(1e3 * (t3 - t2))
-        dev["step_ms"].append(1e3 * (t4 - t3))
-        dev["pairs"].append(int(n_pairs))
-        dev["clusters"].append(int(n_cl))
-    ctx.close()
-    dev["pair_queries_per_s"] = sum(dev["pairs"]) / (sum(dev["broad_phase_ms"]) + sum(dev["clusters_ms"]) +
-                                                     sum(dev["step_ms"])) * 1e3
-    log(f"[c5] device pipeline: pairs {dev['pairs']}, broad phase {dev['broad_phase_ms']} ms, clusters "
-        f"{dev['clusters_ms']} ms, step {dev['step_ms']} ms")
-    out["device_pipeline_fp32"] = dev
-    best = out["learned_fp32"]
-    res = {"metric": METRIC, "value": best["value"], "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": 1, "ms_per_step": best["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
-           "config": {"workload": "65536 subdiv-2 icospheres U(0.5,1.5) (21 M triangles) at 0.1 bodies/m^3 in and above "
-                                  "a static 40 m funnel (32768 triangles), v ~ N(0,1) - 3 z m/s, dt = 1/60, "
-                                  f"{args.steps} ballistic steps; broad phase on the host (untimed)",
-                      "pairs_per_step": [int(p.shape[0]) for _, p, _ in steps], "weights": src},
-           "variants": out, "learned_fp32_vs_plain_fp64": best["value"] / out["plain_fp64"]["value"]}
-    print(json.dumps(res), flush=True)
-
-
-def timed_steps(ctx, dt, steps, warmup):
-    for _ in range(warmup):
-        ctx.step_run(dt, True)
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx.step_run(dt, True)
-    ctx.sync()
-    return (time.perf_counter() - t0) / steps
-
-
-def end_to_end(ctx, scene, clusters, steps=5):
-    """The same step with host buffers on both sides: bodies, pairs and
-    clusters uploaded, contacts and per-pair CCD values downloaded."""
-    from delta2_amd import scenes
-    cl, n_cl = clusters
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        ctx.upload_bodies(scene.tree_of_body, scene.records)
-        ctx.upload_pairs(scene.pairs)
-        ctx.upload_clusters(cl, n_cl)
-        ctx.step_run(scenes.DT, True)
-        ctx.result(with_pairs=True)
-    sec = (time.perf_counter() - t0) / steps
-    return {"value": scene.pairs.shape[0] / sec, "unit": "pair queries/s", "ms_per_step": 1e3 * sec, "steps": steps,
-            "includes": "H2D of body records, pairs, clusters; D2H of contacts and per-pair CCD values"}
-
-
-def device_pipeline(scene, prec, weights, steps=5):
-    """Broad phase, cluster separation and the step, all on the device, over
-    the scene's bodies (the pair list never leaves the GPU)."""
-    import delta2_amd as d2
-    from delta2_amd import scenes
-    ctx = d2.Context(0, prec)
-    ctx.upload_trees(scene.trees)
-    ctx.upload_bodies(scene.tree_of_body, scene.records)
-    ctx.project_future(np.minimum(2.0 * scene.records[:, 39], scenes.DT))  # broad_phase_embree_cluster.cpp:121-127
-    if weights is not None:
-        ctx.mlp_set_params(weights)
-        ctx.set_learned(True)
-    ctx.set_graph(True)
-    t_bp = t_cl = t_st = 0.0
-    for k in range(steps + 1):
-        t0 = time.perf_counter()
-        n_pairs = ctx.broad_phase(mode=1)
-        t1 = time.perf_counter()
-        n_cl = ctx.build_clusters()
-        t2 = time.perf_counter()
-        ctx.step_run(scenes.DT, True)
-        ctx.sync()
-        t3 = time.perf_counter()
-        if k:  # the first step sizes buffers and captures graphs
-            t_bp += t1 - t0
-            t_cl += t2 - t1
-            t_st += t3 - t2
-    ctx.close()
-    ms = lambda x: 1e3 * x / steps  # noqa: E731
-    total = ms(t_bp + t_cl + t_st)
-    return {"bodies": scene.n_bodies, "pairs": n_pairs, "clusters": n_cl, "broad_phase_ms": ms(t_bp),
-            "clusters_ms": ms(t_cl), "step_ms": ms(t_st), "total_ms": total,
-            "bodies_per_s_broad_phase": scene.n_bodies / (t_bp / steps), "pair_queries_per_s": n_pairs / (total / 1e3)}
 
 
 def run_c3(args):
@@ -386,9 +560,7 @@ def run_c3(args):
                 ctx.step_run(scenes.DT, True)
             ctx.sync()
             ctx.set_profiling(False)
-            for k, (n, ms, items) in sorted(ctx.kernel_stats().items(), key=lambda kv: -kv[1][1]):
-                if n:
-                    log(f"[c3]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")
+            kernel_table(ctx, "c3")
         ctx.close()
     # plain: one scene at a time
     ctx = d2.Context(0, d2.D2_FP64)
@@ -436,14 +608,19 @@ def run_c3(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--bodies", type=int, default=65536, help="c5: bodies besides the funnel")
+    ap.add_argument("--subdiv", type=int, default=3, help="c5: icosphere subdivision (3: 1280 triangles per body)")
+    ap.add_argument("--plain-steps", type=int, default=3, help="c5: timed steps of the plain FP64 path (0: all)")
+    ap.add_argument("--c2", type=int, default=1, help="c5: also measure the 4096-body workload (extra key)")
+    ap.add_argument("--c2-deadline", type=float, default=330.0, help="c5: skip the 4096-body figures after this many s")
     ap.add_argument("--scenes", type=int, default=256, help="c3: number of independent scenes")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--learned", default="on", choices=["on", "off"],
                     help="the learned pair gate (weights from delta2_amd/weights; trained here when absent)")
-    ap.add_argument("--graph", default="on", choices=["on", "off"])
+    ap.add_argument("--graph", default="on", choices=["on", "off"], help="c1-c4: graph replay of the step")
     ap.add_argument("--profile-steps", type=int, default=5, help="eager steps timed per kernel for the roofline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="default launch shapes (profiling runs)")
@@ -459,106 +636,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no multi-GPU sharding",
               file=sys.stderr)
         sys.exit(2)
-
     import delta2_amd as d2
-    from delta2_amd import scenes
     d2.lib()
-    if args.workload == "c3":
-        run_c3(args)
-        return
-    if args.workload == "c4":
-        run_c4(args)
-        return
-    if args.workload == "c5":
-        run_c5(args)
-        return
-    t0 = time.time()
-    scene = scenes.cached(args.workload, scenes.scene_c2 if args.workload == "c2" else scenes.scene_c1)
-    log(f"[bench] scene {scene.name}: {scene.n_bodies} bodies, {scene.n_triangles()} triangles, "
-        f"{scene.pairs.shape[0]} pairs, built in {time.time() - t0:.1f}s")
-    pairs = scene.pairs
-    max_time = np.minimum(scene.records[pairs[:, 0], 39], scene.records[pairs[:, 1], 39])
-    prec = d2.D2_FP64 if args.precision == "fp64" else d2.D2_FP32
-    clusters = scenes.clusters(pairs, scene.n_bodies, scene.records)
-    n_cl = clusters[1]
-    weights, wsrc = gate_weights() if args.learned == "on" else (None, None)
-    # the plain FP64 path on the same inputs (reference point of the speedup)
-    plain_sec, plain_res = None, None
-    if not args.no_plain:
-        plain = make_ctx(scene, d2.D2_FP64, True, None, clusters, tune=not args.no_autotune)
-        plain_sec = timed_steps(plain, scenes.DT, args.steps, args.warmup)
-        plain_res = step_results(plain)
-        plain.close()
-    ctx = make_ctx(scene, prec, args.graph == "on", weights, clusters, tune=not args.no_autotune)
-    n_contacts = ctx.step_run(scenes.DT, True)
-    sec = timed_steps(ctx, scenes.DT, args.steps, args.warmup)
-    identical = same_results(step_results(ctx), plain_res) if plain_res else None
-    ms_per_step = 1e3 * sec
-    value = pairs.shape[0] / sec
-    gate = ctx.gate_counts() if weights is not None else None
-    log(f"[bench] plain fp64: {1e3 * plain_sec if plain_sec else float('nan'):.3f} ms/step; this form: {ms_per_step:.3f} ms/step "
-        f"({n_cl} clusters, {n_contacts} filtered contacts, graph {args.graph}, gate {gate}, identical {identical})")
-    # per-kernel times: eager steps, every kernel bracketed by HIP events on the context's stream
-    ctx.set_graph(False)
-    ctx.reset_stats()
-    ctx.set_profiling(True)
-    for _ in range(args.profile_steps):
-        ctx.step_run(scenes.DT, True)
-    ctx.sync()
-    ctx.set_profiling(False)
-    stats = ctx.kernel_stats()
-    for k, (n, ms, items) in sorted(stats.items(), key=lambda kv: -kv[1][1]):
-        if n:
-            log(f"[bench]   {k:12s} launches {n:6d}  total {ms:9.3f} ms  avg {ms / n * 1e3:9.2f} us  items {items:.0f}")
-
-    dom = max((k for k in stats if k in ALG_BYTES and stats[k][0]), key=lambda k: stats[k][1])
-    n, ms, items = stats[dom]
-    avg_s = ms / n / 1e3
-    bytes_per_launch = ALG_BYTES[dom] * items / n
-    achieved = bytes_per_launch / avg_s / 1e9
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):  # measured HBM bytes per dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)
-        with open(TRAFFIC_FILE) as f:
-            key = f"{scene.name}/{args.precision}/{'learned' if weights is not None else 'plain'}"
-            t = json.load(f).get(key, {}).get(dom)
-        traffic = t.get("bytes_per_dispatch") if t else None
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch}
-    if dom in ALU_OPS:
-        tf = ALU_OPS[dom] * items / n / avg_s / 1e12
-        roofline["alu"] = {"achieved_tflops_est": tf, "peak_fp32_tflops": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
-                           "ops_per_unit_est": ALU_OPS[dom]}
-    ctx.set_graph(args.graph == "on")
-    e2e = end_to_end(ctx, scene, clusters)
-    log(f"[bench] end to end (host buffers): {e2e['ms_per_step']:.3f} ms/step")
-    dev = device_pipeline(scene, prec, weights)
-    log(f"[bench] device pipeline: broad phase {dev['broad_phase_ms']:.3f} ms ({dev['pairs']} pairs), clusters "
-        f"{dev['clusters_ms']:.3f} ms ({dev['clusters']}), step {dev['step_ms']:.3f} ms")
-    cpu = None
-    if not args.no_cpu_baseline:
-        t0 = time.time()
-        cpu = cpu_baseline(scene, max_time)
-        log(f"[bench] cpu baseline in {time.time() - t0:.1f}s: {cpu['value']:.1f} pairs/s on {cpu['cores']} cores, "
-            f"{cpu['value_1_thread']:.1f} on 1")
-    out = {
-        "metric": METRIC, "value": value, "unit": "pair queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if prec == d2.D2_FP64 else "f32+f64", "data": "synthetic",
-        "form": {"precision": args.precision, "learned_gate": args.learned == "on", "graph": args.graph == "on",
-                 "weights": wsrc},
-        "plain_fp64": {"value": pairs.shape[0] / plain_sec, "ms_per_step": 1e3 * plain_sec} if plain_sec else None,
-        "speedup_vs_plain_fp64": plain_sec / sec if plain_sec else None, "identical_to_plain_fp64": identical,
-        "gate": gate,
-        "config": {"workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
-                               "Poisson-disk 30% packing; v~N(0,2); dt=1/60; CCD+step rule then discrete+filter"
-                   if scene.name == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60",
-                   "bodies": scene.n_bodies, "triangles": scene.n_triangles(), "pairs": int(pairs.shape[0]),
-                   "precision": args.precision},
-        "roofline": roofline, "cpu_baseline": cpu, "end_to_end": e2e, "device_pipeline": dev,
-    }
-    print(json.dumps(out), flush=True)
-    ctx.close()
+    {"c3": run_c3, "c4": run_c4, "c5": run_pile}.get(args.workload, run_small)(args)
 
 
 if __name__ == "__main__":

@@ -118,6 +118,8 @@ def lib():
         "d2_pairs_download": ([_vp, _ip, _dp], ctypes.c_int),
         "d2_clusters_build": ([_vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "d2_clusters_download": ([_vp, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
+        "d2_set_touch_stats": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_touch_stats": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -238,12 +240,16 @@ class Context:
         self._check(lib().d2_set_auto_grow(self.h, int(bool(on))))
 
     def upload_trees(self, trees):
-        """trees: list of (nodes, buckets)."""
-        offs = np.zeros(len(trees) + 1, dtype=np.int32)
-        offs[1:] = np.cumsum([t[0].shape[0] for t in trees])
-        nodes = _i32(np.concatenate([t[0] for t in trees]))
-        buckets = _f64(np.concatenate([t[1] for t in trees]))
-        self._check(lib().d2_trees_upload(self.h, len(trees), _iptr(offs), _iptr(nodes), _dptr(buckets)))
+        """trees: a list of (nodes, buckets), or flat arrays with attributes
+        offs, nodes and buckets (scenes.FlatTrees; uploaded without a copy)."""
+        if hasattr(trees, "offs"):
+            offs, nodes, buckets = _i32(trees.offs), _i32(trees.nodes), _f64(trees.buckets)
+        else:
+            offs = np.zeros(len(trees) + 1, dtype=np.int32)
+            offs[1:] = np.cumsum([t[0].shape[0] for t in trees])
+            nodes = _i32(np.concatenate([t[0] for t in trees]))
+            buckets = _f64(np.concatenate([t[1] for t in trees]))
+        self._check(lib().d2_trees_upload(self.h, offs.size - 1, _iptr(offs), _iptr(nodes), _dptr(buckets)))
 
     def upload_bodies(self, tree_of_body, records):
         tob = _i32(tree_of_body)
@@ -415,6 +421,19 @@ class Context:
 
     def sync(self):
         self._check(lib().d2_sync(self.h))
+
+    def set_touch_stats(self, on=True):
+        """Accounting runs: record the (body, node) records each level tests."""
+        self._check(lib().d2_set_touch_stats(self.h, int(bool(on))))
+
+    def touch_stats(self):
+        """(per_level[2, 17, 4], run[4]) of the last accounting run: distinct
+        inner nodes, leaf nodes, leaf triangles and bodies, per phase (0
+        discrete, 1 CCD) and level, and over the whole run."""
+        n = lib().d2_touch_stats(self.h, None, 0)
+        buf = np.zeros(n, dtype=np.int64)
+        lib().d2_touch_stats(self.h, buf.ctypes.data_as(_i64p), n)
+        return buf[:-4].reshape(2, 17, 4), buf[-4:]
 
     # --- broad phase and clusters ---------------------------------------------
     def upload_tree_bounds(self, aabb6, radius):

@@ -1557,6 +1557,82 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
   }
 }
 
+// ------------------------------------------- algorithmic-byte accounting
+// Instrumentation only (d2_set_touch_stats): which (body, node) records a
+// level's traversal tests.  Bit node_base[body] + node of a bitmap stands for
+// node `node` of body `body`; one bitmap per level and one per run (the union
+// over the run's levels and phases), and one bit per body for each.
+struct TouchArgs {
+  const NP* soup;
+  const NP* conn;
+  const unsigned long long* lvl;
+  const int* np_hit_prev;
+  const int32_t* pair_info;
+  const long long* node_base;
+  unsigned *lv_nodes, *run_nodes, *lv_bodies, *run_bodies;
+  long long cap_np;
+  int level, discrete;
+};
+__device__ __forceinline__ void touch_bit(unsigned* bits, long long idx) {
+  atomicOr(bits + (idx >> 5), 1u << (unsigned)(idx & 31));
+}
+// Marks the nodes of every node pair the level tests: a discrete node pair
+// whose parent missed is dropped unread by k_tri_mask (same test here); every
+// CCD soup and connected node pair is tested.
+__global__ void __launch_bounds__(256) k_touch(TouchArgs T) {
+  const unsigned long long* lv = T.lvl + (size_t)T.level * LV_STRIDE;
+  const long long ns = (long long)lv[LV_SOUP] < T.cap_np ? (long long)lv[LV_SOUP] : T.cap_np;
+  const long long nc = T.discrete ? 0 : ((long long)lv[LV_CONN] < T.cap_np ? (long long)lv[LV_CONN] : T.cap_np);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < ns + nc; t += (long long)gridDim.x * blockDim.x) {
+    const NP np = t < ns ? T.soup[t] : T.conn[t - ns];
+    if (T.discrete && np.pad != 0u && T.np_hit_prev[np.pad - 1u] == 0) continue;
+    const int a = T.pair_info[4 * np.pair], b = T.pair_info[4 * np.pair + 1];
+    const long long ia = T.node_base[a] + np.na, ib = T.node_base[b] + np.nb;
+    touch_bit(T.lv_nodes, ia), touch_bit(T.lv_nodes, ib);
+    touch_bit(T.run_nodes, ia), touch_bit(T.run_nodes, ib);
+    touch_bit(T.lv_bodies, a), touch_bit(T.lv_bodies, b);
+    touch_bit(T.run_bodies, a), touch_bit(T.run_bodies, b);
+  }
+}
+// out[0..3] += inner nodes, leaf nodes, triangles of those leaves, bodies
+// marked in the bitmaps.
+__global__ void __launch_bounds__(256) k_touch_count(const unsigned* nodes_bits, long long n_words, const unsigned* body_bits,
+                                                     long long n_body_words, const long long* node_base, int n_bodies,
+                                                     const int32_t* body_tree, const int32_t* tree_off,
+                                                     const NodeRec* nodes, unsigned long long* out) {
+  unsigned long long inner = 0, leaf = 0, tris = 0, bodies = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n_words; w += stride) {
+    unsigned m = nodes_bits[w];
+    while (m) {
+      const int k = __ffs(m) - 1;
+      m &= m - 1u;
+      const long long idx = 32 * w + k;
+      int lo = 0, hi = n_bodies - 1;  // the body: the last node_base <= idx
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (node_base[mid] <= idx) lo = mid;
+        else hi = mid - 1;
+      }
+      const NodeRec r = nodes[tree_off[body_tree[lo]] + (idx - node_base[lo])];
+      if (r.inner) ++inner;
+      else ++leaf, tris += (unsigned long long)r.ntris;
+    }
+  }
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < n_body_words; w += stride)
+    bodies += (unsigned long long)__popc(body_bits[w]);
+  for (int o = 32; o > 0; o >>= 1) {
+    inner += __shfl_down(inner, o), leaf += __shfl_down(leaf, o);
+    tris += __shfl_down(tris, o), bodies += __shfl_down(bodies, o);
+  }
+  if (lane_id() == 0) {
+    if (inner) atomicAdd(out + 0, inner);
+    if (leaf) atomicAdd(out + 1, leaf);
+    if (tris) atomicAdd(out + 2, tris);
+    if (bodies) atomicAdd(out + 3, bodies);
+  }
+}
+
 // -------------------------------------------------------- post-processing
 __device__ __forceinline__ long long n_contacts(const unsigned long long* cnt, long long cap) {
   long long n = (long long)cnt[C_CONTACTS];
@@ -2711,6 +2787,15 @@ struct d2_ctx {
   long long prev_lv[2][MAX_LEVELS + 1][LV_STRIDE] = {};
   bool prev_ok[2] = {false, false};
   long long prev_contacts[2] = {-1, -1};  // contacts of the last run's discrete / CCD phase (-1: none yet)
+  // algorithmic-byte accounting (d2_set_touch_stats): bitmaps over (body, node)
+  // and bodies, per level and per run; counts per phase and level, then per run
+  bool touch_on = false;
+  std::vector<int32_t> host_tree_off;
+  DBuf<long long> touch_base;
+  DBuf<unsigned> touch_lv, touch_run, touch_lvb, touch_runb;
+  DBuf<unsigned long long> touch_acc;
+  long long touch_words = 0, touch_body_words = 0;
+  std::vector<unsigned long long> touch_host;
   // profiling
   bool profiling = false;
   long long k_launch[K_NUM] = {};
@@ -2963,7 +3048,63 @@ unsigned long long grids_hash(const d2_ctx* c, int discrete, int levels) {
   }
   return h;
 }
+// Accounting runs (eager only): counts the (body, node) records this level
+// tests into touch_acc[phase][level], and marks them in the run's bitmaps.
+constexpr int TOUCH_FIELDS = 4;  // inner nodes, leaf nodes, leaf triangles, bodies
+constexpr int TOUCH_RUN = 2 * (MAX_LEVELS + 1) * TOUCH_FIELDS;  // offset of the run's union
+void touch_level(d2_ctx* c, int level, int discrete) {
+  (void)hipMemsetAsync(c->touch_lv.p, 0, sizeof(unsigned) * c->touch_words, c->stream);
+  (void)hipMemsetAsync(c->touch_lvb.p, 0, sizeof(unsigned) * c->touch_body_words, c->stream);
+  TouchArgs T;
+  T.soup = c->soup_list[level & 1].p, T.conn = c->conn_list[level & 1].p, T.lvl = c->lvl.p;
+  T.np_hit_prev = (level & 1) ? c->np_hit.p : c->np_hit2.p;
+  T.pair_info = c->pair_info.p, T.node_base = c->touch_base.p;
+  T.lv_nodes = c->touch_lv.p, T.run_nodes = c->touch_run.p, T.lv_bodies = c->touch_lvb.p, T.run_bodies = c->touch_runb.p;
+  T.cap_np = c->cap, T.level = level, T.discrete = discrete;
+  hipLaunchKernelGGL(k_touch, dim3(grid_for(c, c->cap)), dim3(256), 0, c->stream, T);
+  unsigned long long* out = c->touch_acc.p + ((discrete ? 0 : 1) * (MAX_LEVELS + 1) + level) * TOUCH_FIELDS;
+  hipLaunchKernelGGL(k_touch_count, dim3(grid_for(c, c->touch_words)), dim3(256), 0, c->stream, c->touch_lv.p,
+                     c->touch_words, c->touch_lvb.p, c->touch_body_words, c->touch_base.p, c->n_bodies,
+                     c->body_tree.p, c->tree_off.p, c->nodes.p, out);
+}
+
+// Start of an accounting run: the (body, node) numbering of the current
+// bodies, cleared bitmaps and counts.
+int touch_begin(d2_ctx* c) {
+  std::vector<long long> base(c->n_bodies + 1, 0);
+  for (int b = 0; b < c->n_bodies; ++b) {
+    const int t = c->host_body_tree[b];
+    base[b + 1] = base[b] + (c->host_tree_off[t + 1] - c->host_tree_off[t]);
+  }
+  c->touch_words = std::max<long long>((base[c->n_bodies] + 31) / 32, 1);
+  c->touch_body_words = std::max<long long>((c->n_bodies + 31) / 32, 1);
+  RC(ensure(c, c->touch_base, base.size()));
+  RC(ensure(c, c->touch_lv, c->touch_words)); RC(ensure(c, c->touch_run, c->touch_words));
+  RC(ensure(c, c->touch_lvb, c->touch_body_words)); RC(ensure(c, c->touch_runb, c->touch_body_words));
+  RC(ensure(c, c->touch_acc, TOUCH_RUN + TOUCH_FIELDS));
+  HIPCHK(c, hipMemcpyAsync(c->touch_base.p, base.data(), sizeof(long long) * base.size(), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemsetAsync(c->touch_run.p, 0, sizeof(unsigned) * c->touch_words, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->touch_runb.p, 0, sizeof(unsigned) * c->touch_body_words, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->touch_acc.p, 0, sizeof(unsigned long long) * (TOUCH_RUN + TOUCH_FIELDS), c->stream));
+  return D2_OK;
+}
+
+// End of an accounting run: the union's counts, and every count to the host.
+int touch_end(d2_ctx* c) {
+  hipLaunchKernelGGL(k_touch_count, dim3(grid_for(c, c->touch_words)), dim3(256), 0, c->stream, c->touch_run.p,
+                     c->touch_words, c->touch_runb.p, c->touch_body_words, c->touch_base.p, c->n_bodies,
+                     c->body_tree.p, c->tree_off.p, c->nodes.p, c->touch_acc.p + TOUCH_RUN);
+  HIPCHK(c, hipGetLastError());
+  c->touch_host.assign(TOUCH_RUN + TOUCH_FIELDS, 0ull);
+  HIPCHK(c, hipMemcpyAsync(c->touch_host.data(), c->touch_acc.p, sizeof(unsigned long long) * c->touch_host.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
 void enqueue_level(d2_ctx* c, int level, int discrete) {
+  if (c->touch_on && !c->capturing) touch_level(c, level, discrete);
   Dev D = make_dev(c, level, discrete);
   const LevelGrids G = level_grids(c, discrete, level);
   const int cg = G.cull, eg = G.expand, sg = G.stream;
@@ -3412,10 +3553,13 @@ int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out)
   if (kind == RUN_STEP) RC(ensure_clusters(c, c->n_clusters));
   c->ccd_valid = false;
   c->last_mode = -1;
-  const bool graph = c->graph_on && graph_ready(c, kind, filter);
+  // accounting runs are eager: their extra kernels sit between the levels
+  const bool graph = c->graph_on && !c->touch_on && graph_ready(c, kind, filter);
+  if (c->touch_on) RC(touch_begin(c));
   if (graph) RC(run_graph(c, kind, filter, dt));
   else RC(run_eager(c, kind, filter, dt));
   RC(finish_run(c, kind, filter));
+  if (c->touch_on) RC(touch_end(c));
   if (graph && graph_cut_short(c, kind, filter)) {
     RC(run_eager(c, kind, filter, dt));
     RC(finish_run(c, kind, filter));
@@ -3701,6 +3845,7 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->n_trees = n_trees;
   c->tree_depth = depth;
+  c->host_tree_off.assign(tree_node_offset, tree_node_offset + n_trees + 1);
   c->pair_depth = std::min(MAX_LEVELS - 1, 2 * *std::max_element(depth.begin(), depth.end()));  // until pairs come
   // body records name trees by index: bodies, pairs and clusters must come again
   c->host_body_tree.clear();
@@ -4261,6 +4406,21 @@ int d2_set_learned(d2_ctx* c, int on, float t_gate, float s_gate) {
   c->gate_t = t_gate;
   c->gate_s = s_gate;
   return D2_OK;
+}
+
+int d2_set_touch_stats(d2_ctx* c, int on) {
+  if (!c) return D2_ERR_ARG;
+  c->touch_on = on != 0;
+  c->touch_host.clear();
+  return D2_OK;
+}
+
+int d2_touch_stats(const d2_ctx* c, int64_t* out, int n) {
+  if (!c) return D2_ERR_ARG;
+  const int total = TOUCH_RUN + TOUCH_FIELDS;
+  if (out)
+    for (int i = 0; i < std::min(n, total); ++i) out[i] = i < (int)c->touch_host.size() ? (int64_t)c->touch_host[i] : 0;
+  return total;
 }
 
 int d2_set_profiling(d2_ctx* c, int on) {

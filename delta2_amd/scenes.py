@@ -127,18 +127,15 @@ class Scene:
         return self._n_tris
 
     def save(self, path):
-        offs = np.zeros(len(self.trees) + 1, dtype=np.int64)
-        offs[1:] = np.cumsum([t[0].shape[0] for t in self.trees])
-        np.savez(path, name=self.name, offs=offs, nodes=np.concatenate([t[0] for t in self.trees]),
-                 buckets=np.concatenate([t[1] for t in self.trees]), records=self.records, radii=self.radii,
-                 scene_of_body=self.scene_of_body, pairs=self.pairs, n_tris=self.n_triangles(),
+        t = self.trees if isinstance(self.trees, FlatTrees) else FlatTrees.from_list(list(self.trees))
+        np.savez(path, name=self.name, offs=t.offs, nodes=t.nodes, buckets=t.buckets, records=self.records,
+                 radii=self.radii, scene_of_body=self.scene_of_body, pairs=self.pairs, n_tris=self.n_triangles(),
                  box=getattr(self, "box", 0.0))
 
     @staticmethod
     def load(path):
         z = np.load(path)
-        offs, nodes, buckets = z["offs"], z["nodes"], z["buckets"]
-        trees = [(nodes[offs[i]:offs[i + 1]], buckets[offs[i]:offs[i + 1]]) for i in range(offs.size - 1)]
+        trees = FlatTrees(z["offs"].astype(np.int32), z["nodes"], z["buckets"])
         s = Scene(str(z["name"]), [], trees, z["records"], z["radii"], z["scene_of_body"])
         s.pairs = z["pairs"]
         s._n_tris = int(z["n_tris"])
@@ -322,6 +319,66 @@ def scene_c2(seed=1002, n_bodies=4096, packing=0.30, vel_sigma=2.0):
     return s
 
 
+class FlatTrees:
+    """Many trees in the flat layout d2_trees_upload takes: tree t owns node
+    and bucket records offs[t]..offs[t+1].  Indexing gives (nodes, buckets)
+    views, so it stands in for a list of trees."""
+
+    def __init__(self, offs, nodes, buckets):
+        self.offs, self.nodes, self.buckets = offs, nodes, buckets
+
+    def __len__(self):
+        return self.offs.size - 1
+
+    def __getitem__(self, t):
+        if isinstance(t, slice):
+            return [self[i] for i in range(*t.indices(len(self)))]
+        a, b = self.offs[t], self.offs[t + 1]
+        return self.nodes[a:b], self.buckets[a:b]
+
+    def __iter__(self):
+        return (self[t] for t in range(len(self)))
+
+    @staticmethod
+    def from_list(trees):
+        """Packs a list of trees, freeing each list entry as it is copied (the
+        list then holds None), so the peak stays near one copy."""
+        offs = np.zeros(len(trees) + 1, dtype=np.int32)
+        offs[1:] = np.cumsum([t[0].shape[0] for t in trees])
+        nodes = np.empty((int(offs[-1]), 8), dtype=np.int32)
+        buckets = np.empty((int(offs[-1]), 88))
+        for k in range(len(trees)):
+            nodes[offs[k]:offs[k + 1]], buckets[offs[k]:offs[k + 1]] = trees[k]
+            trees[k] = None
+        return FlatTrees(offs, nodes, buckets)
+
+
+def build_trees_flat(mesh_of, n, stream_base, threads=None, progress=None, first=None):
+    """Trees of meshes mesh_of(0..n-1) on a thread pool, packed flat.  Mesh k
+    uses its own PCG32 stream (42, stream_base + k), so the result does not
+    depend on the thread count; ``first`` optionally prepends a prebuilt tree.
+    The meshes are made on the fly and not kept.  progress(msg) is called
+    about every 10% of the trees."""
+    import os
+    import time
+    from concurrent.futures import ThreadPoolExecutor
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+
+    def one(k):
+        V, F = mesh_of(k)
+        return tree_build(V, F, pcg_seed(42, stream_base + k))
+
+    trees = [first] if first is not None else []
+    t0 = time.time()
+    step = max(1, n // 10)
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        for k, t in enumerate(ex.map(one, range(n), chunksize=64)):
+            trees.append(t)
+            if progress and (k + 1) % step == 0:
+                progress(f"trees {k + 1}/{n} in {time.time() - t0:.0f}s")
+    return FlatTrees.from_list(trees)
+
+
 def build_trees_parallel(meshes, stream_base, threads=None):
     """Trees of many meshes on a thread pool (the library call releases the
     GIL); mesh k uses its own PCG32 stream (42, stream_base + k), so the
@@ -439,17 +496,20 @@ def funnel_distance(p, r_top=20.0, r_bottom=4.0, height=40.0):
     return np.linalg.norm(q - t[:, None] * ab, axis=1)
 
 
-def scene_c5(seed=1005, n_bodies=65536, threads=None, density=0.1):
-    """Large pile: n_bodies C1 meshes (subdivision-2 icospheres scaled
-    U(0.5,1.5), 320 triangles each) at the C1 number density (0.1 per m^3)
+def scene_c5(seed=1005, n_bodies=65536, threads=None, density=0.1, subdiv=3, progress=None):
+    """Large pile: n_bodies icospheres scaled U(0.5,1.5) -- subdivision 3
+    (1280 triangles, 84 M in all at 65536 bodies) by default, subdivision 2
+    (the 320 of C1) on request -- at the C1 number density (0.1 per m^3)
     inside a static 40 m funnel (top radius 20 m, outlet radius 4 m) and in
     the column of radius 20 m above it, falling at v ~ N(0,1) + (0,0,-3) m/s.
     Body 0 is the funnel (static, 32768 triangles).  A body pairs with the
     funnel when its bounding sphere comes within reach of the cone surface
     (what a triangle BVH broad phase decides); body pairs use the swept
-    bounding-sphere test."""
+    bounding-sphere test.  Trees use one PCG32 stream per mesh (42, 40000 +
+    body), built in parallel and packed flat (FlatTrees); the meshes are not
+    kept."""
     rng = np.random.default_rng(seed)
-    V0, F0 = icosphere(2, 1.0)
+    V0, F0 = icosphere(subdiv, 1.0)
     _, _, I0 = mass_properties(V0, F0)
     r0 = float(np.max(np.linalg.norm(V0, axis=1)))
     funnel_v = np.pi * 40.0 / 3.0 * (20.0 ** 2 + 20.0 * 4.0 + 4.0 ** 2)
@@ -466,19 +526,22 @@ def scene_c5(seed=1005, n_bodies=65536, threads=None, density=0.1):
         pos[k:k + take] = c[:take]
         k += take
     FV, FF = funnel_mesh()
-    meshes = [(FV, FF)]
     recs = [body_record(np.array([1.0, 0.0, 0.0, 0.0]), np.zeros(3), np.zeros(3), np.zeros((3, 3)), static=True)]
     radii = [float(np.max(np.linalg.norm(FV, axis=1)))]
+    scale = np.zeros(n_bodies + 1)
     for i in range(n_bodies):
         sc = rng.uniform(0.5, 1.5)
-        meshes.append((V0 * sc, F0))
+        scale[i + 1] = sc
         v = rng.normal(0.0, 1.0, size=3) + np.array([0.0, 0.0, -3.0])
         recs.append(body_record(random_rotation(rng), pos[i], v, np.linalg.inv(I0 * sc ** 5)))
         radii.append(r0 * sc)
-    trees = build_trees_parallel(meshes, 40000, threads)
+    trees = build_trees_flat(lambda k: (FV, FF) if k == 0 else (V0 * scale[k], F0), n_bodies + 1, 40000, threads,
+                             progress)
     rec = np.array(recs)
     radii = np.array(radii)
-    s = Scene("c5", meshes, trees, rec, radii)
+    s = Scene("c5", [], trees, rec, radii)
+    s._n_tris = int(FF.shape[0] + n_bodies * F0.shape[0])
+    s.subdiv = subdiv
     s.pairs = c5_pairs(rec, radii)
     return s
 

@@ -319,6 +319,16 @@ int d2_reset_stats(d2_ctx* ctx);
  * contact count in the last field of level 0).  Copies min(n, total) values;
  * returns the total. */
 int d2_run_info(const d2_ctx* ctx, int64_t* info, int n);
+/* Algorithmic-byte accounting (no reference counterpart: instrumentation).
+ * When on, runs are eager and every level also records which (body, node)
+ * records its traversal tests (a discrete node pair whose parent missed is
+ * not tested).  d2_touch_stats copies, for the last run, per phase (0
+ * discrete, 1 CCD) and level 0..16 four counts -- distinct inner nodes,
+ * distinct leaf nodes, the triangles of those leaves, distinct bodies --
+ * then the same four counts over the run's union of levels and phases
+ * (144 values).  Copies min(n, total) values; returns the total. */
+int d2_set_touch_stats(d2_ctx* ctx, int on);
+int d2_touch_stats(const d2_ctx* ctx, int64_t* out, int n);
 /* Synchronise the context's stream. */
 int d2_sync(d2_ctx* ctx);
 

@@ -139,14 +139,17 @@ def bucket_discrete(a, b, na, nb, Ta, Tb):
     return out
 
 
-def compare_trees_full(a, b):
+def compare_trees_full(a, b, tested=None):
     """Depth-first restatement of compareTreesFull; returns (ints[n,8], reals[n,11])
-    in the reference's order (pair column 0)."""
+    in the reference's order (pair column 0).  tested, a list, receives
+    (level, node a, node b) of every node pair whose buckets are compared."""
     Ta = d2o.state_transformation(a.rec[14:28])
     Tb = d2o.state_transformation(b.rec[14:28])
     found = []
 
     def visit(na, nb, path):
+        if tested is not None:
+            tested.append((len(path), na, nb))
         hits = bucket_discrete(a, b, na, nb, Ta, Tb)
         if not hits:
             return
@@ -256,7 +259,7 @@ def ccd_frames(a, b, max_time):
             d2o.state_transformation(b.rec[0:14]), d2o.state_transformation(fb))
 
 
-def compare_trees_full_continuous(a, b, max_time):
+def compare_trees_full_continuous(a, b, max_time, tested=None):
     """Depth-first restatement of compareTreesFullContinuous; returns
     (ints, reals, min_time_out).  The roots are a soup pair whatever they are;
     below them a pair of two leaves is a connected pair, every other a soup
@@ -269,6 +272,8 @@ def compare_trees_full_continuous(a, b, max_time):
     pass_leaf = set()  # passes in which a leaf-leaf node pair hit
 
     def visit(na, nb, path, root):
+        if tested is not None:
+            tested.append((len(path), na, nb))
         lst = 0 if (root or a.inner(na) or b.inner(nb)) else 1
         key = (len(path), lst)
         hits = (bucket_soup if lst == 0 else bucket_connected)(a, b, na, nb, T)
