@@ -66,7 +66,7 @@ def step_alg_bytes(run_counts, n_pairs, raw_hits, P=P64):
 
 
 # phase whose levels each traversal-cull kernel runs (0 discrete, 1 CCD)
-KERNEL_PHASE = {"tri_mask": 0, "soup_cull": 1}
+KERNEL_PHASE = {"tri_mask": 0, "tri_full": 0, "soup_cull": 1}
 
 
 def roofline_of(stats, per_level, traffic_key):
@@ -173,6 +173,47 @@ def same_results(a, b):
     return all(np.array_equal(x, y) for x, y in zip(a, b))
 
 
+def tune_step(ctx, learned, budget_s, tag):
+    """Coordinate descent over the launch shapes (d2_set_tune) -- the
+    candidates of the library's d2_autotune, one sweep -- timing one
+    d2_step_run per candidate on the resident inputs and logging each, until
+    the time budget is spent.  Leaves the best shape set."""
+    from delta2_amd import scenes
+    t_start = time.time()
+    best = ctx.get_tune()
+    u = max(1, best[0] // 4)  # the default shape is (4, 8, 16) blocks per CU
+
+    def run(shape):
+        ctx.set_tune(*shape)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step_run(scenes.DT, True)
+        ctx.sync()
+        return 1e3 * (time.perf_counter() - t0)
+
+    best_ms = run(best)
+    log(f"[{tag}] tune {best}: {best_ms:.1f} ms")
+    dims = [[u, 2 * u, 4 * u, 8 * u, 16 * u], [2 * u, 4 * u, 8 * u, 16 * u, 32 * u], [4 * u, 16 * u, 64 * u]]
+    if learned:
+        dims.append([1, 2, 3])
+    for d, vals in enumerate(dims):
+        for v in vals:
+            if time.time() - t_start > budget_s:
+                log(f"[{tag}] tune budget of {budget_s:.0f}s spent")
+                ctx.set_tune(*best)
+                return best, best_ms
+            shape = list(best)
+            if shape[d] == v or v > (4096 if d == 0 else 16384):
+                continue
+            shape[d] = v
+            ms = run(tuple(shape))
+            log(f"[{tag}] tune {tuple(shape)}: {ms:.1f} ms")
+            if ms < best_ms:
+                best, best_ms = tuple(shape), ms
+    ctx.set_tune(*best)
+    return best, best_ms
+
+
 # --- the large pile (default) ------------------------------------------------------
 class PileStepper:
     """One context stepping the pile: per step, the body states are uploaded,
@@ -253,12 +294,14 @@ def run_pile(args):
     # picks launch shapes on its resident inputs), then the K timed steps
     fast = PileStepper(sc, prec, weights)
     log(f"[c5] trees uploaded in {fast.upload_s:.1f}s")
-    fast.step(states[0])
+    n, s = fast.step(states[0])
+    log(f"[c5] first step (sizes the buffers): {n} pairs, {1e3 * s:.1f} ms")
     if not args.no_autotune:
-        shape, ms = fast.ctx.autotune("step", reps=1)
-        log(f"[c5] autotuned {shape}: {ms:.1f} ms")
+        shape, ms = tune_step(fast.ctx, weights is not None, args.tune_budget, "c5")
+        log(f"[c5] tuned {shape}: {ms:.1f} ms")
     for k in range(1, W):
-        fast.step(states[k])
+        n, s = fast.step(states[k])
+        log(f"[c5] warm-up step {k}: {n} pairs, {1e3 * s:.1f} ms")
     pairs_t, secs, ref = [], [], []
     for k in range(W, W + K):
         n, s = fast.step(states[k])
@@ -285,6 +328,7 @@ def run_pile(args):
     fast.ctx.step_run(scenes.DT, True)
     fast.ctx.set_touch_stats(False)
     per_level, _ = fast.ctx.touch_stats()
+    log("[c5] accounting step done")
     tag = f"c5/{args.precision}/{'learned' if weights is not None else 'plain'}"
     roofline = roofline_of(stats, per_level, tag)
     log(f"[c5] roofline {roofline['kernel']}: {roofline['achieved']:.0f} GB/s ({100 * roofline['frac']:.1f}% of HBM)")
@@ -294,6 +338,7 @@ def run_pile(args):
     fast.ctx.step_run(scenes.DT, True)
     fast.ctx.result(with_pairs=True)
     e2e_ms = 1e3 * (time.perf_counter() - t0)
+    log(f"[c5] end to end: {e2e_ms:.1f} ms")
     # the CPU baseline's inputs: the first timed step's pairs and future states
     fast.prepare(states[W])
     cpu_pairs, cpu_mt = fast.ctx.download_pairs()
@@ -307,7 +352,9 @@ def run_pile(args):
     step_bytes = None
     if not args.no_plain:
         plain = PileStepper(sc, d2.D2_FP64)
-        plain.step(states[0])
+        log(f"[c5] plain fp64: trees uploaded in {plain.upload_s:.1f}s")
+        n, s = plain.step(states[0])
+        log(f"[c5] plain fp64 first step (sizes the buffers): {1e3 * s:.1f} ms")
         psecs, ppairs, identical = [], [], True
         for k in range(W, W + n_plain):
             n, s = plain.step(states[k])
@@ -616,6 +663,7 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=3, help="c5: timed steps of the plain FP64 path (0: all)")
     ap.add_argument("--c2", type=int, default=1, help="c5: also measure the 4096-body workload (extra key)")
     ap.add_argument("--c2-deadline", type=float, default=330.0, help="c5: skip the 4096-body figures after this many s")
+    ap.add_argument("--tune-budget", type=float, default=45.0, help="c5: seconds spent tuning launch shapes")
     ap.add_argument("--scenes", type=int, default=256, help="c3: number of independent scenes")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32"])
     ap.add_argument("--learned", default="on", choices=["on", "off"],

@@ -120,6 +120,10 @@ def lib():
         "d2_clusters_download": ([_vp, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
         "d2_set_touch_stats": ([_vp, ctypes.c_int], ctypes.c_int),
         "d2_touch_stats": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
+        "d2_find_contacts_bucket": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int64, _ip, _dp,
+                                     _i64p], ctypes.c_int),
+        "d2_filter_contacts": ([_vp, ctypes.c_int, _dp, ctypes.c_double, _ip, _ip, ctypes.POINTER(ctypes.c_int)],
+                               ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -421,6 +425,37 @@ class Context:
 
     def sync(self):
         self._check(lib().d2_sync(self.h))
+
+    def find_contacts_bucket(self, body_a, body_b, node_pairs, max_hits=None, entry="d2_find_contacts_bucket"):
+        """findContactsBucketComparison over node pairs (node of a's tree,
+        node of b's tree) at the bodies' future states: (pair_used[n],
+        ints[h, 8], reals[h, 11]).  Raises CapacityError when more than
+        max_hits hits exist (self.last_hits holds the count)."""
+        nps = _i32(node_pairs).reshape(-1, 2)
+        n = nps.shape[0]
+        cap = max(64 * n, 1) if max_hits is None else int(max_hits)
+        used = np.zeros(max(n, 1), dtype=np.int32)
+        ints = np.zeros((max(cap, 1), 8), dtype=np.int32)
+        reals = np.zeros((max(cap, 1), 11))
+        hits = ctypes.c_int64(0)
+        rc = getattr(lib(), entry)(self.h, int(body_a), int(body_b), n, _iptr(nps), _iptr(used), cap, _iptr(ints),
+                                   _dptr(reals), ctypes.byref(hits))
+        self.last_hits = hits.value
+        self._check(rc)
+        h = hits.value
+        return used[:n], ints[:h], reals[:h]
+
+    def filter_contacts(self, reals, margin=0.0):
+        """filterContacts(unfiltered, margin) on the device: (sorted order of
+        the inputs, kept inputs in sorted order)."""
+        r = _f64(reals).reshape(-1, 11)
+        n = r.shape[0]
+        order = np.zeros(max(n, 1), dtype=np.int32)
+        kept = np.zeros(max(n, 1), dtype=np.int32)
+        k = ctypes.c_int(0)
+        self._check(lib().d2_filter_contacts(self.h, n, _dptr(r), float(margin), _iptr(order), _iptr(kept),
+                                             ctypes.byref(k)))
+        return order[:n], kept[:k.value]
 
     def set_touch_stats(self, on=True):
         """Accounting runs: record the (body, node) records each level tests."""

@@ -36,7 +36,6 @@
 #include <hip/hip_runtime.h>
 
 #include <hip/hip_ext.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <array>
@@ -270,14 +269,16 @@ struct ClusterAcc {
   float dt;
   unsigned long long* step_bits;  // null: nothing to do
   int *first_any, *first_ns;
+  int* read_later;  // per body: its step is read or changed after its cluster's first pair
   const double* bodies;
   double* body_step;
   const int32_t *pairs, *cluster_of_pair;
 };
 __device__ __forceinline__ void cluster_init_range(const ClusterAcc& A, long long t0, long long stride) {
-  const int n = A.body_step && A.n_bodies > A.n_clusters ? A.n_bodies : A.n_clusters;
+  const int n = A.n_bodies > A.n_clusters ? A.n_bodies : A.n_clusters;
   for (long long k = t0; k < n; k += stride) {
     if (k < A.n_clusters) A.step_bits[k] = ~0ull, A.first_any[k] = 0x7fffffff, A.first_ns[k] = 0x7fffffff;
+    if (k < A.n_bodies) A.read_later[k] = 0;
     if (A.body_step && k < A.n_bodies) {
       double s = A.bodies[40 * (size_t)k + 39];
       const double hi = (double)A.dt;
@@ -1211,6 +1212,117 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   }
 }
 
+// FP64 discrete levels in one pass, with no item list (the reference's own
+// form: every triangle pair of a node pair tested in place,
+// full_tree_comparison.h:26-62 with bucketCompare, discrete_comparison.h).
+// A wavefront takes 64 node pairs at a time and packs their valid triangle
+// pairs (n_a * n_b each) densely over its lanes: a prefix sum over the counts,
+// then each lane finds the node pair of its triangle pair by binary search
+// over the sums.  The FP64 test, the contact record and its emission key are
+// k_tri_dist's, so the contacts are the same bit for bit; hits set the node
+// pair's flag and expansion descriptor for k_disc_expand.
+struct FullSlot {
+  int pair, ga, gb, ntb, excl, inner;
+  long long w;
+};
+__global__ void __launch_bounds__(256) k_tri_full(Dev D) {
+  __shared__ FullSlot s_np[4][64];
+  __shared__ int s_hit[4][64];
+  __shared__ unsigned long long s_tested;
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const long long n = D.n_soup();
+  const long long n_groups = (n + 63) / 64;
+  const long long nwaves = (long long)gridDim.x * (blockDim.x >> 6);
+  if (threadIdx.x == 0) s_tested = 0ull;
+  __syncthreads();
+  unsigned long long tested = 0ull;  // kept by lane 0
+  for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < n_groups; g += nwaves) {
+    const long long w = g * 64 + lane;
+    int cnt = 0, desc = 0;
+    FullSlot f{0, 0, 0, 1, 0, 0, w};
+    if (w < n) {
+      const NP np = D.soup[w];
+      // a child of a parent that did not hit: no work, no children
+      if (np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) {
+        const int32_t* pi = D.pair_info + 4 * np.pair;
+        const NodeRec na = D.nodes[pi[2] + np.na];
+        const NodeRec nb = D.nodes[pi[3] + np.nb];
+        f.pair = (int)np.pair, f.ga = pi[2] + na.bucket, f.gb = pi[3] + nb.bucket;
+        f.ntb = nb.ntris > 0 ? nb.ntris : 1;
+        f.inner = (na.inner || nb.inner) ? 1 : 0;
+        cnt = na.ntris * nb.ntris;
+        desc = expansion_desc(na, nb);
+      }
+    }
+    const int incl = wave_incl_scan(cnt);
+    f.excl = incl - cnt;
+    s_np[wid][lane] = f;
+    s_hit[wid][lane] = 0;
+    const int total = rl(incl, 63);
+    tested += (unsigned long long)total;
+    wave_sync_lds();
+    for (int t0 = 0; t0 < total; t0 += 64) {
+      const int t = t0 + lane;
+      // the owning node pair: how many inclusive sums are <= t (every lane
+      // takes part in the shuffles)
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (__shfl(incl, k + step - 1) <= t) k += step;
+      bool hit = false, emit = false;
+      int i = 0, j = 0;
+      v3<double> P{}, Q{};
+      double ea = 0.0, eb = 0.0;
+      const FullSlot& o = s_np[wid][k];
+      if (t < total) {
+        const int s = t - o.excl;
+        i = s / o.ntb, j = s - i * o.ntb;
+        const double* pr = D.pair_rec + (size_t)PR_STRIDE * o.pair;
+        const double* ba = D.buckets + (size_t)BUCKET * o.ga;
+        const double* bb = D.buckets + (size_t)BUCKET * o.gb;
+        const tri<double> A = xform(pr + PR_TA1, load_tri<double>(ba, i));
+        const tri<double> B = xform(pr + PR_TB1, load_tri<double>(bb, j));
+        const double d = tri_tri_dist(A, B, P, Q);
+        ea = ba[72 + i], eb = bb[72 + j];
+        hit = d < (pr[PR_SEARCH] + ea) + eb;
+        if (hit) s_hit[wid][k] = 1;
+        emit = hit && !o.inner;  // an inner node: the hit only expands the node pair
+      }
+      const long long slot = wave_alloc(emit, &D.cnt[C_CONTACTS]);
+      if (!emit) continue;
+      if (slot >= D.cap_contacts) {
+        atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+        continue;
+      }
+      atomicAdd(&D.pair_cnt[o.pair], 1);
+      const NP np = D.soup[o.w];
+      const double* pr = D.pair_rec + (size_t)PR_STRIDE * o.pair;
+      const double* ba = D.buckets + (size_t)BUCKET * o.ga;
+      const double* bb = D.buckets + (size_t)BUCKET * o.gb;
+      ContactRec c;
+      c.key = make_key(D.level, 0, np.key, 8 * i + j);
+      c.pair = (int32_t)np.pair, c.na = (int32_t)np.na, c.nb = (int32_t)np.nb, c.kind = 0;
+      c.ia = i, c.ib = j, c.level = D.level, c.pad = 0;
+      c.r[0] = P.x, c.r[1] = P.y, c.r[2] = P.z, c.r[3] = Q.x, c.r[4] = Q.y, c.r[5] = Q.z;
+      c.r[6] = pr[PR_GEOA] + ea, c.r[7] = pr[PR_GEOB] + eb, c.r[8] = ba[80 + i], c.r[9] = bb[80 + j], c.r[10] = 0.0;
+      D.contacts[slot] = c;
+    }
+    wave_sync_lds();
+    if (w < n) {
+      const int h = s_hit[wid][lane];
+      D.np_hit[w] = h;
+      if (h) D.exp_desc[w] = desc;
+    }
+    wave_sync_lds();  // s_np and s_hit are rewritten by the next group
+  }
+  if (lane == 0 && tested) atomicAdd(&s_tested, tested);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_tested) {
+    atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_ITEMS], s_tested);
+    atomicAdd(&D.cnt[C_LVL_ITEMS], s_tested);
+  }
+}
+
 // CCD levels: children of hit node pairs (full_tree_comparison.h:26-62,
 // 99-150) and the sub-tests of the connected (leaf-leaf) list, in two chunked
 // passes: counts + block partials, then writes at scanned offsets, which
@@ -1633,6 +1745,139 @@ __global__ void __launch_bounds__(256) k_touch_count(const unsigned* nodes_bits,
   }
 }
 
+// ---------------------------------------------------- bucket-level entry
+// The plugin point's unit of work (strategy::ContactDetectionStrategy::
+// findContactsBucketComparison, contact_detection_strategy.h:11, and its
+// implementation comparison_dist.h:205-240): a list of node pairs of two
+// bodies compared at the bodies' future states.  One wavefront per node pair,
+// lane 8 i + j tests triangle i of a against triangle j of b in FP64
+// (tri_tri_dist); a node pair's hits take one slot range (one atomic), and
+// k_bucket_scan / k_bucket_place put them in the reference's order: node-pair
+// order, then i, then j -- the lane order.
+struct BucketArgs {
+  const int32_t* np_list;  // [n][2] node indices within the two trees
+  int n;
+  const NodeRec* nodes;
+  const double* buckets;
+  int off_a, off_b;        // first node / bucket record of each tree
+  const double* body_a;    // body records (future state at [14..27])
+  const double* body_b;
+  ContactRec* hits;        // hit records at provisional slots
+  long long cap;
+  unsigned long long* counter;  // slots taken
+  long long* base;         // per node pair, its first slot
+  int* count;              // per node pair, its hits
+};
+__global__ void __launch_bounds__(256) k_bucket_pairs(BucketArgs A) {
+  __shared__ double sT[32];  // future poses of a and b (basic_utils.h:79-86)
+  if (threadIdx.x == 0) {
+    const double* sa = A.body_a + 14;
+    const double* sb = A.body_b + 14;
+    pose(quat{sa[0], sa[1], sa[2], sa[3]}, v3<double>{sa[4], sa[5], sa[6]}, sT);
+    pose(quat{sb[0], sb[1], sb[2], sb[3]}, v3<double>{sb[4], sb[5], sb[6]}, sT + 16);
+  }
+  __syncthreads();
+  const int lane = lane_id();
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long n_waves = ((long long)gridDim.x * blockDim.x) >> 6;
+  const double geo_a = A.body_a[37], geo_b = A.body_b[37];
+  const double search = (double)(float)(geo_a + geo_b);  // float search distance (comparison_dist.h:207)
+  const int i = lane >> 3, j = lane & 7;
+  for (long long k = wave; k < A.n; k += n_waves) {
+    const int la = A.np_list[2 * k], lb = A.np_list[2 * k + 1];
+    const NodeRec na = A.nodes[A.off_a + la];
+    const NodeRec nb = A.nodes[A.off_b + lb];
+    const double* ba = A.buckets + (size_t)BUCKET * (A.off_a + na.bucket);
+    const double* bb = A.buckets + (size_t)BUCKET * (A.off_b + nb.bucket);
+    bool hit = false;
+    v3<double> P{0, 0, 0}, Q{0, 0, 0};
+    if (i < na.ntris && j < nb.ntris) {
+      const tri<double> TA = xform(sT, load_tri<double>(ba, i));
+      const tri<double> TB = xform(sT + 16, load_tri<double>(bb, j));
+      const double d = tri_tri_dist(TA, TB, P, Q);
+      hit = d < (search + ba[72 + i]) + bb[72 + j];
+    }
+    const unsigned long long mask = __ballot(hit);
+    const int cnt = __popcll(mask);
+    unsigned long long b0 = 0;
+    if (lane == 0 && cnt) b0 = atomicAdd(A.counter, (unsigned long long)cnt);
+    b0 = readlane_u64(b0, 0);
+    if (lane == 0) A.base[k] = (long long)b0, A.count[k] = cnt;
+    if (hit) {
+      const long long slot = (long long)b0 + __popcll(mask & lanes_below());
+      if (slot < A.cap) {
+        ContactRec c;
+        c.key = 0ull;
+        c.pair = 0, c.na = la, c.nb = lb, c.kind = 0, c.ia = i, c.ib = j, c.level = 0, c.pad = 0;
+        c.r[0] = P.x, c.r[1] = P.y, c.r[2] = P.z, c.r[3] = Q.x, c.r[4] = Q.y, c.r[5] = Q.z;
+        c.r[6] = geo_a + ba[72 + i], c.r[7] = geo_b + bb[72 + j];  // Contact(P, Q, geo + eps, ..., inner eps)
+        c.r[8] = ba[80 + i], c.r[9] = bb[80 + j], c.r[10] = 0.0;
+        A.hits[slot] = c;
+      }
+    }
+  }
+}
+
+// The standalone filter's input as one pair: contact i keeps input position i
+// as its emission key, so the per-pair finishing kernels see the caller's
+// order (the unstable sort's result depends on it).
+__global__ void __launch_bounds__(256) k_filter_load(int n, const double* reals, ContactRec* c, int* perm, int* pair_off,
+                                                     int* n_big) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    ContactRec r;
+    r.key = (unsigned long long)i;
+    r.pair = 0, r.na = 0, r.nb = 0, r.kind = 0, r.ia = i, r.ib = 0, r.level = 0, r.pad = 0;
+    for (int q = 0; q < 11; ++q) r.r[q] = reals[11 * (size_t)i + q];
+    c[i] = r;
+    perm[i] = i;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) pair_off[0] = 0, pair_off[1] = n, n_big[0] = 0, n_big[1] = 0;
+}
+
+// Exclusive offsets of the node pairs' hits (node-pair order) and the
+// reference's cumulative counts (pair_used_out[k] = hits up to and including
+// node pair k, comparison_dist.h:238); one block.
+__global__ void __launch_bounds__(1024) k_bucket_scan(int n, const int* count, long long* off, int32_t* used) {
+  __shared__ long long s[1024];
+  const int t = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int b = t * per, e = b + per < n ? b + per : n;
+  long long sum = 0;
+  for (int k = b; k < e; ++k) sum += count[k];
+  s[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {  // inclusive scan of the chunk sums
+    const long long v = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  long long run = t ? s[t - 1] : 0;
+  for (int k = b; k < e; ++k) {
+    off[k] = run;
+    run += count[k];
+    used[k] = (int32_t)run;
+  }
+}
+
+// Each node pair's hits from their provisional slots to their final rows, in
+// the flat contact layout (8 int32 + 11 doubles per hit).
+__global__ void __launch_bounds__(256) k_bucket_place(int n, const long long* base, const int* count,
+                                                      const long long* off, const ContactRec* hits, int32_t* out_i,
+                                                      double* out_r) {
+  const int lane = lane_id();
+  const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long n_waves = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long k = wave; k < n; k += n_waves) {
+    if (lane >= count[k]) continue;
+    const ContactRec& c = hits[base[k] + lane];
+    const long long row = off[k] + lane;
+    int32_t* oi = out_i + 8 * row;
+    oi[0] = c.pair, oi[1] = c.na, oi[2] = c.nb, oi[3] = c.kind, oi[4] = c.ia, oi[5] = c.ib, oi[6] = c.level, oi[7] = 0;
+    for (int q = 0; q < 11; ++q) out_r[11 * row + q] = c.r[q];
+  }
+}
+
 // -------------------------------------------------------- post-processing
 __device__ __forceinline__ long long n_contacts(const unsigned long long* cnt, long long cap) {
   long long n = (long long)cnt[C_CONTACTS];
@@ -1861,7 +2106,7 @@ __device__ __forceinline__ bool step_rule_scale(const ContactRec& ct, const Step
 }
 
 __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
-                                int* kept_cnt, const double* pair_rec, double* scale) {
+                                int* kept_cnt, const double* pair_rec, double* scale, double margin) {
   {
     int b = pair_off[p], e = pair_off[p + 1];
     {  // heapsort of perm[b, e) by the (unique) emission key
@@ -1887,14 +2132,14 @@ __device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off,
         sift(0, len);
       }
     }
-    if (mode == 1) {  // filterContacts(.., 0.0)
+    if (mode == 1) {  // filterContacts(.., margin)
       Sorter s{perm, c};
       s.sort(b, e);
       int kept = 0;
       for (int i = b; i < e; ++i) {
         const ContactRec& ct = c[perm[i]];
         v3<double> m1 = dvs(add(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}), 2.0);
-        double search = (ct.r[6] + ct.r[7]) + 0.0;
+        double search = (ct.r[6] + ct.r[7]) + margin;
         bool fresh = true;
         for (int h = b; h < i && fresh; ++h) {
           if (!keep[h]) continue;
@@ -2269,7 +2514,8 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
                                                            int mode, int* keep, int* kept_cnt,
                                                            const double* pair_rec, double* scale, int* over,
                                                            int* n_over, int* huge, int* n_huge, const int* tail,
-                                                           const int* n_tail, unsigned long long* out_total) {
+                                                           const int* n_tail, unsigned long long* out_total,
+                                                           double margin) {
   constexpr int SEGS = blk_segs(CAP);
   __shared__ unsigned long long skey[CAP];  // emission keys, then the partition lists of the tie replay
   __shared__ int sidx[CAP];
@@ -2369,7 +2615,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
         scale[p] = m;
       }
       for (int i = tid; i < n; i += 256) perm[b + i] = eo[i];
-    } else if (mode == 1) {  // filterContacts(.., 0.0)
+    } else if (mode == 1) {  // filterContacts(.., margin)
       for (int i = tid; i < n; i += 256) {
         const ContactRec& ct = c[eo[i]];
         const v3<double> A{ct.r[0], ct.r[1], ct.r[2]}, B{ct.r[3], ct.r[4], ct.r[5]};
@@ -2418,7 +2664,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
         const bool valid = t < n;
         const int pi = valid ? ord[t] : 0;
         const double mx = smid[3 * pi], my = smid[3 * pi + 1], mz = smid[3 * pi + 2];
-        const double search = sk[pi].eps + 0.0;
+        const double search = sk[pi].eps + margin;
         bool near = false;
         for (int q0 = 0; q0 < nk; q0 += 64) {  // contacts kept before the block
           const int h = q0 + lane < nk ? mark[q0 + lane] : 0;
@@ -2465,7 +2711,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
     if (tail) {  // pairs above every LDS capacity (listed by the first launch)
       if (mode == 1) {  // the filter's std::sort replay: one lane per pair
         for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x) {
-          pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale);
+          pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale, margin);
           if (out_total) atomicAdd(out_total, (unsigned long long)kept_cnt[tail[t]]);
         }
       } else {  // one block per pair
@@ -2558,12 +2804,22 @@ __global__ void __launch_bounds__(1024) k_scan_small(const int* in, int* out, lo
 }
 
 // ------------------------------------------------------ cluster step sizes
-// The cluster step (separate_clusters.cpp:541-598,
-// time_step_selection_dynamic_continuous.cpp:17-21) is a minimum over its
-// pairs' max_time*scale (pairs with CCD contacts) and bodies' last steps, the
-// first pair reading body a (:589-590); a minimum does not depend on the
-// order of the sequential loop.  Steps are non-negative, so they order like
-// their bit patterns.
+// The cluster step (separate_clusters.cpp:555-598,
+// time_step_selection_dynamic_continuous.cpp:17-21).  The reference runs two
+// loops over the cluster's pairs in order: the first lowers both dynamic
+// bodies' steps to use = min(max_time * scale, step[a], step[b]) for a pair
+// with CCD contacts; the second starts from step[a] of the first pair that is
+// not static-static (:589-590) and takes min(., step[id]) over the others (id
+// = a unless a is static).  Since max_time * scale <= max_time <= both last
+// steps, every pair's use reaches the step its own pair reads in the second
+// loop, so the result is the minimum over pairs of max_time * scale (with
+// contacts) and last[id] -- order does not matter -- except for the first
+// pair when its body a is static: the second loop reads a's unchanged last
+// step there, and that pair's use only counts when its body b is read or
+// lowered again later (a later pair whose id is b, or a later pair with
+// contacts touching b: every other pair comes later).  read_later marks
+// those bodies.  Steps are non-negative, so they order like their bit
+// patterns.
 __global__ void k_cluster_init(ClusterAcc CA) {
   cluster_init_range(CA, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
 }
@@ -2571,51 +2827,72 @@ __global__ void k_cluster_first(int n_pairs, ClusterAcc CA) {
   cluster_first_range(CA, n_pairs, (long long)blockIdx.x * blockDim.x + threadIdx.x, (long long)gridDim.x * blockDim.x);
 }
 __global__ void k_cluster_min(int n_pairs, const int32_t* pairs, const int32_t* cluster_of_pair, const double* bodies,
-                              const int* pair_cnt, const double* scale, const int* first_ns, unsigned long long* step_bits) {
+                              const int* pair_cnt, const double* scale, const int* first_ns, int* read_later,
+                              unsigned long long* step_bits) {
   for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n_pairs; p += gridDim.x * blockDim.x) {
     const int k = cluster_of_pair[p], a = pairs[2 * p], b = pairs[2 * p + 1];
     const double* ra = bodies + 40 * (size_t)a;
     const double* rb = bodies + 40 * (size_t)b;
     const bool sa = ra[38] != 0.0, sb = rb[38] != 0.0;
     if (sa && sb) continue;
+    const bool first = p == first_ns[k];
+    const int id = !sa ? a : b;
+    if (!first) {  // the second loop reads step[id]; a pair with contacts lowers a's and b's steps
+      read_later[id] = 1;
+      if (pair_cnt[p] > 0) read_later[a] = 1, read_later[b] = 1;
+    }
     const double mt = (rb[39] < ra[39]) ? rb[39] : ra[39];
     double v = 1e300;
-    if (pair_cnt[p] > 0) {
+    if (pair_cnt[p] > 0 && !(first && sa)) {  // a static first pair's own use: see cluster_step_of
       const double s = mt * scale[p];
       v = (s < v) ? s : v;
     }
-    const int id = !sa ? a : b;
-    const double last = p == first_ns[k] ? ra[39] : bodies[40 * (size_t)id + 39];
+    const double last = first ? ra[39] : bodies[40 * (size_t)id + 39];
     v = (last < v) ? last : v;
     atomicMin(&step_bits[k], (unsigned long long)__double_as_longlong(v));
   }
 }
+struct ClusterFinal {
+  const int32_t* pairs;
+  const double* bodies;
+  const unsigned long long* step_bits;
+  const int *first_any, *first_ns, *first_member, *pair_cnt, *read_later;
+  const double* scale;
+};
 // A cluster without pairs keeps its first particle's last step (:594-596);
-// first_member (clusters built on the device) names it, otherwise dt.
-__device__ __forceinline__ double cluster_step_of(int k, float dt, const int32_t* pairs, const double* bodies,
-                                                  const unsigned long long* step_bits, const int* first_any,
-                                                  const int* first_member) {
+// first_member (clusters built on the device) names it, otherwise dt.  A first
+// pair with a static body a adds its use only when its body b is read later.
+__device__ __forceinline__ double cluster_step_of(int k, float dt, const ClusterFinal& F) {
   double s;
-  if (step_bits[k] != ~0ull) s = __longlong_as_double((long long)step_bits[k]);
-  else if (first_any[k] != 0x7fffffff) s = bodies[40 * (size_t)pairs[2 * first_any[k]] + 39];
-  else if (first_member) s = bodies[40 * (size_t)first_member[k] + 39];
+  if (F.step_bits[k] != ~0ull) s = __longlong_as_double((long long)F.step_bits[k]);
+  else if (F.first_any[k] != 0x7fffffff) s = F.bodies[40 * (size_t)F.pairs[2 * F.first_any[k]] + 39];
+  else if (F.first_member) s = F.bodies[40 * (size_t)F.first_member[k] + 39];
   else s = (double)dt;
+  const int p0 = F.first_ns[k];
+  if (p0 != 0x7fffffff && F.pair_cnt[p0] > 0) {
+    const double* ra = F.bodies + 40 * (size_t)F.pairs[2 * p0];
+    const double* rb = F.bodies + 40 * (size_t)F.pairs[2 * p0 + 1];
+    if (ra[38] != 0.0 && F.read_later[F.pairs[2 * p0 + 1]]) {
+      const double mt = (rb[39] < ra[39]) ? rb[39] : ra[39];
+      const double u = mt * F.scale[p0];
+      s = (u < s) ? u : s;
+    }
+  }
   const double hi = (double)dt;
   s = (hi < s) ? hi : s;         // std::min(s, dt)
   return (s < 1e-3) ? 1e-3 : s;  // std::max(1e-3, .)
 }
 // Cluster steps, and with body_step the per-body steps of bodies in pairs
 // (their cluster's step, computed per pair from the same accumulators).
-__global__ void k_cluster_final(int n_clusters, float dt, const int32_t* pairs, const double* bodies,
-                                const unsigned long long* step_bits, const int* first_any, const int* first_member,
-                                double* out, int n_pairs, const int32_t* cluster_of_pair, double* body_step) {
+__global__ void k_cluster_final(int n_clusters, float dt, ClusterFinal F, double* out, int n_pairs,
+                                const int32_t* cluster_of_pair, double* body_step) {
   const int n = body_step && n_pairs > n_clusters ? n_pairs : n_clusters;
   for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-    if (k < n_clusters) out[k] = cluster_step_of(k, dt, pairs, bodies, step_bits, first_any, first_member);
+    if (k < n_clusters) out[k] = cluster_step_of(k, dt, F);
     if (body_step && k < n_pairs) {
-      const double s = cluster_step_of(cluster_of_pair[k], dt, pairs, bodies, step_bits, first_any, first_member);
-      body_step[pairs[2 * k]] = s;
-      body_step[pairs[2 * k + 1]] = s;
+      const double s = cluster_step_of(cluster_of_pair[k], dt, F);
+      body_step[F.pairs[2 * k]] = s;
+      body_step[F.pairs[2 * k + 1]] = s;
     }
   }
 }
@@ -2645,11 +2922,12 @@ namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
                               "leaf_point", "leaf_edge", "tri_mask", "tri_enum", "tri_dist", "post", "post_pairs",
-                              "cluster", "gate"};
+                              "cluster", "gate", "tri_full"};
 // K_TRI_ENUM launches nothing (the enumeration is part of k_tri_mask); its
 // item count is the discrete node pairs received, kept for instrumentation.
+// FP64 contexts run K_TRI_FULL (k_tri_full) in place of K_TRI_MASK + K_TRI_DIST.
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
-           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_NUM };
+           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_TRI_FULL, K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
@@ -2733,7 +3011,7 @@ struct d2_ctx {
   DBuf<double> out_r;
   // clusters
   DBuf<unsigned long long> step_bits;
-  DBuf<int> first_any, first_ns;
+  DBuf<int> first_any, first_ns, read_later;
   DBuf<double> cluster_step, body_step;
   DBuf<int32_t> tmp_cluster;
   // broad phase (per tree: float AABB of the mesh and bounding radius)
@@ -2796,6 +3074,13 @@ struct d2_ctx {
   DBuf<unsigned long long> touch_acc;
   long long touch_words = 0, touch_body_words = 0;
   std::vector<unsigned long long> touch_host;
+  // bucket-level entry and standalone filter
+  DBuf<int32_t> bk_np, bk_used, bk_out_i;
+  DBuf<long long> bk_base, bk_off;
+  DBuf<int> bk_count;
+  DBuf<ContactRec> bk_hits;
+  DBuf<double> bk_out_r;
+  DBuf<unsigned long long> bk_counter;
   // profiling
   bool profiling = false;
   long long k_launch[K_NUM] = {};
@@ -2982,7 +3267,9 @@ int prepare(d2_ctx* c) {
     RC(ensure(c, c->conn_list[k], c->cap));
   }
   RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
-  RC(ensure(c, c->ee_items, c->cap_items)); RC(ensure(c, c->ditems, c->cap_items)); RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->np_hit2, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
+  RC(ensure(c, c->ee_items, c->cap_items));
+  if (c->precision == D2_FP32) RC(ensure(c, c->ditems, c->cap_items));  // FP64: k_tri_full keeps no item list
+  RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->np_hit2, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
@@ -3108,9 +3395,11 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   Dev D = make_dev(c, level, discrete);
   const LevelGrids G = level_grids(c, discrete, level);
   const int cg = G.cull, eg = G.expand, sg = G.stream;
-  if (discrete) {
-    timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, c->precision == D2_FP32 ? 1 : 0);
+  if (discrete && c->precision == D2_FP32) {
+    timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, 1);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
+  } else if (discrete) {  // one wavefront per 64 node pairs
+    timed_launch(c, K_TRI_FULL, k_tri_full, dim3(eg), dim3(256), D);
   } else {
     timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
     timed_launch(c, K_SOUP_EDGE, k_soup_sub, dim3(2 * sg), dim3(256), D);  // point and edge sub-tests
@@ -3142,6 +3431,23 @@ int post_grid(const d2_ctx* c, int mode) {
   return (int)std::min<long long>(b, hi);
 }
 
+// The per-pair finishing launches over contacts, pair_off and perm: pairs up to
+// 512 contacts in the first (wblocks blocks), those up to 2048 in the second
+// launch's LDS pass, larger ones by the second launch's per-pair forms (with
+// the filter, one lane per pair; n_big[0..1] count the two lists, zeroed
+// before).  mode 1 applies filterContacts(.., margin).
+void launch_finish(d2_ctx* c, int np, int wblocks, int mode, unsigned long long* kept_total, double margin) {
+  hipLaunchKernelGGL(k_pair_finish_block<BLK_CAP>, dim3(wblocks), dim3(256), 0, c->stream, np, (const int*)nullptr,
+                     (const int*)nullptr, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
+                     c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p, c->huge.p, c->n_big.p + 1,
+                     (const int*)nullptr, (const int*)nullptr, kept_total, margin);
+  hipLaunchKernelGGL(k_pair_finish_block<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(256), 0,
+                     c->stream, np, (const int*)c->big.p, (const int*)c->n_big.p, c->contacts.p, c->pair_off.p,
+                     c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p,
+                     c->n_big.p + 1, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
+                     (const int*)(c->n_big.p + 1), kept_total, margin);
+}
+
 // Post-processing of the contacts of the last phase: per-pair order, then the
 // contact filter (mode 1), the step rule (mode 2) or nothing (mode 0).
 int enqueue_post(d2_ctx* c, int mode) {
@@ -3167,15 +3473,7 @@ int enqueue_post(d2_ctx* c, int mode) {
     // the filter's kept total accumulates into C_OUT (zeroed by k_setup); the
     // output offsets are scanned when the contacts are read (d2_result_contacts)
     unsigned long long* kept_total = filtered ? c->cnt.p + C_OUT : nullptr;
-    hipLaunchKernelGGL(k_pair_finish_block<BLK_CAP>, dim3(wblocks), dim3(256), 0, c->stream, np, (const int*)nullptr,
-                       (const int*)nullptr, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
-                       c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p, c->huge.p, c->n_big.p + 1,
-                       (const int*)nullptr, (const int*)nullptr, kept_total);
-    hipLaunchKernelGGL(k_pair_finish_block<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(256), 0,
-                       c->stream, np, (const int*)c->big.p, (const int*)c->n_big.p, c->contacts.p, c->pair_off.p,
-                       c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p,
-                       c->n_big.p + 1, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
-                       (const int*)(c->n_big.p + 1), kept_total);
+    launch_finish(c, np, wblocks, mode, kept_total, 0.0);  // filterContacts(contacts, 0.0), PDE_explicit.cpp:88
   }
   HIPCHK(c, hipGetLastError());
   return D2_OK;
@@ -3185,6 +3483,7 @@ ClusterAcc cluster_acc(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters
   ClusterAcc A{};
   A.n_clusters = n_clusters, A.n_bodies = c->n_bodies, A.dt = dt;
   A.step_bits = c->step_bits.p, A.first_any = c->first_any.p, A.first_ns = c->first_ns.p;
+  A.read_later = c->read_later.p;
   A.bodies = c->bodies.p, A.body_step = project ? c->body_step.p : nullptr;
   A.pairs = c->pairs.p, A.cluster_of_pair = cluster_of_pair;
   return A;
@@ -3198,17 +3497,18 @@ int enqueue_clusters(d2_ctx* c, const int32_t* cluster_of_pair, int n_clusters, 
   double* body_step = project ? c->body_step.p : nullptr;
   const ClusterAcc A = cluster_acc(c, cluster_of_pair, n_clusters, dt, project);
   if (!started)
-    hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, project ? std::max(n_clusters, c->n_bodies) : n_clusters)),
-                       dim3(256), 0, c->stream, A);
+    hipLaunchKernelGGL(k_cluster_init, dim3(grid_for(c, std::max(n_clusters, c->n_bodies))), dim3(256), 0, c->stream, A);
   if (np > 0) {
     if (!started) hipLaunchKernelGGL(k_cluster_first, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, A);
     hipLaunchKernelGGL(k_cluster_min, dim3(grid_for(c, np)), dim3(256), 0, c->stream, np, c->pairs.p, cluster_of_pair,
-                       c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->step_bits.p);
+                       c->bodies.p, c->pair_cnt.p, c->scale.p, c->first_ns.p, c->read_later.p, c->step_bits.p);
   }
+  ClusterFinal F;
+  F.pairs = c->pairs.p, F.bodies = c->bodies.p, F.step_bits = c->step_bits.p, F.first_any = c->first_any.p;
+  F.first_ns = c->first_ns.p, F.first_member = first_member ? c->cc_first.p : (const int*)nullptr;
+  F.pair_cnt = c->pair_cnt.p, F.read_later = c->read_later.p, F.scale = c->scale.p;
   hipLaunchKernelGGL(k_cluster_final, dim3(grid_for(c, project ? std::max(n_clusters, np) : n_clusters)), dim3(256), 0,
-                     c->stream, n_clusters, dt, c->pairs.p, c->bodies.p, c->step_bits.p, c->first_any.p,
-                     first_member ? c->cc_first.p : (const int*)nullptr, c->cluster_step.p, np, cluster_of_pair,
-                     body_step);
+                     c->stream, n_clusters, dt, F, c->cluster_step.p, np, cluster_of_pair, body_step);
   if (project)
     hipLaunchKernelGGL(k_project, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p,
                        c->body_step.p, 1);
@@ -3287,7 +3587,9 @@ void enqueue_gate(d2_ctx* c, const PhaseSpec& ph) {
 // Items of level l per kernel, from its sizes (profiling).
 void add_level_items(d2_ctx* c, int level, int discrete) {
   const unsigned long long* s = c->host_lvl + level * LV_STRIDE;
-  if (discrete) {
+  if (discrete && c->precision != D2_FP32) {
+    c->k_items[K_TRI_FULL] += (double)s[LV_ITEMS];
+  } else if (discrete) {
     c->k_items[K_TRI_MASK] += (double)s[LV_SOUP];
     c->k_items[K_TRI_ENUM] += (double)s[LV_SOUP];
     c->k_items[K_TRI_DIST] += (double)s[LV_ITEMS];
@@ -3454,6 +3756,7 @@ bool graph_cut_short(const d2_ctx* c, int kind, int filter) {
 int ensure_clusters(d2_ctx* c, int n_clusters) {
   RC(ensure(c, c->step_bits, n_clusters)); RC(ensure(c, c->first_any, n_clusters)); RC(ensure(c, c->first_ns, n_clusters));
   RC(ensure(c, c->cluster_step, n_clusters)); RC(ensure(c, c->body_step, std::max(c->n_bodies, 1)));
+  RC(ensure(c, c->read_later, std::max(c->n_bodies, 1)));
   return D2_OK;
 }
 
@@ -3507,7 +3810,7 @@ void grow_capacities(d2_ctx* c, int kind) {
       if (ccd) {
         res = std::max<long long>(res, (long long)v[LV_ITEMS]);
         items = std::max<long long>(items, (long long)std::max(v[LV_PT], v[LV_EE]));
-      } else {
+      } else if (c->precision == D2_FP32) {  // FP64 discrete levels keep no item list (k_tri_full)
         items = std::max<long long>(items, (long long)v[LV_ITEMS]);
       }
     }
@@ -3525,12 +3828,74 @@ void grow_capacities(d2_ctx* c, int kind) {
   if (!any) {  // sizes beyond the recorded ones (an overflow cut a level short): double the phase's lists
     const unsigned long long ov = c->host_cnt[C_OVERFLOW];
     c->cap *= 2, c->cap_contacts *= 2;
-    if (ov & 2ull) c->cap_items *= 2;
+    if ((ov & 2ull) && c->precision == D2_FP32) c->cap_items *= 2;
     if (ov & 1ull) c->cap_res *= 2, c->cap_items *= 2;
   }
 }
 
 int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out);
+
+// ---- bucket-level entry: scratch and one call
+struct BucketScratch {
+  int32_t* np;       // [cap_np][2] staged node pairs
+  long long* base;   // [cap_np] first provisional slot of each node pair
+  int* count;        // [cap_np] hits of each node pair
+  long long* off;    // [cap_np] first output row of each node pair
+  int32_t* used;     // [cap_np] cumulative hit counts
+  ContactRec* hits;  // [cap_hits] provisional hit records
+  int32_t* out_i;    // [cap_hits][8]
+  double* out_r;     // [cap_hits][11]
+  unsigned long long* counter;
+  long long cap_np, cap_hits;
+};
+
+// Validates a bucket call against the resident trees and bodies.
+int bucket_check(d2_ctx* c, int a, int b, int n, const int32_t* node_pairs, const char* who) {
+  if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, std::string(who) + ": upload trees and bodies first");
+  if (a < 0 || a >= c->n_bodies || b < 0 || b >= c->n_bodies || n < 0 || (n > 0 && !node_pairs))
+    return fail(c, D2_ERR_ARG, std::string(who) + ": bad arguments");
+  const int ta = c->host_body_tree[a], tb = c->host_body_tree[b];
+  const int na = c->host_tree_off[ta + 1] - c->host_tree_off[ta], nb = c->host_tree_off[tb + 1] - c->host_tree_off[tb];
+  for (int k = 0; k < n; ++k)
+    if (node_pairs[2 * k] < 0 || node_pairs[2 * k] >= na || node_pairs[2 * k + 1] < 0 || node_pairs[2 * k + 1] >= nb)
+      return fail(c, D2_ERR_ARG, std::string(who) + ": node index out of range");
+  return D2_OK;
+}
+
+// One bucket call on the given scratch.  *total receives the hit count; the
+// cumulative counts are always written, the hits when they fit both the
+// scratch and max_hits.
+int bucket_run(d2_ctx* c, const BucketScratch& S, int a, int b, int n, const int32_t* node_pairs,
+               int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out, long long* total) {
+  *total = 0;
+  if (n == 0) return D2_OK;
+  HIPCHK(c, hipMemcpyAsync(S.np, node_pairs, sizeof(int32_t) * 2 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, S.counter, 1ll, 0ull);
+  BucketArgs A;
+  A.np_list = S.np, A.n = n, A.nodes = c->nodes.p, A.buckets = c->buckets.p;
+  A.off_a = c->host_tree_off[c->host_body_tree[a]], A.off_b = c->host_tree_off[c->host_body_tree[b]];
+  A.body_a = c->bodies.p + 40 * (size_t)a, A.body_b = c->bodies.p + 40 * (size_t)b;
+  A.hits = S.hits, A.cap = S.cap_hits, A.counter = S.counter, A.base = S.base, A.count = S.count;
+  const int blocks = (int)std::min<long long>((n + 3) / 4, (long long)c->cus * 8);
+  hipLaunchKernelGGL(k_bucket_pairs, dim3(blocks), dim3(256), 0, c->stream, A);
+  hipLaunchKernelGGL(k_bucket_scan, dim3(1), dim3(1024), 0, c->stream, n, (const int*)S.count, S.off, S.used);
+  HIPCHK(c, hipGetLastError());
+  unsigned long long hits = 0;
+  HIPCHK(c, hipMemcpyAsync(&hits, S.counter, sizeof(hits), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pair_used_out, S.used, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *total = (long long)hits;
+  if ((long long)hits > S.cap_hits || (long long)hits > max_hits || hits == 0) return D2_OK;
+  hipLaunchKernelGGL(k_bucket_place, dim3(blocks), dim3(256), 0, c->stream, n, (const long long*)S.base,
+                     (const int*)S.count, (const long long*)S.off, (const ContactRec*)S.hits, S.out_i, S.out_r);
+  HIPCHK(c, hipGetLastError());
+  if (ints_out)
+    HIPCHK(c, hipMemcpyAsync(ints_out, S.out_i, sizeof(int32_t) * 8 * hits, hipMemcpyDeviceToHost, c->stream));
+  if (reals_out)
+    HIPCHK(c, hipMemcpyAsync(reals_out, S.out_r, sizeof(double) * 11 * hits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
 
 int run(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out) {
   int rc = run_once(c, kind, filter, dt, n_contacts_out);
@@ -3623,7 +3988,11 @@ int d2_ctx_destroy(d2_ctx* c) {
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
   release(c->big), release(c->huge), release(c->n_big);
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
-      release(c->tmp_cluster);
+      release(c->tmp_cluster), release(c->read_later);
+  release(c->touch_base), release(c->touch_lv), release(c->touch_run), release(c->touch_lvb), release(c->touch_runb),
+      release(c->touch_acc);
+  release(c->bk_np), release(c->bk_used), release(c->bk_out_i), release(c->bk_base), release(c->bk_off),
+      release(c->bk_count), release(c->bk_hits), release(c->bk_out_r), release(c->bk_counter);
   release(c->tree_radius), release(c->tree_eps_out), release(c->hv), release(c->feat), release(c->pred), release(c->hv_off);
   release(c->cert), release(c->mlp_w), release(c->mlp_m), release(c->mlp_v), release(c->mlp_wf), release(c->mlp_grad);
   release(c->mlp_wb), release(c->tr_x), release(c->tr_t), release(c->tr_act), release(c->tr_dlt), release(c->tr_loss);
@@ -3999,6 +4368,62 @@ int d2_step_run(d2_ctx* c, float dt, int filter, double* cluster_step_out, int64
                              c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
+  return D2_OK;
+}
+
+int d2_find_contacts_bucket(d2_ctx* c, int body_a, int body_b, int n_node_pairs, const int32_t* node_pairs,
+                            int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out,
+                            int64_t* n_hits_out) {
+  if (!c) return D2_ERR_ARG;
+  RC(bucket_check(c, body_a, body_b, n_node_pairs, node_pairs, "d2_find_contacts_bucket"));
+  if (n_node_pairs > 0 && !pair_used_out) return fail(c, D2_ERR_ARG, "d2_find_contacts_bucket: pair_used_out is NULL");
+  (void)hipSetDevice(c->device);
+  const size_t n = std::max(n_node_pairs, 1);
+  RC(ensure(c, c->bk_np, 2 * n)); RC(ensure(c, c->bk_base, n)); RC(ensure(c, c->bk_count, n));
+  RC(ensure(c, c->bk_off, n)); RC(ensure(c, c->bk_used, n)); RC(ensure(c, c->bk_counter, 1));
+  long long cap = std::max<long long>((long long)c->bk_hits.n, 8ll * (long long)n);
+  long long total = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    RC(ensure(c, c->bk_hits, cap)); RC(ensure(c, c->bk_out_i, 8 * cap)); RC(ensure(c, c->bk_out_r, 11 * cap));
+    const BucketScratch S{c->bk_np.p, c->bk_base.p, c->bk_count.p, c->bk_off.p, c->bk_used.p, c->bk_hits.p,
+                          c->bk_out_i.p, c->bk_out_r.p, c->bk_counter.p, (long long)n, cap};
+    RC(bucket_run(c, S, body_a, body_b, n_node_pairs, node_pairs, pair_used_out, max_hits, ints_out, reals_out, &total));
+    if (total <= cap) break;
+    cap = total;  // the scratch grows to the recorded count and the call repeats
+  }
+  if (n_hits_out) *n_hits_out = total;
+  if (total > max_hits) return fail(c, D2_ERR_CAPACITY, "d2_find_contacts_bucket: more hits than max_hits");
+  return D2_OK;
+}
+
+int d2_filter_contacts(d2_ctx* c, int n, const double* reals, double margin, int32_t* sorted_out, int32_t* kept_out,
+                       int* n_kept_out) {
+  if (!c) return D2_ERR_ARG;
+  if (n < 0 || (n > 0 && (!reals || !sorted_out || !kept_out)) || !(margin >= 0.0 || margin < 0.0))
+    return fail(c, D2_ERR_ARG, "d2_filter_contacts: bad arguments");
+  if (n_kept_out) *n_kept_out = 0;
+  if (n == 0) return D2_OK;
+  (void)hipSetDevice(c->device);
+  const size_t m = (size_t)n;
+  RC(ensure(c, c->contacts, m)); RC(ensure(c, c->perm, m)); RC(ensure(c, c->keep, m));
+  RC(ensure(c, c->kept_cnt, 2)); RC(ensure(c, c->pair_off, 2)); RC(ensure(c, c->n_big, 2));
+  RC(ensure(c, c->big, 1)); RC(ensure(c, c->huge, 1)); RC(ensure(c, c->pair_rec, PR_STRIDE)); RC(ensure(c, c->scale, 1));
+  RC(ensure(c, c->bk_out_r, 11 * m));
+  // the run buffers now hold this call's contacts: no earlier result stays readable
+  c->last_mode = -1;
+  c->ccd_valid = false;
+  HIPCHK(c, hipMemcpyAsync(c->bk_out_r.p, reals, sizeof(double) * 11 * m, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_filter_load, dim3(grid_for(c, n)), dim3(256), 0, c->stream, n, (const double*)c->bk_out_r.p,
+                     c->contacts.p, c->perm.p, c->pair_off.p, c->n_big.p);
+  launch_finish(c, 1, 1, 1, nullptr, margin);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(sorted_out, c->perm.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(kept_out, c->keep.p, sizeof(int32_t) * m, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int k = 0;  // keep flags per sorted position -> the kept inputs in sorted order
+  for (int t = 0; t < n; ++t)
+    if (kept_out[t]) kept_out[k++] = sorted_out[t];
+  if (n_kept_out) *n_kept_out = k;
   return D2_OK;
 }
 
@@ -4410,8 +4835,9 @@ int d2_set_learned(d2_ctx* c, int on, float t_gate, float s_gate) {
 
 int d2_set_touch_stats(d2_ctx* c, int on) {
   if (!c) return D2_ERR_ARG;
+  // turning it on forgets the last counts; turning it off keeps them readable
+  if (on && !c->touch_on) c->touch_host.clear();
   c->touch_on = on != 0;
-  c->touch_host.clear();
   return D2_OK;
 }
 

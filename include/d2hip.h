@@ -296,6 +296,36 @@ int d2_set_learned(d2_ctx* ctx, int on, float t_gate, float s_gate);
 int d2_result_contacts(d2_ctx* ctx, int32_t* offsets, int32_t* ints, double* reals);
 int d2_result_pairs(d2_ctx* ctx, double* per_pair);
 
+/* The plugin point's bucket comparison:
+ * strategy::ContactDetectionStrategy::findContactsBucketComparison
+ * (strategies/contact_detection/contact_detection_strategy.h:11) as
+ * ContactDetectionComparison implements it (contact_detection_comparison.cpp:
+ * 12-14 -> collision_detection/comparison_dist.h:205-240).  node_pairs[k] =
+ * (node of body_a's tree, node of body_b's tree) -- the DeferredCompare list
+ * of one BFS level; every triangle of the first bucket is compared with every
+ * triangle of the second at the bodies' future states, in FP64.  Hits are the
+ * contact records (layout above; pair 0, kind 0, level 0) in the reference's
+ * order (node pair, then a's triangle, then b's); pair_used_out[k] = hits of
+ * node pairs 0..k (the reference appends to `hits` and records hits.size(), so
+ * a caller appending to a non-empty vector adds its size).  *n_hits_out gets
+ * the hit count; when it exceeds max_hits nothing but pair_used_out and the
+ * count is written and the call fails with D2_ERR_CAPACITY (retry with more
+ * room).  D2_ERR_ARG for a node index outside its tree. */
+int d2_find_contacts_bucket(d2_ctx* ctx, int body_a, int body_b, int n_node_pairs, const int32_t* node_pairs,
+                            int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out,
+                            int64_t* n_hits_out);
+/* filterContacts(unfiltered, margin) (collision_detection/contact.h:44-69) on
+ * the device over n contacts given as reals (11 per contact, layout above;
+ * the filter reads A, B, eps_a and eps_b).  The reference sorts `unfiltered`
+ * in place with libstdc++ std::sort by (eps_a + eps_b, |A - B|) and keeps a
+ * contact unless its midpoint lies within eps_a + eps_b + margin of an
+ * earlier kept one: sorted_out[t] = the input index at sorted position t
+ * (that permutation, n entries), kept_out[0..*n_kept_out) = the kept input
+ * indices in sorted order (room for n).  Uses the context's run buffers: the
+ * last run's results are no longer readable afterwards. */
+int d2_filter_contacts(d2_ctx* ctx, int n, const double* reals, double margin, int32_t* sorted_out, int32_t* kept_out,
+                       int* n_kept_out);
+
 /* Convenience: upload pairs and run, as one call (host buffers in/out). */
 int d2_narrow(d2_ctx* ctx, int n_pairs, const int32_t* pairs, int filter, int64_t* n_contacts_out);
 int d2_ccd(d2_ctx* ctx, int n_pairs, const int32_t* pairs, const double* max_time, int64_t* n_contacts_out);
@@ -326,7 +356,8 @@ int d2_run_info(const d2_ctx* ctx, int64_t* info, int n);
  * discrete, 1 CCD) and level 0..16 four counts -- distinct inner nodes,
  * distinct leaf nodes, the triangles of those leaves, distinct bodies --
  * then the same four counts over the run's union of levels and phases
- * (144 values).  Copies min(n, total) values; returns the total. */
+ * (144 values).  Copies min(n, total) values; returns the total.  Turning
+ * the accounting on forgets the last counts; turning it off keeps them. */
 int d2_set_touch_stats(d2_ctx* ctx, int on);
 int d2_touch_stats(const d2_ctx* ctx, int64_t* out, int n);
 /* Synchronise the context's stream. */

@@ -78,6 +78,8 @@ def lib():
         L.d2o_ccd.argtypes = [_vp, ctypes.c_int, _ip, _dp]
         L.d2o_ccd.restype = ctypes.c_int
         L.d2o_scene_result.argtypes = [_vp, _ip, _ip, _dp, _dp]
+        L.d2o_filter_order.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _ip, _ip]
+        L.d2o_filter_order.restype = ctypes.c_int
         L.d2o_exhaustive_leaves.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.d2o_exhaustive_leaves.restype = ctypes.c_int
         L.d2o_cluster_step.argtypes = [_vp, ctypes.c_int, _ip, ctypes.c_float]
@@ -293,6 +295,17 @@ class Scene:
         cc = np.zeros(1)
         t = lib().d2o_time_pass(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), _p(mt), threads, repeats, _p(cc))
         return t, int(cc[0])
+
+
+def filter_order(reals, margin=0.0):
+    """filterContacts on contact reals (n x 11): (the inputs' order after the
+    in-place std::sort, the kept inputs in that order)."""
+    r = _d(reals)
+    n = r.size // 11
+    order = np.zeros(max(n, 1), dtype=np.int32)
+    kept = np.zeros(max(n, 1), dtype=np.int32)
+    k = lib().d2o_filter_order(_p(r), n, float(margin), order.ctypes.data_as(_ip), kept.ctypes.data_as(_ip))
+    return order[:n], kept[:k]
 
 
 def step_scale(reals, body_a, body_b, max_time):

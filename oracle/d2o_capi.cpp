@@ -410,6 +410,25 @@ extern "C" int d2o_ccd(void* h, int n_pairs, const int32_t* pairs, const double*
   return (int)(sc->out_i.size() / 8);
 }
 
+// filterContacts(unfiltered, margin) (contact.h:44-69) over n contacts given
+// as reals (11 per contact): sorted_out = the inputs' order after the
+// reference's in-place std::sort, kept_out = the kept inputs in that order.
+// Returns the number kept.
+extern "C" int d2o_filter_order(const double* reals, int n, double margin, int32_t* sorted_out, int32_t* kept_out) {
+  std::vector<Contact> cs(n);
+  for (int i = 0; i < n; ++i) {
+    const double* r = reals + 11 * i;
+    cs[i].A = V(r[0], r[1], r[2]);
+    cs[i].B = V(r[3], r[4], r[5]);
+    cs[i].eps_a = r[6], cs[i].eps_b = r[7], cs[i].eps_inner_a = r[8], cs[i].eps_inner_b = r[9], cs[i].toc = r[10];
+    cs[i].ia = i;  // the input position travels with the contact
+  }
+  std::vector<Contact> kept = filterContacts(cs, margin);  // sorts cs in place
+  for (int i = 0; i < n; ++i) sorted_out[i] = cs[i].ia;
+  for (size_t k = 0; k < kept.size(); ++k) kept_out[k] = kept[k].ia;
+  return (int)kept.size();
+}
+
 // Every leaf of body a against every leaf of body b, without the traversal:
 // mode 0 runs the discrete bucket comparison (comparison_dist.h:205-240) at the
 // future states; mode 1 projects both future states at max_time as

@@ -206,3 +206,44 @@ def test_graph_continues_past_captured_levels():
     finally:
         eager.close()
         graph.close()
+
+
+def test_cluster_step_static_first_pair():
+    """separate_clusters.cpp:589-590: the second loop starts from step[a] of a
+    cluster's first pair; with a static body a that is its unchanged last
+    step, and the pair's own scaled step only counts when its dynamic body is
+    read later (a later pair's id) or lowered again (a later pair with
+    contacts).  A plane (body 1) sinks through a static pyramid's apex
+    (body 0) within a 0.01 s interval: scale 0.375; body 2 is a far plane."""
+    from test_oracle_kat import PLANE_F, PLANE_V, PYRAMID_F, PYRAMID_V
+    from test_oracle_pins import record
+    rng = d2.pcg_seed(42, 54)
+    trees = [d2.tree_build(PYRAMID_V, PYRAMID_F, rng), d2.tree_build(PLANE_V, PLANE_F, rng)]
+    import d2o
+    rng = d2o.pcg_seed(42, 54)
+    otrees = [d2o.build_tree(PYRAMID_V, PYRAMID_F, rng), d2o.build_tree(PLANE_V, PLANE_F, rng)]
+    rec = np.stack([record(trans=(0, 0, 1.0), static=True, last=0.01),
+                    record(trans=(0, 0, 1.105), vel=(0, 0, -1.0), last=0.01),
+                    record(trans=(5.0, 0, 1.105), last=0.01)])
+    tob = [0, 1, 1]
+    dt = np.float32(1 / 60)
+    cases = [
+        (np.array([[0, 1]]), None),            # the pair alone: step[a] = 0.01
+        (np.array([[0, 1], [1, 2]]), None),    # body 1 is the id of a later pair: 0.01 * 0.375
+        (np.array([[0, 2], [0, 1]]), None),    # static first pair without contacts, then the sinking pair
+    ]
+    osc = d2o.Scene(otrees, tob, rec)
+    with d2.Context(0) as c:
+        c.upload_trees(trees)
+        c.upload_bodies(tob, rec)
+        for pairs, _ in cases:
+            mt = np.minimum(rec[pairs[:, 0], 39], rec[pairs[:, 1], 39])
+            c.upload_pairs(pairs, mt)
+            c.ccd_run()
+            got = c.cluster_steps(np.zeros(pairs.shape[0], np.int32), 1, dt)
+            want = osc.cluster_step(pairs, dt)
+            print(f"[static first] pairs={pairs.tolist()} gpu={got[0]} oracle={want}", flush=True)
+            assert got[0] == want, (pairs.tolist(), got, want)
+    # the three cases really differ: the quirk is exercised
+    steps = [osc.cluster_step(p, dt) for p, _ in cases]
+    assert steps[0] == 0.01 and abs(steps[1] - 0.00375) < 1e-15
