@@ -122,6 +122,8 @@ def lib():
         "d2_touch_stats": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
         "d2_find_contacts_bucket": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int64, _ip, _dp,
                                      _i64p], ctypes.c_int),
+        "d2_find_contacts_bucket_noalloc": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int64,
+                                             _ip, _dp, _i64p], ctypes.c_int),
         "d2_filter_contacts": ([_vp, ctypes.c_int, _dp, ctypes.c_double, _ip, _ip, ctypes.POINTER(ctypes.c_int)],
                                ctypes.c_int),
     }

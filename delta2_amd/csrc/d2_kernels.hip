@@ -44,6 +44,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1818,6 +1819,17 @@ __global__ void __launch_bounds__(256) k_bucket_pairs(BucketArgs A) {
   }
 }
 
+// Scratch of the allocation-free bucket entry (d2_find_contacts_bucket_noalloc):
+// at most NOALLOC_NP node pairs and NOALLOC_HITS hits per call, laid out as
+// node pairs | base | count | off | used | hits | out ints | out reals |
+// counter.  Each device holds its own copy until the process ends; its
+// address comes from hipGetSymbolAddress (the name is a host-side stand-in).
+constexpr long long NOALLOC_NP = 1ll << 15;
+constexpr long long NOALLOC_HITS = 1ll << 17;
+constexpr size_t NOALLOC_BYTES = (size_t)NOALLOC_NP * (8 + 8 + 4 + 8 + 4) +
+                                 (size_t)NOALLOC_HITS * (sizeof(ContactRec) + 8 * 4 + 11 * 8) + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
 // The standalone filter's input as one pair: contact i keeps input position i
 // as its emission key, so the per-pair finishing kernels see the caller's
 // order (the unstable sort's result depends on it).
@@ -3075,6 +3087,7 @@ struct d2_ctx {
   long long touch_words = 0, touch_body_words = 0;
   std::vector<unsigned long long> touch_host;
   // bucket-level entry and standalone filter
+  unsigned char* noalloc = nullptr;  // this device's copy of `scratch` (resolved at creation)
   DBuf<int32_t> bk_np, bk_used, bk_out_i;
   DBuf<long long> bk_base, bk_off;
   DBuf<int> bk_count;
@@ -3835,6 +3848,23 @@ void grow_capacities(d2_ctx* c, int kind) {
 
 int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out);
 
+// This device's copy of `scratch`, resolved once per device with the device
+// current (the first context created on it) and kept until the process ends;
+// nullptr when the runtime cannot resolve it.
+unsigned char* noalloc_scratch(int device) {
+  static std::mutex mu;
+  static unsigned char* addr[256] = {};
+  static bool tried[256] = {};
+  if (device < 0 || device >= 256) return nullptr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!tried[device]) {
+    tried[device] = true;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) addr[device] = static_cast<unsigned char*>(p);
+  }
+  return addr[device];
+}
+
 // ---- bucket-level entry: scratch and one call
 struct BucketScratch {
   int32_t* np;       // [cap_np][2] staged node pairs
@@ -3966,6 +3996,7 @@ int d2_ctx_create(int device, int precision, d2_ctx** out) {
     return D2_ERR_HIP;
   }
   std::memset(c->host_lvl, 0, sizeof(unsigned long long) * 2 * (MAX_LEVELS + 1) * LV_STRIDE);
+  c->noalloc = noalloc_scratch(device);
   *out = c;
   return D2_OK;
 }
@@ -4393,6 +4424,39 @@ int d2_find_contacts_bucket(d2_ctx* c, int body_a, int body_b, int n_node_pairs,
   }
   if (n_hits_out) *n_hits_out = total;
   if (total > max_hits) return fail(c, D2_ERR_CAPACITY, "d2_find_contacts_bucket: more hits than max_hits");
+  return D2_OK;
+}
+
+int d2_find_contacts_bucket_noalloc(d2_ctx* c, int body_a, int body_b, int n_node_pairs, const int32_t* node_pairs,
+                                    int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out,
+                                    int64_t* n_hits_out) {
+  if (!c) return D2_ERR_ARG;
+  RC(bucket_check(c, body_a, body_b, n_node_pairs, node_pairs, "d2_find_contacts_bucket_noalloc"));
+  if (n_node_pairs > 0 && !pair_used_out)
+    return fail(c, D2_ERR_ARG, "d2_find_contacts_bucket_noalloc: pair_used_out is NULL");
+  if (n_node_pairs > NOALLOC_NP)
+    return fail(c, D2_ERR_CAPACITY, "d2_find_contacts_bucket_noalloc: at most 32768 node pairs per call");
+  if (!c->noalloc) return fail(c, D2_ERR_HIP, "d2_find_contacts_bucket_noalloc: the scratch array has no address");
+  (void)hipSetDevice(c->device);
+  // the scratch array carved into the bucket call's buffers (8-byte fields first)
+  unsigned char* p = c->noalloc;
+  BucketScratch S;
+  S.base = reinterpret_cast<long long*>(p), p += 8 * NOALLOC_NP;
+  S.off = reinterpret_cast<long long*>(p), p += 8 * NOALLOC_NP;
+  S.np = reinterpret_cast<int32_t*>(p), p += 8 * NOALLOC_NP;
+  S.count = reinterpret_cast<int*>(p), p += 4 * NOALLOC_NP;
+  S.used = reinterpret_cast<int32_t*>(p), p += 4 * NOALLOC_NP;
+  S.hits = reinterpret_cast<ContactRec*>(p), p += sizeof(ContactRec) * NOALLOC_HITS;
+  S.out_r = reinterpret_cast<double*>(p), p += 11 * 8 * NOALLOC_HITS;
+  S.out_i = reinterpret_cast<int32_t*>(p), p += 8 * 4 * NOALLOC_HITS;
+  S.counter = reinterpret_cast<unsigned long long*>(p);
+  S.cap_np = NOALLOC_NP, S.cap_hits = NOALLOC_HITS;
+  long long total = 0;
+  RC(bucket_run(c, S, body_a, body_b, n_node_pairs, node_pairs, pair_used_out, max_hits, ints_out, reals_out, &total));
+  if (n_hits_out) *n_hits_out = total;
+  if (total > NOALLOC_HITS)
+    return fail(c, D2_ERR_CAPACITY, "d2_find_contacts_bucket_noalloc: at most 131072 hits per call");
+  if (total > max_hits) return fail(c, D2_ERR_CAPACITY, "d2_find_contacts_bucket_noalloc: more hits than max_hits");
   return D2_OK;
 }
 

@@ -314,6 +314,20 @@ int d2_result_pairs(d2_ctx* ctx, double* per_pair);
 int d2_find_contacts_bucket(d2_ctx* ctx, int body_a, int body_b, int n_node_pairs, const int32_t* node_pairs,
                             int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out,
                             int64_t* n_hits_out);
+/* d2_find_contacts_bucket without any device allocation: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and errors as d2_find_contacts_bucket, on a fixed
+ * scratch array that each GPU holds once for the life of the process: at most
+ * 32768 node pairs per call (more: D2_ERR_CAPACITY before any device work)
+ * and at most 131072 hits (more: D2_ERR_CAPACITY with *n_hits_out and
+ * pair_used_out written, as for max_hits).  Every call on one GPU uses that
+ * GPU's one array, from every context: calls that overlap on one GPU (two
+ * threads, two contexts' streams) race and must be serialised by the
+ * caller.  The call synchronises the context's stream before it returns. */
+int d2_find_contacts_bucket_noalloc(d2_ctx* ctx, int body_a, int body_b, int n_node_pairs, const int32_t* node_pairs,
+                                    int32_t* pair_used_out, int64_t max_hits, int32_t* ints_out, double* reals_out,
+                                    int64_t* n_hits_out);
 /* filterContacts(unfiltered, margin) (collision_detection/contact.h:44-69) on
  * the device over n contacts given as reals (11 per contact, layout above;
  * the filter reads A, B, eps_a and eps_b).  The reference sorts `unfiltered`

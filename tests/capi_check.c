@@ -44,7 +44,8 @@ int main(int argc, char** argv) {
     /* every entry point resolves (the link already proved it; touch them) */
     typedef void (*fn_t)(void);
     fn_t fns[] = {(fn_t)d2_ctx_create, (fn_t)d2_narrow_run, (fn_t)d2_ccd_run, (fn_t)d2_step_run,
-                  (fn_t)d2_find_contacts_bucket, (fn_t)d2_filter_contacts, (fn_t)d2_broad_phase,
+                  (fn_t)d2_find_contacts_bucket, (fn_t)d2_find_contacts_bucket_noalloc,
+                  (fn_t)d2_filter_contacts, (fn_t)d2_broad_phase,
                   (fn_t)d2_clusters_build, (fn_t)d2_tree_build, (fn_t)d2_touch_stats};
     int linked = 0;
     for (size_t i = 0; i < sizeof(fns) / sizeof(fns[0]); ++i) linked += fns[i] != NULL;
