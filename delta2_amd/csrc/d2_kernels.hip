@@ -2117,67 +2117,9 @@ __device__ __forceinline__ bool step_rule_scale(const ContactRec& ct, const Step
   return true;
 }
 
-__device__ void pair_finish_one(int p, const ContactRec* c, const int* pair_off, int* perm, int mode, int* keep,
-                                int* kept_cnt, const double* pair_rec, double* scale, double margin) {
-  {
-    int b = pair_off[p], e = pair_off[p + 1];
-    {  // heapsort of perm[b, e) by the (unique) emission key
-      const int n = e - b;
-      int* a = perm + b;
-      auto sift = [&](int root, int len) {
-        const int v = a[root];
-        const unsigned long long kv = c[v].key;
-        int hole = root;
-        for (int ch = 2 * hole + 1; ch < len; ch = 2 * hole + 1) {
-          if (ch + 1 < len && c[a[ch + 1]].key > c[a[ch]].key) ++ch;
-          if (c[a[ch]].key <= kv) break;
-          a[hole] = a[ch];
-          hole = ch;
-        }
-        a[hole] = v;
-      };
-      for (int r = n / 2 - 1; r >= 0; --r) sift(r, n);
-      for (int len = n - 1; len > 0; --len) {
-        const int t = a[0];
-        a[0] = a[len];
-        a[len] = t;
-        sift(0, len);
-      }
-    }
-    if (mode == 1) {  // filterContacts(.., margin)
-      Sorter s{perm, c};
-      s.sort(b, e);
-      int kept = 0;
-      for (int i = b; i < e; ++i) {
-        const ContactRec& ct = c[perm[i]];
-        v3<double> m1 = dvs(add(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}), 2.0);
-        double search = (ct.r[6] + ct.r[7]) + margin;
-        bool fresh = true;
-        for (int h = b; h < i && fresh; ++h) {
-          if (!keep[h]) continue;
-          const ContactRec& hc = c[perm[h]];
-          v3<double> m2 = dvs(add(v3<double>{hc.r[0], hc.r[1], hc.r[2]}, v3<double>{hc.r[3], hc.r[4], hc.r[5]}), 2.0);
-          if (vlen(sub(m1, m2)) < search) fresh = false;
-        }
-        keep[i] = fresh;
-        kept += fresh;
-      }
-      kept_cnt[p] = kept;
-    } else if (mode == 2) {  // step rule, separate_clusters.cpp:496-564
-      const StepKin k = step_kin(pair_rec + (size_t)PR_STRIDE * p);
-      double ms = 1.0;
-      for (int i = b; i < e; ++i) {
-        double s;
-        if (step_rule_scale(c[perm[i]], k, s) && s < ms) ms = s;  // std::min(ms, s): NaN never wins
-      }
-      scale[p] = ms;
-    }
-  }
-}
-
 // Contacts a pair may have for the LDS passes of the post-processing: the
 // second pass (one 256-thread block per CU, 142 KB of LDS) takes pairs up to
-// POST_CAP_LARGE; above it a pair takes one lane (pair_finish_one).
+// POST_CAP_LARGE; above it a pair takes one block with global scratch.
 constexpr int POST_CAP_LARGE = 2048;
 struct LdsKey {
   double eps, dist;
@@ -2517,9 +2459,137 @@ __device__ void pair_finish_huge(int p, const ContactRec* c, const int* pair_off
   }
 }
 
+// Global scratch of the filter above every LDS capacity: six int arrays of
+// the run's contact capacity, a pair using positions [b, b + n) of each
+// (its contacts' range; the sort's segment queue, at most 6 (n / 17 + 1)
+// ints, fits there for n > 16).
+struct HugeScratch {
+  int *rk, *ord, *apos, *bpos, *mark, *segs;
+};
+
+// filterContacts(.., margin) of one pair above every LDS capacity, one block:
+// the emission order by rank counting (pair_finish_huge), the (eps, dist)
+// ranks by rank counting over LDS key tiles, the std::sort replay
+// (block_std_sort) on global scratch when keys tie, then the greedy pass 64
+// candidates at a time as in the LDS path, with the kept list in scratch.
+// Returns the kept count (every thread).
+template <int CAP>
+__device__ int pair_filter_huge(int p, const ContactRec* c, const int* pair_off, int* perm, int* keep, int* kept_cnt,
+                                double margin, const HugeScratch& H, unsigned long long* skey, LdsKey* sk,
+                                int* s_misc, unsigned long long (*conf_w)[64], int (*near_w)[64]) {
+  const int tid = threadIdx.x, lane = lane_id(), w = tid >> 6;
+  const int b = pair_off[p], n = pair_off[p + 1] - b;
+  pair_finish_huge<CAP>(p, c, pair_off, perm, 0, keep, nullptr, nullptr, skey, nullptr);  // perm: emission order
+  if (tid == 0) s_misc[0] = 0;
+  __syncthreads();
+  int* rk = H.rk + b;
+  int* ord = H.ord + b;
+  int* eo = H.apos + b;  // the emission order, kept while perm is rewritten
+  int* kl = H.mark + b;  // the kept list (emission positions), after the sort
+  constexpr int K = 8;
+  bool tie = false;
+  for (int q0 = 0; q0 < n; q0 += K * (int)blockDim.x) {  // ranks of the (eps, dist) keys; equal keys share one
+    LdsKey mk[K];
+    int r[K], eq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = q0 + k * (int)blockDim.x + tid;
+      const ContactRec& ct = c[perm[b + (i < n ? i : 0)]];
+      mk[k].eps = ct.r[6] + ct.r[7];
+      mk[k].dist = vlen(sub(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}));
+      r[k] = 0, eq[k] = 0;
+    }
+    for (int t0 = 0; t0 < n; t0 += CAP) {
+      const int m = n - t0 < CAP ? n - t0 : CAP;
+      for (int j = tid; j < m; j += blockDim.x) {
+        const ContactRec& ct = c[perm[b + t0 + j]];
+        sk[j].eps = ct.r[6] + ct.r[7];
+        sk[j].dist = vlen(sub(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}));
+      }
+      __syncthreads();
+      for (int j = 0; j < m; ++j) {
+        const LdsKey y = sk[j];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (lds_less(y, mk[k])) ++r[k];
+          else if (!lds_less(mk[k], y)) ++eq[k];
+        }
+      }
+      __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int i = q0 + k * (int)blockDim.x + tid;
+      if (i < n) rk[i] = r[k], tie = tie || eq[k] > 1;
+    }
+  }
+  if (tie) s_misc[0] = 1;
+  __syncthreads();
+  if (s_misc[0] == 0) {
+    for (int i = tid; i < n; i += blockDim.x) ord[rk[i]] = i;
+  } else {
+    for (int i = tid; i < n; i += blockDim.x) ord[i] = i;
+    __syncthreads();
+    block_std_sort(n, ord, rk, H.apos + b, H.bpos + b, H.mark + b, H.segs + b, s_misc + 2, blk_segs(n));
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) eo[i] = perm[b + i];
+  __syncthreads();
+  // greedy pass in sorted order
+  int nk = 0;
+  for (int t0 = 0; t0 < n; t0 += 64) {
+    const int m = n - t0 < 64 ? n - t0 : 64;
+    const int t = t0 + lane;
+    const bool valid = t < n;
+    const ContactRec& ct = c[eo[ord[valid ? t : t0]]];
+    const v3<double> mid = dvs(add(v3<double>{ct.r[0], ct.r[1], ct.r[2]}, v3<double>{ct.r[3], ct.r[4], ct.r[5]}), 2.0);
+    const double mx = mid.x, my = mid.y, mz = mid.z;
+    const double search = (ct.r[6] + ct.r[7]) + margin;
+    bool near = false;
+    for (int q0 = 0; q0 < nk; q0 += 64) {  // contacts kept before this group
+      const ContactRec& hc = c[eo[kl[q0 + lane < nk ? q0 + lane : q0]]];
+      const v3<double> hm =
+          dvs(add(v3<double>{hc.r[0], hc.r[1], hc.r[2]}, v3<double>{hc.r[3], hc.r[4], hc.r[5]}), 2.0);
+      const int mm = nk - q0 < 64 ? nk - q0 : 64;
+      for (int u = w; u < mm; u += 4) {
+        const v3<double> dd{mx - readlane_d(hm.x, u), my - readlane_d(hm.y, u), mz - readlane_d(hm.z, u)};
+        near = near || near_mid(dd, search);
+      }
+    }
+    unsigned long long conf = 0;  // bit u: candidate u (< this one) of the group is near
+    for (int u = w; u < m; u += 4) {
+      const v3<double> dd{mx - readlane_d(mx, u), my - readlane_d(my, u), mz - readlane_d(mz, u)};
+      if (u < lane && near_mid(dd, search)) conf |= 1ull << u;
+    }
+    near_w[w][lane] = near ? 1 : 0;
+    conf_w[w][lane] = conf;
+    __syncthreads();
+    if (w == 0) {
+      const bool nr = (near_w[0][lane] | near_w[1][lane] | near_w[2][lane] | near_w[3][lane]) != 0;
+      const unsigned long long cf = conf_w[0][lane] | conf_w[1][lane] | conf_w[2][lane] | conf_w[3][lane];
+      const unsigned long long blocked = __ballot(nr);
+      unsigned long long kept = 0;
+      for (int i = 0; i < m; ++i) {
+        const unsigned long long ci = readlane_u64(cf, i);
+        if (!((blocked >> i) & 1ull) && !(ci & kept)) kept |= 1ull << i;
+      }
+      const bool fresh = (kept >> lane) & 1ull;
+      if (valid) keep[b + t] = fresh ? 1 : 0;
+      if (fresh) kl[nk + __popcll(kept & lanes_below())] = ord[t];
+      if (lane == 0) s_misc[1] = nk + __popcll(kept);
+    }
+    __syncthreads();
+    nk = s_misc[1];
+  }
+  for (int t = tid; t < n; t += blockDim.x) perm[b + t] = eo[ord[t]];
+  if (tid == 0) kept_cnt[p] = nk;
+  __syncthreads();
+  return nk;
+}
+
 // CAP = BLK_CAP: every pair (list null); pairs above CAP go to `over` (up to
 // POST_CAP_LARGE contacts) or `huge`.  CAP = POST_CAP_LARGE: the pairs of
-// `list`, then the pairs of `tail` one lane each (above every LDS capacity).
+// `list`, then the pairs of `tail` one block each (above every LDS capacity).
 template <int CAP>
 __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const int* list, const int* n_list,
                                                            const ContactRec* c, const int* pair_off, int* perm,
@@ -2527,7 +2597,7 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
                                                            const double* pair_rec, double* scale, int* over,
                                                            int* n_over, int* huge, int* n_huge, const int* tail,
                                                            const int* n_tail, unsigned long long* out_total,
-                                                           double margin) {
+                                                           double margin, HugeScratch H) {
   constexpr int SEGS = blk_segs(CAP);
   __shared__ unsigned long long skey[CAP];  // emission keys, then the partition lists of the tie replay
   __shared__ int sidx[CAP];
@@ -2720,13 +2790,14 @@ __global__ void __launch_bounds__(256) k_pair_finish_block(int n_pairs, const in
     __syncthreads();
   }
   if constexpr (CAP == POST_CAP_LARGE) {  // (kept out of the first launch's code: registers, scratch)
-    if (tail) {  // pairs above every LDS capacity (listed by the first launch)
-      if (mode == 1) {  // the filter's std::sort replay: one lane per pair
-        for (int t = blockIdx.x * blockDim.x + tid; t < *n_tail; t += gridDim.x * blockDim.x) {
-          pair_finish_one(tail[t], c, pair_off, perm, mode, keep, kept_cnt, pair_rec, scale, margin);
-          if (out_total) atomicAdd(out_total, (unsigned long long)kept_cnt[tail[t]]);
+    if (tail) {  // pairs above every LDS capacity (listed by the first launch), one block each
+      if (mode == 1) {
+        for (int t = blockIdx.x; t < *n_tail; t += gridDim.x) {
+          const int nk = pair_filter_huge<CAP>(tail[t], c, pair_off, perm, keep, kept_cnt, margin, H, skey, sk,
+                                               s_misc, conf_w, near_w);
+          if (tid == 0) kept_sum += (unsigned long long)nk;
         }
-      } else {  // one block per pair
+      } else {
         for (int t = blockIdx.x; t < *n_tail; t += gridDim.x) {
           pair_finish_huge<CAP>(tail[t], c, pair_off, perm, mode, keep, pair_rec, scale, skey, s_min);
           __syncthreads();
@@ -3019,6 +3090,7 @@ struct d2_ctx {
   DBuf<double> min_time, scale;
   // post
   DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, huge, n_big, scan_part;
+  DBuf<int> hscratch;  // 6 x contact capacity: the filter of pairs above every LDS capacity
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
   // clusters
@@ -3287,6 +3359,7 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
   RC(ensure(c, c->conn_mask, (size_t)c->cap * CONN_MASK_WORDS));
   RC(ensure(c, c->contacts, c->cap_contacts)); RC(ensure(c, c->cnt, C_COUNT));
+  RC(ensure(c, c->hscratch, 6 * (size_t)c->cap_contacts));
   RC(ensure(c, c->lvl, (size_t)2 * (MAX_LEVELS + 1) * LV_STRIDE));
   RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
   RC(ensure(c, c->pass_min_s, np)); RC(ensure(c, c->pass_min_c, np)); RC(ensure(c, c->pass_flag_s, np));
@@ -3449,16 +3522,21 @@ int post_grid(const d2_ctx* c, int mode) {
 // launch's LDS pass, larger ones by the second launch's per-pair forms (with
 // the filter, one lane per pair; n_big[0..1] count the two lists, zeroed
 // before).  mode 1 applies filterContacts(.., margin).
+HugeScratch huge_scratch(d2_ctx* c) {
+  const size_t m = c->hscratch.n / 6;
+  int* h = c->hscratch.p;
+  return HugeScratch{h, h + m, h + 2 * m, h + 3 * m, h + 4 * m, h + 5 * m};
+}
 void launch_finish(d2_ctx* c, int np, int wblocks, int mode, unsigned long long* kept_total, double margin) {
   hipLaunchKernelGGL(k_pair_finish_block<BLK_CAP>, dim3(wblocks), dim3(256), 0, c->stream, np, (const int*)nullptr,
                      (const int*)nullptr, c->contacts.p, c->pair_off.p, c->perm.p, mode, c->keep.p, c->kept_cnt.p,
                      c->pair_rec.p, c->scale.p, c->big.p, c->n_big.p, c->huge.p, c->n_big.p + 1,
-                     (const int*)nullptr, (const int*)nullptr, kept_total, margin);
+                     (const int*)nullptr, (const int*)nullptr, kept_total, margin, huge_scratch(c));
   hipLaunchKernelGGL(k_pair_finish_block<POST_CAP_LARGE>, dim3(std::max(1, std::min(np, c->cus))), dim3(256), 0,
                      c->stream, np, (const int*)c->big.p, (const int*)c->n_big.p, c->contacts.p, c->pair_off.p,
                      c->perm.p, mode, c->keep.p, c->kept_cnt.p, c->pair_rec.p, c->scale.p, c->huge.p,
                      c->n_big.p + 1, c->huge.p, c->n_big.p + 1, (const int*)c->huge.p,
-                     (const int*)(c->n_big.p + 1), kept_total, margin);
+                     (const int*)(c->n_big.p + 1), kept_total, margin, huge_scratch(c));
 }
 
 // Post-processing of the contacts of the last phase: per-pair order, then the
@@ -4017,7 +4095,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->min_time), release(c->scale);
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
-  release(c->big), release(c->huge), release(c->n_big);
+  release(c->big), release(c->huge), release(c->n_big), release(c->hscratch);
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster), release(c->read_later);
   release(c->touch_base), release(c->touch_lv), release(c->touch_run), release(c->touch_lvb), release(c->touch_runb),
@@ -4472,7 +4550,7 @@ int d2_filter_contacts(d2_ctx* c, int n, const double* reals, double margin, int
   RC(ensure(c, c->contacts, m)); RC(ensure(c, c->perm, m)); RC(ensure(c, c->keep, m));
   RC(ensure(c, c->kept_cnt, 2)); RC(ensure(c, c->pair_off, 2)); RC(ensure(c, c->n_big, 2));
   RC(ensure(c, c->big, 1)); RC(ensure(c, c->huge, 1)); RC(ensure(c, c->pair_rec, PR_STRIDE)); RC(ensure(c, c->scale, 1));
-  RC(ensure(c, c->bk_out_r, 11 * m));
+  RC(ensure(c, c->bk_out_r, 11 * m)); RC(ensure(c, c->hscratch, 6 * m));
   // the run buffers now hold this call's contacts: no earlier result stays readable
   c->last_mode = -1;
   c->ccd_valid = false;

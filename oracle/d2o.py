@@ -99,6 +99,14 @@ def lib():
         L.d2o_clusters.restype = ctypes.c_int
         L.d2o_interpolate.argtypes = [_dp, _dp, ctypes.c_double, _dp]
         L.d2o_interpolate.restype = ctypes.c_int
+        L.d2o_solver_props.argtypes = [_vp, _dp, _dp]
+        L.d2o_solve_cluster.argtypes = [_vp, ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_double, _dp,
+                                        ctypes.POINTER(ctypes.c_int)]
+        L.d2o_solve_cluster.restype = ctypes.c_int
+        L.d2o_solve_results.argtypes = [_vp, _dp]
+        L.d2o_warm_get.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
+        L.d2o_warm_get.restype = ctypes.c_int
+        L.d2o_commit.argtypes = [_vp, _dp]
         _lib = L
     return _lib
 
@@ -288,6 +296,36 @@ class Scene:
     def cluster_step(self, pairs, dt):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         return lib().d2o_cluster_step(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), float(dt))
+
+    def solver_props(self, mass, friction):
+        """Per-body mass and friction coefficient for the solver."""
+        self._mass, self._fric = _d(mass, self.n_bodies), _d(friction, self.n_bodies)
+        lib().d2o_solver_props(self.h, _p(self._mass), _p(self._fric))
+
+    def solve_cluster(self, bodies, pairs, t, g=(0.0, 0.0, -9.8)):
+        """SequentialImpulses::solve for one cluster (its bodies, its pairs in
+        order) at step t: (solved, per-hit [n, 11]: normal, impulse, impulse
+        offset, friction impulse, tangential velocity, in gathered order)."""
+        bodies = np.ascontiguousarray(bodies, dtype=np.int32).ravel()
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        gv = _d(g, 3)
+        nh = ctypes.c_int(0)
+        ok = lib().d2o_solve_cluster(self.h, bodies.size, bodies.ctypes.data_as(_ip), pairs.shape[0],
+                                     pairs.ctypes.data_as(_ip), float(t), _p(gv), ctypes.byref(nh))
+        out = np.zeros((nh.value, 11))
+        if ok:
+            lib().d2o_solve_results(self.h, _p(out))
+        return bool(ok), out
+
+    def warm(self, a, b):
+        """The warm store's impulse for bodies (a, b), or None."""
+        v = np.zeros(1)
+        return float(v[0]) if lib().d2o_warm_get(self.h, int(a), int(b), _p(v)) else None
+
+    def commit(self, step):
+        """current <- future, future <- extrapolate(step) for dynamic bodies."""
+        st = _d(step, self.n_bodies)
+        lib().d2o_commit(self.h, _p(st))
 
     def time_pass(self, pairs, max_time, threads=1, repeats=5):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)

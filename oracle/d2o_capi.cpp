@@ -21,6 +21,7 @@
 #include <atomic>
 
 #include "d2o_broad.hpp"
+#include "d2o_solver.hpp"
 #include "d2o_tree.hpp"
 
 using namespace d2o;
@@ -69,6 +70,9 @@ struct Scene {
   std::vector<double> out_r;
   std::vector<int32_t> out_off;
   std::vector<double> out_pair;  // per pair: step scale (CCD) / min_time_out
+  std::vector<SolverBody> props;  // per body: mass, friction coefficient (the solver)
+  WarmStore store;                // the solver's warm start, kept across solves
+  std::vector<SolverOut> sout;    // per hit of the last solve
 };
 
 void exportContacts(Scene& sc, int pair, const std::vector<Contact>& cs) {
@@ -517,6 +521,70 @@ extern "C" double d2o_cluster_step(void* h, int n_pairs, const int32_t* pairs, f
   }
   if (cluster_step < 0.0) cluster_step = local.empty() ? (double)dt : sc->bodies[local[0]].last_time_step_size;
   return std::max(1e-3, std::min(cluster_step, (double)dt));
+}
+
+// Sequential impulses (sequential_impulses.cpp:104-638) for one cluster.
+// mass/friction per scene body must be set first (d2o_solver_props).  The
+// cluster's hits are gathered as PDEExplicit::step gathers them
+// (PDE_explicit.cpp:77-95): per pair in order, compareTreesFull then
+// filterContacts(.., 0.0), at the bodies' current future states.  Returns 1
+// (solved: future states replaced, warm store updated) or 0 (the reference
+// returns false), and the hit count in *n_hits; d2o_solve_results reads the
+// per-hit values (11 doubles: normal, impulse, impulse offset, friction
+// impulse, tangential velocity) in gathered order.
+extern "C" void d2o_solver_props(void* h, const double* mass, const double* friction) {
+  Scene* sc = static_cast<Scene*>(h);
+  sc->props.resize(sc->bodies.size());
+  for (size_t i = 0; i < sc->bodies.size(); ++i) sc->props[i].mass = mass[i], sc->props[i].friction = friction[i];
+}
+extern "C" int d2o_solve_cluster(void* h, int n_bodies, const int32_t* bodies, int n_pairs, const int32_t* pairs,
+                                 double t, const double* g, int* n_hits) {
+  Scene* sc = static_cast<Scene*>(h);
+  std::vector<SolverHit> hits;
+  for (int p = 0; p < n_pairs; ++p) {
+    Body& a = sc->bodies[pairs[2 * p]];
+    Body& b = sc->bodies[pairs[2 * p + 1]];
+    std::vector<Contact> cs = compareTreesFull(a, b);
+    cs = filterContacts(cs, 0.0);
+    for (const Contact& c : cs) {
+      SolverHit sh;
+      sh.ct = c, sh.a = pairs[2 * p], sh.b = pairs[2 * p + 1], sh.row = (int)hits.size();
+      hits.push_back(sh);
+    }
+  }
+  std::vector<int> cluster(bodies, bodies + n_bodies);
+  sc->sout.assign(hits.size(), SolverOut());
+  if (n_hits) *n_hits = (int)hits.size();
+  return solveCluster(sc->bodies, sc->props, cluster, hits, t, V(g[0], g[1], g[2]), sc->store, sc->sout) ? 1 : 0;
+}
+extern "C" void d2o_solve_results(void* h, double* out) {
+  Scene* sc = static_cast<Scene*>(h);
+  for (size_t i = 0; i < sc->sout.size(); ++i) {
+    const SolverOut& o = sc->sout[i];
+    double v[11] = {o.normal.x, o.normal.y, o.normal.z, o.impulse, o.impulse_offset, o.friction.x, o.friction.y,
+                    o.friction.z, o.v_tangent.x, o.v_tangent.y, o.v_tangent.z};
+    std::memcpy(out + 11 * i, v, sizeof(v));
+  }
+}
+// The warm store's value for the body pair (a, b); returns 0 when absent.
+extern "C" int d2o_warm_get(void* h, int a, int b, double* value) {
+  Scene* sc = static_cast<Scene*>(h);
+  auto it = sc->store.find(std::make_pair(std::min(a, b), std::max(a, b)));
+  if (it == sc->store.end()) return 0;
+  *value = it->second;
+  return 1;
+}
+// PDEExplicit::step's tail for the dynamic bodies (PDE_explicit.cpp:117-131):
+// current <- future, future <- current.extrapolate(step); step per body.
+extern "C" void d2o_commit(void* h, const double* step) {
+  Scene* sc = static_cast<Scene*>(h);
+  for (size_t i = 0; i < sc->bodies.size(); ++i) {
+    Body& b = sc->bodies[i];
+    if (b.is_static) continue;
+    b.current = b.future;
+    b.last_time_step_size = step[i];
+    b.projectFutureState(step[i]);
+  }
 }
 
 // Timed CPU baseline: runs CCD + step rule, then discrete + filter, for every

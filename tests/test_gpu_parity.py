@@ -15,7 +15,7 @@ import pytest
 
 import d2o
 import delta2_amd as d2
-from gpu_cases import c1_scene, dense_scene, hull_scene, oracle_scene, with_future
+from gpu_cases import c1_scene, dense_scene, hull_scene, oracle_scene, sparse_scene, with_future
 from delta2_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -73,7 +73,8 @@ def load(ctx, s, records):
     ctx.upload_bodies(s.tree_of_body, records)
 
 
-SCENES = [("c1", c1_scene), ("dense", dense_scene), ("hulls", hull_scene)]
+# c2_256: the 4096-body mixed workload's generator at 256 bodies
+SCENES = [("c1", c1_scene), ("dense", dense_scene), ("hulls", hull_scene), ("c2_256", sparse_scene)]
 
 
 @pytest.mark.parametrize("name,make", SCENES)

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import delta2_amd as d2
-from gpu_cases import c1_scene, dense_scene, oracle_scene, with_future
+from gpu_cases import c1_scene, dense_scene, oracle_scene, sparse_scene, with_future
 from delta2_amd import scenes
 
 pytestmark = pytest.mark.gpu
@@ -58,7 +58,7 @@ def ctx():
     c.close()
 
 
-@pytest.mark.parametrize("name,make", [("c1", c1_scene), ("dense", dense_scene)])
+@pytest.mark.parametrize("name,make", [("c1", c1_scene), ("dense", dense_scene), ("c2_256", sparse_scene)])
 def test_step_matches_oracle(ctx, name, make):
     s = make()
     print(f"[step] scene={name} pairs={s.pairs.shape[0]}", flush=True)
