@@ -221,10 +221,11 @@ class PileStepper:
     (broad_phase_embree_cluster.cpp:126-132), then broad phase and cluster
     separation run on the device, then the timed d2_step_run."""
 
-    def __init__(self, sc, prec, weights=None):
+    def __init__(self, sc, prec, weights=None, prune=True):
         import delta2_amd as d2
         self.sc = sc
         self.ctx = d2.Context(0, prec)
+        self.ctx.set_pruning(prune)
         t0 = time.time()
         self.ctx.upload_trees(sc.trees)
         self.upload_s = time.time() - t0
@@ -351,7 +352,7 @@ def run_pile(args):
     plain_out, identical = None, None
     step_bytes = None
     if not args.no_plain:
-        plain = PileStepper(sc, d2.D2_FP64)
+        plain = PileStepper(sc, d2.D2_FP64, prune=False)  # the reference's traversal, every node pair
         log(f"[c5] plain fp64: trees uploaded in {plain.upload_s:.1f}s")
         n, s = plain.step(states[0])
         log(f"[c5] plain fp64 first step (sizes the buffers): {1e3 * s:.1f} ms")
@@ -425,6 +426,7 @@ def make_ctx(scene, prec, graph, weights=None, clusters=None, tune=True):
     import delta2_amd as d2
     from delta2_amd import scenes
     ctx = d2.Context(0, prec)
+    ctx.set_pruning(prec == d2.D2_FP32)  # FP64 contexts are the plain path: every node pair of the reference
     ctx.upload_trees(scene.trees)
     ctx.upload_bodies(scene.tree_of_body, scene.records)
     ctx.upload_pairs(scene.pairs)
@@ -586,6 +588,7 @@ def run_c3(args):
     out = {}
     for name, prec, graph in (("batched_fp32_graph", d2.D2_FP32, True), ("batched_fp64_graph", d2.D2_FP64, True)):
         ctx = d2.Context(0, prec)  # capacities grow on demand
+        ctx.set_pruning(prec == d2.D2_FP32)
         ctx.upload_trees(sc.trees)
         ctx.upload_bodies(sc.tree_of_body, sc.records)
         ctx.upload_pairs(sc.pairs)
@@ -611,6 +614,7 @@ def run_c3(args):
         ctx.close()
     # plain: one scene at a time
     ctx = d2.Context(0, d2.D2_FP64)
+    ctx.set_pruning(False)
     ctx.upload_trees(sc.trees)
     ctx.upload_bodies(sc.tree_of_body, sc.records)
     sob = sc.scene_of_body[sc.pairs[:, 0]]

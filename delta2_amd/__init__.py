@@ -78,6 +78,11 @@ def lib():
         "d2_bodies_upload": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
         "d2_bodies_download": ([_vp, _dp], ctypes.c_int),
         "d2_project_future": ([_vp, _dp], ctypes.c_int),
+        "d2_solver_bodies": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
+        "d2_solve": ([_vp, _dp, _ip, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "d2_solve_results": ([_vp, _dp], ctypes.c_int),
+        "d2_commit_step": ([_vp], ctypes.c_int),
+        "d2_solver_reset": ([_vp], ctypes.c_int),
         "d2_pairs_upload": ([_vp, ctypes.c_int, _ip, _dp], ctypes.c_int),
         "d2_narrow_run": ([_vp, ctypes.c_int, _i64p], ctypes.c_int),
         "d2_ccd_run": ([_vp, _i64p], ctypes.c_int),
@@ -119,6 +124,7 @@ def lib():
         "d2_clusters_build": ([_vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
         "d2_clusters_download": ([_vp, _ip, _ip, _ip, _dp, _dp], ctypes.c_int),
         "d2_set_touch_stats": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_set_pruning": ([_vp, ctypes.c_int], ctypes.c_int),
         "d2_touch_stats": ([_vp, _i64p, ctypes.c_int], ctypes.c_int),
         "d2_find_contacts_bucket": ([_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _ip, ctypes.c_int64, _ip, _dp,
                                      _i64p], ctypes.c_int),
@@ -321,6 +327,36 @@ class Context:
         self.n_out = n.value
         return (n.value, steps) if want_steps else n.value
 
+    def solver_bodies(self, mass, friction):
+        """Per-body mass and friction coefficient for the solver."""
+        m, f = _f64(mass), _f64(friction)
+        self._check(lib().d2_solver_bodies(self.h, m.size, _dptr(m), _dptr(f)))
+
+    def solve(self, gravity=(0.0, 0.0, -9.8)):
+        """SequentialImpulses::solve over every cluster of the last filtered
+        step run (the future states are replaced): (per-cluster status, 1
+        solved / 0 refused; number refused)."""
+        g = _f64(gravity)
+        status = np.zeros(max(self.n_clusters, 1), dtype=np.int32)
+        nf = ctypes.c_int(0)
+        self._check(lib().d2_solve(self.h, _dptr(g), _iptr(status), ctypes.byref(nf)))
+        return status[:self.n_clusters], nf.value
+
+    def solve_results(self):
+        """Per contact of the last step run's filtered result (its CSR order):
+        normal, impulse, impulse offset, friction impulse, tangential velocity."""
+        out = np.zeros((max(self.n_out, 1), 11))
+        self._check(lib().d2_solve_results(self.h, _dptr(out)))
+        return out[:self.n_out]
+
+    def commit_step(self):
+        """current <- future, last step <- the step, future <- extrapolate (dynamic bodies)."""
+        self._check(lib().d2_commit_step(self.h))
+
+    def solver_reset(self):
+        """Empty the solver's warm store."""
+        self._check(lib().d2_solver_reset(self.h))
+
     def set_graph(self, on=True):
         self._check(lib().d2_set_graph(self.h, int(bool(on))))
 
@@ -458,6 +494,10 @@ class Context:
         self._check(lib().d2_filter_contacts(self.h, n, _dptr(r), float(margin), _iptr(order), _iptr(kept),
                                              ctypes.byref(k)))
         return order[:n], kept[:k.value]
+
+    def set_pruning(self, on=True):
+        """Skip discrete node pairs whose leaves are provably apart (results unchanged)."""
+        self._check(lib().d2_set_pruning(self.h, int(bool(on))))
 
     def set_touch_stats(self, on=True):
         """Accounting runs: record the (body, node) records each level tests."""

@@ -98,6 +98,86 @@ template <class R> D2_HD void inv44(const R* a, R* b) {
   b[15] = ((a[8] * s3 - a[9] * s1) + a[10] * s0) * id;
 }
 
+// A leaf triangle's reach beyond the search distance: its bucket eps plus the
+// largest distance a float-barycentric "inside" test can accept a point
+// outside it, 64 eps_f kappa L (kappa = d00 d11 / (d00 d11 - d01^2), L the
+// longest edge; 4x the 16 eps_f kappa L rounding bound).
+D2_HD double leaf_slack(const double* a, const double* b, const double* c, double eps) {
+  double e0[3], e1[3], e2[3];
+  for (int d = 0; d < 3; ++d) e0[d] = b[d] - a[d], e1[d] = c[d] - a[d], e2[d] = c[d] - b[d];
+  const double d00 = e0[0] * e0[0] + e0[1] * e0[1] + e0[2] * e0[2];
+  const double d11 = e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2];
+  const double d01 = e0[0] * e1[0] + e0[1] * e1[1] + e0[2] * e1[2];
+  const double d22 = e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2];
+  const double den = d00 * d11 - d01 * d01;
+  const double kappa = den > 0.0 ? d00 * d11 / den : 1e30;
+  const double L2 = d00 > d11 ? (d00 > d22 ? d00 : d22) : (d11 > d22 ? d11 : d22);
+  return eps + 64.0 * 5.9604645e-08 * kappa * sqrt(L2);
+}
+// Whether no leaf triangle under node a (pose Ta) comes within reach of one
+// under node b (pose Tb), so that the node pair's subtree holds no contact:
+// its discrete leaf test accepts a triangle pair when the computed distance
+// is below search + eps_i + eps_j (comparison_dist.h:233), and the computed
+// distance undercuts the true one by at most the float-barycentric
+// acceptance bound, both folded into each node's slack E.  The lower bound on
+// the true distance is the separating-axis gap of the two oriented leaf
+// boxes over the 15 axes of two OBBs (3 + 3 face normals, 9 edge
+// directions, each gap divided by the axis length), in FP64 with a 1e-9
+// relative and absolute margin.  Pruning such a node pair leaves every
+// contact of the traversal unchanged: the reference reaches no contact
+// below it.
+D2_HD bool nodes_apart(const double* Ta, const double* Tb, const float* ba, const float* bb,
+                         double search) {
+  const double thr = ((search + (double)ba[6]) + (double)bb[6]) * (1.0 + 1e-9) + 1e-9;
+  double ca[3], ea[3], cb[3], eb[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    ca[k] = 0.5 * ((double)ba[k] + (double)ba[3 + k]), ea[k] = 0.5 * ((double)ba[3 + k] - (double)ba[k]);
+    cb[k] = 0.5 * ((double)bb[k] + (double)bb[3 + k]), eb[k] = 0.5 * ((double)bb[3 + k] - (double)bb[k]);
+  }
+  // centres in the world, then R = Ra^T Rb and t = Ra^T (Cb - Ca)
+  double d[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double wa = ((Ta[4 * r] * ca[0] + Ta[4 * r + 1] * ca[1]) + Ta[4 * r + 2] * ca[2]) + Ta[4 * r + 3];
+    const double wb = ((Tb[4 * r] * cb[0] + Tb[4 * r + 1] * cb[1]) + Tb[4 * r + 2] * cb[2]) + Tb[4 * r + 3];
+    d[r] = wb - wa;
+  }
+  double R[3][3], A[3][3], t[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    t[i] = (Ta[i] * d[0] + Ta[4 + i] * d[1]) + Ta[8 + i] * d[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      R[i][j] = (Ta[i] * Tb[j] + Ta[4 + i] * Tb[4 + j]) + Ta[8 + i] * Tb[8 + j];
+      A[i][j] = fabs(R[i][j]);
+    }
+  }
+  double gap = -1.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)  // a's face normals
+    gap = fmax(gap, fabs(t[i]) - (ea[i] + ((eb[0] * A[i][0] + eb[1] * A[i][1]) + eb[2] * A[i][2])));
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {  // b's face normals
+    const double tj = (t[0] * R[0][j] + t[1] * R[1][j]) + t[2] * R[2][j];
+    gap = fmax(gap, fabs(tj) - (((ea[0] * A[0][j] + ea[1] * A[1][j]) + ea[2] * A[2][j]) + eb[j]));
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {  // a_i x b_j, length sqrt(1 - R_ij^2)
+    const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+      const double len2 = 1.0 - R[i][j] * R[i][j];
+      if (!(len2 > 1e-12)) continue;  // (nearly) parallel edges: the face axes cover them
+      const double ra = ea[i1] * A[i2][j] + ea[i2] * A[i1][j];
+      const double rb = eb[j1] * A[i][j2] + eb[j2] * A[i][j1];
+      const double s = fabs(t[i2] * R[i1][j] - t[i1] * R[i2][j]) - (ra + rb);
+      gap = fmax(gap, s / sqrt(len2));
+    }
+  }
+  return gap > thr;
+}
 // ---- rigid-body state (model/state.cpp) -----------------------------------
 struct quat {
   double w, x, y, z;

@@ -39,7 +39,9 @@
 
 #include <algorithm>
 #include <array>
+#include <cfloat>
 #include <chrono>
+#include <cmath>
 #include <random>
 #include <cstdio>
 #include <cstring>
@@ -51,6 +53,8 @@
 #include "../../include/d2hip.h"
 #define D2_HD __host__ __device__ __forceinline__
 #include "d2_geom.hpp"
+#include "d2_sort.hpp"
+#include "d2_solver.hpp"
 
 namespace d2 {
 
@@ -191,6 +195,8 @@ struct Dev {
   int *pass_flag_s, *pass_flag_c;
   double* min_time;
   const float* tree_eps_out;   // float-barycentric acceptance bound per tree
+  const float* nbound;         // per node: leaf box and slack (nodes_apart)
+  int prune;                   // skip node pairs nodes_apart proves contact-free
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
   const int32_t* body_tree;
   int fp32;                    // FP32 culls of the CCD leaf sub-tests
@@ -1058,8 +1064,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
       l_pair = (int)np.pair, l_offa = pi[2], l_offb = pi[3];
       const NodeRec na = D.nodes[l_offa + np.na];
       const NodeRec nb = D.nodes[l_offb + np.nb];
-      // a child of a parent that did not hit: no work, no children
-      if (np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) {
+      // a child of a parent that did not hit, or a node pair whose leaves are
+      // provably apart: no work, no children
+      const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+      if ((np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) &&
+          !(D.prune && nodes_apart(pr + PR_TA1, pr + PR_TB1, D.nbound + 8 * ((size_t)l_offa + np.na),
+                                   D.nbound + 8 * ((size_t)l_offb + np.nb), pr[PR_SEARCH]))) {
         l_np = np;
         l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
         l_inner = (na.inner || nb.inner) ? 1 : 0;
@@ -1244,8 +1254,11 @@ __global__ void __launch_bounds__(256) k_tri_full(Dev D) {
     if (w < n) {
       const NP np = D.soup[w];
       // a child of a parent that did not hit: no work, no children
-      if (np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) {
-        const int32_t* pi = D.pair_info + 4 * np.pair;
+      const int32_t* pi = D.pair_info + 4 * np.pair;
+      const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+      if ((np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) &&
+          !(D.prune && nodes_apart(pr + PR_TA1, pr + PR_TB1, D.nbound + 8 * ((size_t)pi[2] + np.na),
+                                   D.nbound + 8 * ((size_t)pi[3] + np.nb), pr[PR_SEARCH]))) {
         const NodeRec na = D.nodes[pi[2] + np.na];
         const NodeRec nb = D.nodes[pi[3] + np.nb];
         f.pair = (int)np.pair, f.ga = pi[2] + na.bucket, f.gb = pi[3] + nb.bucket;
@@ -1937,142 +1950,6 @@ __global__ void __launch_bounds__(256) k_scatter(const unsigned long long* cnt, 
   }
 }
 
-// contact.h:35-43
-__device__ __forceinline__ bool contact_less(const ContactRec& x, const ContactRec& y) {
-  double xe = x.r[6] + x.r[7], ye = y.r[6] + y.r[7];
-  if (xe == ye) {
-    double dx = vlen(sub(v3<double>{x.r[0], x.r[1], x.r[2]}, v3<double>{x.r[3], x.r[4], x.r[5]}));
-    double dy = vlen(sub(v3<double>{y.r[0], y.r[1], y.r[2]}, v3<double>{y.r[3], y.r[4], y.r[5]}));
-    return dx < dy;
-  }
-  return xe < ye;
-}
-
-// The permutation libstdc++'s std::sort produces (introsort: median-of-three
-// pivot to the front, unguarded Hoare partition, depth limit 2*floor(log2 n),
-// heapsort fallback, insertion sort threshold 16).  Reproducing it keeps the
-// contact filter's choices among exactly tied contacts identical.
-struct Sorter {
-  int* a;
-  const ContactRec* c;
-  __device__ bool lt(int x, int y) const { return contact_less(c[x], c[y]); }
-  __device__ void swp(int i, int j) {
-    int t = a[i];
-    a[i] = a[j];
-    a[j] = t;
-  }
-  __device__ void median_to_first(int res, int x, int y, int z) {
-    if (lt(a[x], a[y])) {
-      if (lt(a[y], a[z])) swp(res, y);
-      else if (lt(a[x], a[z])) swp(res, z);
-      else swp(res, x);
-    } else if (lt(a[x], a[z])) swp(res, x);
-    else if (lt(a[y], a[z])) swp(res, z);
-    else swp(res, y);
-  }
-  __device__ int partition(int first, int last, int pivot) {
-    while (true) {
-      while (lt(a[first], a[pivot])) ++first;
-      --last;
-      while (lt(a[pivot], a[last])) --last;
-      if (!(first < last)) return first;
-      swp(first, last);
-      ++first;
-    }
-  }
-  __device__ void push_heap(int first, int hole, int top, int val) {
-    int parent = (hole - 1) / 2;
-    while (hole > top && lt(a[first + parent], val)) {
-      a[first + hole] = a[first + parent];
-      hole = parent;
-      parent = (hole - 1) / 2;
-    }
-    a[first + hole] = val;
-  }
-  __device__ void adjust_heap(int first, int hole, int len, int val) {
-    int top = hole, child = hole;
-    while (child < (len - 1) / 2) {
-      child = 2 * (child + 1);
-      if (lt(a[first + child], a[first + child - 1])) child--;
-      a[first + hole] = a[first + child];
-      hole = child;
-    }
-    if ((len & 1) == 0 && child == (len - 2) / 2) {
-      child = 2 * (child + 1);
-      a[first + hole] = a[first + child - 1];
-      hole = child - 1;
-    }
-    push_heap(first, hole, top, val);
-  }
-  __device__ void heap_sort(int first, int last) {
-    int len = last - first;
-    if (len >= 2) {
-      for (int parent = (len - 2) / 2;; --parent) {
-        adjust_heap(first, parent, len, a[first + parent]);
-        if (parent == 0) break;
-      }
-    }
-    while (last - first > 1) {
-      --last;
-      int v = a[last];
-      a[last] = a[first];
-      adjust_heap(first, 0, last - first, v);
-    }
-  }
-  __device__ void linear_insert(int last) {
-    int v = a[last], next = last - 1;
-    while (lt(v, a[next])) {
-      a[last] = a[next];
-      last = next;
-      --next;
-    }
-    a[last] = v;
-  }
-  __device__ void insertion(int first, int last) {
-    if (first == last) return;
-    for (int i = first + 1; i != last; ++i) {
-      if (lt(a[i], a[first])) {
-        int v = a[i];
-        for (int k = i; k > first; --k) a[k] = a[k - 1];
-        a[first] = v;
-      } else {
-        linear_insert(i);
-      }
-    }
-  }
-  __device__ void sort(int first, int last) {
-    if (first == last) return;
-    int n = last - first;
-    int depth = 2 * (63 - __clzll((long long)n));
-    int st_f[64], st_l[64], st_d[64], sp = 0;
-    st_f[sp] = first, st_l[sp] = last, st_d[sp] = depth, ++sp;
-    while (sp) {
-      --sp;
-      int f = st_f[sp], l = st_l[sp], d = st_d[sp];
-      while (l - f > 16) {
-        if (d == 0) {
-          heap_sort(f, l);
-          break;
-        }
-        --d;
-        int mid = f + (l - f) / 2;
-        median_to_first(f, f + 1, mid, l - 1);
-        int cut = partition(f + 1, l, f);
-        st_f[sp] = cut, st_l[sp] = l, st_d[sp] = d, ++sp;  // processed independently
-        l = cut;
-      }
-    }
-    if (n > 16) {
-      insertion(first, first + 16);
-      for (int i = first + 16; i != last; ++i) linear_insert(i);
-    } else {
-      insertion(first, last);
-    }
-  }
-};
-
-// Per pair: restore the breadth-first order (insertion sort on the key), then
-// either the contact filter (discrete) or the step rule (CCD).
 // The step rule's scale for one contact (separate_clusters.cpp:496-564):
 // false when the contact sets none.  k: the pair's future-state kinematics.
 struct StepKin {
@@ -2128,126 +2005,6 @@ __device__ __forceinline__ bool lds_less(const LdsKey& x, const LdsKey& y) {
   if (x.eps == y.eps) return x.dist < y.dist;
   return x.eps < y.eps;
 }
-// Sorter over LDS operands (same algorithm): keys are integer ranks, equal
-// exactly when the (eps, dist) keys are equal, so comparisons are unchanged.
-struct LdsSorter {
-  int* a;
-  const int* k;
-  __device__ bool lt(int x, int y) const { return k[x] < k[y]; }
-  __device__ void swp(int i, int j) {
-    int t = a[i];
-    a[i] = a[j];
-    a[j] = t;
-  }
-  __device__ void median_to_first(int res, int x, int y, int z) {
-    if (lt(a[x], a[y])) {
-      if (lt(a[y], a[z])) swp(res, y);
-      else if (lt(a[x], a[z])) swp(res, z);
-      else swp(res, x);
-    } else if (lt(a[x], a[z])) swp(res, x);
-    else if (lt(a[y], a[z])) swp(res, z);
-    else swp(res, y);
-  }
-  __device__ int partition(int first, int last, int pivot) {
-    while (true) {
-      while (lt(a[first], a[pivot])) ++first;
-      --last;
-      while (lt(a[pivot], a[last])) --last;
-      if (!(first < last)) return first;
-      swp(first, last);
-      ++first;
-    }
-  }
-  __device__ void push_heap(int first, int hole, int top, int val) {
-    int parent = (hole - 1) / 2;
-    while (hole > top && lt(a[first + parent], val)) {
-      a[first + hole] = a[first + parent];
-      hole = parent;
-      parent = (hole - 1) / 2;
-    }
-    a[first + hole] = val;
-  }
-  __device__ void adjust_heap(int first, int hole, int len, int val) {
-    int top = hole, child = hole;
-    while (child < (len - 1) / 2) {
-      child = 2 * (child + 1);
-      if (lt(a[first + child], a[first + child - 1])) child--;
-      a[first + hole] = a[first + child];
-      hole = child;
-    }
-    if ((len & 1) == 0 && child == (len - 2) / 2) {
-      child = 2 * (child + 1);
-      a[first + hole] = a[first + child - 1];
-      hole = child - 1;
-    }
-    push_heap(first, hole, top, val);
-  }
-  __device__ void heap_sort(int first, int last) {
-    int len = last - first;
-    if (len >= 2)
-      for (int parent = (len - 2) / 2;; --parent) {
-        adjust_heap(first, parent, len, a[first + parent]);
-        if (parent == 0) break;
-      }
-    while (last - first > 1) {
-      --last;
-      int v = a[last];
-      a[last] = a[first];
-      adjust_heap(first, 0, last - first, v);
-    }
-  }
-  __device__ void linear_insert(int last) {
-    int v = a[last], next = last - 1;
-    while (lt(v, a[next])) {
-      a[last] = a[next];
-      last = next;
-      --next;
-    }
-    a[last] = v;
-  }
-  __device__ void insertion(int first, int last) {
-    if (first == last) return;
-    for (int i = first + 1; i != last; ++i) {
-      if (lt(a[i], a[first])) {
-        int v = a[i];
-        for (int q = i; q > first; --q) a[q] = a[q - 1];
-        a[first] = v;
-      } else {
-        linear_insert(i);
-      }
-    }
-  }
-  __device__ void sort(int first, int last) {
-    if (first == last) return;
-    int n = last - first;
-    int depth = 2 * (63 - __clzll((long long)n));
-    int st_f[64], st_l[64], st_d[64], sp = 0;
-    st_f[sp] = first, st_l[sp] = last, st_d[sp] = depth, ++sp;
-    while (sp) {
-      --sp;
-      int f = st_f[sp], l = st_l[sp], d = st_d[sp];
-      while (l - f > 16) {
-        if (d == 0) {
-          heap_sort(f, l);
-          break;
-        }
-        --d;
-        int mid = f + (l - f) / 2;
-        median_to_first(f, f + 1, mid, l - 1);
-        int cut = partition(f + 1, l, f);
-        st_f[sp] = cut, st_l[sp] = l, st_d[sp] = d, ++sp;
-        l = cut;
-      }
-    }
-    if (n > 16) {
-      insertion(first, first + 16);
-      for (int i = first + 16; i != last; ++i) linear_insert(i);
-    } else {
-      insertion(first, last);
-    }
-  }
-};
-
 // The same permutation computed by the whole wavefront.  Each Hoare partition
 // of [lo, hi) around the pivot at lo-1 is derived from ranks: with A_k the k-th
 // position from the left holding an element not less than the pivot and B_k
@@ -2329,7 +2086,7 @@ __device__ void block_std_sort(int n, int* ord, const int* rk, int* Apos, int* B
       int d = L[3 * q + 2];
       if (d == 0) {  // depth exhausted: heapsort (libstdc++ __partial_sort) on one lane
         if (lane == 0) {
-          LdsSorter srt{ord, rk};
+          StdSort<RankLess> srt{ord, RankLess{rk}};
           srt.heap_sort(f, l);
           mark[f] = 2;
         }
@@ -2994,6 +2751,164 @@ __global__ void k_project(int n, double* bodies, const double* step, int skip_st
   }
 }
 
+// ------------------------------------------- sequential impulses (the solver)
+// After a filtered step: the clusters' pair lists and hit counts, then one
+// thread per cluster walks SequentialImpulses::solve (d2_solver.hpp).
+struct SolveArgs {
+  int n_pairs, n_clusters, n_bodies;
+  const int32_t* pairs;
+  const int32_t* cluster_of_pair;
+  const int* kept_cnt;    // per pair: filtered contacts
+  const int* pair_off;    // per pair: first contact position
+  const int* out_off;     // per pair: first output row (filtered CSR)
+  const int* perm;        // contact at each position
+  const int* keep;        // kept flag at each position
+  const ContactRec* contacts;
+  const double* cluster_step;
+  const double* body_step;
+  double* bodies;
+  const double* mass;
+  const double* fric;
+  double gx, gy, gz;
+  int* ccount;  // [n_clusters + 1] pairs per cluster, then their offsets (poff)
+  int* chits;   // [n_clusters + 1] hits per cluster, then offsets (hoff)
+  int* cfill;   // [n_clusters]
+  int* cpairs;  // [n_pairs]
+  int* inpair;  // [n_bodies] 1 + the cluster of the body's pairs, 0: in no pair
+  int* loc;     // [n_bodies] local particle index of a dynamic body
+  int* flag;    // set when a dynamic body's pairs lie in two clusters
+  int* status;  // [n_clusters]
+  int* hl;      // [hits][2]
+  int* ord;     // [hits]
+  int* hrow;    // [hits] output row
+  int* hpc;     // [hits] contacts of the hit's pair
+  int* pbody;   // [2 n_pairs]
+  double* hA;   // [hits][3]
+  double* hB;   // [hits][3]
+  double* heps; // [hits]
+  double* hkey; // [hits] |A - B| / (eps_a + eps_b)
+  double* hs;   // [hits][SH]
+  double* ps;   // [2 n_pairs][SP]
+  double* out;  // [hits][11], cluster-gathered order
+  WarmTable warm;
+};
+__global__ void k_fill_i32(int* p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_sv_count(SolveArgs A) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_pairs; p += gridDim.x * blockDim.x) {
+    const int k = A.cluster_of_pair[p];
+    atomicAdd(&A.ccount[k], 1);
+    atomicAdd(&A.chits[k], A.kept_cnt[p]);
+    for (int e = 0; e < 2; ++e) {  // a dynamic body belongs to the one cluster of its pairs
+      const int b = A.pairs[2 * p + e];
+      const int old = atomicCAS(&A.inpair[b], 0, k + 1);
+      if (old != 0 && old != k + 1 && A.bodies[40 * (size_t)b + 38] == 0.0) atomicOr(A.flag, 1);
+    }
+  }
+}
+// ccount / chits hold the exclusive offsets here (poff / hoff)
+__global__ void k_sv_place(SolveArgs A) {
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < A.n_pairs; p += gridDim.x * blockDim.x) {
+    const int k = A.cluster_of_pair[p];
+    A.cpairs[A.ccount[k] + atomicAdd(&A.cfill[k], 1)] = p;
+  }
+}
+struct IdLess {
+  D2_HD bool operator()(int x, int y) const { return x < y; }
+};
+struct KeyLess {
+  const double* k;
+  D2_HD bool operator()(int x, int y) const { return k[x] < k[y]; }
+};
+// One thread per cluster.  The cluster's pairs in index order (the order its
+// interactions were gathered in, separate_clusters.cpp:405-434), its hits per
+// pair in the filtered order (PDE_explicit.cpp:77-95), the std::sort replay
+// of the hits by Contact::operator< (contact.h:14-18), then the solve.
+__global__ void __launch_bounds__(64) k_sv_solve(SolveArgs A) {
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < A.n_clusters; k += gridDim.x * blockDim.x) {
+    const int p0 = A.ccount[k], np_k = A.ccount[k + 1] - p0;
+    const int h0 = A.chits[k];
+    int* cp = A.cpairs + p0;
+    StdSort<IdLess>{cp, IdLess{}}.sort(0, np_k);
+    int* pb = A.pbody + 2 * (size_t)p0;
+    int n_part = 0, nh = 0;
+    auto local_of = [&](int b) {
+      if (A.bodies[40 * (size_t)b + 38] == 0.0) {  // a dynamic body belongs to one cluster
+        if (A.loc[b] < 0) A.loc[b] = n_part, pb[n_part++] = b;
+        return A.loc[b];
+      }
+      for (int q = 0; q < n_part; ++q)  // a static body: one particle per cluster it touches
+        if (pb[q] == b) return q;
+      pb[n_part] = b;
+      return n_part++;
+    };
+    for (int i = 0; i < np_k; ++i) {
+      const int p = cp[i];
+      const int la = local_of(A.pairs[2 * p]), lb = local_of(A.pairs[2 * p + 1]);
+      int row = A.out_off[p];
+      for (int t = A.pair_off[p]; t < A.pair_off[p + 1]; ++t) {
+        if (!A.keep[t]) continue;
+        const ContactRec& c = A.contacts[A.perm[t]];
+        const size_t H = (size_t)h0 + nh;
+        for (int j = 0; j < 3; ++j) A.hA[3 * H + j] = c.r[j], A.hB[3 * H + j] = c.r[3 + j];
+        A.heps[H] = c.r[6] + c.r[7];
+        A.hkey[H] = vlen(sub(v3<double>{c.r[0], c.r[1], c.r[2]}, v3<double>{c.r[3], c.r[4], c.r[5]})) / A.heps[H];
+        A.hl[2 * H] = la, A.hl[2 * H + 1] = lb;
+        A.hrow[H] = row++;
+        A.hpc[H] = A.kept_cnt[p];
+        A.ord[H] = nh;
+        ++nh;
+      }
+    }
+    StdSort<KeyLess>{A.ord + h0, KeyLess{A.hkey + h0}}.sort(0, nh);
+    ClusterSolve S;
+    S.nh = nh, S.np = n_part;
+    S.hA = A.hA + 3 * (size_t)h0, S.hB = A.hB + 3 * (size_t)h0, S.heps = A.heps + h0, S.hl = A.hl + 2 * (size_t)h0;
+    S.hpc = A.hpc + h0;
+    S.ord = A.ord + h0, S.pbody = pb, S.bodies = A.bodies, S.mass = A.mass, S.fric = A.fric;
+    S.t = A.cluster_step[k], S.g = v3<double>{A.gx, A.gy, A.gz}, S.warm = A.warm;
+    S.hs = A.hs + (size_t)SH * h0, S.ps = A.ps + (size_t)SP * 2 * p0;
+    S.out_h = A.out + 11 * (size_t)h0;
+    A.status[k] = solve_cluster(S) ? 1 : 0;
+  }
+}
+// Bodies in no pair: a cluster of their own without contacts (gravity only).
+__global__ void k_sv_free(SolveArgs A) {
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < A.n_bodies; b += gridDim.x * blockDim.x) {
+    double* r = A.bodies + 40 * (size_t)b;
+    if (A.inpair[b] || r[38] != 0.0) continue;
+    const double t = A.body_step[b];
+    const v3<double> g{A.gx, A.gy, A.gz}, z{0.0, 0.0, 0.0};
+    const SolverState u = sv_update(sv_load(r + 14), t, scl(sclL(A.mass[b], g), t), z, z, z, A.mass[b], r + 28);
+    r[14] = u.q.w, r[15] = u.q.x, r[16] = u.q.y, r[17] = u.q.z;
+    r[18] = u.x.x, r[19] = u.x.y, r[20] = u.x.z, r[21] = u.v.x, r[22] = u.v.y, r[23] = u.v.z;
+    r[24] = u.L.x, r[25] = u.L.y, r[26] = u.L.z;
+  }
+}
+// Per-hit results to the filtered CSR rows.
+__global__ void k_sv_scatter(long long n, const int* hrow, const double* in, double* out) {
+  for (long long h = (long long)blockIdx.x * blockDim.x + threadIdx.x; h < n; h += (long long)gridDim.x * blockDim.x)
+    for (int j = 0; j < 11; ++j) out[11 * (size_t)hrow[h] + j] = in[11 * (size_t)h + j];
+}
+// PDEExplicit::step's tail (PDE_explicit.cpp:117-131) for the dynamic bodies:
+// current <- future, last step <- the body's step, future <- extrapolate.
+__global__ void k_commit(int n, double* bodies, const double* step) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double* r = bodies + 40 * (size_t)i;
+    if (r[38] != 0.0) continue;
+    for (int j = 0; j < 14; ++j) r[j] = r[14 + j];
+    r[39] = step[i];
+    store_state(extrapolate(load_state(r), step[i], r + 28), r + 14);
+  }
+}
+// Re-inserts the keys of an old warm table into a new one.
+__global__ void k_warm_rehash(const unsigned long long* keys, const double* vals, long long n, WarmTable W) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    if (keys[i] != WARM_EMPTY) warm_put(W, keys[i], vals[i]);
+}
+
 }  // namespace d2
 
 // =========================================================================
@@ -3091,6 +3006,19 @@ struct d2_ctx {
   // post
   DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, huge, n_big, scan_part;
   DBuf<int> hscratch;  // 6 x contact capacity: the filter of pairs above every LDS capacity
+  DBuf<float> nbound;  // per node: leaf box lo, hi (body frame), largest leaf eps + slack, 0
+  bool prune_on = true;  // skip node pairs whose leaf triangles are provably apart
+  // the solver (d2_solve): per-body mass and friction, per-cluster and per-hit scratch, the warm store
+  DBuf<double> sv_mass, sv_fric;
+  DBuf<int> sv_ccount, sv_chits, sv_poff, sv_hoff, sv_cfill, sv_cpairs, sv_inpair, sv_loc, sv_status, sv_flag;
+  DBuf<int> sv_hl, sv_ord, sv_hrow, sv_hpc, sv_pbody;
+  DBuf<double> sv_hA, sv_hB, sv_heps, sv_hkey, sv_hs, sv_ps, sv_out, sv_out_csr;
+  DBuf<unsigned long long> warm_keys, warm_used;
+  DBuf<double> warm_vals;
+  int last_kind = -1;         // kind of the last finished run
+  long long run_serial = 0;   // finished runs
+  long long solved_serial = -1;  // run_serial the last d2_solve worked on
+  long long solved_hits = 0;
   DBuf<int32_t> out_i;
   DBuf<double> out_r;
   // clusters
@@ -3326,6 +3254,8 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.pass_flag_s = c->pass_flag_s.p, D.pass_flag_c = c->pass_flag_c.p;
   D.min_time = c->min_time.p;
   D.tree_eps_out = c->tree_eps_out.p;
+  D.nbound = c->nbound.p;
+  D.prune = c->prune_on && !c->touch_on && c->nbound.p ? 1 : 0;  // accounting runs walk the reference's node pairs
   D.conn_mask = c->conn_mask.p;
   D.body_tree = c->body_tree.p;
   D.fp32 = c->precision == D2_FP32 ? 1 : 0;
@@ -3880,6 +3810,8 @@ int finish_run(d2_ctx* c, int kind, int filter) {
     if (kind == RUN_STEP) c->prev_contacts[1] = (long long)c->host_lvl[half + LV_STRIDE - 1];
   }
   c->n_out = (long long)c->host_cnt[C_OUT];
+  c->last_kind = kind;
+  ++c->run_serial;
   c->last_mode = kind == RUN_CCD ? 1 : 0;
   c->ccd_valid = kind != RUN_NARROW;
   c->out_filtered = kind != RUN_CCD && filter != 0;
@@ -4041,6 +3973,41 @@ int run_once(d2_ctx* c, int kind, int filter, float dt, int64_t* n_contacts_out)
   return D2_OK;
 }
 
+// ---- the solver's warm store
+constexpr size_t WARM_MIN = 1u << 20;
+WarmTable warm_table(d2_ctx* c) {
+  return WarmTable{c->warm_keys.p, c->warm_vals.p, (unsigned long long)c->warm_keys.n - 1, c->warm_used.p};
+}
+// Room for `extra` more keys at a load factor of at most one half: the table
+// doubles (re-inserting its keys) until it fits.  Synchronises the stream.
+int warm_reserve(d2_ctx* c, long long extra) {
+  if (!c->warm_keys.p) {
+    RC(ensure(c, c->warm_keys, WARM_MIN)); RC(ensure(c, c->warm_vals, WARM_MIN)); RC(ensure(c, c->warm_used, 1));
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c, WARM_MIN)), dim3(256), 0, c->stream, c->warm_keys.p,
+                       (long long)WARM_MIN, WARM_EMPTY);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->warm_used.p, 1ll, 0ull);
+  }
+  unsigned long long used = 0;
+  HIPCHK(c, hipMemcpyAsync(&used, c->warm_used.p, sizeof(used), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t cap = c->warm_keys.n;
+  while ((long long)used + extra > (long long)(cap / 2)) cap *= 2;
+  if (cap == c->warm_keys.n) return D2_OK;
+  DBuf<unsigned long long> old_k = c->warm_keys;
+  DBuf<double> old_v = c->warm_vals;
+  c->warm_keys = DBuf<unsigned long long>{}, c->warm_vals = DBuf<double>{};
+  RC(ensure(c, c->warm_keys, cap)); RC(ensure(c, c->warm_vals, cap));
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c, (long long)cap)), dim3(256), 0, c->stream, c->warm_keys.p,
+                     (long long)cap, WARM_EMPTY);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->warm_used.p, 1ll, 0ull);
+  hipLaunchKernelGGL(k_warm_rehash, dim3(grid_for(c, (long long)old_k.n)), dim3(256), 0, c->stream,
+                     (const unsigned long long*)old_k.p, (const double*)old_v.p, (long long)old_k.n, warm_table(c));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  release(old_k), release(old_v);
+  return D2_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI
@@ -4096,6 +4063,13 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
   release(c->big), release(c->huge), release(c->n_big), release(c->hscratch);
+  release(c->nbound);
+  release(c->sv_mass), release(c->sv_fric), release(c->sv_ccount), release(c->sv_chits), release(c->sv_poff),
+      release(c->sv_hoff), release(c->sv_cfill), release(c->sv_cpairs), release(c->sv_inpair), release(c->sv_loc),
+      release(c->sv_status), release(c->sv_flag), release(c->sv_hl), release(c->sv_ord), release(c->sv_hrow),
+      release(c->sv_hpc), release(c->sv_pbody), release(c->sv_hA), release(c->sv_hB), release(c->sv_heps),
+      release(c->sv_hkey), release(c->sv_hs), release(c->sv_ps), release(c->sv_out), release(c->sv_out_csr),
+      release(c->warm_keys), release(c->warm_used), release(c->warm_vals);
   release(c->step_bits), release(c->first_any), release(c->first_ns), release(c->cluster_step), release(c->body_step),
       release(c->tmp_cluster), release(c->read_later);
   release(c->touch_base), release(c->touch_lv), release(c->touch_run), release(c->touch_lvb), release(c->touch_runb),
@@ -4288,6 +4262,48 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
     // rounded up so the float margin is never below the double one
     eps_out[t] = (float)std::min((eo + leaf_eps) * (1.0 + 1e-6), 1e30);
   }
+  // Per node, for pruning: the body-frame box of every leaf-triangle vertex
+  // under the node and the largest (leaf eps + float-barycentric acceptance
+  // bound 64 eps_f kappa L) of those triangles, as floats rounded outward;
+  // children follow their parent, so one pass from the back merges them.
+  std::vector<float> nb((size_t)total * 8);
+  for (int t = 0; t < n_trees; ++t) {
+    const int b0 = tree_node_offset[t];
+    for (int i = tree_node_offset[t + 1] - 1; i >= b0; --i) {
+      const int32_t* r = nodes + 8 * (size_t)i;
+      double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, E = 0.0;
+      if (!r[2]) {
+        const double* bk = buckets + (size_t)BUCKET * (b0 + r[1]);
+        for (int k = 0; k < r[7]; ++k) {
+          double v[3][3];
+          for (int q = 0; q < 3; ++q)
+            for (int d = 0; d < 3; ++d) v[q][d] = bk[(3 * q + d) * 8 + k];
+          E = std::max(E, leaf_slack(v[0], v[1], v[2], bk[72 + k]));
+          for (int q = 0; q < 3; ++q)
+            for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], v[q][d]), hi[d] = std::max(hi[d], v[q][d]);
+        }
+      } else {
+        for (int ch = 0; ch < r[6]; ++ch) {
+          const float* cb = nb.data() + 8 * ((size_t)b0 + r[5] + ch);
+          for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], (double)cb[d]), hi[d] = std::max(hi[d], (double)cb[3 + d]);
+          E = std::max(E, (double)cb[6]);
+        }
+      }
+      float* o = nb.data() + 8 * (size_t)i;
+      const bool empty = !(lo[0] <= hi[0]);
+      for (int d = 0; d < 3; ++d) {  // an empty node never prunes
+        float fl = empty ? -FLT_MAX : (float)lo[d], fh = empty ? FLT_MAX : (float)hi[d];
+        if ((double)fl > lo[d]) fl = std::nextafter(fl, -INFINITY);
+        if ((double)fh < hi[d]) fh = std::nextafter(fh, INFINITY);
+        o[d] = fl, o[3 + d] = fh;
+      }
+      float fe = (float)std::min(E, 1e30);
+      if ((double)fe < E) fe = std::nextafter(fe, INFINITY);
+      o[6] = fe, o[7] = 0.0f;
+    }
+  }
+  RC(ensure(c, c->nbound, nb.size()));
+  HIPCHK(c, hipMemcpyAsync(c->nbound.p, nb.data(), sizeof(float) * nb.size(), hipMemcpyHostToDevice, c->stream));
   // Per tree for the broad phase, as MeshData::initBoudingData (model/mesh.cpp:44-51)
   // computes it over the mesh vertices (here: the leaf triangles' vertices,
   // which are every vertex of a mesh whose vertices all belong to a face):
@@ -4388,6 +4404,128 @@ int d2_bodies_download(d2_ctx* c, double* records) {
   if (!c || !records) return D2_ERR_ARG;
   if (c->n_bodies <= 0) return fail(c, D2_ERR_STATE, "no bodies uploaded");
   HIPCHK(c, hipMemcpyAsync(records, c->bodies.p, sizeof(double) * 40 * (size_t)c->n_bodies, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_solver_bodies(d2_ctx* c, int n_bodies, const double* mass, const double* friction) {
+  if (!c) return D2_ERR_ARG;
+  if (c->n_bodies <= 0 || n_bodies != c->n_bodies) return fail(c, D2_ERR_ARG, "d2_solver_bodies: one value per uploaded body");
+  if (!mass || !friction) return fail(c, D2_ERR_ARG, "d2_solver_bodies: bad arguments");
+  for (int i = 0; i < n_bodies; ++i)
+    if (!(mass[i] > 0.0) || !(friction[i] >= 0.0))
+      return fail(c, D2_ERR_ARG, "d2_solver_bodies: masses must be > 0 and friction coefficients >= 0");
+  (void)hipSetDevice(c->device);
+  RC(ensure(c, c->sv_mass, n_bodies)); RC(ensure(c, c->sv_fric, n_bodies));
+  HIPCHK(c, hipMemcpyAsync(c->sv_mass.p, mass, sizeof(double) * n_bodies, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->sv_fric.p, friction, sizeof(double) * n_bodies, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_solve(d2_ctx* c, const double* gravity, int32_t* cluster_status, int* n_failed) {
+  if (!c) return D2_ERR_ARG;
+  if (!gravity) return fail(c, D2_ERR_ARG, "d2_solve: gravity is NULL");
+  if (c->last_mode != 0 || c->last_kind != RUN_STEP || !c->out_filtered)
+    return fail(c, D2_ERR_STATE, "d2_solve needs a preceding d2_step_run with the contact filter");
+  if (c->sv_mass.n < (size_t)c->n_bodies || !c->sv_mass.p)
+    return fail(c, D2_ERR_STATE, "d2_solve: set the bodies' masses and friction first (d2_solver_bodies)");
+  (void)hipSetDevice(c->device);
+  const int np = c->n_pairs, nk = c->n_clusters, nb = c->n_bodies;
+  const long long nh = c->n_out;
+  RC(warm_reserve(c, nh));
+  RC(ensure(c, c->sv_ccount, nk + 1)); RC(ensure(c, c->sv_chits, nk + 1)); RC(ensure(c, c->sv_poff, nk + 1));
+  RC(ensure(c, c->sv_hoff, nk + 1)); RC(ensure(c, c->sv_cfill, nk)); RC(ensure(c, c->sv_status, nk));
+  RC(ensure(c, c->sv_cpairs, std::max(np, 1))); RC(ensure(c, c->sv_pbody, 2 * (size_t)std::max(np, 1)));
+  RC(ensure(c, c->sv_inpair, nb)); RC(ensure(c, c->sv_loc, nb)); RC(ensure(c, c->sv_flag, 1));
+  RC(ensure(c, c->out_off, (size_t)np + 1));
+  const size_t H = (size_t)std::max<long long>(nh, 1);
+  RC(ensure(c, c->sv_hl, 2 * H)); RC(ensure(c, c->sv_ord, H)); RC(ensure(c, c->sv_hrow, H)); RC(ensure(c, c->sv_hpc, H));
+  RC(ensure(c, c->sv_hA, 3 * H)); RC(ensure(c, c->sv_hB, 3 * H)); RC(ensure(c, c->sv_heps, H)); RC(ensure(c, c->sv_hkey, H));
+  RC(ensure(c, c->sv_hs, (size_t)SH * H)); RC(ensure(c, c->sv_out, 11 * H));
+  RC(ensure(c, c->sv_ps, (size_t)SP * 2 * std::max(np, 1)));
+  SolveArgs A;
+  A.n_pairs = np, A.n_clusters = nk, A.n_bodies = nb;
+  A.pairs = c->pairs.p, A.cluster_of_pair = c->cluster_of_pair.p, A.kept_cnt = c->kept_cnt.p;
+  A.pair_off = c->pair_off.p, A.out_off = c->out_off.p, A.perm = c->perm.p, A.keep = c->keep.p;
+  A.contacts = c->contacts.p, A.cluster_step = c->cluster_step.p, A.body_step = c->body_step.p;
+  A.bodies = c->bodies.p, A.mass = c->sv_mass.p, A.fric = c->sv_fric.p;
+  A.gx = gravity[0], A.gy = gravity[1], A.gz = gravity[2];
+  A.ccount = c->sv_ccount.p, A.chits = c->sv_chits.p, A.cfill = c->sv_cfill.p, A.cpairs = c->sv_cpairs.p;
+  A.inpair = c->sv_inpair.p, A.loc = c->sv_loc.p, A.status = c->sv_status.p;
+  A.hl = c->sv_hl.p, A.ord = c->sv_ord.p, A.hrow = c->sv_hrow.p, A.hpc = c->sv_hpc.p, A.pbody = c->sv_pbody.p;
+  A.hA = c->sv_hA.p, A.hB = c->sv_hB.p, A.heps = c->sv_heps.p, A.hkey = c->sv_hkey.p, A.hs = c->sv_hs.p;
+  A.ps = c->sv_ps.p, A.out = c->sv_out.p, A.warm = warm_table(c), A.flag = c->sv_flag.p;
+  auto fill = [&](int* p, long long n, int v) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(c, n)), dim3(256), 0, c->stream, p, n, v);
+  };
+  fill(A.ccount, nk + 1, 0), fill(A.chits, nk + 1, 0), fill(A.cfill, nk, 0);
+  fill(A.inpair, nb, 0), fill(A.loc, nb, -1), fill(c->sv_flag.p, 1, 0);
+  hipLaunchKernelGGL(k_sv_count, dim3(grid_for(c, np)), dim3(256), 0, c->stream, A);
+  scan_host_n(c, c->sv_ccount.p, c->sv_poff.p, nk, nullptr);
+  scan_host_n(c, c->sv_chits.p, c->sv_hoff.p, nk, nullptr);
+  scan_host_n(c, c->kept_cnt.p, c->out_off.p, np, nullptr);
+  SolveArgs B = A;
+  B.ccount = c->sv_poff.p, B.chits = c->sv_hoff.p;  // the offsets from here on
+  hipLaunchKernelGGL(k_sv_place, dim3(grid_for(c, np)), dim3(256), 0, c->stream, B);
+  int bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, c->sv_flag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (bad) return fail(c, D2_ERR_ARG, "d2_solve: a dynamic body's pairs lie in more than one cluster");
+  hipLaunchKernelGGL(k_sv_solve, dim3((nk + 63) / 64 > 0 ? (nk + 63) / 64 : 1), dim3(64), 0, c->stream, B);
+  hipLaunchKernelGGL(k_sv_free, dim3(grid_for(c, nb)), dim3(256), 0, c->stream, B);
+  HIPCHK(c, hipGetLastError());
+  std::vector<int32_t> st(std::max(nk, 1));
+  if (nk > 0) HIPCHK(c, hipMemcpyAsync(st.data(), c->sv_status.p, sizeof(int32_t) * nk, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int failed = 0;
+  for (int k = 0; k < nk; ++k) failed += st[k] == 0;
+  if (cluster_status && nk > 0) std::memcpy(cluster_status, st.data(), sizeof(int32_t) * nk);
+  if (n_failed) *n_failed = failed;
+  c->solved_serial = c->run_serial;
+  c->solved_hits = nh;
+  return D2_OK;
+}
+
+int d2_solve_results(d2_ctx* c, double* per_contact) {
+  if (!c || !per_contact) return D2_ERR_ARG;
+  if (c->last_mode != 0 || c->solved_serial != c->run_serial)
+    return fail(c, D2_ERR_STATE, "d2_solve_results: no solve of the last step run");
+  const long long nh = c->solved_hits;
+  if (nh == 0) return D2_OK;
+  (void)hipSetDevice(c->device);
+  RC(ensure(c, c->sv_out_csr, 11 * (size_t)nh));
+  hipLaunchKernelGGL(k_sv_scatter, dim3(grid_for(c, nh)), dim3(256), 0, c->stream, nh, (const int*)c->sv_hrow.p,
+                     (const double*)c->sv_out.p, c->sv_out_csr.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(per_contact, c->sv_out_csr.p, sizeof(double) * 11 * (size_t)nh, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return D2_OK;
+}
+
+int d2_commit_step(d2_ctx* c) {
+  if (!c) return D2_ERR_ARG;
+  if (c->last_kind != RUN_STEP || c->last_mode != 0 || c->body_step.n < (size_t)c->n_bodies)
+    return fail(c, D2_ERR_STATE, "d2_commit_step needs a preceding d2_step_run");
+  (void)hipSetDevice(c->device);
+  hipLaunchKernelGGL(k_commit, dim3(grid_for(c, c->n_bodies)), dim3(256), 0, c->stream, c->n_bodies, c->bodies.p,
+                     (const double*)c->body_step.p);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->last_mode = -1;  // the step's results no longer describe the bodies
+  c->ccd_valid = false;
+  return D2_OK;
+}
+
+int d2_solver_reset(d2_ctx* c) {
+  if (!c) return D2_ERR_ARG;
+  if (!c->warm_keys.p) return D2_OK;
+  (void)hipSetDevice(c->device);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(c, (long long)c->warm_keys.n)), dim3(256), 0, c->stream,
+                     c->warm_keys.p, (long long)c->warm_keys.n, WARM_EMPTY);
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, c->warm_used.p, 1ll, 0ull);
+  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return D2_OK;
 }
@@ -4972,6 +5110,13 @@ int d2_set_learned(d2_ctx* c, int on, float t_gate, float s_gate) {
   c->gate_phases = on;
   c->gate_t = t_gate;
   c->gate_s = s_gate;
+  return D2_OK;
+}
+
+int d2_set_pruning(d2_ctx* c, int on) {
+  if (!c) return D2_ERR_ARG;
+  if (c->prune_on != (on != 0)) drop_graphs(c);  // captured launches carry the flag
+  c->prune_on = on != 0;
   return D2_OK;
 }
 

@@ -206,6 +206,40 @@ int d2_clusters_upload(d2_ctx* ctx, const int32_t* cluster_of_pair, int n_cluste
  * overwritten (as in the reference); current states and last steps are not. */
 int d2_step_run(d2_ctx* ctx, float dt, int filter, double* cluster_step_out, int64_t* n_contacts_out);
 
+/* ---- the contact solver (consumer of the filtered contacts) ------------- */
+/* Per-body mass (> 0) and friction coefficient (>= 0), the solver's view of
+ * Particle::getMass / friction_coeff (model/particle.h:17, :47); n_bodies
+ * must equal the uploaded count.  The inverse inertia is the body record's. */
+int d2_solver_bodies(d2_ctx* ctx, int n_bodies, const double* mass, const double* friction);
+/* strategy::SequentialImpulses::solve (strategies/contact_force/
+ * sequential_impulses.cpp:104-638, warm start sequential_impulses_warm_start.
+ * cpp:12-54) for every cluster of the last d2_step_run, which must have run
+ * with the filter: each cluster's hits are its pairs' filtered contacts,
+ * pairs in index order (PDE_explicit.cpp:77-95), solved at the cluster's step
+ * with external force mass * gravity (contact_force_strategy.cpp:10-13);
+ * bodies in no pair are clusters without contacts (gravity only).  Solved
+ * clusters replace their dynamic bodies' future states; a cluster the
+ * reference refuses (a contact closer than 1e-6 with a step above 1e-4)
+ * keeps them and gets status 0.  cluster_status (n_clusters, may be NULL)
+ * receives 1 / 0 per cluster, *n_failed the refusals.  The warm store (the
+ * impulse of each body pair's last contact) persists across calls.  Every
+ * dynamic body's pairs must lie in one cluster (D2_ERR_ARG otherwise). */
+int d2_solve(d2_ctx* ctx, const double* gravity, int32_t* cluster_status, int* n_failed);
+/* Per contact of the solved step's filtered result, in d2_result_contacts'
+ * order, 11 doubles: the contact normal normalized((B - A) / 2), the normal
+ * impulse, the position-offset impulse, the friction impulse (on body a) and
+ * the tangential relative velocity v_ab - (v_ab . n) n of the last friction
+ * sweep (sequential_impulses.cpp:124, :197, :455). */
+int d2_solve_results(d2_ctx* ctx, double* per_contact);
+/* PDEExplicit::step's tail for the dynamic bodies (PDE_explicit.cpp:117-131):
+ * current state <- future state, last step <- the body's step of the last
+ * d2_step_run, future state <- extrapolate(current, step).  Static bodies
+ * keep theirs (the reference's last step of a static body depends on the
+ * order its clusters are stepped in). */
+int d2_commit_step(d2_ctx* ctx);
+/* Empties the solver's warm store. */
+int d2_solver_reset(d2_ctx* ctx);
+
 /* ---- broad phase and clusters -------------------------------------------- */
 /* Per-tree mesh bounds the broad phase reads: the float AABB (lo xyz, hi xyz)
  * and the bounding radius of MeshData::initBoudingData (model/mesh.cpp:44-51,
@@ -363,6 +397,15 @@ int d2_reset_stats(d2_ctx* ctx);
  * contact count in the last field of level 0).  Copies min(n, total) values;
  * returns the total. */
 int d2_run_info(const d2_ctx* ctx, int64_t* info, int n);
+/* Subtree pruning of the discrete traversal (no reference counterpart; on by
+ * default): a node pair whose leaf triangles are provably farther apart than
+ * any leaf contact could be (separating-axis gap of the oriented leaf boxes
+ * against the search distance plus the nodes' largest leaf eps and
+ * float-barycentric slack) is not tested and not expanded.  The reference
+ * finds no contact below such a node pair, so every result is unchanged;
+ * only the work per level shrinks.  Accounting runs (d2_set_touch_stats)
+ * walk the reference's node pairs without it. */
+int d2_set_pruning(d2_ctx* ctx, int on);
 /* Algorithmic-byte accounting (no reference counterpart: instrumentation).
  * When on, runs are eager and every level also records which (body, node)
  * records its traversal tests (a discrete node pair whose parent missed is

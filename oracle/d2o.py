@@ -107,6 +107,7 @@ def lib():
         L.d2o_warm_get.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _dp]
         L.d2o_warm_get.restype = ctypes.c_int
         L.d2o_commit.argtypes = [_vp, _dp]
+        L.d2o_scene_set_bodies.argtypes = [_vp, _dp]
         _lib = L
     return _lib
 
@@ -296,6 +297,11 @@ class Scene:
     def cluster_step(self, pairs, dt):
         pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
         return lib().d2o_cluster_step(self.h, pairs.shape[0], pairs.ctypes.data_as(_ip), float(dt))
+
+    def set_bodies(self, records):
+        """Replace every body's record (the warm store stays)."""
+        self._bodies = _d(records, 40 * self.n_bodies)
+        lib().d2o_scene_set_bodies(self.h, _p(self._bodies))
 
     def solver_props(self, mass, friction):
         """Per-body mass and friction coefficient for the solver."""

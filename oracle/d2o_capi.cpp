@@ -369,6 +369,22 @@ extern "C" void d2o_scene_get_bodies(void* h, double* body_records) {
   }
 }
 
+// Replaces every body's states, last step and flags from records (trees and
+// the solver's warm store stay).
+extern "C" void d2o_scene_set_bodies(void* h, const double* body_records) {
+  Scene* sc = static_cast<Scene*>(h);
+  for (size_t i = 0; i < sc->bodies.size(); ++i) {
+    const double* r = body_records + 40 * i;
+    Body& b = sc->bodies[i];
+    b.current = stateFrom(r);
+    b.future = stateFrom(r + 14);
+    for (int k = 0; k < 9; ++k) b.inv_inertia.m[k] = r[28 + k];
+    b.geo_eps = r[37];
+    b.is_static = r[38] != 0.0;
+    b.last_time_step_size = r[39];
+  }
+}
+
 // Discrete narrow phase over a pair list (PDE_explicit.cpp:77-95): for each
 // pair compareTreesFull, then filterContacts(.., 0.0) when filter != 0.
 // Results are kept in the scene; see d2o_scene_result_*.  Returns the number

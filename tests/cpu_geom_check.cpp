@@ -143,5 +143,74 @@ int main() {
     }
   }
   printf("near_mid %d %d\n", cases, bad);
+
+  // nodes_apart never prunes a node pair that holds a leaf contact: two random
+  // clusters of 1-8 triangles (some thin) with random leaf eps, their float
+  // boxes and slack as d2_trees_upload computes them, poses placing them
+  // around the contact distance; a prune while some triangle pair's oracle
+  // distance is below search + eps_i + eps_j is a violation.
+  {
+    int cases = 0, violations = 0, pruned = 0;
+    const double search = (double)(float)0.02;
+    for (int k = 0; k < 40000; ++k) {
+      const double sz = std::pow(10.0, U(-2.0, 0.5));
+      std::vector<d2o::Tri<double>> tri[2];
+      std::vector<double> eps[2];
+      float box[2][8];
+      for (int s = 0; s < 2; ++s) {
+        const int n = 1 + (int)(rng() % 8);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, E = 0.0;
+        for (int i = 0; i < n; ++i) {
+          d2o::V3<double> a(U(-sz, sz), U(-sz, sz), U(-sz, sz)), b(U(-sz, sz), U(-sz, sz), U(-sz, sz));
+          d2o::V3<double> c = (rng() % 4 == 0) ? a + (b - a) * U(0.2, 0.8) + d2o::V3<double>(U(-1e-4, 1e-4), U(-1e-4, 1e-4), U(-1e-4, 1e-4)) * sz
+                                               : d2o::V3<double>(U(-sz, sz), U(-sz, sz), U(-sz, sz));
+          tri[s].push_back(d2o::Tri<double>(a, b, c));
+          const double e = (rng() % 2) ? 0.0 : U(0.0, 0.01);
+          eps[s].push_back(e);
+          const double va[3] = {a.x, a.y, a.z}, vb[3] = {b.x, b.y, b.z}, vc[3] = {c.x, c.y, c.z};
+          E = std::max(E, d2::leaf_slack(va, vb, vc, e));
+          for (const double* v : {va, vb, vc})
+            for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], v[d]), hi[d] = std::max(hi[d], v[d]);
+        }
+        for (int d = 0; d < 3; ++d) {
+          float fl = (float)lo[d], fh = (float)hi[d];
+          if ((double)fl > lo[d]) fl = std::nextafter(fl, -INFINITY);
+          if ((double)fh < hi[d]) fh = std::nextafter(fh, INFINITY);
+          box[s][d] = fl, box[s][3 + d] = fh;
+        }
+        float fe = (float)E;
+        if ((double)fe < E) fe = std::nextafter(fe, INFINITY);
+        box[s][6] = fe, box[s][7] = 0.0f;
+      }
+      double Ta[16], Tb[16];
+      rand_pose(Ta, 5.0, 3.2);
+      // b near a: mostly just beyond the two boxes' bounding radii (gaps from
+      // -2 to +6 cm, around the contact distance), else anywhere within a few
+      // cluster sizes
+      double ra = 0.0, rb = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double ha = std::max(std::fabs(box[0][d]), std::fabs(box[0][3 + d]));
+        const double hb = std::max(std::fabs(box[1][d]), std::fabs(box[1][3 + d]));
+        ra += ha * ha, rb += hb * hb;
+      }
+      d2o::V3<double> dir(U(-1, 1), U(-1, 1), U(-1, 1));
+      dir = dir / d2o::norm(dir);
+      const double dist = (rng() % 4) ? std::sqrt(ra) + std::sqrt(rb) + U(-0.02, 0.06) : U(0.0, 3.0) * sz;
+      d2o::M4<double> Mb = d2o::transformationMatrixEuler(d2o::V3<double>(U(-3.2, 3.2), U(-3.2, 3.2), U(-3.2, 3.2)),
+                                                          d2o::V3<double>(Ta[3] + dir.x * dist, Ta[7] + dir.y * dist,
+                                                                          Ta[11] + dir.z * dist));
+      std::memcpy(Tb, Mb.m, sizeof(Tb));
+      ++cases;
+      if (!d2::nodes_apart(Ta, Tb, box[0], box[1], search)) continue;
+      ++pruned;
+      for (size_t i = 0; i < tri[0].size(); ++i)
+        for (size_t j = 0; j < tri[1].size(); ++j) {
+          d2o::V3<double> P, Q;
+          const double d = d2o::comparisonDist(tri[0][i].transformed(m4(Ta)), tri[1][j].transformed(m4(Tb)), P, Q);
+          if (d < (search + eps[0][i]) + eps[1][j]) ++violations;
+        }
+    }
+    printf("nodes_apart %d %d (pruned %d)\n", cases, violations, pruned);
+  }
   return 0;
 }

@@ -3,7 +3,8 @@
 d2_geom.hpp is the arithmetic of every kernel; compiling it for the host and
 comparing with the oracle on random inputs pins the kernels' FP64 expression
 order without a GPU (tri-tri distance, point-triangle CCD with the per-pair
-frame changes, edge-edge CCD incl. bisection cases, state extrapolation)."""
+frame changes, edge-edge CCD incl. bisection cases, state extrapolation), and
+the subtree-pruning bound never hides a leaf contact."""
 import os
 import subprocess
 
@@ -20,6 +21,9 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
         assert int(rows[name][2]) == 0, out
         assert int(rows[name][1]) >= 10000
     assert "bisection cases" in out
+    # subtree pruning: no violation, and the random placements do prune often
+    assert int(rows["nodes_apart"][2]) == 0, out
+    assert int(rows["nodes_apart"][4].rstrip(")")) > 2000, out
 
 
 def test_partition_rank_sort_matches_std_sort(tmp_path):

@@ -3,13 +3,13 @@
 - the 4096-body mixed workload (subdivision-3 icospheres and ellipsoid hulls,
   2.9 M triangles), the batch of 256 independent scenes of 16-512 bodies, the
   4096-body held-out scene of the learned gate, and the 65536-body funnel
-  pile (84 M triangles): one composite step (CCD, step rule,
-  cluster steps, projection, discrete narrow phase + filter) in the FP32 form
-  with the learned gate (committed weights) equals the plain FP64 step bit for
-  bit over every pair, and a seeded sample of 64 pairs equals the oracle: the
-  per-pair CCD values (relative 1e-9) and the discrete contacts after the
-  filter on the future states the device used (index tuples exact, reals
-  relative 1e-9).
+  pile (84 M triangles): one composite step (CCD, step rule, cluster steps,
+  projection, discrete narrow phase + filter) in the FP32 form with the
+  learned gate (committed weights) and subtree pruning equals the plain FP64
+  step without pruning bit for bit over every pair, and a seeded sample of 64
+  pairs equals the oracle: the per-pair CCD values (relative 1e-9) and the
+  discrete contacts after the filter on the future states the device used
+  (index tuples exact, reals relative 1e-9).
 The oracle gets only the sampled pairs' bodies and trees.
 """
 import numpy as np
@@ -40,9 +40,10 @@ def clusters_of(pairs, n_bodies):
     return cl.astype(np.int32), len(used)
 
 
-def one_step(s, rec, pairs, cl, k, precision, weights=None):
+def one_step(s, rec, pairs, cl, k, precision, weights=None, prune=True):
     """One d2_step_run on a fresh context: (result, cluster steps, bodies after)."""
     with d2.Context(0, precision) as c:
+        c.set_pruning(prune)
         if weights is not None:
             c.mlp_set_params(weights)
             c.set_learned(True)
@@ -72,11 +73,13 @@ def check_config(s, label, seed):
     pairs = s.pairs
     cl, k = clusters_of(pairs, s.n_bodies)
     print(f"[{label}] {s.n_bodies} bodies, {pairs.shape[0]} pairs, {k} clusters", flush=True)
-    plain, steps64, bodies64, n64, _ = one_step(s, rec, pairs, cl, k, d2.D2_FP64)
+    plain, steps64, bodies64, n64, _ = one_step(s, rec, pairs, cl, k, d2.D2_FP64, prune=False)
     print(f"[{label}] plain FP64 step: {n64} contacts", flush=True)
     fast, steps32, bodies32, n32, gate = one_step(s, rec, pairs, cl, k, d2.D2_FP32, weights)
     print(f"[{label}] FP32 + gate step: {n32} contacts, gate {gate}", flush=True)
-    # the fast form equals the plain one bit for bit, over every pair
+    # the fast form (FP32 culls, learned gate, subtree pruning) equals the
+    # plain one (FP64, every node pair of the reference's traversal) bit for
+    # bit, over every pair
     for name, a, b in zip(("offsets", "ints", "reals", "per_pair"), fast, plain):
         assert np.array_equal(a, b), f"{label}: FP32 + gate {name} differ from plain FP64"
     assert np.array_equal(steps32, steps64) and np.array_equal(bodies32, bodies64)
