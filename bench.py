@@ -315,6 +315,23 @@ def run_pile(args):
     ms_per_step = 1e3 * sum(secs) / K
     gate = fast.ctx.gate_counts() if weights is not None else None
 
+    # what each part of this form buys: the last timed state stepped again
+    # with the learned gate off, then with subtree pruning off (same results)
+    def ab(label, off, on):
+        off()
+        fast.step(states[W + K - 1])  # sizes the buffers for this form
+        _, s_off = fast.step(states[W + K - 1])
+        on()
+        _, s_on = fast.step(states[W + K - 1])
+        log(f"[c5] {label}: {1e3 * s_on:.1f} ms with, {1e3 * s_off:.1f} ms without")
+        return {"ms_with": 1e3 * s_on, "ms_without": 1e3 * s_off}
+    parts = {}
+    if weights is not None:
+        parts["learned_gate"] = ab("learned gate", lambda: fast.ctx.set_learned(False),
+                                   lambda: fast.ctx.set_learned(True))
+    parts["subtree_pruning"] = ab("subtree pruning", lambda: fast.ctx.set_pruning(False),
+                                  lambda: fast.ctx.set_pruning(True))
+
     # per-kernel times (one eager step with HIP events around every kernel)
     # and the (body, node) records its levels test, on the last timed state
     last = states[W + K - 1]
@@ -397,7 +414,7 @@ def run_pile(args):
         "dtype": "f32+f64" if prec == d2.D2_FP32 else "f64", "data": "synthetic",
         "form": {"precision": args.precision, "learned_gate": weights is not None, "graph": False, "weights": wsrc},
         "plain_fp64": plain_out, "speedup_vs_plain_fp64": plain_out["ms_per_step"] / ms_per_step if plain_out else None,
-        "identical_to_plain_fp64": identical, "gate": gate,
+        "identical_to_plain_fp64": identical, "gate": gate, "parts": parts,
         "config": {"workload": f"{args.bodies} subdivision-{args.subdiv} icospheres U(0.5,1.5) ({sc_tris(args)} "
                                f"triangles) in and above a static 40 m funnel (32768 triangles) at 0.1 bodies/m^3, "
                                f"v ~ N(0,1) - 3z m/s, dt = 1/60; per step: device broad phase + clusters over the "

@@ -178,6 +178,40 @@ D2_HD bool nodes_apart(const double* Ta, const double* Tb, const float* ba, cons
   }
   return gap > thr;
 }
+// The same question answered with the nodes' bounding spheres (centre
+// b[8..10], radius b[7] around the same leaf vertices): |Ca - Cb| - ra - rb
+// against the same threshold, in FP64.  Cheaper than the boxes (a few dozen
+// operations and registers); either test proving a pair apart is enough.
+D2_HD bool spheres_apart(const double* Ta, const double* Tb, const float* ba, const float* bb, double search) {
+  const double thr = ((search + (double)ba[6]) + (double)bb[6]) * (1.0 + 1e-9) + 1e-9;
+  double s2 = 0.0;
+  for (int r = 0; r < 3; ++r) {
+    const double wa = ((Ta[4 * r] * (double)ba[8] + Ta[4 * r + 1] * (double)ba[9]) + Ta[4 * r + 2] * (double)ba[10]) + Ta[4 * r + 3];
+    const double wb = ((Tb[4 * r] * (double)bb[8] + Tb[4 * r + 1] * (double)bb[9]) + Tb[4 * r + 2] * (double)bb[10]) + Tb[4 * r + 3];
+    s2 += (wb - wa) * (wb - wa);
+  }
+  return sqrt(s2) - ((double)ba[7] + (double)bb[7]) > thr;
+}
+// spheres_apart in FP32 for the FP32 form's mask kernel (it runs at five
+// wavefronts per SIMD; the FP64 poses would spill there).  Each world
+// coordinate w = R c + t is off by at most 8 u (|R c|_1 + |t|) after the
+// rounding of the pose, the product and the sums (u = 2^-24), so the centre
+// distance by at most 16 u m + 5 u |d| with m the sum of |w| over both
+// centres; the test demands 32 u (m + |d|) beyond the FP64 threshold.
+D2_HD bool spheres_apart_f32(const double* Ta, const double* Tb, const float* ba, const float* bb, double search) {
+  const double thr = ((search + (double)ba[6]) + (double)bb[6]) * (1.0 + 1e-9) + 1e-9;
+  float s2 = 0.0f, mag = 0.0f;
+  for (int r = 0; r < 3; ++r) {
+    const float wa = (((float)Ta[4 * r] * ba[8] + (float)Ta[4 * r + 1] * ba[9]) + (float)Ta[4 * r + 2] * ba[10]) +
+                     (float)Ta[4 * r + 3];
+    const float wb = (((float)Tb[4 * r] * bb[8] + (float)Tb[4 * r + 1] * bb[9]) + (float)Tb[4 * r + 2] * bb[10]) +
+                     (float)Tb[4 * r + 3];
+    s2 += (wb - wa) * (wb - wa);
+    mag += fabsf(wa) + fabsf(wb) + fabsf((float)Ta[4 * r + 3]) + fabsf((float)Tb[4 * r + 3]);
+  }
+  const double dist = (double)sqrtf(s2);
+  return dist - ((double)ba[7] + (double)bb[7]) > thr + 32.0 * 5.9604645e-08 * ((double)mag + dist);
+}
 // ---- rigid-body state (model/state.cpp) -----------------------------------
 struct quat {
   double w, x, y, z;

@@ -59,6 +59,7 @@
 namespace d2 {
 
 // ------------------------------------------------------------------ records
+constexpr int NB_STRIDE = 12;  // per-node pruning bounds (d2_trees_upload)
 struct NodeRec {
   int32_t id, bucket, inner, depth, parent, child_first, nchild, ntris;
 };
@@ -195,7 +196,7 @@ struct Dev {
   int *pass_flag_s, *pass_flag_c;
   double* min_time;
   const float* tree_eps_out;   // float-barycentric acceptance bound per tree
-  const float* nbound;         // per node: leaf box and slack (nodes_apart)
+  const float* nbound;         // per node: leaf box, slack, sphere (nodes_apart, spheres_apart)
   int prune;                   // skip node pairs nodes_apart proves contact-free
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
   const int32_t* body_tree;
@@ -1027,14 +1028,45 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 // not hit.  The discrete phase's contacts carry their emission keys (child
 // index paths) and are ordered by them afterwards, so its lists need no
 // breadth-first order and children take slots by atomics.
-__device__ __forceinline__ void write_children(const Dev& D, long long slot, const NP& np, int desc, uint32_t parent) {
+// Only the children in cmask are written (the others are provably apart:
+// child_mask); the emission keys keep every child's own index.
+__device__ __forceinline__ void write_children(const Dev& D, long long slot, const NP& np, int desc, uint32_t parent,
+                                               unsigned cmask) {
   const bool adv_a = desc & 1;
   const int nc = (desc >> 1) & 15, first = desc >> 5;
-  for (int c = 0; c < nc; ++c, ++slot) {
+  for (int c = 0; c < nc; ++c) {
+    if (!((cmask >> c) & 1u)) continue;
     const uint32_t ca = adv_a ? (uint32_t)(first + c) : np.na;
     const uint32_t cb = adv_a ? np.nb : (uint32_t)(first + c);
     if (slot < D.cap_np) D.soup_next[slot] = NP{np.pair, ca, cb, parent, np.key * 8ull + (unsigned long long)c};
+    ++slot;
   }
+}
+// The children of a node pair worth writing: all of them, or (pruning) those
+// nodes_apart does not prove contact-free.  offa/offb: the two trees' first
+// node records.
+template <bool F32>
+__device__ __forceinline__ unsigned child_mask(const Dev& D, const NP& np, int desc, int offa, int offb) {
+  const int nc = (desc >> 1) & 15;
+  unsigned m = (1u << nc) - 1u;
+  if (!D.prune) return m;
+  const bool adv_a = desc & 1;
+  const int first = desc >> 5;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+  for (int c = 0; c < nc; ++c) {
+    const size_t ga = (size_t)offa + (adv_a ? (uint32_t)(first + c) : np.na);
+    const size_t gb = (size_t)offb + (adv_a ? np.nb : (uint32_t)(first + c));
+    const float* ba = D.nbound + NB_STRIDE * ga;
+    const float* bb = D.nbound + NB_STRIDE * gb;
+    // the FP32 form's mask kernel runs at five wavefronts per SIMD: the FP32
+    // sphere test only (the boxes' test, or FP64 poses, would spill there);
+    // elsewhere both FP64 tests
+    const bool apart = F32 ? spheres_apart_f32(pr + PR_TA1, pr + PR_TB1, ba, bb, pr[PR_SEARCH])
+                           : spheres_apart(pr + PR_TA1, pr + PR_TB1, ba, bb, pr[PR_SEARCH]) ||
+                                 nodes_apart(pr + PR_TA1, pr + PR_TB1, ba, bb, pr[PR_SEARCH]);
+    if (apart) m &= ~(1u << c);
+  }
+  return m;
 }
 // Five wavefronts per SIMD (at most 96 VGPRs, no spills) instead of the four
 // its natural 101 VGPRs allow: the kernel waits on dependent loads, and more
@@ -1064,12 +1096,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
       l_pair = (int)np.pair, l_offa = pi[2], l_offb = pi[3];
       const NodeRec na = D.nodes[l_offa + np.na];
       const NodeRec nb = D.nodes[l_offb + np.nb];
-      // a child of a parent that did not hit, or a node pair whose leaves are
-      // provably apart: no work, no children
+      // a child of a parent that did not hit, or a root pair whose leaves are
+      // provably apart (deeper node pairs are tested when written): no work,
+      // no children
       const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
       if ((np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) &&
-          !(D.prune && nodes_apart(pr + PR_TA1, pr + PR_TB1, D.nbound + 8 * ((size_t)l_offa + np.na),
-                                   D.nbound + 8 * ((size_t)l_offb + np.nb), pr[PR_SEARCH]))) {
+          !(D.prune && D.level == 0 &&
+            spheres_apart_f32(pr + PR_TA1, pr + PR_TB1, D.nbound + NB_STRIDE * ((size_t)l_offa + np.na),
+                              D.nbound + NB_STRIDE * ((size_t)l_offb + np.nb), pr[PR_SEARCH]))) {
         l_np = np;
         l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
         l_inner = (na.inner || nb.inner) ? 1 : 0;
@@ -1141,7 +1175,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     const int mine = lane < batch ? __popcll(my_keep) : 0;
     // (FP32 mode; in FP64 mode every node pair keeps its pairs, so k_disc_expand
     // expands the hit ones after k_tri_dist)
-    const int nch = fp32 && lane < batch && (my_settled || my_keep != 0ull) ? (l_desc >> 1) & 15 : 0;
+    const unsigned cmask =
+        fp32 && lane < batch && (my_settled || my_keep != 0ull) ? child_mask<true>(D, l_np, l_desc, l_offa, l_offb) : 0u;
+    const int nch = __popc(cmask);
     const int incl = wave_incl_scan(mine), ich = wave_incl_scan(nch);
     const int par = it & 1;
     if (lane == 63) s_tot[par][wid] = incl, s_ch[par][wid] = ich;
@@ -1163,7 +1199,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     if (nch > 0) {
       long long slot = (long long)s_chb[par] + ich - nch;
       for (int v = 0; v < wid; ++v) slot += s_ch[par][v];
-      write_children(D, slot, l_np, l_desc, my_settled ? 0u : (uint32_t)(wl + 1));
+      write_children(D, slot, l_np, l_desc, my_settled ? 0u : (uint32_t)(wl + 1), cmask);
     }
     long long base = (long long)s_base[par];
     for (int v = 0; v < wid; ++v) base += s_tot[par][v];
@@ -1257,8 +1293,9 @@ __global__ void __launch_bounds__(256) k_tri_full(Dev D) {
       const int32_t* pi = D.pair_info + 4 * np.pair;
       const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
       if ((np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) &&
-          !(D.prune && nodes_apart(pr + PR_TA1, pr + PR_TB1, D.nbound + 8 * ((size_t)pi[2] + np.na),
-                                   D.nbound + 8 * ((size_t)pi[3] + np.nb), pr[PR_SEARCH]))) {
+          !(D.prune && D.level == 0 &&
+            nodes_apart(pr + PR_TA1, pr + PR_TB1, D.nbound + NB_STRIDE * ((size_t)pi[2] + np.na),
+                        D.nbound + NB_STRIDE * ((size_t)pi[3] + np.nb), pr[PR_SEARCH]))) {
         const NodeRec na = D.nodes[pi[2] + np.na];
         const NodeRec nb = D.nodes[pi[3] + np.nb];
         f.pair = (int)np.pair, f.ga = pi[2] + na.bucket, f.gb = pi[3] + nb.bucket;
@@ -1658,9 +1695,14 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
   for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < n; b0 += stride) {  // block-uniform
     const long long w = b0 + threadIdx.x;
     int desc = 0, nc = 0;
+    unsigned cmask = 0u;
+    NP np{};
     if (w < n && D.np_hit[w]) {
       desc = D.exp_desc[w];
-      nc = (desc >> 1) & 15;
+      np = D.soup[w];
+      const int32_t* pi = D.pair_info + 4 * np.pair;
+      cmask = child_mask<false>(D, np, desc, pi[2], pi[3]);
+      nc = __popc(cmask);
     }
     const int incl = wave_incl_scan(nc);
     if (lane == 63) s_tot[wid] = incl;
@@ -1679,7 +1721,7 @@ __global__ void __launch_bounds__(256) k_disc_expand(Dev D) {
     long long slot = (long long)s_base + incl - nc;
     for (int v = 0; v < wid; ++v) slot += s_tot[v];
     __syncthreads();  // s_tot and s_base are rewritten by the next round
-    if (nc > 0) write_children(D, slot, D.soup[w], desc, 0u);
+    if (nc > 0) write_children(D, slot, np, desc, 0u, cmask);
   }
 }
 
@@ -3006,7 +3048,7 @@ struct d2_ctx {
   // post
   DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, huge, n_big, scan_part;
   DBuf<int> hscratch;  // 6 x contact capacity: the filter of pairs above every LDS capacity
-  DBuf<float> nbound;  // per node: leaf box lo, hi (body frame), largest leaf eps + slack, 0
+  DBuf<float> nbound;  // per node: leaf box lo, hi (body frame), largest leaf eps + slack, sphere r, centre, 0
   bool prune_on = true;  // skip node pairs whose leaf triangles are provably apart
   // the solver (d2_solve): per-body mass and friction, per-cluster and per-hit scratch, the warm store
   DBuf<double> sv_mass, sv_fric;
@@ -4262,11 +4304,14 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
     // rounded up so the float margin is never below the double one
     eps_out[t] = (float)std::min((eo + leaf_eps) * (1.0 + 1e-6), 1e30);
   }
-  // Per node, for pruning: the body-frame box of every leaf-triangle vertex
-  // under the node and the largest (leaf eps + float-barycentric acceptance
-  // bound 64 eps_f kappa L) of those triangles, as floats rounded outward;
-  // children follow their parent, so one pass from the back merges them.
-  std::vector<float> nb((size_t)total * 8);
+  // Per node, for pruning (NB_STRIDE floats): the body-frame box of every
+  // leaf-triangle vertex under the node, the largest (leaf eps +
+  // float-barycentric acceptance bound 64 eps_f kappa L) of those triangles,
+  // and a sphere around the same vertices (their centroid, the largest
+  // distance from it), all rounded outward; children follow their parent, so
+  // one pass from the back merges the boxes and slacks, and a second pass
+  // per node over its leaves' vertices gives the spheres.
+  std::vector<float> nb((size_t)total * NB_STRIDE);
   for (int t = 0; t < n_trees; ++t) {
     const int b0 = tree_node_offset[t];
     for (int i = tree_node_offset[t + 1] - 1; i >= b0; --i) {
@@ -4284,12 +4329,12 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
         }
       } else {
         for (int ch = 0; ch < r[6]; ++ch) {
-          const float* cb = nb.data() + 8 * ((size_t)b0 + r[5] + ch);
+          const float* cb = nb.data() + NB_STRIDE * ((size_t)b0 + r[5] + ch);
           for (int d = 0; d < 3; ++d) lo[d] = std::min(lo[d], (double)cb[d]), hi[d] = std::max(hi[d], (double)cb[3 + d]);
           E = std::max(E, (double)cb[6]);
         }
       }
-      float* o = nb.data() + 8 * (size_t)i;
+      float* o = nb.data() + NB_STRIDE * (size_t)i;
       const bool empty = !(lo[0] <= hi[0]);
       for (int d = 0; d < 3; ++d) {  // an empty node never prunes
         float fl = empty ? -FLT_MAX : (float)lo[d], fh = empty ? FLT_MAX : (float)hi[d];
@@ -4299,7 +4344,53 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
       }
       float fe = (float)std::min(E, 1e30);
       if ((double)fe < E) fe = std::nextafter(fe, INFINITY);
-      o[6] = fe, o[7] = 0.0f;
+      o[6] = fe, o[7] = FLT_MAX, o[8] = o[9] = o[10] = o[11] = 0.0f;
+    }
+    // spheres: the leaves under a node are a contiguous run of the tree's
+    // leaves in depth-first order, so each node's vertex set is walked once
+    // per level of the tree (leaf vertices gathered bottom-up by child lists)
+    std::vector<std::vector<int>> leaves_of(tree_node_offset[t + 1] - b0);
+    for (int i = tree_node_offset[t + 1] - 1; i >= b0; --i) {
+      const int32_t* r = nodes + 8 * (size_t)i;
+      std::vector<int>& L = leaves_of[i - b0];
+      if (!r[2]) L.push_back(i);
+      else
+        for (int ch = 0; ch < r[6]; ++ch) {
+          std::vector<int>& C = leaves_of[r[5] + ch];
+          L.insert(L.end(), C.begin(), C.end());
+        }
+      double cen[3] = {0.0, 0.0, 0.0};
+      long long nv = 0;
+      for (int leaf : L) {
+        const int32_t* lr = nodes + 8 * (size_t)leaf;
+        const double* bk = buckets + (size_t)BUCKET * (b0 + lr[1]);
+        for (int k = 0; k < lr[7]; ++k)
+          for (int q = 0; q < 3; ++q, ++nv)
+            for (int d = 0; d < 3; ++d) cen[d] += bk[(3 * q + d) * 8 + k];
+      }
+      if (nv == 0) continue;  // no vertices: the sphere never prunes
+      float fc[3];
+      for (int d = 0; d < 3; ++d) fc[d] = (float)(cen[d] / (double)nv);
+      double rad = 0.0;
+      for (int leaf : L) {
+        const int32_t* lr = nodes + 8 * (size_t)leaf;
+        const double* bk = buckets + (size_t)BUCKET * (b0 + lr[1]);
+        for (int k = 0; k < lr[7]; ++k)
+          for (int q = 0; q < 3; ++q) {
+            double s2 = 0.0;
+            for (int d = 0; d < 3; ++d) {
+              const double x = bk[(3 * q + d) * 8 + k] - (double)fc[d];
+              s2 += x * x;
+            }
+            rad = std::max(rad, std::sqrt(s2));
+          }
+      }
+      float* o = nb.data() + NB_STRIDE * (size_t)i;
+      float fr = (float)(rad * (1.0 + 1e-12));
+      if ((double)fr < rad) fr = std::nextafter(fr, INFINITY);
+      o[7] = fr, o[8] = fc[0], o[9] = fc[1], o[10] = fc[2];
+      if (!r[2]) continue;
+      for (int ch = 0; ch < r[6]; ++ch) std::vector<int>().swap(leaves_of[r[5] + ch]);  // children are done
     }
   }
   RC(ensure(c, c->nbound, nb.size()));

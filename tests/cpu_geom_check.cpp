@@ -144,19 +144,20 @@ int main() {
   }
   printf("near_mid %d %d\n", cases, bad);
 
-  // nodes_apart never prunes a node pair that holds a leaf contact: two random
+  // nodes_apart, spheres_apart and spheres_apart_f32 never prune a node pair
+  // that holds a leaf contact: two random
   // clusters of 1-8 triangles (some thin) with random leaf eps, their float
   // boxes and slack as d2_trees_upload computes them, poses placing them
   // around the contact distance; a prune while some triangle pair's oracle
   // distance is below search + eps_i + eps_j is a violation.
   {
-    int cases = 0, violations = 0, pruned = 0;
+    int cases = 0, violations = 0, pruned = 0, pruned_by[3] = {0, 0, 0};
     const double search = (double)(float)0.02;
     for (int k = 0; k < 40000; ++k) {
       const double sz = std::pow(10.0, U(-2.0, 0.5));
       std::vector<d2o::Tri<double>> tri[2];
       std::vector<double> eps[2];
-      float box[2][8];
+      float box[2][12];
       for (int s = 0; s < 2; ++s) {
         const int n = 1 + (int)(rng() % 8);
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, E = 0.0;
@@ -180,7 +181,23 @@ int main() {
         }
         float fe = (float)E;
         if ((double)fe < E) fe = std::nextafter(fe, INFINITY);
-        box[s][6] = fe, box[s][7] = 0.0f;
+        box[s][6] = fe;
+        // the sphere as d2_trees_upload forms it: the vertices' centroid
+        // (rounded to float) and the largest distance from it, rounded up
+        double cen[3] = {0.0, 0.0, 0.0};
+        for (const auto& t : tri[s])
+          for (const auto* v : {&t.A, &t.B, &t.C}) cen[0] += v->x, cen[1] += v->y, cen[2] += v->z;
+        float fc[3];
+        for (int d = 0; d < 3; ++d) fc[d] = (float)(cen[d] / (3.0 * tri[s].size()));
+        double rad = 0.0;
+        for (const auto& t : tri[s])
+          for (const auto* v : {&t.A, &t.B, &t.C}) {
+            const double x = v->x - (double)fc[0], y = v->y - (double)fc[1], z = v->z - (double)fc[2];
+            rad = std::max(rad, std::sqrt(x * x + y * y + z * z));
+          }
+        float fr = (float)(rad * (1.0 + 1e-12));
+        if ((double)fr < rad) fr = std::nextafter(fr, INFINITY);
+        box[s][7] = fr, box[s][8] = fc[0], box[s][9] = fc[1], box[s][10] = fc[2], box[s][11] = 0.0f;
       }
       double Ta[16], Tb[16];
       rand_pose(Ta, 5.0, 3.2);
@@ -201,16 +218,22 @@ int main() {
                                                                           Ta[11] + dir.z * dist));
       std::memcpy(Tb, Mb.m, sizeof(Tb));
       ++cases;
-      if (!d2::nodes_apart(Ta, Tb, box[0], box[1], search)) continue;
-      ++pruned;
-      for (size_t i = 0; i < tri[0].size(); ++i)
-        for (size_t j = 0; j < tri[1].size(); ++j) {
+      const bool apart[3] = {d2::nodes_apart(Ta, Tb, box[0], box[1], search),
+                             d2::spheres_apart(Ta, Tb, box[0], box[1], search),
+                             d2::spheres_apart_f32(Ta, Tb, box[0], box[1], search)};
+      if (!apart[0] && !apart[1] && !apart[2]) continue;
+      bool contact = false;
+      for (size_t i = 0; i < tri[0].size() && !contact; ++i)
+        for (size_t j = 0; j < tri[1].size() && !contact; ++j) {
           d2o::V3<double> P, Q;
           const double d = d2o::comparisonDist(tri[0][i].transformed(m4(Ta)), tri[1][j].transformed(m4(Tb)), P, Q);
-          if (d < (search + eps[0][i]) + eps[1][j]) ++violations;
+          contact = d < (search + eps[0][i]) + eps[1][j];
         }
+      for (int v = 0; v < 3; ++v) pruned_by[v] += apart[v], violations += apart[v] && contact;
+      pruned += 1;
     }
-    printf("nodes_apart %d %d (pruned %d)\n", cases, violations, pruned);
+    printf("nodes_apart %d %d (pruned %d: boxes %d, spheres %d, spheres f32 %d)\n", cases, violations, pruned,
+           pruned_by[0], pruned_by[1], pruned_by[2]);
   }
   return 0;
 }
