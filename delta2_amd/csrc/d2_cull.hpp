@@ -1,0 +1,188 @@
+// d2_cull.hpp -- FP32 culls of the CCD sub-tests (gfx950 kernels and the CPU
+// soundness check in tests/cpu_geom_check.cpp).
+//
+// A cull decides only whether an FP64 sub-test runs: a culled sub-test
+// provably reports no distance below the search distance, so it cannot be the
+// winner of a hit, and results are identical with the culls on or off.
+//   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
+//     point moves on the segment p0 -> p1 in the triangle's frame and the
+//     reported distance is 0 (the segment crosses the triangle), the plane
+//     distance at t = 0 or 1 (projection inside), or infinite.  Culled when
+//     p0 and p1 lie on one side of the plane farther than the search distance
+//     plus margin (no crossing; both plane distances large), or both lie
+//     outside one edge's half-plane by more than the margin plus the
+//     barycentric acceptance bound (no projection or crossing inside).
+//   edge-edge (edgeEdgeCCD, :565-673): the reported distance is |P - Q| for
+//     points of the two lerped segments at one t, so it is at least the
+//     separation of the two segments along an axis that separates them on the
+//     same side at t = 0 and at t = 1 (point offsets are linear in t).  Axes:
+//     the midpoint offsets and the edge-direction cross products at both ends.
+// margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products).
+//
+// The includer defines D2_HD (see d2_geom.hpp, included first).
+#pragma once
+
+namespace d2 {
+
+// FP32 culls only decide which FP64 tests run, so their arithmetic may fuse:
+// an FMA rounds once where a multiply-add rounds twice, so the error bounds
+// the margins cover (gamma_n |a||b| for an n-term dot product) still hold.
+D2_HD float f3dot(const float* a, const float* b) {
+  return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
+}
+// Poses as float 3x4 matrices, translations taken relative to an origin
+// (b's start position) so that world coordinates stay small.
+struct M34f {
+  float m[12];
+};
+D2_HD M34f to_m34f(const double* m, const double* org) {
+  M34f o;
+  for (int r = 0; r < 3; ++r) {
+    o.m[4 * r] = (float)m[4 * r], o.m[4 * r + 1] = (float)m[4 * r + 1], o.m[4 * r + 2] = (float)m[4 * r + 2];
+    o.m[4 * r + 3] = (float)(m[4 * r + 3] - (org ? org[4 * r + 3] : 0.0));
+  }
+  return o;
+}
+D2_HD void xf(const M34f& M, const v3<double>& p, float* o) {
+  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
+  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
+  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
+  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
+}
+D2_HD float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
+// Separation of segment Q above segment P along u at one instant (negative
+// when none), scaled by 1/|u|.
+D2_HD float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0, const float* q1) {
+  const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
+  return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
+}
+D2_HD bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
+                          const float* bs1, const float* be0, const float* be1, float thr) {
+  const float EPSF = 5.9604645e-08f;
+  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
+                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
+  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
+  float ax[4][3];
+  for (int r = 0; r < 3; ++r) {
+    ax[0][r] = 0.5f * ((bs0[r] + bs1[r]) - (as0[r] + as1[r]));
+    ax[1][r] = 0.5f * ((be0[r] + be1[r]) - (ae0[r] + ae1[r]));
+  }
+  const float da[3] = {as1[0] - as0[0], as1[1] - as0[1], as1[2] - as0[2]};
+  const float db[3] = {bs1[0] - bs0[0], bs1[1] - bs0[1], bs1[2] - bs0[2]};
+  const float ea[3] = {ae1[0] - ae0[0], ae1[1] - ae0[1], ae1[2] - ae0[2]};
+  const float eb[3] = {be1[0] - be0[0], be1[1] - be0[1], be1[2] - be0[2]};
+  ax[2][0] = da[1] * db[2] - da[2] * db[1], ax[2][1] = da[2] * db[0] - da[0] * db[2], ax[2][2] = da[0] * db[1] - da[1] * db[0];
+  ax[3][0] = ea[1] * eb[2] - ea[2] * eb[1], ax[3][1] = ea[2] * eb[0] - ea[0] * eb[2], ax[3][2] = ea[0] * eb[1] - ea[1] * eb[0];
+  for (int k = 0; k < 4; ++k) {
+    const float nu = sqrtf(f3dot(ax[k], ax[k]));
+    if (!(nu > 1e-20f)) continue;
+    const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
+    const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
+    if (fmaxf(up, dn) > lim) return true;
+  }
+  return false;
+}
+
+// A soup (surrogate) triangle's plane and outward edge normals in its body
+// frame, computed in FP64 and rounded once to float: unit normal n, offset
+// d = n.A, in-plane edge normals m[q] pointing away from the opposite corner
+// with offsets o[q], and eo = 64 u kappa L (u = 2^-53, kappa = d00 d11 /
+// (d00 d11 - d01^2), L the longest edge), the FP64 barycentric acceptance
+// bound of the reference's inside test.  A degenerate triangle gets n = 0
+// and o = +huge: its sub-tests are never culled.
+struct SoupTri {
+  float n[3], d, m[9], o[3], eo;
+};
+D2_HD SoupTri soup_tri(const tri<double>& T) {
+  SoupTri s;
+  for (int r = 0; r < 3; ++r) s.n[r] = 0.f, s.m[r] = s.m[3 + r] = s.m[6 + r] = 0.f;
+  s.d = 0.f;
+  s.o[0] = s.o[1] = s.o[2] = 3.4e38f;  // never outside
+  s.eo = 3.4e38f;
+  const v3<double> e0 = sub(T.b, T.a), e1 = sub(T.c, T.a), e2 = sub(T.c, T.b);
+  const v3<double> cr = cross(e0, e1);
+  const double cl = vlen(cr);
+  const double d00 = dot(e0, e0), d11 = dot(e1, e1), d01 = dot(e0, e1), d22 = dot(e2, e2);
+  const double den = d00 * d11 - d01 * d01;
+  if (!(cl > 0.0) || !(den > 0.0)) return s;  // degenerate: n = 0 keeps every sub-test
+  const double L = sqrt(d00 > d11 ? (d00 > d22 ? d00 : d22) : (d11 > d22 ? d11 : d22));
+  s.eo = (float)(64.0 * 1.1102230246251565e-16 * (d00 * d11 / den) * L * (1.0 + 1e-6));
+  const v3<double> n = normal(T);  // the reference's normal() bit for bit, rounded once to float below
+  s.n[0] = (float)n.x, s.n[1] = (float)n.y, s.n[2] = (float)n.z;
+  s.d = (float)dot(n, T.a);
+  for (int q = 0; q < 3; ++q) {
+    const v3<double> x = corner(T, q), y = corner(T, (q + 1) % 3), z = corner(T, (q + 2) % 3);
+    v3<double> mm = cross(sub(y, x), n);
+    if (dot(mm, sub(z, x)) > 0.0) mm = v3<double>{-mm.x, -mm.y, -mm.z};
+    const double ml = vlen(mm);
+    if (!(ml > 0.0)) continue;
+    mm = dvs(mm, ml);
+    s.m[3 * q] = (float)mm.x, s.m[3 * q + 1] = (float)mm.y, s.m[3 * q + 2] = (float)mm.z;
+    s.o[q] = (float)dot(mm, x);
+  }
+  return s;
+}
+D2_HD bool soup_point_keep(const float* p0, const float* p1, const SoupTri& t, float thr, float margin) {
+  if (t.n[0] == 0.f && t.n[1] == 0.f && t.n[2] == 0.f) return true;
+  const float h0 = f3dot(t.n, p0) - t.d, h1 = f3dot(t.n, p1) - t.d;
+  const float lim = thr + margin;
+  if ((h0 > lim && h1 > lim) || (h0 < -lim && h1 < -lim)) return false;
+  const float lat = margin + t.eo;
+  for (int q = 0; q < 3; ++q)
+    if (f3dot(t.m + 3 * q, p0) - t.o[q] > lat && f3dot(t.m + 3 * q, p1) - t.o[q] > lat) return false;
+  return true;
+}
+
+// The surviving sub-tests of one soup triangle pair (ta of body a, tb of b,
+// body coordinates) as a 15-bit mask in triTriCCDLinear's slot order
+// (continuous_comparison.h:680-779): slot 5k is corner k of ta against tb,
+// 5k+1 corner k of tb against ta, 5k+2+l edge k of ta against edge l of tb.
+// Poses: Ta0/Ta1, Tb0/Tb1 (start / end), Mab = inverse(Tb) Ta and Mba =
+// inverse(Ta) Tb at both ends (the frame changes of the point sub-tests).
+D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1,
+                              const double* Mab0, const double* Mab1, const double* Mba0, const double* Mba1,
+                              const tri<double>& ta, const tri<double>& tb, float thr) {
+  unsigned mask = 0u;
+  {  // edge-edge: world coordinates relative to b's start position
+    float wa0[3][3], wa1[3][3], wb0[3][3], wb1[3][3];
+    const M34f A0 = to_m34f(Ta0, Tb0), A1 = to_m34f(Ta1, Tb0);
+    const M34f B0 = to_m34f(Tb0, Tb0), B1 = to_m34f(Tb1, Tb0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xf(A0, corner(ta, k), wa0[k]), xf(A1, corner(ta, k), wa1[k]);
+      xf(B0, corner(tb, k), wb0[k]), xf(B1, corner(tb, k), wb1[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l)
+        if (!cull_edge_edge(wa0[k], wa0[(k + 1) % 3], wa1[k], wa1[(k + 1) % 3], wb0[l], wb0[(l + 1) % 3], wb1[l],
+                            wb1[(l + 1) % 3], thr))
+          mask |= 1u << (5 * k + 2 + l);
+  }
+  {  // point-triangle: a's corners in b's frame against tb, b's corners in a's frame against ta
+    float pa0[3][3], pa1[3][3], pb0[3][3], pb1[3][3];
+    const M34f F0 = to_m34f(Mab0, nullptr), F1 = to_m34f(Mab1, nullptr);
+    const M34f G0 = to_m34f(Mba0, nullptr), G1 = to_m34f(Mba1, nullptr);
+    float mag = 0.f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      xf(F0, corner(ta, k), pa0[k]), xf(F1, corner(ta, k), pa1[k]);
+      xf(G0, corner(tb, k), pb0[k]), xf(G1, corner(tb, k), pb1[k]);
+      mag = fmaxf(mag, fmaxf(fmaxf(amax3(pa0[k]), amax3(pa1[k])), fmaxf(amax3(pb0[k]), amax3(pb1[k]))));
+      const v3<double> ca = corner(ta, k), cb = corner(tb, k);
+      mag = fmaxf(mag, fmaxf(fmaxf(fabsf((float)ca.x), fabsf((float)ca.y)), fabsf((float)ca.z)));
+      mag = fmaxf(mag, fmaxf(fmaxf(fabsf((float)cb.x), fabsf((float)cb.y)), fabsf((float)cb.z)));
+    }
+    const float margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
+    const SoupTri sa = soup_tri(ta), sb = soup_tri(tb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (soup_point_keep(pa0[k], pa1[k], sb, thr, margin)) mask |= 1u << (5 * k);
+      if (soup_point_keep(pb0[k], pb1[k], sa, thr, margin)) mask |= 1u << (5 * k + 1);
+    }
+  }
+  return mask;
+}
+
+}  // namespace d2

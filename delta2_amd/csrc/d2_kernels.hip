@@ -53,6 +53,7 @@
 #include "../../include/d2hip.h"
 #define D2_HD __host__ __device__ __forceinline__
 #include "d2_geom.hpp"
+#include "d2_cull.hpp"
 #include "d2_sort.hpp"
 #include "d2_solver.hpp"
 
@@ -102,7 +103,11 @@ enum Counter : int {
 // LV_PACK (discrete levels): the level's item count (low 32 bits) and its
 // children count (high 32 bits), taken together by k_tri_mask's allocation
 // atomic and unpacked into LV_ITEMS and the next level's LV_SOUP by k_tri_dist.
-enum LevelSlot : int { LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_PACK = 5, LV_STRIDE = 8 };
+// LV_SPT / LV_SEE (CCD levels, FP32 form): the soup sub-tests that survive
+// the FP32 culls, point and edge (slot 5 is LV_PACK on discrete levels).
+enum LevelSlot : int {
+  LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_PACK = 5, LV_SPT = 5, LV_SEE = 6, LV_STRIDE = 8
+};
 constexpr int MAX_LEVELS = 16;  // contact keys hold 16 base-8 path digits
 // Key layout: level(5) | list(1) | path(48) | sub(10)
 __device__ __forceinline__ unsigned long long make_key(int level, int list, unsigned long long path, int sub) {
@@ -134,6 +139,15 @@ __device__ __forceinline__ void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 __device__ __forceinline__ unsigned long long lanes_below() { return (1ull << lane_id()) - 1ull; }
+// Inclusive prefix sum over the wavefront.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d);
+    if (lane_id() >= d) v += o;
+  }
+  return v;
+}
 
 // One atomic per wavefront; returns this lane's slot (or -1 when !pred).
 __device__ __forceinline__ long long wave_alloc(bool pred, unsigned long long* counter) {
@@ -179,6 +193,7 @@ struct Dev {
   Item* pt_items;
   Item* ee_items;
   double* res;  // [items][15][8]
+  unsigned* res_mask;  // FP32 form: per soup item, the slots its FP32 culls keep
   int* np_hit;
   int* pair_cnt;               // contacts per pair, counted as they are written
   int* exp_desc;               // per discrete node pair: expansion_desc (FP64 mode)
@@ -536,138 +551,152 @@ __global__ void __launch_bounds__(256) k_soup_cull(Dev D) {
   }
 }
 
-// FP32 culls only decide which FP64 tests run, so their arithmetic may fuse:
-// an FMA rounds once where a multiply-add rounds twice, so the error bounds
-// the margins cover (gamma_n |a||b| for an n-term dot product) still hold.
-__device__ __forceinline__ float f3dot(const float* a, const float* b) {
-  return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0]));
-}
-// FP32 bounds of the CCD edge-edge sub-tests (soup and leaf levels): see
-// the FP32 culls of CCD leaf sub-tests below for the argument and margin.
-struct M34f {
-  float m[12];
-};
-__device__ __forceinline__ M34f to_m34f(const double* m, const double* org) {
-  M34f o;
-  for (int r = 0; r < 3; ++r) {
-    o.m[4 * r] = (float)m[4 * r], o.m[4 * r + 1] = (float)m[4 * r + 1], o.m[4 * r + 2] = (float)m[4 * r + 2];
-    o.m[4 * r + 3] = (float)(m[4 * r + 3] - (org ? org[4 * r + 3] : 0.0));
-  }
-  return o;
-}
-__device__ __forceinline__ void xf(const M34f& M, const v3<double>& p, float* o) {
-  const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
-  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
-  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
-  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
-}
-__device__ __forceinline__ float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
-// Separation of segment Q above segment P along u at one instant (negative
-// when none), scaled by 1/|u|.
-__device__ __forceinline__ float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0,
-                                         const float* q1) {
-  const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
-  return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
-}
-__device__ bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
-                               const float* bs1, const float* be0, const float* be1, float thr) {
-  const float EPSF = 5.9604645e-08f;
-  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
-                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
-  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
-  float ax[4][3];
-  for (int r = 0; r < 3; ++r) {
-    ax[0][r] = 0.5f * ((bs0[r] + bs1[r]) - (as0[r] + as1[r]));
-    ax[1][r] = 0.5f * ((be0[r] + be1[r]) - (ae0[r] + ae1[r]));
-  }
-  const float da[3] = {as1[0] - as0[0], as1[1] - as0[1], as1[2] - as0[2]};
-  const float db[3] = {bs1[0] - bs0[0], bs1[1] - bs0[1], bs1[2] - bs0[2]};
-  const float ea[3] = {ae1[0] - ae0[0], ae1[1] - ae0[1], ae1[2] - ae0[2]};
-  const float eb[3] = {be1[0] - be0[0], be1[1] - be0[1], be1[2] - be0[2]};
-  ax[2][0] = da[1] * db[2] - da[2] * db[1], ax[2][1] = da[2] * db[0] - da[0] * db[2], ax[2][2] = da[0] * db[1] - da[1] * db[0];
-  ax[3][0] = ea[1] * eb[2] - ea[2] * eb[1], ax[3][1] = ea[2] * eb[0] - ea[0] * eb[2], ax[3][2] = ea[0] * eb[1] - ea[1] * eb[0];
-  for (int k = 0; k < 4; ++k) {
-    const float nu = sqrtf(f3dot(ax[k], ax[k]));
-    if (!(nu > 1e-20f)) continue;
-    const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
-    const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
-    if (fmaxf(up, dn) > lim) return true;
-  }
-  return false;
-}
-
 __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<double> Q, double toc) {
   r[0] = d, r[1] = toc, r[2] = P.x, r[3] = P.y, r[4] = P.z, r[5] = Q.x, r[6] = Q.y, r[7] = Q.z;
 }
 
-// Point-triangle sub-tests of soup items; slot 5k+dir (k: vertex index,
-// dir 0: vertex of a vs triangle of b, dir 1: vertex of b vs triangle of a).
-__device__ __forceinline__ void soup_point_body(Dev& D, long long blk, long long nblk) {
-  const long long total = D.n_res() * 6;
-  for (long long t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
-    long long it = t / 6;
-    int s = (int)(t % 6), k = s >> 1, dir = s & 1;
-    Item item = D.items[it];
-    NP np = D.soup[item.np];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    int i = item.sub >> 3, j = item.sub & 7;
-    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-    v3<double> P, Q;
-    double toc = 1.0, d;
-    if (dir == 0)
-      d = point_tri(load_corner(ba, i, k), pr + PR_MAB0, pr + PR_MAB1, load_tri<double>(bb, j), pr + PR_TB0, pr + PR_TB1,
-                    P, Q, toc);
-    else
-      d = point_tri(load_corner(bb, j, k), pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(ba, i), pr + PR_TA0, pr + PR_TA1,
-                    P, Q, toc);
-    store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + dir) * 8, d, P, Q, toc);
+// ------------------------------------------- FP32 culls of CCD soup sub-tests
+// FP32 form: one thread per soup item (triangle pair) runs the FP32 bounds of
+// its 15 sub-tests (k_soup_keep), keeps a 15-bit mask of the survivors per
+// item and appends them to two compact lists (point and edge sub-tests, the
+// leaf level's item lists, which this level's connected pass fills only
+// later); the FP64 sub-tests then run over those lists only, in full
+// wavefronts, and k_soup_reduce selects over the surviving slots.  A culled
+// sub-test provably reports no distance below the search distance, so it
+// cannot be the winner of a hit (its slot would hold +inf, toc 1, which the
+// selection never takes).  The bounds (d2_cull.hpp) are those of the leaf
+// level: point-triangle by the plane side and the edge half-planes of the
+// triangle in its frame (soup_point_keep), edge-edge by cull_edge_edge on
+// the lerped world-space edges.  Soup triangles are surrogates, so their
+// plane and edge normals are computed per item in FP64 (soup_tri, rounded
+// once to float) and the lateral margin adds the FP64 barycentric acceptance
+// bound of that triangle.
+constexpr unsigned SOUP_PT_BITS = 0xC63u;   // slots 5k and 5k+1 (point-triangle, k = 0..2)
+constexpr unsigned SOUP_EE_BITS = 0x739Cu;  // slots 5k+2+l (edge k of a, edge l of b)
+// Surviving sub-test slots of soup item it (bit s: slot s of triTriCCDLinear's order).
+__device__ unsigned soup_keep_mask(const Dev& D, long long it) {
+  const Item item = D.items[it];
+  const NP np = D.soup[item.np];
+  const int32_t* pi = D.pair_info + 4 * np.pair;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+  const NodeRec na = D.nodes[pi[2] + np.na];
+  const NodeRec nb = D.nodes[pi[3] + np.nb];
+  const int i = item.sub >> 3, j = item.sub & 7;
+  const tri<double> ta = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[2] + na.bucket), i);
+  const tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), j);
+  return soup_cull_mask(pr + PR_TA0, pr + PR_TA1, pr + PR_TB0, pr + PR_TB1, pr + PR_MAB0, pr + PR_MAB1, pr + PR_MBA0,
+                        pr + PR_MBA1, ta, tb, (float)pr[PR_SEARCH]);
+}
+// FP32 form: the survivor masks of the level's soup items and the two
+// compact sub-test lists (wavefront prefix sums, one atomic per wavefront
+// and list; the lists' order is free, results go to fixed slots).
+__global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
+  const long long n = D.n_res();
+  const int lane = lane_id();
+  unsigned long long* cpt = D.lvl + D.level * LV_STRIDE + LV_SPT;
+  unsigned long long* cee = D.lvl + D.level * LV_STRIDE + LV_SEE;
+  for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < n; b0 += (long long)gridDim.x * blockDim.x) {
+    const long long it = b0 + threadIdx.x;
+    unsigned mask = 0u;
+    if (it < n) {
+      mask = soup_keep_mask(D, it);
+      D.res_mask[it] = mask;
+    }
+    const int mp = __popc(mask & SOUP_PT_BITS), me = __popc(mask & SOUP_EE_BITS);
+    const int ip = wave_incl_scan(mp), ie = wave_incl_scan(me);
+    unsigned long long bp = 0ull, be = 0ull;
+    if (lane == 63) {
+      if (ip) bp = atomicAdd(cpt, (unsigned long long)ip);
+      if (ie) be = atomicAdd(cee, (unsigned long long)ie);
+      if ((long long)(bp + ip) > D.cap_items || (long long)(be + ie) > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 1ull);
+    }
+    long long sp = (long long)readlane_u64(bp, 63) + (ip - mp), se = (long long)readlane_u64(be, 63) + (ie - me);
+    for (unsigned m = mask; m; m &= m - 1u) {
+      const int s = __ffs(m) - 1;
+      const bool pt = (SOUP_PT_BITS >> s) & 1u;
+      const long long slot = pt ? sp++ : se++;
+      if (slot < D.cap_items) (pt ? D.pt_items : D.ee_items)[slot] = Item{(uint32_t)it, (uint32_t)s};
+    }
   }
 }
 
-// Edge-edge sub-tests of soup items; slot 5k+2+l (edge k of a, edge l of b).
+// Point-triangle sub-test of soup item it, slot 5k+dir (k: vertex index,
+// dir 0: vertex of a vs triangle of b, dir 1: vertex of b vs triangle of a).
+__device__ __forceinline__ void soup_point_one(Dev& D, long long it, int k, int dir) {
+  Item item = D.items[it];
+  NP np = D.soup[item.np];
+  const int32_t* pi = D.pair_info + 4 * np.pair;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+  const NodeRec na = D.nodes[pi[2] + np.na];
+  const NodeRec nb = D.nodes[pi[3] + np.nb];
+  int i = item.sub >> 3, j = item.sub & 7;
+  const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+  const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+  v3<double> P, Q;
+  double toc = 1.0, d;
+  if (dir == 0)
+    d = point_tri(load_corner(ba, i, k), pr + PR_MAB0, pr + PR_MAB1, load_tri<double>(bb, j), pr + PR_TB0, pr + PR_TB1,
+                  P, Q, toc);
+  else
+    d = point_tri(load_corner(bb, j, k), pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(ba, i), pr + PR_TA0, pr + PR_TA1,
+                  P, Q, toc);
+  store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + dir) * 8, d, P, Q, toc);
+}
+
+// Edge-edge sub-test of soup item it, slot 5k+2+l (edge k of a, edge l of b).
+__device__ __forceinline__ void soup_edge_one(Dev& D, long long it, int k, int l) {
+  Item item = D.items[it];
+  NP np = D.soup[item.np];
+  const int32_t* pi = D.pair_info + 4 * np.pair;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+  const NodeRec na = D.nodes[pi[2] + np.na];
+  const NodeRec nb = D.nodes[pi[3] + np.nb];
+  int i = item.sub >> 3, j = item.sub & 7;
+  const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
+  const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+  v3<double> a0 = load_corner(ba, i, k), a1 = load_corner(ba, i, (k + 1) % 3);
+  v3<double> b0 = load_corner(bb, j, l), b1 = load_corner(bb, j, (l + 1) % 3);
+  v3<double> P, Q;
+  double toc = 1.0;
+  double d = edge_edge(xform(pr + PR_TA0, a0), xform(pr + PR_TA0, a1), xform(pr + PR_TA1, a0), xform(pr + PR_TA1, a1),
+                       xform(pr + PR_TB0, b0), xform(pr + PR_TB0, b1), xform(pr + PR_TB1, b0), xform(pr + PR_TB1, b1),
+                       P, Q, toc);
+  store_res(D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8, d, P, Q, toc);
+}
+
+// The point sub-tests of the level: every item's six (FP64 form), or the
+// FP32 form's surviving list.
+__device__ __forceinline__ void soup_point_body(Dev& D, long long blk, long long nblk) {
+  if (D.fp32) {
+    const long long n = D.n_lv(LV_SPT);
+    for (long long t = blk * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
+      const Item e = D.pt_items[t];
+      soup_point_one(D, e.np, (int)e.sub / 5, (int)e.sub % 5);
+    }
+    return;
+  }
+  const long long total = D.n_res() * 6;
+  for (long long t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
+    const int s = (int)(t % 6);
+    soup_point_one(D, t / 6, s >> 1, s & 1);
+  }
+}
+
+// The edge sub-tests of the level: every item's nine (FP64 form), or the
+// FP32 form's surviving list.
 __device__ __forceinline__ void soup_edge_body(Dev& D, long long blk, long long nblk) {
+  if (D.fp32) {
+    const long long n = D.n_lv(LV_SEE);
+    for (long long t = blk * blockDim.x + threadIdx.x; t < n; t += nblk * blockDim.x) {
+      const Item e = D.ee_items[t];
+      soup_edge_one(D, e.np, (int)e.sub / 5, (int)e.sub % 5 - 2);
+    }
+    return;
+  }
   const long long total = D.n_res() * 9;
   for (long long t = blk * blockDim.x + threadIdx.x; t < total; t += nblk * blockDim.x) {
-    long long it = t / 9;
-    int s = (int)(t % 9), k = s / 3, l = s % 3;
-    Item item = D.items[it];
-    NP np = D.soup[item.np];
-    const int32_t* pi = D.pair_info + 4 * np.pair;
-    const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-    const NodeRec na = D.nodes[pi[2] + np.na];
-    const NodeRec nb = D.nodes[pi[3] + np.nb];
-    int i = item.sub >> 3, j = item.sub & 7;
-    const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-    const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
-    v3<double> a0 = load_corner(ba, i, k), a1 = load_corner(ba, i, (k + 1) % 3);
-    v3<double> b0 = load_corner(bb, j, l), b1 = load_corner(bb, j, (l + 1) % 3);
-    double* slot = D.res + ((size_t)it * RES_SLOTS + 5 * k + 2 + l) * 8;
-    if (D.fp32) {
-      // FP32 form: the separating-axis bound of the leaf level's edge-edge
-      // cull (cull_edge_edge, same lerped world-space edges).  A culled
-      // sub-test reports no distance below the search distance, so it cannot
-      // be the winner of a hit; its slot stays at +inf, toc 1, which the
-      // selection never takes.
-      const M34f A0 = to_m34f(pr + PR_TA0, pr + PR_TB0), A1 = to_m34f(pr + PR_TA1, pr + PR_TB0);
-      const M34f B0 = to_m34f(pr + PR_TB0, pr + PR_TB0), B1 = to_m34f(pr + PR_TB1, pr + PR_TB0);
-      float as0[3], as1[3], ae0[3], ae1[3], bs0[3], bs1[3], be0[3], be1[3];
-      xf(A0, a0, as0), xf(A0, a1, as1), xf(A1, a0, ae0), xf(A1, a1, ae1);
-      xf(B0, b0, bs0), xf(B0, b1, bs1), xf(B1, b0, be0), xf(B1, b1, be1);
-      if (cull_edge_edge(as0, as1, ae0, ae1, bs0, bs1, be0, be1, (float)pr[PR_SEARCH])) {
-        store_res(slot, 1.0 / 0.0, v3<double>{0, 0, 0}, v3<double>{0, 0, 0}, 1.0);
-        continue;
-      }
-    }
-    v3<double> P, Q;
-    double toc = 1.0;
-    double d = edge_edge(xform(pr + PR_TA0, a0), xform(pr + PR_TA0, a1), xform(pr + PR_TA1, a0), xform(pr + PR_TA1, a1),
-                         xform(pr + PR_TB0, b0), xform(pr + PR_TB0, b1), xform(pr + PR_TB1, b0), xform(pr + PR_TB1, b1),
-                         P, Q, toc);
-    store_res(slot, d, P, Q, toc);
+    const int s = (int)(t % 9);
+    soup_edge_one(D, t / 9, s / 3, s % 3);
   }
 }
 
@@ -690,7 +719,9 @@ __global__ void __launch_bounds__(256) k_soup_reduce(Dev D) {
     const double* r = D.res + (size_t)t * RES_SLOTS * 8;
     double best = 1.0 / 0.0, toc = 1.0;
     int win = -1;
+    const unsigned live = D.fp32 ? D.res_mask[t] : (1u << RES_SLOTS) - 1u;  // FP32 form: culled slots hold nothing
     for (int s = 0; s < RES_SLOTS; ++s) {
+      if (!((live >> s) & 1u)) continue;
       double d = r[8 * s], tc = r[8 * s + 1];
       if (d < best || (d == best && tc < toc)) best = d, toc = tc, win = s;
     }
@@ -1013,14 +1044,6 @@ __device__ __forceinline__ int expansion_desc(const NodeRec& na, const NodeRec& 
 }
 __device__ __forceinline__ int rl(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 // Inclusive prefix sum over the wavefront (every lane active).
-__device__ __forceinline__ int wave_incl_scan(int v) {
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int o = __shfl_up(v, d);
-    if (lane_id() >= d) v += o;
-  }
-  return v;
-}
 // Discrete lists: the children of a node pair that hits or may hit, at slot,
 // slot + 1, ... of the next list, each naming its parent: parent 0 for a
 // certain hit, list index + 1 when the parent's FP64 tests (k_tri_dist, its
@@ -1380,21 +1403,10 @@ __global__ void __launch_bounds__(256) k_tri_full(Dev D) {
 // keeps every list in the reference's breadth-first order (discrete levels
 // are expanded by k_tri_mask, or by k_disc_expand in FP64 mode).
 // ------------------------------------------- FP32 culls of CCD leaf sub-tests
-// A culled sub-test provably reports no distance below the search distance.
-//   point-triangle (pointTriCCDLinear, continuous_comparison.h:41-98): the
-//     point moves on the segment p0 -> p1 in the triangle's frame and the
-//     reported distance is 0 (segment crosses the triangle), the plane
-//     distance at t = 0 or 1 (projection inside), or infinite.  Culled when
-//     p0 and p1 lie on one side of the plane farther than the search distance
-//     plus margin (no crossing; both plane distances large), or both lie
-//     outside one edge's half-plane by more than the margin plus the
-//     float-barycentric acceptance bound (no projection or crossing inside).
-//   edge-edge (edgeEdgeCCD, :565-673): the reported distance is |P - Q| for
-//     points of the two lerped segments at one t, so it is at least the
-//     separation of the two segments along an axis that separates them on the
-//     same side at t = 0 and at t = 1 (point offsets are linear in t).  Axes:
-//     the midpoint offsets and the edge-direction cross products at both ends.
-// margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products).
+// The bounds of d2_cull.hpp (plane side and edge half-planes for
+// point-triangle, separating axes for edge-edge), with the leaf triangles'
+// float-barycentric acceptance bound per tree (tree_eps_out) as the lateral
+// margin.
 // Per connected node pair, shared across its wavefront: lane v < 24 holds
 // vertex v of a's leaf (triangle v/3, corner v%3), lanes 24-47 those of b:
 // the vertex at both ends in world coordinates (relative to b's start
@@ -2034,6 +2046,25 @@ __device__ __forceinline__ bool step_rule_scale(const ContactRec& ct, const Step
     s = lerp(toc, (1.0 < fp) ? 1.0 : fp, 0.8);  // std::min(fp, 1.0)
   }
   return true;
+}
+
+// The step rule of a step's CCD phase (separate_clusters.cpp:496-564): a
+// pair's scale is the minimum of 1 and its contacts' scales, which no order
+// of the contacts changes, and a step never reads its CCD contacts, so they
+// stay where the emitting kernels put them (no scatter, no emission order):
+// one thread per contact, an atomic minimum on the scale's bits (k_setup sets
+// 1.0).  Scales are non-negative (toc >= 0, st >= 0, and the last two forms
+// need pv > 0 and depth < pd), so their bit patterns order as their values;
+// a NaN never passes s < 1, as std::min(min_scaling, NaN) keeps min_scaling.
+__global__ void __launch_bounds__(256) k_step_rule(const unsigned long long* cnt, long long cap, const ContactRec* c,
+                                                   const double* pair_rec, double* scale) {
+  const long long n = n_contacts(cnt, cap);
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const ContactRec& ct = c[i];
+    double s;
+    if (!step_rule_scale(ct, step_kin(pair_rec + (size_t)PR_STRIDE * ct.pair), s) || !(s < 1.0)) continue;
+    atomicMin(reinterpret_cast<unsigned long long*>(scale + ct.pair), (unsigned long long)__double_as_longlong(s));
+  }
 }
 
 // Contacts a pair may have for the LDS passes of the post-processing: the
@@ -2962,12 +2993,12 @@ namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
                               "leaf_point", "leaf_edge", "tri_mask", "tri_enum", "tri_dist", "post", "post_pairs",
-                              "cluster", "gate", "tri_full"};
+                              "cluster", "gate", "tri_full", "soup_keep"};
 // K_TRI_ENUM launches nothing (the enumeration is part of k_tri_mask); its
 // item count is the discrete node pairs received, kept for instrumentation.
 // FP64 contexts run K_TRI_FULL (k_tri_full) in place of K_TRI_MASK + K_TRI_DIST.
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
-           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_TRI_FULL, K_NUM };
+           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_TRI_FULL, K_SOUP_KEEP, K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
@@ -3038,6 +3069,7 @@ struct d2_ctx {
   DBuf<Item> items, pt_items, ee_items;
   DBuf<DItem> ditems;
   DBuf<double> res;
+  DBuf<unsigned> res_mask;  // FP32 form: per soup item, the slots its FP32 culls keep
   DBuf<int> np_hit, np_hit2, exp_desc;
   DBuf<ContactRec> contacts;
   DBuf<unsigned long long> cnt, lvl, masks, conn_mask;
@@ -3280,6 +3312,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.pt_items = c->pt_items.p;
   D.ee_items = c->ee_items.p;
   D.res = c->res.p;
+  D.res_mask = c->res_mask.p;
   D.np_hit = (level & 1) ? c->np_hit2.p : c->np_hit.p;
   D.np_hit_prev = (level & 1) ? c->np_hit.p : c->np_hit2.p;
   D.exp_desc = c->exp_desc.p;
@@ -3334,6 +3367,7 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->hscratch, 6 * (size_t)c->cap_contacts));
   RC(ensure(c, c->lvl, (size_t)2 * (MAX_LEVELS + 1) * LV_STRIDE));
   RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
+  if (c->precision == D2_FP32) RC(ensure(c, c->res_mask, (size_t)c->cap_res));
   RC(ensure(c, c->pass_min_s, np)); RC(ensure(c, c->pass_min_c, np)); RC(ensure(c, c->pass_flag_s, np));
   RC(ensure(c, c->pass_flag_c, np)); RC(ensure(c, c->min_time, np)); RC(ensure(c, c->scale, np));
   RC(ensure(c, c->pair_cnt, np + 1)); RC(ensure(c, c->pair_off, np + 1)); RC(ensure(c, c->pair_fill, np));
@@ -3460,6 +3494,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
     timed_launch(c, K_TRI_FULL, k_tri_full, dim3(eg), dim3(256), D);
   } else {
     timed_launch(c, K_SOUP_CULL, k_soup_cull, dim3(cg), dim3(256), D);
+    if (D.fp32) timed_launch(c, K_SOUP_KEEP, k_soup_keep, dim3(sg), dim3(256), D);  // FP32 culls + survivor lists
     timed_launch(c, K_SOUP_EDGE, k_soup_sub, dim3(2 * sg), dim3(256), D);  // point and edge sub-tests
     timed_launch(c, K_SOUP_REDUCE, k_soup_reduce, dim3(sg), dim3(256), D);
   }
@@ -3482,7 +3517,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
 // headroom, at least 16 blocks, at most the tuned stream grid.
 int post_grid(const d2_ctx* c, int mode) {
   const int hi = stream_grid(c);
-  const long long n = c->prev_contacts[mode == 2 ? 1 : 0];
+  const long long n = c->prev_contacts[mode >= 2 ? 1 : 0];
   if (n < 0) return hi;
   long long b = 16;
   while (b < hi && b * 256 < n + n / 4) b <<= 1;
@@ -3512,9 +3547,17 @@ void launch_finish(d2_ctx* c, int np, int wblocks, int mode, unsigned long long*
 }
 
 // Post-processing of the contacts of the last phase: per-pair order, then the
-// contact filter (mode 1), the step rule (mode 2) or nothing (mode 0).
+// contact filter (mode 1), the step rule (mode 2) or nothing (mode 0); mode 3
+// (a step's CCD phase) runs the step rule alone, on the unordered contacts.
 int enqueue_post(d2_ctx* c, int mode) {
   const int np = c->n_pairs;
+  if (mode == 3) {
+    Timer tm(c, K_POST_PAIRS, (double)np);
+    hipLaunchKernelGGL(k_step_rule, dim3(post_grid(c, mode)), dim3(256), 0, c->stream, c->cnt.p, c->cap_contacts,
+                       c->contacts.p, c->pair_rec.p, c->scale.p);
+    HIPCHK(c, hipGetLastError());
+    return D2_OK;
+  }
   const int sg = post_grid(c, mode);
   const bool filtered = mode == 1;
   {
@@ -3596,7 +3639,7 @@ int phases_of(int kind, int filter, PhaseSpec* out) {
     out[0] = {0, 0, 0, 2, 1};
     return 1;
   }
-  out[0] = {0, 0, 1, 2, 2};
+  out[0] = {0, 0, 1, 3, 2};  // the step never reads its CCD contacts: step rule only
   out[1] = {1, 1, 0, fmode, 3};
   return 2;
 }
@@ -3658,8 +3701,10 @@ void add_level_items(d2_ctx* c, int level, int discrete) {
     c->k_items[K_TRI_DIST] += (double)s[LV_ITEMS];
   } else {
     c->k_items[K_SOUP_CULL] += (double)s[LV_SOUP];
-    c->k_items[K_SOUP_POINT] += 6.0 * (double)s[LV_ITEMS];
-    c->k_items[K_SOUP_EDGE] += 9.0 * (double)s[LV_ITEMS];
+    const bool fp32 = c->precision == D2_FP32;  // FP32 form: the sub-tests its culls keep
+    if (fp32) c->k_items[K_SOUP_KEEP] += (double)s[LV_ITEMS];
+    c->k_items[K_SOUP_POINT] += fp32 ? (double)s[LV_SPT] : 6.0 * (double)s[LV_ITEMS];
+    c->k_items[K_SOUP_EDGE] += fp32 ? (double)s[LV_SPT] + (double)s[LV_SEE] : 15.0 * (double)s[LV_ITEMS];
     c->k_items[K_SOUP_REDUCE] += (double)s[LV_ITEMS];
     c->k_items[K_LEAF_POINT] += (double)s[LV_PT];
     c->k_items[K_LEAF_EDGE] += (double)s[LV_EE];
@@ -3875,6 +3920,7 @@ void grow_capacities(d2_ctx* c, int kind) {
       if (ccd) {
         res = std::max<long long>(res, (long long)v[LV_ITEMS]);
         items = std::max<long long>(items, (long long)std::max(v[LV_PT], v[LV_EE]));
+        items = std::max<long long>(items, (long long)std::max(v[LV_SPT], v[LV_SEE]));  // FP32 form's soup survivors
       } else if (c->precision == D2_FP32) {  // FP64 discrete levels keep no item list (k_tri_full)
         items = std::max<long long>(items, (long long)v[LV_ITEMS]);
       }
@@ -4097,7 +4143,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
-  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
+  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->res_mask), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
   release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);

@@ -14,6 +14,7 @@ using std::fabs;
 using std::sin;
 using std::cos;
 #include "../delta2_amd/csrc/d2_geom.hpp"
+#include "../delta2_amd/csrc/d2_cull.hpp"
 #include "../oracle/d2o_tree.hpp"
 
 static std::mt19937_64 rng(12345);
@@ -234,6 +235,54 @@ int main() {
     }
     printf("nodes_apart %d %d (pruned %d: boxes %d, spheres %d, spheres f32 %d)\n", cases, violations, pruned,
            pruned_by[0], pruned_by[1], pruned_by[2]);
+  }
+  // FP32 culls of the CCD soup sub-tests (soup_cull_mask): a culled sub-test
+  // never reports a distance below the search distance.  Triangle pairs of
+  // two nearby bodies (world offsets up to 40 m), small motions over the
+  // step, search distances 2-50 mm.
+  {
+    long long subtests = 0, culled = 0, hits = 0, viol = 0;
+    for (int k = 0; k < N; ++k) {
+      double Ta0[16], Ta1[16], Tb0[16], Tb1[16], rel[16], da[16], db[16], inv[16];
+      double Mab0[16], Mab1[16], Mba0[16], Mba1[16];
+      rand_pose(Ta0, (k % 4 == 0) ? 40.0 : 2.0, 3.2);
+      rand_pose(rel, (k % 2) ? 0.6 : 0.15, 3.2);  // half of the pairs nearly touching
+      d2::mul44(Ta0, rel, Tb0);
+      rand_pose(da, 0.15, 0.2);
+      rand_pose(db, 0.15, 0.2);
+      d2::mul44(Ta0, da, Ta1);
+      d2::mul44(Tb0, db, Tb1);
+      d2::inv44(Tb0, inv), d2::mul44(inv, Ta0, Mab0);
+      d2::inv44(Tb1, inv), d2::mul44(inv, Ta1, Mab1);
+      d2::inv44(Ta0, inv), d2::mul44(inv, Tb0, Mba0);
+      d2::inv44(Ta1, inv), d2::mul44(inv, Tb1, Mba1);
+      const double sa = U(0.05, 0.5), sb = U(0.05, 0.5);
+      d2o::Tri<double> oa = rand_tri(sa, U(-0.3, 0.3)), ob = rand_tri(sb, U(-0.3, 0.3));
+      if (k % 8 == 0) ob.C = ob.A + (ob.B - ob.A) * U(0.0, 1.0);  // degenerate (collinear) triangles too
+      const d2::tri<double> ta = cvt(oa), tb = cvt(ob);
+      const float thr = (float)U(0.002, 0.05);
+      const double thr_d = (double)thr;
+      const unsigned mask = d2::soup_cull_mask(Ta0, Ta1, Tb0, Tb1, Mab0, Mab1, Mba0, Mba1, ta, tb, thr);
+      for (int s = 0; s < 15; ++s) {
+        const int kk = s / 5, r = s % 5;
+        d2::v3<double> P, Q;
+        double toc = 1.0, d;
+        if (r == 0) {
+          d = d2::point_tri(d2::corner(ta, kk), Mab0, Mab1, tb, Tb0, Tb1, P, Q, toc);
+        } else if (r == 1) {
+          d = d2::point_tri(d2::corner(tb, kk), Mba0, Mba1, ta, Ta0, Ta1, P, Q, toc);
+        } else {
+          const int l = r - 2;
+          const d2::v3<double> a0 = d2::corner(ta, kk), a1 = d2::corner(ta, (kk + 1) % 3);
+          const d2::v3<double> b0 = d2::corner(tb, l), b1 = d2::corner(tb, (l + 1) % 3);
+          d = d2::edge_edge(d2::xform(Ta0, a0), d2::xform(Ta0, a1), d2::xform(Ta1, a0), d2::xform(Ta1, a1),
+                            d2::xform(Tb0, b0), d2::xform(Tb0, b1), d2::xform(Tb1, b0), d2::xform(Tb1, b1), P, Q, toc);
+        }
+        const bool hit = d < thr_d, keep = (mask >> s) & 1u;
+        ++subtests, hits += hit, culled += !keep, viol += hit && !keep;
+      }
+    }
+    printf("soup_culls %lld %lld (culled %lld, hits %lld)\n", subtests, viol, culled, hits);
   }
   return 0;
 }

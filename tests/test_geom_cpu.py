@@ -4,7 +4,8 @@ d2_geom.hpp is the arithmetic of every kernel; compiling it for the host and
 comparing with the oracle on random inputs pins the kernels' FP64 expression
 order without a GPU (tri-tri distance, point-triangle CCD with the per-pair
 frame changes, edge-edge CCD incl. bisection cases, state extrapolation), and
-the subtree-pruning bound never hides a leaf contact."""
+the subtree-pruning bound never hides a leaf contact, and the FP32 culls of
+the CCD soup sub-tests never drop a sub-test that reports a hit."""
 import os
 import re
 import subprocess
@@ -27,6 +28,12 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
     line = next(l for l in out.splitlines() if l.startswith("nodes_apart"))
     prunes = [int(x) for x in re.findall(r"(?:boxes|spheres|f32) (\d+)", line)]  # boxes, spheres, spheres f32
     assert len(prunes) == 3 and min(prunes) > 1000, out
+    # FP32 culls of the CCD soup sub-tests: no culled sub-test reports a hit,
+    # on draws with thousands of hits and most sub-tests culled
+    assert int(rows["soup_culls"][2]) == 0, out
+    line = next(l for l in out.splitlines() if l.startswith("soup_culls"))
+    culled, hits = (int(x) for x in re.findall(r"(?:culled|hits) (\d+)", line))
+    assert int(rows["soup_culls"][1]) >= 100000 and culled > int(rows["soup_culls"][1]) // 2 and hits > 1000, out
 
 
 def test_partition_rank_sort_matches_std_sort(tmp_path):
