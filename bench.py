@@ -195,7 +195,7 @@ def tune_step(ctx, learned, budget_s, tag):
     log(f"[{tag}] tune {best}: {best_ms:.1f} ms")
     dims = [[u, 2 * u, 4 * u, 8 * u, 16 * u], [2 * u, 4 * u, 8 * u, 16 * u, 32 * u], [4 * u, 16 * u, 64 * u]]
     if learned:
-        dims.append([1, 2, 3])
+        dims.append([0, 1, 2, 3])  # phase mask of the gate; 0 = it gates no phase
     for d, vals in enumerate(dims):
         for v in vals:
             if time.time() - t_start > budget_s:
@@ -300,6 +300,7 @@ def run_pile(args):
     if not args.no_autotune:
         shape, ms = tune_step(fast.ctx, weights is not None, args.tune_budget, "c5")
         log(f"[c5] tuned {shape}: {ms:.1f} ms")
+    tuned = fast.ctx.get_tune()  # the gate's phase mask is its last entry (0: it gates no phase)
     for k in range(1, W):
         n, s = fast.step(states[k])
         log(f"[c5] warm-up step {k}: {n} pairs, {1e3 * s:.1f} ms")
@@ -316,19 +317,28 @@ def run_pile(args):
     gate = fast.ctx.gate_counts() if weights is not None else None
 
     # what each part of this form buys: the last timed state stepped again
-    # with the learned gate off, then with subtree pruning off (same results)
+    # with it off (same results); the learned gate in each phase mask
+    def timed_last():
+        fast.step(states[W + K - 1])  # sizes the buffers for this form
+        return 1e3 * fast.step(states[W + K - 1])[1]
+
     def ab(label, off, on):
         off()
-        fast.step(states[W + K - 1])  # sizes the buffers for this form
-        _, s_off = fast.step(states[W + K - 1])
+        s_off = timed_last()
         on()
-        _, s_on = fast.step(states[W + K - 1])
-        log(f"[c5] {label}: {1e3 * s_on:.1f} ms with, {1e3 * s_off:.1f} ms without")
-        return {"ms_with": 1e3 * s_on, "ms_without": 1e3 * s_off}
+        s_on = timed_last()
+        log(f"[c5] {label}: {s_on:.1f} ms with, {s_off:.1f} ms without")
+        return {"ms_with": s_on, "ms_without": s_off}
     parts = {}
     if weights is not None:
-        parts["learned_gate"] = ab("learned gate", lambda: fast.ctx.set_learned(False),
-                                   lambda: fast.ctx.set_learned(True))
+        by = {}
+        for name, ph in (("off", 0), ("ccd", 1), ("discrete", 2), ("both", 3)):
+            fast.ctx.set_learned(ph != 0, phases=ph)
+            by[name] = timed_last()
+        fast.ctx.set_learned(True)
+        fast.ctx.set_tune(*tuned)  # back to the tuned phase mask
+        log(f"[c5] learned gate by phases (ms): {by}, tuned mask {tuned[3]}")
+        parts["learned_gate"] = {"ms_by_phases": by, "tuned_phases": tuned[3]}
     parts["subtree_pruning"] = ab("subtree pruning", lambda: fast.ctx.set_pruning(False),
                                   lambda: fast.ctx.set_pruning(True))
 
@@ -412,7 +422,8 @@ def run_pile(args):
         "metric": METRIC, "value": value, "unit": "pair queries/s", "n_gpus": 1, "steps": K, "warmup": W,
         "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if prec == d2.D2_FP32 else "f64", "data": "synthetic",
-        "form": {"precision": args.precision, "learned_gate": weights is not None, "graph": False, "weights": wsrc},
+        "form": {"precision": args.precision, "learned_gate": weights is not None,
+                 "gate_phases": tuned[3] if weights is not None else None, "graph": False, "weights": wsrc},
         "plain_fp64": plain_out, "speedup_vs_plain_fp64": plain_out["ms_per_step"] / ms_per_step if plain_out else None,
         "identical_to_plain_fp64": identical, "gate": gate, "parts": parts,
         "config": {"workload": f"{args.bodies} subdivision-{args.subdiv} icospheres U(0.5,1.5) ({sc_tris(args)} "

@@ -185,4 +185,112 @@ D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double
   return mask;
 }
 
+// ---------------------------------------------- discrete triangle-pair cull
+// Classifies one triangle pair of the discrete phase in FP32:
+//   0 = certain miss, 1 = needs the FP64 test, 2 = certain hit.
+// Lower bound: the reference's distance is the length of a pair of points on
+// the two triangles, except that a point projected onto a plane is accepted
+// as "inside" by float barycentrics (triangle.h:92-100), which can accept a
+// point up to eps_out ~ 16*eps_f*kappa*L outside (kappa = d00*d11/denom of the
+// barycentric system, L the longest edge).  So dist_ref >= true distance -
+// eps_out, and the true distance is >= the projection gap of the two vertex
+// sets on any axis (here both normals and the centroid direction).
+// Upper bound: every edge-edge candidate (comparison_dist.h:80-140) returns
+// |P - Q| with Q the closest point of one segment to a point X of the other
+// and P the closest point to Q, so it is <= max over the four endpoint pairs;
+// dist_ref <= the smallest such bound over the 9 edge pairs.
+// margin = 64*eps_f*kappa*L (4x the barycentric bound) + 16*eps_f*|coords|
+// (FP32 rounding of the relative coordinates and projections) + 1e-6 m.
+// Ill-conditioned barycentric systems (denominator <= 1e-4 d00 d11) get no
+// cull.
+//
+// The per-triangle terms are computed once (cull_prep): vertices in the
+// frame of the pair (b's position as origin), normal, kappa, longest squared
+// edge, own-normal extent, centroid, largest coordinate, and whether the
+// barycentric system is well conditioned.
+struct alignas(16) CullTri {
+  float c3[3], kap, L2, crd, ok;
+  float v[9], n[3], nn, pmin, pmax;
+  double eps;  // the triangle's eps from its bucket (read with the triangle)
+};
+D2_HD void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
+  const v3<double> vv[3] = {t.a, t.b, t.c};
+  float crd = 0.f;
+  for (int v = 0; v < 3; ++v) {
+    const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
+    for (int r = 0; r < 3; ++r) {
+      o.v[3 * v + r] = fmaf(rot[3 * r + 2], z, fmaf(rot[3 * r + 1], y, fmaf(rot[3 * r], x, tr[r])));
+      crd = fmaxf(crd, fabsf(o.v[3 * v + r]));
+    }
+  }
+  float e0[3], e1[3], e2[3];
+  for (int r = 0; r < 3; ++r)
+    e0[r] = o.v[3 + r] - o.v[r], e1[r] = o.v[6 + r] - o.v[r], e2[r] = o.v[6 + r] - o.v[3 + r];
+  o.n[0] = e0[1] * e1[2] - e0[2] * e1[1], o.n[1] = e0[2] * e1[0] - e0[0] * e1[2], o.n[2] = e0[0] * e1[1] - e0[1] * e1[0];
+  const float d00 = f3dot(e0, e0), d11 = f3dot(e1, e1), d01 = f3dot(e0, e1);
+  const float den = d00 * d11 - d01 * d01;
+  o.ok = (den > 1e-4f * d00 * d11) ? 1.f : 0.f;
+  o.kap = o.ok != 0.f ? d00 * d11 / den : 0.f;
+  o.L2 = fmaxf(fmaxf(d00, d11), f3dot(e2, e2));
+  o.nn = f3dot(o.n, o.n);
+  const float p0 = f3dot(o.v, o.n), p1 = f3dot(o.v + 3, o.n), p2 = f3dot(o.v + 6, o.n);
+  o.pmin = fminf(p0, fminf(p1, p2)), o.pmax = fmaxf(p0, fmaxf(p1, p2));
+  for (int r = 0; r < 3; ++r) o.c3[r] = ((o.v[r] + o.v[3 + r]) + o.v[6 + r]) * (1.f / 3.f);
+  o.crd = crd;
+}
+D2_HD float cull_margin(const CullTri& A, const CullTri& B, float thr) {
+  const float EPSF = 5.9604645e-08f;
+  const float kappa = fmaxf(A.kap, B.kap);
+  const float L = sqrtf(fmaxf(A.L2, B.L2));
+  return 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * fmaxf(A.crd, B.crd) + EPSF * thr;
+}
+// 0 certain miss, 1 needs the FP64 test, 2 certain hit.  The lower bound
+// compares g^2 > T^2 |n|^2 per axis instead of g / |n| > T, with T = (thr +
+// margin)(1 + 2^-20) covering the few roundings of the squared form.  The upper bound
+// (certain hit) is only computed when want_hit: a certain hit only matters
+// for node pairs with an inner node.
+D2_HD int cull_pair(const CullTri& A, const CullTri& B, float thr, bool want_hit = true) {
+  if (A.ok == 0.f || B.ok == 0.f) return 1;
+  const float margin = cull_margin(A, B, thr);
+  const float T = (thr + margin) * (1.f + 9.5367432e-07f);
+  const float T2 = T * T;
+  bool sep = false;
+  {  // axis nA
+    const float b0 = f3dot(B.v, A.n), b1 = f3dot(B.v + 3, A.n), b2 = f3dot(B.v + 6, A.n);
+    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - A.pmax, A.pmin - fmaxf(b0, fmaxf(b1, b2)));
+    sep = sep || (A.nn > 0.f && g > 0.f && g * g > T2 * A.nn);
+  }
+  {  // axis nB
+    const float a0 = f3dot(A.v, B.n), a1 = f3dot(A.v + 3, B.n), a2 = f3dot(A.v + 6, B.n);
+    const float g = fmaxf(B.pmin - fmaxf(a0, fmaxf(a1, a2)), fminf(a0, fminf(a1, a2)) - B.pmax);
+    sep = sep || (B.nn > 0.f && g > 0.f && g * g > T2 * B.nn);
+  }
+  {  // centroid direction
+    const float cd[3] = {B.c3[0] - A.c3[0], B.c3[1] - A.c3[1], B.c3[2] - A.c3[2]};
+    const float nn = f3dot(cd, cd);
+    const float a0 = f3dot(A.v, cd), a1 = f3dot(A.v + 3, cd), a2 = f3dot(A.v + 6, cd);
+    const float b0 = f3dot(B.v, cd), b1 = f3dot(B.v + 3, cd), b2 = f3dot(B.v + 6, cd);
+    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - fmaxf(a0, fmaxf(a1, a2)),
+                          fminf(a0, fminf(a1, a2)) - fmaxf(b0, fmaxf(b1, b2)));
+    sep = sep || (nn > 0.f && g > 0.f && g * g > T2 * nn);
+  }
+  if (sep) return 0;
+  if (!want_hit) return 1;
+  float ub2 = 3.4e38f;
+  float dd[3][3];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      const float t[3] = {A.v[3 * x] - B.v[3 * y], A.v[3 * x + 1] - B.v[3 * y + 1], A.v[3 * x + 2] - B.v[3 * y + 2]};
+      dd[x][y] = f3dot(t, t);
+    }
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) {
+      const int x1 = x == 2 ? 0 : x + 1, y1 = y == 2 ? 0 : y + 1;
+      ub2 = fminf(ub2, fmaxf(fmaxf(dd[x][y], dd[x][y1]), fmaxf(dd[x1][y], dd[x1][y1])));
+    }
+  const float lim = thr - margin;
+  if (lim > 0.f && ub2 < lim * lim) return 2;
+  return 1;
+}
+
 }  // namespace d2

@@ -849,183 +849,9 @@ __global__ void __launch_bounds__(256) k_leaf_sub(Dev D) {
 
 // ------------------------------------------- ordered enumeration/expansion
 // ------------------------------------------------ FP32 conservative cull
-// Classifies one triangle pair of the discrete phase in FP32:
-//   0 = certain miss, 1 = needs the FP64 test, 2 = certain hit.
-// Lower bound: the reference's distance is the length of a pair of points on
-// the two triangles, except that a point projected onto a plane is accepted
-// as "inside" by float barycentrics (triangle.h:92-100), which can accept a
-// point up to eps_out ~ 16*eps_f*kappa*L outside (kappa = d00*d11/denom of the
-// barycentric system, L the longest edge).  So dist_ref >= true distance -
-// eps_out, and the true distance is >= the projection gap of the two vertex
-// sets on any axis (here both normals and the centroid direction).
-// Upper bound: every edge-edge candidate (comparison_dist.h:80-140) returns
-// |P - Q| with Q the closest point of one segment to a point X of the other
-// and P the closest point to Q, so it is <= max over the four endpoint pairs;
-// dist_ref <= the smallest such bound over the 9 edge pairs.
-// margin = 64*eps_f*kappa*L (4x the barycentric bound) + 16*eps_f*|coords|
-// (FP32 rounding of the relative coordinates and projections) + 1e-6 m.
-__device__ int cull_class(const double* pa, const double* pb, const tri<double>& la, const tri<double>& lt, double thr_d) {
-  const float EPSF = 5.9604645e-08f;
-  float ra[9], rb[9], ta[3];
-  for (int r = 0; r < 3; ++r) {
-    for (int k = 0; k < 3; ++k) ra[3 * r + k] = (float)pa[4 * r + k], rb[3 * r + k] = (float)pb[4 * r + k];
-    ta[r] = (float)(pa[4 * r + 3] - pb[4 * r + 3]);
-  }
-  float A[3][3], B[3][3];
-  const v3<double> av[3] = {la.a, la.b, la.c}, bv[3] = {lt.a, lt.b, lt.c};
-  float coord = 0.f;
-  for (int v = 0; v < 3; ++v) {
-    float x = (float)av[v].x, y = (float)av[v].y, z = (float)av[v].z;
-    float u = (float)bv[v].x, w = (float)bv[v].y, q = (float)bv[v].z;
-    for (int r = 0; r < 3; ++r) {
-      A[v][r] = ((ra[3 * r] * x + ra[3 * r + 1] * y) + ra[3 * r + 2] * z) + ta[r];
-      B[v][r] = (rb[3 * r] * u + rb[3 * r + 1] * w) + rb[3 * r + 2] * q;
-      coord = fmaxf(coord, fmaxf(fabsf(A[v][r]), fabsf(B[v][r])));
-    }
-  }
-  float ea0[3], ea1[3], eb0[3], eb1[3], nA[3], nB[3], cd[3];
-  for (int r = 0; r < 3; ++r) {
-    ea0[r] = A[1][r] - A[0][r], ea1[r] = A[2][r] - A[0][r];
-    eb0[r] = B[1][r] - B[0][r], eb1[r] = B[2][r] - B[0][r];
-    cd[r] = (B[0][r] + B[1][r] + B[2][r]) - (A[0][r] + A[1][r] + A[2][r]);
-  }
-  auto crs = [](const float* a, const float* b, float* o) {
-    o[0] = a[1] * b[2] - a[2] * b[1], o[1] = a[2] * b[0] - a[0] * b[2], o[2] = a[0] * b[1] - a[1] * b[0];
-  };
-  crs(ea0, ea1, nA);
-  crs(eb0, eb1, nB);
-  float d00 = f3dot(ea0, ea0), d11 = f3dot(ea1, ea1), d01 = f3dot(ea0, ea1);
-  float e00 = f3dot(eb0, eb0), e11 = f3dot(eb1, eb1), e01 = f3dot(eb0, eb1);
-  float dena = d00 * d11 - d01 * d01, denb = e00 * e11 - e01 * e01;
-  // ill-conditioned barycentric systems get no cull
-  if (!(dena > 1e-4f * d00 * d11) || !(denb > 1e-4f * e00 * e11)) return 1;
-  float kappa = fmaxf(d00 * d11 / dena, e00 * e11 / denb);
-  float L2 = fmaxf(fmaxf(d00, d11), fmaxf(e00, e11));
-  float e12[3], f12[3];
-  for (int r = 0; r < 3; ++r) e12[r] = A[2][r] - A[1][r], f12[r] = B[2][r] - B[1][r];
-  L2 = fmaxf(L2, fmaxf(f3dot(e12, e12), f3dot(f12, f12)));
-  float L = sqrtf(L2);
-  float thr = (float)thr_d;
-  float margin = 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * coord + EPSF * thr;
-  // lower bound
-  float lb = 0.f;
-  const float* axes[3] = {nA, nB, cd};
-  for (int k = 0; k < 3; ++k) {
-    const float* n = axes[k];
-    float nn = f3dot(n, n);
-    if (!(nn > 0.f)) continue;
-    float a0 = f3dot(A[0], n), a1 = f3dot(A[1], n), a2 = f3dot(A[2], n);
-    float b0 = f3dot(B[0], n), b1 = f3dot(B[1], n), b2 = f3dot(B[2], n);
-    float amin = fminf(a0, fminf(a1, a2)), amax = fmaxf(a0, fmaxf(a1, a2));
-    float bmin = fminf(b0, fminf(b1, b2)), bmax = fmaxf(b0, fmaxf(b1, b2));
-    float g = fmaxf(bmin - amax, amin - bmax);
-    if (g > 0.f) lb = fmaxf(lb, g / sqrtf(nn));
-  }
-  if (lb > thr + margin) return 0;
-  // upper bound
-  float dd[3][3];
-  for (int x = 0; x < 3; ++x)
-    for (int y = 0; y < 3; ++y) {
-      float t[3] = {A[x][0] - B[y][0], A[x][1] - B[y][1], A[x][2] - B[y][2]};
-      dd[x][y] = f3dot(t, t);
-    }
-  float ub2 = 3.4e38f;
-  for (int x = 0; x < 3; ++x)
-    for (int y = 0; y < 3; ++y) {
-      int x1 = x == 2 ? 0 : x + 1, y1 = y == 2 ? 0 : y + 1;
-      ub2 = fminf(ub2, fmaxf(fmaxf(dd[x][y], dd[x][y1]), fmaxf(dd[x1][y], dd[x1][y1])));
-    }
-  float lim = thr - margin;
-  if (lim > 0.f && ub2 < lim * lim) return 2;
-  return 1;
-}
-
-// The same classification split in two: the per-triangle terms are computed
-// once per node pair (lanes 0-7: triangles of a, lanes 8-15: triangles of b)
-// and shared by shuffles with the 64 triangle-pair lanes.  Per triangle:
-// vertices (relative frame of cull_class), normal, kappa, longest squared
-// edge, own-normal extent, vertex sum, largest coordinate, and whether its
-// barycentric system is well conditioned.
-struct __align__(16) CullTri {
-  float v[9], n[3], nn, kap, L2, pmin, pmax, cen[3], crd, ok;
-  double eps;  // the triangle's eps from its bucket (read with the triangle)
-};
-__device__ __forceinline__ void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
-  const v3<double> vv[3] = {t.a, t.b, t.c};
-  float crd = 0.f;
-  for (int v = 0; v < 3; ++v) {
-    const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
-    for (int r = 0; r < 3; ++r) {
-      o.v[3 * v + r] = fmaf(rot[3 * r + 2], z, fmaf(rot[3 * r + 1], y, fmaf(rot[3 * r], x, tr[r])));
-      crd = fmaxf(crd, fabsf(o.v[3 * v + r]));
-    }
-  }
-  float e0[3], e1[3], e2[3];
-  for (int r = 0; r < 3; ++r)
-    e0[r] = o.v[3 + r] - o.v[r], e1[r] = o.v[6 + r] - o.v[r], e2[r] = o.v[6 + r] - o.v[3 + r];
-  o.n[0] = e0[1] * e1[2] - e0[2] * e1[1], o.n[1] = e0[2] * e1[0] - e0[0] * e1[2], o.n[2] = e0[0] * e1[1] - e0[1] * e1[0];
-  const float d00 = f3dot(e0, e0), d11 = f3dot(e1, e1), d01 = f3dot(e0, e1);
-  const float den = d00 * d11 - d01 * d01;
-  o.ok = (den > 1e-4f * d00 * d11) ? 1.f : 0.f;
-  o.kap = o.ok != 0.f ? d00 * d11 / den : 0.f;
-  o.L2 = fmaxf(fmaxf(d00, d11), f3dot(e2, e2));
-  o.nn = f3dot(o.n, o.n);
-  const float p0 = f3dot(o.v, o.n), p1 = f3dot(o.v + 3, o.n), p2 = f3dot(o.v + 6, o.n);
-  o.pmin = fminf(p0, fminf(p1, p2)), o.pmax = fmaxf(p0, fmaxf(p1, p2));
-  for (int r = 0; r < 3; ++r) o.cen[r] = (o.v[r] + o.v[3 + r]) + o.v[6 + r];
-  o.crd = crd;
-}
-// 0 certain miss, 1 needs the FP64 test, 2 certain hit (the bounds of cull_class).
-// The lower bound compares g^2 > T'^2 |n|^2 per axis instead of g / |n| > T,
-// with T' = T (1 + 2^-20) covering the few roundings of the squared form.
-// The upper bound (certain hit) is only computed when want_hit: a certain
-// hit only matters for node pairs with an inner node.
-__device__ __forceinline__ int cull_pair(const CullTri& A, const CullTri& B, float thr, bool want_hit = true) {
-  const float EPSF = 5.9604645e-08f;
-  if (A.ok == 0.f || B.ok == 0.f) return 1;
-  const float kappa = fmaxf(A.kap, B.kap);
-  const float L = sqrtf(fmaxf(A.L2, B.L2));
-  const float margin = 1e-6f + 64.f * EPSF * kappa * L + 16.f * EPSF * fmaxf(A.crd, B.crd) + EPSF * thr;
-  const float T = (thr + margin) * (1.f + 9.5367432e-07f);
-  const float T2 = T * T;
-  bool sep = false;
-  {  // axis nA
-    const float b0 = f3dot(B.v, A.n), b1 = f3dot(B.v + 3, A.n), b2 = f3dot(B.v + 6, A.n);
-    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - A.pmax, A.pmin - fmaxf(b0, fmaxf(b1, b2)));
-    sep = sep || (A.nn > 0.f && g > 0.f && g * g > T2 * A.nn);
-  }
-  {  // axis nB
-    const float a0 = f3dot(A.v, B.n), a1 = f3dot(A.v + 3, B.n), a2 = f3dot(A.v + 6, B.n);
-    const float g = fmaxf(B.pmin - fmaxf(a0, fmaxf(a1, a2)), fminf(a0, fminf(a1, a2)) - B.pmax);
-    sep = sep || (B.nn > 0.f && g > 0.f && g * g > T2 * B.nn);
-  }
-  {  // centroid direction
-    const float cd[3] = {B.cen[0] - A.cen[0], B.cen[1] - A.cen[1], B.cen[2] - A.cen[2]};
-    const float nn = f3dot(cd, cd);
-    const float a0 = f3dot(A.v, cd), a1 = f3dot(A.v + 3, cd), a2 = f3dot(A.v + 6, cd);
-    const float b0 = f3dot(B.v, cd), b1 = f3dot(B.v + 3, cd), b2 = f3dot(B.v + 6, cd);
-    const float g = fmaxf(fminf(b0, fminf(b1, b2)) - fmaxf(a0, fmaxf(a1, a2)),
-                          fminf(a0, fminf(a1, a2)) - fmaxf(b0, fmaxf(b1, b2)));
-    sep = sep || (nn > 0.f && g > 0.f && g * g > T2 * nn);
-  }
-  if (sep) return 0;
-  if (!want_hit) return 1;
-  float ub2 = 3.4e38f;
-  float dd[3][3];
-  for (int x = 0; x < 3; ++x)
-    for (int y = 0; y < 3; ++y) {
-      const float t[3] = {A.v[3 * x] - B.v[3 * y], A.v[3 * x + 1] - B.v[3 * y + 1], A.v[3 * x + 2] - B.v[3 * y + 2]};
-      dd[x][y] = f3dot(t, t);
-    }
-  for (int x = 0; x < 3; ++x)
-    for (int y = 0; y < 3; ++y) {
-      const int x1 = x == 2 ? 0 : x + 1, y1 = y == 2 ? 0 : y + 1;
-      ub2 = fminf(ub2, fmaxf(fmaxf(dd[x][y], dd[x][y1]), fmaxf(dd[x1][y], dd[x1][y1])));
-    }
-  const float lim = thr - margin;
-  if (lim > 0.f && ub2 < lim * lim) return 2;
-  return 1;
-}
+// The discrete phase's FP32 classification of triangle pairs (certain miss,
+// FP64 test, certain hit) is in d2_cull.hpp: cull_prep, cull_margin,
+// cull_pair.
 
 // One wavefront per node pair (chunked ranges): which triangle pairs need the
 // FP64 test.  FP64 mode keeps every valid pair; FP32 mode drops certain
@@ -1094,7 +920,7 @@ __device__ __forceinline__ unsigned child_mask(const Dev& D, const NP& np, int d
 // Five wavefronts per SIMD (at most 96 VGPRs, no spills) instead of the four
 // its natural 101 VGPRs allow: the kernel waits on dependent loads, and more
 // resident wavefronts hide more of them.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_tri_mask(Dev D, int fp32) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_tri_mask(Dev D) {
   __shared__ int s_tot[2][4], s_ch[2][4];
   __shared__ CullTri s_tri[4][4][16];  // per wavefront: 4 node pairs' 8 + 8 prepared triangles
   __shared__ unsigned long long s_base[2], s_chb[2];
@@ -1131,9 +957,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
         l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
         l_inner = (na.inner || nb.inner) ? 1 : 0;
         l_desc = expansion_desc(na, nb);
-        if (fp32) l_search = D.pair_rec[(size_t)PR_STRIDE * np.pair + PR_SEARCH];
+        l_search = D.pair_rec[(size_t)PR_STRIDE * np.pair + PR_SEARCH];
       }
-      if (!fp32) D.exp_desc[wl] = l_desc;
     }
     const long long left = w0 < re ? (re - w0 + nw - 1) / nw : 0;
     const int batch = left < 64 ? (int)left : 64;
@@ -1141,7 +966,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     int my_settled = 0;
     const int i = lane >> 3, j = lane & 7;
     for (int k0 = 0; k0 < batch; k0 += 4) {
-      if (fp32) {
+      {
         // the triangles of four node pairs at once: lane l prepares triangle
         // (l & 15) of node pair k0 + (l >> 4) (lanes 0-7 of a group: bucket a,
         // 8-15: bucket b), so no lane idles in the preparation
@@ -1171,24 +996,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
       for (int k = k0; k < kend; ++k) {
         const int nta = rl(l_nta, k), ntb = rl(l_ntb, k);
         const bool inner = rl(l_inner, k) != 0;
-        const bool valid = i < nta && j < ntb;
-        int cls = valid ? 1 : 0;
-        if (fp32 && valid) {  // operands read from LDS where they are used (fewer live registers)
+        int cls = 0;
+        if (i < nta && j < ntb) {  // operands read from LDS where they are used (fewer live registers)
           // no global load on the node pair's chain: the search distance came
           // with the node pair, the eps values with the prepared triangles
           const CullTri& A = s_tri[wid][k - k0][i];
           const CullTri& B = s_tri[wid][k - k0][8 + j];
           const double search = __longlong_as_double((long long)readlane_u64(
               (unsigned long long)__double_as_longlong(l_search), k));
-          const float thr = (float)((search + A.eps) + B.eps);
-          cls = cull_pair(A, B, thr, inner);
+          cls = cull_pair(A, B, (float)((search + A.eps) + B.eps), inner);
         }
         unsigned long long keep = __ballot(cls == 1 || (cls == 2 && !inner));
         const bool settled = inner && __ballot(cls == 2) != 0ull;
         if (settled) keep = 0ull;
         if (lane == k) my_keep = keep, my_settled = settled ? 1 : 0;
       }
-      if (fp32) wave_sync_lds();  // read before the next group overwrites
+      wave_sync_lds();  // read before the next group overwrites
     }
     if (lane < batch) D.np_hit[wl] = my_settled;
     // The kept triangle pairs go straight to the level's item list, and the
@@ -1196,10 +1019,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     // the next list: one slot range each per block and round, one atomic
     // each (order does not matter: contacts carry their emission keys).
     const int mine = lane < batch ? __popcll(my_keep) : 0;
-    // (FP32 mode; in FP64 mode every node pair keeps its pairs, so k_disc_expand
-    // expands the hit ones after k_tri_dist)
     const unsigned cmask =
-        fp32 && lane < batch && (my_settled || my_keep != 0ull) ? child_mask<true>(D, l_np, l_desc, l_offa, l_offb) : 0u;
+        lane < batch && (my_settled || my_keep != 0ull) ? child_mask<true>(D, l_np, l_desc, l_offa, l_offb) : 0u;
     const int nch = __popc(cmask);
     const int incl = wave_incl_scan(mine), ich = wave_incl_scan(nch);
     const int par = it & 1;
@@ -3488,7 +3309,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   const LevelGrids G = level_grids(c, discrete, level);
   const int cg = G.cull, eg = G.expand, sg = G.stream;
   if (discrete && c->precision == D2_FP32) {
-    timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D, 1);
+    timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
   } else if (discrete) {  // one wavefront per 64 node pairs
     timed_launch(c, K_TRI_FULL, k_tri_full, dim3(eg), dim3(256), D);
@@ -4214,7 +4035,7 @@ int d2_set_auto_grow(d2_ctx* c, int on) {
 int d2_set_tune(d2_ctx* c, const d2_tune* t) {
   if (!c || !t) return D2_ERR_ARG;
   if (t->chunk_blocks < 1 || t->chunk_blocks > 4096 || t->stream_blocks < 1 || t->stream_blocks > 16384 ||
-      t->post_blocks < 1 || t->post_blocks > 16384 || t->gate_phases < 1 || t->gate_phases > 3)
+      t->post_blocks < 1 || t->post_blocks > 16384 || t->gate_phases < 0 || t->gate_phases > 3)
     return fail(c, D2_ERR_ARG, "d2_set_tune: value out of range");
   c->tune = *t;
   if (c->learned_on) c->gate_phases = t->gate_phases;
@@ -4254,7 +4075,7 @@ int d2_autotune(d2_ctx* c, int kind, float dt, int filter, int reps, d2_tune* be
   d2_tune best = start;
   double best_ms = 0.0;
   int rc = time_it(best, &best_ms);
-  const std::vector<int> gate = {1, 2, 3};
+  const std::vector<int> gate = {0, 1, 2, 3};  // 0: the gate gated in no phase (it may not pay for itself)
   const int dims = c->learned_on ? 4 : 3;
   for (int sweep = 0; sweep < 2 && !rc; ++sweep)
     for (int dim = 0; dim < dims && !rc; ++dim) {

@@ -95,7 +95,7 @@ typedef struct d2_tune {
   int chunk_blocks;
   int stream_blocks;
   int post_blocks;
-  int gate_phases; /* phase mask of the learned gate when it is on (1 CCD, 2 discrete, 3 both) */
+  int gate_phases; /* phase mask of the learned gate when it is on (1 CCD, 2 discrete, 3 both; 0 none) */
 } d2_tune;
 int d2_set_tune(d2_ctx* ctx, const d2_tune* tune);
 int d2_get_tune(const d2_ctx* ctx, d2_tune* tune);

@@ -284,5 +284,53 @@ int main() {
     }
     printf("soup_culls %lld %lld (culled %lld, hits %lld)\n", subtests, viol, culled, hits);
   }
+  // FP32 classification of discrete triangle pairs (cull_prep, cull_pair): a
+  // certain miss never hits in FP64, a certain hit
+  // always does.  Pairs placed at the hit threshold within a relative 1e-5
+  // (moved along their closest-point direction), skinny triangles, world
+  // offsets up to 40 m.
+  {
+    long long cases = 0, miss = 0, two = 0, hits = 0, viol = 0;
+    for (int k = 0; k < 2 * N; ++k) {
+      double Ta[16], Tb[16];
+      rand_pose(Ta, (k % 4 == 0) ? 40.0 : 2.0, 3.2);
+      double rel[16];
+      rand_pose(rel, (k % 3 == 0) ? 0.01 : 0.5, 3.2);
+      d2::mul44(Ta, rel, Tb);
+      const double sz = (k % 3 == 0) ? U(0.002, 0.01) : U(0.02, 0.5);  // small ones give certain hits
+      d2o::Tri<double> oa = rand_tri(sz, 0.0), ob = rand_tri(sz, 0.0);
+      if (k % 7 == 0) oa.C = oa.A + (oa.B - oa.A) * U(0.0, 1.0) + d2o::V3<double>(U(-1, 1), U(-1, 1), U(-1, 1)) * 1e-3;
+      const double search = U(0.001, 0.03), ea = U(0.0, 0.01), eb = U(0.0, 0.01);
+      const double thr_d = (search + ea) + eb;
+      d2::tri<double> ta = cvt(oa), tb = cvt(ob);
+      d2::v3<double> P, Q;
+      double d = d2::tri_tri_dist(d2::xform(Ta, ta), d2::xform(Tb, tb), P, Q);
+      if (k % 2 == 0 && d > 0.0) {  // move b so that the distance sits at the threshold
+        const double target = thr_d * (1.0 + U(-1e-5, 1e-5));
+        const d2::v3<double> dir = d2::dvs(d2::sub(Q, P), d);
+        const double mv = target - d;
+        for (int r = 0; r < 3; ++r) Tb[4 * r + 3] += mv * (r == 0 ? dir.x : r == 1 ? dir.y : dir.z);
+        d = d2::tri_tri_dist(d2::xform(Ta, ta), d2::xform(Tb, tb), P, Q);
+      }
+      const bool hit = d < thr_d;
+      d2::CullTri A, B;
+      float rot[9], tr[3];
+      for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)Ta[4 * r + q];
+        tr[r] = (float)(Ta[4 * r + 3] - Tb[4 * r + 3]);
+      }
+      d2::cull_prep(rot, tr, ta, A);
+      for (int r = 0; r < 3; ++r) {
+        for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)Tb[4 * r + q];
+        tr[r] = 0.f;
+      }
+      d2::cull_prep(rot, tr, tb, B);
+      const float thr = (float)thr_d;
+      const int cls = d2::cull_pair(A, B, thr, true);
+      ++cases, hits += hit, miss += cls == 0, two += cls == 2;
+      viol += (cls == 0 && hit) || (cls == 2 && !hit);
+    }
+    printf("disc_culls %lld %lld (miss %lld, hit %lld, fp64 hits %lld)\n", cases, viol, miss, two, hits);
+  }
   return 0;
 }

@@ -4,8 +4,9 @@ d2_geom.hpp is the arithmetic of every kernel; compiling it for the host and
 comparing with the oracle on random inputs pins the kernels' FP64 expression
 order without a GPU (tri-tri distance, point-triangle CCD with the per-pair
 frame changes, edge-edge CCD incl. bisection cases, state extrapolation), and
-the subtree-pruning bound never hides a leaf contact, and the FP32 culls of
-the CCD soup sub-tests never drop a sub-test that reports a hit."""
+the subtree-pruning bound never hides a leaf contact, and the FP32 culls
+(CCD soup sub-tests, discrete triangle pairs at the hit threshold) never
+drop a sub-test or pair that reports a hit in FP64."""
 import os
 import re
 import subprocess
@@ -34,6 +35,12 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
     line = next(l for l in out.splitlines() if l.startswith("soup_culls"))
     culled, hits = (int(x) for x in re.findall(r"(?:culled|hits) (\d+)", line))
     assert int(rows["soup_culls"][1]) >= 100000 and culled > int(rows["soup_culls"][1]) // 2 and hits > 1000, out
+    # FP32 classification of discrete triangle pairs near the hit threshold:
+    # no certain miss hits, every certain hit hits; each outcome occurs often
+    assert int(rows["disc_culls"][2]) == 0, out
+    line = next(l for l in out.splitlines() if l.startswith("disc_culls"))
+    counts = [int(x) for x in re.findall(r"(?:miss|hit|fp64 hits) (\d+)", line)]
+    assert int(rows["disc_culls"][1]) >= 20000 and len(counts) == 3 and min(counts) > 1000, out
 
 
 def test_partition_rank_sort_matches_std_sort(tmp_path):
