@@ -17,7 +17,11 @@
 //     separation of the two segments along an axis that separates them on the
 //     same side at t = 0 and at t = 1 (point offsets are linear in t).  Axes:
 //     the midpoint offsets and the edge-direction cross products at both ends.
-// margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products).
+// margin = 1e-6 m + 64 eps_f |coords| (FP32 transforms and dot products),
+// where |coords| bounds every magnitude the transforms involve: the results,
+// the body coordinates transformed and the matrices' translations (a frame
+// change between two far-apart body origins cancels large terms, and its
+// rounding scales with them, not with the small result).
 //
 // The includer defines D2_HD (see d2_geom.hpp, included first).
 #pragma once
@@ -50,17 +54,24 @@ D2_HD void xf(const M34f& M, const v3<double>& p, float* o) {
   o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
 }
 D2_HD float amax3(const float* v) { return fmaxf(fabsf(v[0]), fmaxf(fabsf(v[1]), fabsf(v[2]))); }
+// The largest translation entry of a pose, and of a body-coordinate point:
+// magnitudes the FP32 transform's rounding scales with.
+D2_HD float m34_mag(const M34f& M) { return fmaxf(fabsf(M.m[3]), fmaxf(fabsf(M.m[7]), fabsf(M.m[11]))); }
+D2_HD float v3_mag(const v3<double>& p) { return fmaxf(fabsf((float)p.x), fmaxf(fabsf((float)p.y), fabsf((float)p.z))); }
 // Separation of segment Q above segment P along u at one instant (negative
 // when none), scaled by 1/|u|.
 D2_HD float seg_sep(const float* u, float nu, const float* p0, const float* p1, const float* q0, const float* q1) {
   const float a0 = f3dot(u, p0), a1 = f3dot(u, p1), b0 = f3dot(u, q0), b1 = f3dot(u, q1);
   return (fminf(b0, b1) - fmaxf(a0, a1)) / nu;
 }
+// extra: the magnitudes of the transforms' inputs (body coordinates,
+// translations) that made these points.
 D2_HD bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, const float* ae1, const float* bs0,
-                          const float* bs1, const float* be0, const float* be1, float thr) {
+                          const float* bs1, const float* be0, const float* be1, float thr, float extra) {
   const float EPSF = 5.9604645e-08f;
-  const float mag = fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
-                          fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1))));
+  const float mag = fmaxf(fmaxf(fmaxf(fmaxf(amax3(as0), amax3(as1)), fmaxf(amax3(ae0), amax3(ae1))),
+                                fmaxf(fmaxf(amax3(bs0), amax3(bs1)), fmaxf(amax3(be0), amax3(be1)))),
+                          extra);
   const float lim = thr + 1e-6f + 64.f * EPSF * mag;
   float ax[4][3];
   for (int r = 0; r < 3; ++r) {
@@ -79,6 +90,54 @@ D2_HD bool cull_edge_edge(const float* as0, const float* as1, const float* ae0, 
     const float up = fminf(seg_sep(ax[k], nu, as0, as1, bs0, bs1), seg_sep(ax[k], nu, ae0, ae1, be0, be1));
     const float dn = fminf(seg_sep(ax[k], nu, bs0, bs1, as0, as1), seg_sep(ax[k], nu, be0, be1, ae0, ae1));
     if (fmaxf(up, dn) > lim) return true;
+  }
+  return false;
+}
+
+// All nine edge-edge sub-tests of a triangle pair at once: the vertices of
+// triangle a (a0 at t = 0, a1 at t = 1, world coordinates as in
+// cull_edge_edge) and of b.  A point of any edge of a at time t is a convex
+// combination of a's lerped vertices, so along a fixed axis u its projection
+// lies within the lerp of a's extremes at t = 0 and t = 1, and likewise for b;
+// an axis that separates the two vertex sets on the same side at both ends
+// by more than cull_edge_edge's limit therefore culls every edge pair.  Axes:
+// the centroid offset over the step and both triangles' normals at both ends.
+D2_HD bool cull_tri_edges(const float (*a0)[3], const float (*a1)[3], const float (*b0)[3], const float (*b1)[3],
+                          float thr, float extra) {
+  const float EPSF = 5.9604645e-08f;
+  float mag = extra;
+  for (int k = 0; k < 3; ++k)
+    mag = fmaxf(mag, fmaxf(fmaxf(amax3(a0[k]), amax3(a1[k])), fmaxf(amax3(b0[k]), amax3(b1[k]))));
+  const float lim = thr + 1e-6f + 64.f * EPSF * mag;
+  float ax[5][3];
+  for (int r = 0; r < 3; ++r)
+    ax[0][r] = ((b0[0][r] + b0[1][r]) + (b0[2][r] + b1[0][r]) + (b1[1][r] + b1[2][r])) -
+               ((a0[0][r] + a0[1][r]) + (a0[2][r] + a1[0][r]) + (a1[1][r] + a1[2][r]));
+  const float (*T[4])[3] = {a0, a1, b0, b1};
+  for (int q = 0; q < 4; ++q) {
+    const float (*p)[3] = T[q];
+    const float e0[3] = {p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]};
+    const float e1[3] = {p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]};
+    ax[1 + q][0] = e0[1] * e1[2] - e0[2] * e1[1], ax[1 + q][1] = e0[2] * e1[0] - e0[0] * e1[2];
+    ax[1 + q][2] = e0[0] * e1[1] - e0[1] * e1[0];
+  }
+  for (int k = 0; k < 5; ++k) {
+    const float nu = sqrtf(f3dot(ax[k], ax[k]));
+    if (!(nu > 1e-20f)) continue;
+    float amin0 = 3.4e38f, amax0 = -3.4e38f, amin1 = 3.4e38f, amax1 = -3.4e38f;
+    float bmin0 = 3.4e38f, bmax0 = -3.4e38f, bmin1 = 3.4e38f, bmax1 = -3.4e38f;
+    for (int v = 0; v < 3; ++v) {
+      float x = f3dot(ax[k], a0[v]);
+      amin0 = fminf(amin0, x), amax0 = fmaxf(amax0, x);
+      x = f3dot(ax[k], a1[v]);
+      amin1 = fminf(amin1, x), amax1 = fmaxf(amax1, x);
+      x = f3dot(ax[k], b0[v]);
+      bmin0 = fminf(bmin0, x), bmax0 = fmaxf(bmax0, x);
+      x = f3dot(ax[k], b1[v]);
+      bmin1 = fminf(bmin1, x), bmax1 = fmaxf(bmax1, x);
+    }
+    const float up = fminf(bmin0 - amax0, bmin1 - amax1), dn = fminf(amin0 - bmax0, amin1 - bmax1);
+    if (fmaxf(up, dn) / nu > lim) return true;
   }
   return false;
 }
@@ -139,49 +198,67 @@ D2_HD bool soup_point_keep(const float* p0, const float* p1, const SoupTri& t, f
 // 5k+1 corner k of tb against ta, 5k+2+l edge k of ta against edge l of tb.
 // Poses: Ta0/Ta1, Tb0/Tb1 (start / end), Mab = inverse(Tb) Ta and Mba =
 // inverse(Ta) Tb at both ends (the frame changes of the point sub-tests).
+// soup_cull_mask composes the three parts below; the kernel computes the
+// same parts, with the per-edge culls packed over a wavefront.
+
+// The triangles' corners in world coordinates relative to b's start
+// position at both ends (w[0..2] a at t = 0, w[3..5] a at t = 1, w[6..8] b at
+// t = 0, w[9..11] b at t = 1); returns the magnitude of the transforms'
+// inputs (the edge culls' extra).
+D2_HD float soup_world(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1,
+                       const tri<double>& ta, const tri<double>& tb, float (*w)[3]) {
+  const M34f A0 = to_m34f(Ta0, Tb0), A1 = to_m34f(Ta1, Tb0);
+  const M34f B0 = to_m34f(Tb0, Tb0), B1 = to_m34f(Tb1, Tb0);
+  float extra = fmaxf(fmaxf(m34_mag(A0), m34_mag(A1)), m34_mag(B1));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    xf(A0, corner(ta, k), w[k]), xf(A1, corner(ta, k), w[3 + k]);
+    xf(B0, corner(tb, k), w[6 + k]), xf(B1, corner(tb, k), w[9 + k]);
+    extra = fmaxf(extra, fmaxf(v3_mag(corner(ta, k)), v3_mag(corner(tb, k))));
+  }
+  return extra;
+}
+// The point-triangle slots: a's corners in b's frame against tb, b's
+// corners in a's frame against ta.
+D2_HD unsigned soup_point_mask(const double* Mab0, const double* Mab1, const double* Mba0, const double* Mba1,
+                               const tri<double>& ta, const tri<double>& tb, float thr) {
+  unsigned mask = 0u;
+  float pa0[3][3], pa1[3][3], pb0[3][3], pb1[3][3];
+  const M34f F0 = to_m34f(Mab0, nullptr), F1 = to_m34f(Mab1, nullptr);
+  const M34f G0 = to_m34f(Mba0, nullptr), G1 = to_m34f(Mba1, nullptr);
+  float mag = fmaxf(fmaxf(m34_mag(F0), m34_mag(F1)), fmaxf(m34_mag(G0), m34_mag(G1)));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    xf(F0, corner(ta, k), pa0[k]), xf(F1, corner(ta, k), pa1[k]);
+    xf(G0, corner(tb, k), pb0[k]), xf(G1, corner(tb, k), pb1[k]);
+    mag = fmaxf(mag, fmaxf(fmaxf(amax3(pa0[k]), amax3(pa1[k])), fmaxf(amax3(pb0[k]), amax3(pb1[k]))));
+    mag = fmaxf(mag, fmaxf(v3_mag(corner(ta, k)), v3_mag(corner(tb, k))));
+  }
+  const float margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
+  const SoupTri sa = soup_tri(ta), sb = soup_tri(tb);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (soup_point_keep(pa0[k], pa1[k], sb, thr, margin)) mask |= 1u << (5 * k);
+    if (soup_point_keep(pb0[k], pb1[k], sa, thr, margin)) mask |= 1u << (5 * k + 1);
+  }
+  return mask;
+}
+// Edge pair (k, l)'s slot survives cull_edge_edge (world corners w and their
+// inputs' magnitude extra as soup_world gives them).
+D2_HD bool soup_edge_keep(const float (*w)[3], int k, int l, float thr, float extra) {
+  const int k1 = k == 2 ? 0 : k + 1, l1 = l == 2 ? 0 : l + 1;
+  return !cull_edge_edge(w[k], w[k1], w[3 + k], w[3 + k1], w[6 + l], w[6 + l1], w[9 + l], w[9 + l1], thr, extra);
+}
 D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1,
                               const double* Mab0, const double* Mab1, const double* Mba0, const double* Mba1,
                               const tri<double>& ta, const tri<double>& tb, float thr) {
-  unsigned mask = 0u;
-  {  // edge-edge: world coordinates relative to b's start position
-    float wa0[3][3], wa1[3][3], wb0[3][3], wb1[3][3];
-    const M34f A0 = to_m34f(Ta0, Tb0), A1 = to_m34f(Ta1, Tb0);
-    const M34f B0 = to_m34f(Tb0, Tb0), B1 = to_m34f(Tb1, Tb0);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      xf(A0, corner(ta, k), wa0[k]), xf(A1, corner(ta, k), wa1[k]);
-      xf(B0, corner(tb, k), wb0[k]), xf(B1, corner(tb, k), wb1[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-#pragma unroll
-      for (int l = 0; l < 3; ++l)
-        if (!cull_edge_edge(wa0[k], wa0[(k + 1) % 3], wa1[k], wa1[(k + 1) % 3], wb0[l], wb0[(l + 1) % 3], wb1[l],
-                            wb1[(l + 1) % 3], thr))
-          mask |= 1u << (5 * k + 2 + l);
-  }
-  {  // point-triangle: a's corners in b's frame against tb, b's corners in a's frame against ta
-    float pa0[3][3], pa1[3][3], pb0[3][3], pb1[3][3];
-    const M34f F0 = to_m34f(Mab0, nullptr), F1 = to_m34f(Mab1, nullptr);
-    const M34f G0 = to_m34f(Mba0, nullptr), G1 = to_m34f(Mba1, nullptr);
-    float mag = 0.f;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      xf(F0, corner(ta, k), pa0[k]), xf(F1, corner(ta, k), pa1[k]);
-      xf(G0, corner(tb, k), pb0[k]), xf(G1, corner(tb, k), pb1[k]);
-      mag = fmaxf(mag, fmaxf(fmaxf(amax3(pa0[k]), amax3(pa1[k])), fmaxf(amax3(pb0[k]), amax3(pb1[k]))));
-      const v3<double> ca = corner(ta, k), cb = corner(tb, k);
-      mag = fmaxf(mag, fmaxf(fmaxf(fabsf((float)ca.x), fabsf((float)ca.y)), fabsf((float)ca.z)));
-      mag = fmaxf(mag, fmaxf(fmaxf(fabsf((float)cb.x), fabsf((float)cb.y)), fabsf((float)cb.z)));
-    }
-    const float margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
-    const SoupTri sa = soup_tri(ta), sb = soup_tri(tb);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (soup_point_keep(pa0[k], pa1[k], sb, thr, margin)) mask |= 1u << (5 * k);
-      if (soup_point_keep(pb0[k], pb1[k], sa, thr, margin)) mask |= 1u << (5 * k + 1);
-    }
-  }
+  unsigned mask = soup_point_mask(Mab0, Mab1, Mba0, Mba1, ta, tb, thr);
+  float w[12][3];
+  const float extra = soup_world(Ta0, Ta1, Tb0, Tb1, ta, tb, w);
+  if (cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra)) return mask;  // all nine edge pairs culled at once
+  for (int k = 0; k < 3; ++k)
+    for (int l = 0; l < 3; ++l)
+      if (soup_edge_keep(w, k, l, thr, extra)) mask |= 1u << (5 * k + 2 + l);
   return mask;
 }
 
@@ -206,8 +283,9 @@ D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double
 //
 // The per-triangle terms are computed once (cull_prep): vertices in the
 // frame of the pair (b's position as origin), normal, kappa, longest squared
-// edge, own-normal extent, centroid, largest coordinate, and whether the
-// barycentric system is well conditioned.
+// edge, own-normal extent, centroid, largest coordinate (of the results and
+// of the transform's inputs), and whether the barycentric system is well
+// conditioned.
 struct alignas(16) CullTri {
   float c3[3], kap, L2, crd, ok;
   float v[9], n[3], nn, pmin, pmax;
@@ -215,7 +293,9 @@ struct alignas(16) CullTri {
 };
 D2_HD void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
   const v3<double> vv[3] = {t.a, t.b, t.c};
-  float crd = 0.f;
+  // the magnitudes the transform's rounding scales with: inputs and results
+  float crd = fmaxf(fabsf(tr[0]), fmaxf(fabsf(tr[1]), fabsf(tr[2])));
+  for (int v = 0; v < 3; ++v) crd = fmaxf(crd, v3_mag(vv[v]));
   for (int v = 0; v < 3; ++v) {
     const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
     for (int r = 0; r < 3; ++r) {

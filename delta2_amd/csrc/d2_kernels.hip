@@ -573,35 +573,76 @@ __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<
 // bound of that triangle.
 constexpr unsigned SOUP_PT_BITS = 0xC63u;   // slots 5k and 5k+1 (point-triangle, k = 0..2)
 constexpr unsigned SOUP_EE_BITS = 0x739Cu;  // slots 5k+2+l (edge k of a, edge l of b)
-// Surviving sub-test slots of soup item it (bit s: slot s of triTriCCDLinear's order).
-__device__ unsigned soup_keep_mask(const Dev& D, long long it) {
-  const Item item = D.items[it];
-  const NP np = D.soup[item.np];
-  const int32_t* pi = D.pair_info + 4 * np.pair;
-  const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-  const NodeRec na = D.nodes[pi[2] + np.na];
-  const NodeRec nb = D.nodes[pi[3] + np.nb];
-  const int i = item.sub >> 3, j = item.sub & 7;
-  const tri<double> ta = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[2] + na.bucket), i);
-  const tri<double> tb = load_tri<double>(D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket), j);
-  return soup_cull_mask(pr + PR_TA0, pr + PR_TA1, pr + PR_TB0, pr + PR_TB1, pr + PR_MAB0, pr + PR_MAB1, pr + PR_MBA0,
-                        pr + PR_MBA1, ta, tb, (float)pr[PR_SEARCH]);
-}
-// FP32 form: the survivor masks of the level's soup items and the two
-// compact sub-test lists (wavefront prefix sums, one atomic per wavefront
-// and list; the lists' order is free, results go to fixed slots).
+// FP32 form: the survivor masks of the level's soup items (soup_cull_mask's
+// parts: point slots per item; edge slots by the triangle-level test per item
+// and, for the items it leaves, the per-edge culls packed over the
+// wavefront, nine per item, with the items' world corners staged in LDS),
+// then the two compact sub-test lists (wavefront prefix sums, one atomic per
+// wavefront and list; the lists' order is free, results go to fixed slots).
 __global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
+  __shared__ float s_w[4][64][36];        // per wavefront: world corners of the items left for the per-edge culls
+  __shared__ float s_thr[4][64];          // their search distances
+  __shared__ float s_ex[4][64];           // and their transforms' input magnitudes
+  __shared__ unsigned char s_fl[4][64];   // their lanes
+  __shared__ unsigned s_em[4][64];        // per lane: surviving edge slots
   const long long n = D.n_res();
-  const int lane = lane_id();
+  const int lane = lane_id(), wv = threadIdx.x >> 6;
   unsigned long long* cpt = D.lvl + D.level * LV_STRIDE + LV_SPT;
   unsigned long long* cee = D.lvl + D.level * LV_STRIDE + LV_SEE;
   for (long long b0 = (long long)blockIdx.x * blockDim.x; b0 < n; b0 += (long long)gridDim.x * blockDim.x) {
     const long long it = b0 + threadIdx.x;
-    unsigned mask = 0u;
+    bool need = false;
+    const double* pr = nullptr;
+    const double *bka = nullptr, *bkb = nullptr;
+    int ia = 0, ib = 0;
     if (it < n) {
-      mask = soup_keep_mask(D, it);
-      D.res_mask[it] = mask;
+      const Item item = D.items[it];
+      const NP np = D.soup[item.np];
+      const int32_t* pi = D.pair_info + 4 * np.pair;
+      pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+      bka = D.buckets + (size_t)BUCKET * (pi[2] + D.nodes[pi[2] + np.na].bucket);
+      bkb = D.buckets + (size_t)BUCKET * (pi[3] + D.nodes[pi[3] + np.nb].bucket);
+      ia = (int)(item.sub >> 3), ib = (int)(item.sub & 7);
     }
+    {  // the triangle-level edge test; the corners of the items it leaves to LDS
+      float w[12][3];
+      float thr = 0.f, extra = 0.f;
+      if (it < n) {
+        thr = (float)pr[PR_SEARCH];
+        extra = soup_world(pr + PR_TA0, pr + PR_TA1, pr + PR_TB0, pr + PR_TB1, load_tri<double>(bka, ia),
+                           load_tri<double>(bkb, ib), w);
+        need = !cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra);
+      }
+      const unsigned long long tm = __ballot(need);
+      if (need) {
+        const int r = __popcll(tm & lanes_below());
+#pragma unroll
+        for (int q = 0; q < 12; ++q)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) s_w[wv][r][3 * q + c] = w[q][c];
+        s_thr[wv][r] = thr;
+        s_ex[wv][r] = extra;
+        s_fl[wv][r] = (unsigned char)lane;
+      }
+    }
+    s_em[wv][lane] = 0u;
+    unsigned mask = 0u;  // the point slots (the triangles are read again: fewer live registers)
+    if (it < n)
+      mask = soup_point_mask(pr + PR_MAB0, pr + PR_MAB1, pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(bka, ia),
+                             load_tri<double>(bkb, ib), (float)pr[PR_SEARCH]);
+    wave_sync_lds();
+    const int jobs = 9 * __popcll(__ballot(need));
+    for (int q0 = 0; q0 < jobs; q0 += 64) {
+      const int q = q0 + lane;
+      if (q < jobs) {
+        const int r = q / 9, e = q % 9, k = e / 3, l = e % 3;
+        if (soup_edge_keep(reinterpret_cast<const float(*)[3]>(s_w[wv][r]), k, l, s_thr[wv][r], s_ex[wv][r]))
+          atomicOr(&s_em[wv][s_fl[wv][r]], 1u << (5 * k + 2 + l));
+      }
+    }
+    wave_sync_lds();
+    mask |= s_em[wv][lane];
+    if (it < n) D.res_mask[it] = mask;
     const int mp = __popc(mask & SOUP_PT_BITS), me = __popc(mask & SOUP_EE_BITS);
     const int ip = wave_incl_scan(mp), ie = wave_incl_scan(me);
     unsigned long long bp = 0ull, be = 0ull;
@@ -617,6 +658,7 @@ __global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
       const long long slot = pt ? sp++ : se++;
       if (slot < D.cap_items) (pt ? D.pt_items : D.ee_items)[slot] = Item{(uint32_t)it, (uint32_t)s};
     }
+    wave_sync_lds();  // LDS read before the next round overwrites it
   }
 }
 
@@ -1241,7 +1283,7 @@ struct LeafShare {
 };
 struct LeafInfo {
   int n, m;
-  float thr, eo_a, eo_b, margin;
+  float thr, eo_a, eo_b, margin, mag;  // mag: every magnitude of the node pair's FP32 transforms
 };
 __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
   const int lane = lane_id();
@@ -1270,6 +1312,7 @@ __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
       const M34f F1 = to_m34f(of_b ? pr + PR_MBA1 : pr + PR_MAB1, nullptr);
       xf(W0, p, sh.w0), xf(W1, p, sh.w1), xf(F0, p, sh.f0), xf(F1, p, sh.f1);
       mag = fmaxf(fmaxf(amax3(sh.w0), amax3(sh.w1)), fmaxf(amax3(sh.f0), amax3(sh.f1)));
+      mag = fmaxf(mag, fmaxf(fmaxf(fmaxf(m34_mag(W0), m34_mag(W1)), fmaxf(m34_mag(F0), m34_mag(F1))), v3_mag(p)));
     }
   } else {
     const bool of_b = lane >= 56;
@@ -1308,6 +1351,7 @@ __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
     }
   }
   for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_xor(mag, off));
+  I.mag = mag;
   I.margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
   return I;
 }
@@ -1350,7 +1394,7 @@ __device__ bool leaf_edge_keep(const LeafShare& sh, const LeafInfo& I, int s, bo
   sh3(sh.w0, a0, as0), sh3(sh.w0, a1, as1), sh3(sh.w1, a0, ae0), sh3(sh.w1, a1, ae1);
   sh3(sh.w0, b0, bs0), sh3(sh.w0, b1, bs1), sh3(sh.w1, b0, be0), sh3(sh.w1, b1, be1);
   if (!active) return false;
-  return !cull_edge_edge(as0, as1, ae0, ae1, bs0, bs1, be0, be1, I.thr);
+  return !cull_edge_edge(as0, as1, ae0, ae1, bs0, bs1, be0, be1, I.thr, I.mag);
 }
 constexpr int CONN_MASK_WORDS = 16;  // 6 words of point sub-tests (<= 384), 9 of edge sub-tests (<= 576)
 

@@ -241,12 +241,22 @@ int main() {
   // two nearby bodies (world offsets up to 40 m), small motions over the
   // step, search distances 2-50 mm.
   {
-    long long subtests = 0, culled = 0, hits = 0, viol = 0;
+    long long subtests = 0, culled = 0, hits = 0, viol = 0, tri_cut = 0, tri_viol = 0;
     for (int k = 0; k < N; ++k) {
       double Ta0[16], Ta1[16], Tb0[16], Tb1[16], rel[16], da[16], db[16], inv[16];
       double Mab0[16], Mab1[16], Mba0[16], Mba1[16];
       rand_pose(Ta0, (k % 4 == 0) ? 40.0 : 2.0, 3.2);
       rand_pose(rel, (k % 2) ? 0.6 : 0.15, 3.2);  // half of the pairs nearly touching
+      // a third of the pairs with one triangle 10-30 m from its body's origin
+      // (a funnel wall) and the other body's origin next to it: the frame
+      // changes then cancel large terms
+      const int far = k % 3;  // 1: a's triangle far out, 2: b's
+      const double off = far ? U(10.0, 30.0) : 0.0;
+      if (far == 1) {
+        rel[3] = off + U(-0.3, 0.3), rel[7] = U(-0.3, 0.3), rel[11] = U(-0.3, 0.3);
+      } else if (far == 2) {
+        for (int r = 0; r < 3; ++r) rel[4 * r + 3] = -rel[4 * r] * off + U(-0.3, 0.3);
+      }
       d2::mul44(Ta0, rel, Tb0);
       rand_pose(da, 0.15, 0.2);
       rand_pose(db, 0.15, 0.2);
@@ -257,12 +267,18 @@ int main() {
       d2::inv44(Ta0, inv), d2::mul44(inv, Tb0, Mba0);
       d2::inv44(Ta1, inv), d2::mul44(inv, Tb1, Mba1);
       const double sa = U(0.05, 0.5), sb = U(0.05, 0.5);
-      d2o::Tri<double> oa = rand_tri(sa, U(-0.3, 0.3)), ob = rand_tri(sb, U(-0.3, 0.3));
+      d2o::Tri<double> oa = rand_tri(sa, U(-0.3, 0.3) + (far == 1 ? off : 0.0));
+      d2o::Tri<double> ob = rand_tri(sb, U(-0.3, 0.3) + (far == 2 ? off : 0.0));
       if (k % 8 == 0) ob.C = ob.A + (ob.B - ob.A) * U(0.0, 1.0);  // degenerate (collinear) triangles too
       const d2::tri<double> ta = cvt(oa), tb = cvt(ob);
       const float thr = (float)U(0.002, 0.05);
       const double thr_d = (double)thr;
       const unsigned mask = d2::soup_cull_mask(Ta0, Ta1, Tb0, Tb1, Mab0, Mab1, Mba0, Mba1, ta, tb, thr);
+      // the triangle-level edge cull (cull_tri_edges) on the same float world points
+      float w[12][3];
+      const float extra = d2::soup_world(Ta0, Ta1, Tb0, Tb1, ta, tb, w);
+      const bool tcut = d2::cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra);
+      tri_cut += tcut;
       for (int s = 0; s < 15; ++s) {
         const int kk = s / 5, r = s % 5;
         d2::v3<double> P, Q;
@@ -280,9 +296,11 @@ int main() {
         }
         const bool hit = d < thr_d, keep = (mask >> s) & 1u;
         ++subtests, hits += hit, culled += !keep, viol += hit && !keep;
+        tri_viol += r >= 2 && tcut && hit;
       }
     }
     printf("soup_culls %lld %lld (culled %lld, hits %lld)\n", subtests, viol, culled, hits);
+    printf("tri_edges %d %lld (culled %lld)\n", N, tri_viol, tri_cut);
   }
   // FP32 classification of discrete triangle pairs (cull_prep, cull_pair): a
   // certain miss never hits in FP64, a certain hit
@@ -296,9 +314,11 @@ int main() {
       rand_pose(Ta, (k % 4 == 0) ? 40.0 : 2.0, 3.2);
       double rel[16];
       rand_pose(rel, (k % 3 == 0) ? 0.01 : 0.5, 3.2);
+      const double off = (k % 5 == 1) ? U(10.0, 30.0) : 0.0;  // a's triangle far from a's origin, b's origin near it
+      if (off > 0.0) rel[3] += off;
       d2::mul44(Ta, rel, Tb);
       const double sz = (k % 3 == 0) ? U(0.002, 0.01) : U(0.02, 0.5);  // small ones give certain hits
-      d2o::Tri<double> oa = rand_tri(sz, 0.0), ob = rand_tri(sz, 0.0);
+      d2o::Tri<double> oa = rand_tri(sz, off), ob = rand_tri(sz, 0.0);
       if (k % 7 == 0) oa.C = oa.A + (oa.B - oa.A) * U(0.0, 1.0) + d2o::V3<double>(U(-1, 1), U(-1, 1), U(-1, 1)) * 1e-3;
       const double search = U(0.001, 0.03), ea = U(0.0, 0.01), eb = U(0.0, 0.01);
       const double thr_d = (search + ea) + eb;

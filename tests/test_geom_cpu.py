@@ -35,6 +35,9 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
     line = next(l for l in out.splitlines() if l.startswith("soup_culls"))
     culled, hits = (int(x) for x in re.findall(r"(?:culled|hits) (\d+)", line))
     assert int(rows["soup_culls"][1]) >= 100000 and culled > int(rows["soup_culls"][1]) // 2 and hits > 1000, out
+    # the triangle-level edge cull (nine edge-edge sub-tests at once) is sound
+    assert int(rows["tri_edges"][2]) == 0, out
+    assert int(re.search(r"tri_edges .*culled (\d+)", out).group(1)) > 1000, out
     # FP32 classification of discrete triangle pairs near the hit threshold:
     # no certain miss hits, every certain hit hits; each outcome occurs often
     assert int(rows["disc_culls"][2]) == 0, out
