@@ -47,6 +47,12 @@ D2_HD M34f to_m34f(const double* m, const double* org) {
   }
   return o;
 }
+D2_HD void xf_f(const M34f& M, const float* p, float* o) {  // p: body coordinates rounded to float
+  const float x = p[0], y = p[1], z = p[2];
+  o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
+  o[1] = (M.m[4] * x + M.m[5] * y) + (M.m[6] * z + M.m[7]);
+  o[2] = (M.m[8] * x + M.m[9] * y) + (M.m[10] * z + M.m[11]);
+}
 D2_HD void xf(const M34f& M, const v3<double>& p, float* o) {
   const float x = (float)p.x, y = (float)p.y, z = (float)p.z;
   o[0] = (M.m[0] * x + M.m[1] * y) + (M.m[2] * z + M.m[3]);
@@ -142,13 +148,15 @@ D2_HD bool cull_tri_edges(const float (*a0)[3], const float (*a1)[3], const floa
   return false;
 }
 
-// A soup (surrogate) triangle's plane and outward edge normals in its body
-// frame, computed in FP64 and rounded once to float: unit normal n, offset
-// d = n.A, in-plane edge normals m[q] pointing away from the opposite corner
-// with offsets o[q], and eo = 64 u kappa L (u = 2^-53, kappa = d00 d11 /
-// (d00 d11 - d01^2), L the longest edge), the FP64 barycentric acceptance
-// bound of the reference's inside test.  A degenerate triangle gets n = 0
-// and o = +huge: its sub-tests are never culled.
+// A triangle's plane and outward edge normals in its body frame, computed
+// in FP64 and rounded once to float: unit normal n (the reference's normal()
+// bit for bit before the rounding), offset d = n.A, in-plane edge normals
+// m[q] pointing away from the opposite corner with offsets o[q], and eo =
+// 64 eps_f kappa L (kappa = d00 d11 / (d00 d11 - d01^2), L the longest edge),
+// the acceptance bound of the reference's float-barycentric inside test
+// (huge when the system is ill-conditioned, denominator <= 1e-4 d00 d11).  A
+// degenerate triangle gets n = 0 and o = +huge: its sub-tests are never
+// culled.
 struct SoupTri {
   float n[3], d, m[9], o[3], eo;
 };
@@ -165,7 +173,9 @@ D2_HD SoupTri soup_tri(const tri<double>& T) {
   const double den = d00 * d11 - d01 * d01;
   if (!(cl > 0.0) || !(den > 0.0)) return s;  // degenerate: n = 0 keeps every sub-test
   const double L = sqrt(d00 > d11 ? (d00 > d22 ? d00 : d22) : (d11 > d22 ? d11 : d22));
-  s.eo = (float)(64.0 * 1.1102230246251565e-16 * (d00 * d11 / den) * L * (1.0 + 1e-6));
+  // the reference's barycentrics are float (triangle.h:92-100): an
+  // ill-conditioned system gets no lateral cull, the others 64 eps_f kappa L
+  if (den > 1e-4 * d00 * d11) s.eo = (float)(64.0 * 5.9604644775390625e-08 * (d00 * d11 / den) * L * (1.0 + 1e-6));
   const v3<double> n = normal(T);  // the reference's normal() bit for bit, rounded once to float below
   s.n[0] = (float)n.x, s.n[1] = (float)n.y, s.n[2] = (float)n.z;
   s.d = (float)dot(n, T.a);
@@ -201,27 +211,32 @@ D2_HD bool soup_point_keep(const float* p0, const float* p1, const SoupTri& t, f
 // soup_cull_mask composes the three parts below; the kernel computes the
 // same parts, with the per-edge culls packed over a wavefront.
 
-// The triangles' corners in world coordinates relative to b's start
-// position at both ends (w[0..2] a at t = 0, w[3..5] a at t = 1, w[6..8] b at
-// t = 0, w[9..11] b at t = 1); returns the magnitude of the transforms'
-// inputs (the edge culls' extra).
-D2_HD float soup_world(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1,
-                       const tri<double>& ta, const tri<double>& tb, float (*w)[3]) {
+// The corners of a triangle rounded to float, corner by corner.
+D2_HD void corners_f(const tri<double>& T, float* c9) {
+  const v3<double> v[3] = {T.a, T.b, T.c};
+  for (int k = 0; k < 3; ++k) c9[3 * k] = (float)v[k].x, c9[3 * k + 1] = (float)v[k].y, c9[3 * k + 2] = (float)v[k].z;
+}
+// The triangles' corners (ca, cb: rounded to float, as corners_f gives them)
+// in world coordinates relative to b's start position at both ends (w[0..2]
+// a at t = 0, w[3..5] a at t = 1, w[6..8] b at t = 0, w[9..11] b at t = 1);
+// returns the magnitude of the transforms' inputs (the edge culls' extra).
+D2_HD float soup_world(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1, const float* ca,
+                       const float* cb, float (*w)[3]) {
   const M34f A0 = to_m34f(Ta0, Tb0), A1 = to_m34f(Ta1, Tb0);
   const M34f B0 = to_m34f(Tb0, Tb0), B1 = to_m34f(Tb1, Tb0);
   float extra = fmaxf(fmaxf(m34_mag(A0), m34_mag(A1)), m34_mag(B1));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    xf(A0, corner(ta, k), w[k]), xf(A1, corner(ta, k), w[3 + k]);
-    xf(B0, corner(tb, k), w[6 + k]), xf(B1, corner(tb, k), w[9 + k]);
-    extra = fmaxf(extra, fmaxf(v3_mag(corner(ta, k)), v3_mag(corner(tb, k))));
+    xf_f(A0, ca + 3 * k, w[k]), xf_f(A1, ca + 3 * k, w[3 + k]);
+    xf_f(B0, cb + 3 * k, w[6 + k]), xf_f(B1, cb + 3 * k, w[9 + k]);
+    extra = fmaxf(extra, fmaxf(amax3(ca + 3 * k), amax3(cb + 3 * k)));
   }
   return extra;
 }
-// The point-triangle slots: a's corners in b's frame against tb, b's
-// corners in a's frame against ta.
+// The point-triangle slots: a's corners in b's frame against tb (its planes
+// sb, soup_tri), b's corners in a's frame against ta (sa).
 D2_HD unsigned soup_point_mask(const double* Mab0, const double* Mab1, const double* Mba0, const double* Mba1,
-                               const tri<double>& ta, const tri<double>& tb, float thr) {
+                               const float* ca, const float* cb, const SoupTri& sa, const SoupTri& sb, float thr) {
   unsigned mask = 0u;
   float pa0[3][3], pa1[3][3], pb0[3][3], pb1[3][3];
   const M34f F0 = to_m34f(Mab0, nullptr), F1 = to_m34f(Mab1, nullptr);
@@ -229,13 +244,12 @@ D2_HD unsigned soup_point_mask(const double* Mab0, const double* Mab1, const dou
   float mag = fmaxf(fmaxf(m34_mag(F0), m34_mag(F1)), fmaxf(m34_mag(G0), m34_mag(G1)));
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    xf(F0, corner(ta, k), pa0[k]), xf(F1, corner(ta, k), pa1[k]);
-    xf(G0, corner(tb, k), pb0[k]), xf(G1, corner(tb, k), pb1[k]);
+    xf_f(F0, ca + 3 * k, pa0[k]), xf_f(F1, ca + 3 * k, pa1[k]);
+    xf_f(G0, cb + 3 * k, pb0[k]), xf_f(G1, cb + 3 * k, pb1[k]);
     mag = fmaxf(mag, fmaxf(fmaxf(amax3(pa0[k]), amax3(pa1[k])), fmaxf(amax3(pb0[k]), amax3(pb1[k]))));
-    mag = fmaxf(mag, fmaxf(v3_mag(corner(ta, k)), v3_mag(corner(tb, k))));
+    mag = fmaxf(mag, fmaxf(amax3(ca + 3 * k), amax3(cb + 3 * k)));
   }
   const float margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
-  const SoupTri sa = soup_tri(ta), sb = soup_tri(tb);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     if (soup_point_keep(pa0[k], pa1[k], sb, thr, margin)) mask |= 1u << (5 * k);
@@ -252,9 +266,11 @@ D2_HD bool soup_edge_keep(const float (*w)[3], int k, int l, float thr, float ex
 D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double* Tb0, const double* Tb1,
                               const double* Mab0, const double* Mab1, const double* Mba0, const double* Mba1,
                               const tri<double>& ta, const tri<double>& tb, float thr) {
-  unsigned mask = soup_point_mask(Mab0, Mab1, Mba0, Mba1, ta, tb, thr);
+  float ca[9], cb[9];
+  corners_f(ta, ca), corners_f(tb, cb);
+  unsigned mask = soup_point_mask(Mab0, Mab1, Mba0, Mba1, ca, cb, soup_tri(ta), soup_tri(tb), thr);
   float w[12][3];
-  const float extra = soup_world(Ta0, Ta1, Tb0, Tb1, ta, tb, w);
+  const float extra = soup_world(Ta0, Ta1, Tb0, Tb1, ca, cb, w);
   if (cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra)) return mask;  // all nine edge pairs culled at once
   for (int k = 0; k < 3; ++k)
     for (int l = 0; l < 3; ++l)
@@ -289,15 +305,17 @@ D2_HD unsigned soup_cull_mask(const double* Ta0, const double* Ta1, const double
 struct alignas(16) CullTri {
   float c3[3], kap, L2, crd, ok;
   float v[9], n[3], nn, pmin, pmax;
-  double eps;  // the triangle's eps from its bucket (read with the triangle)
+  float pad[4];  // 112 bytes: 28 dwords, so 8 consecutive records start in 8 distinct 4-bank groups of LDS
+  double eps;    // the triangle's eps from its bucket (read with the triangle)
 };
-D2_HD void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
-  const v3<double> vv[3] = {t.a, t.b, t.c};
+// From the triangle's corners rounded to float (body coordinates, corner
+// by corner: a.x a.y a.z b.x ...), as the float copy of the buckets holds them.
+D2_HD void cull_prep_f(const float* rot, const float* tr, const float* c9, CullTri& o) {
   // the magnitudes the transform's rounding scales with: inputs and results
   float crd = fmaxf(fabsf(tr[0]), fmaxf(fabsf(tr[1]), fabsf(tr[2])));
-  for (int v = 0; v < 3; ++v) crd = fmaxf(crd, v3_mag(vv[v]));
+  for (int k = 0; k < 9; ++k) crd = fmaxf(crd, fabsf(c9[k]));
   for (int v = 0; v < 3; ++v) {
-    const float x = (float)vv[v].x, y = (float)vv[v].y, z = (float)vv[v].z;
+    const float x = c9[3 * v], y = c9[3 * v + 1], z = c9[3 * v + 2];
     for (int r = 0; r < 3; ++r) {
       o.v[3 * v + r] = fmaf(rot[3 * r + 2], z, fmaf(rot[3 * r + 1], y, fmaf(rot[3 * r], x, tr[r])));
       crd = fmaxf(crd, fabsf(o.v[3 * v + r]));
@@ -317,6 +335,11 @@ D2_HD void cull_prep(const float* rot, const float* tr, const tri<double>& t, Cu
   o.pmin = fminf(p0, fminf(p1, p2)), o.pmax = fmaxf(p0, fmaxf(p1, p2));
   for (int r = 0; r < 3; ++r) o.c3[r] = ((o.v[r] + o.v[3 + r]) + o.v[6 + r]) * (1.f / 3.f);
   o.crd = crd;
+}
+D2_HD void cull_prep(const float* rot, const float* tr, const tri<double>& t, CullTri& o) {
+  const float c9[9] = {(float)t.a.x, (float)t.a.y, (float)t.a.z, (float)t.b.x, (float)t.b.y,
+                       (float)t.b.z, (float)t.c.x, (float)t.c.y, (float)t.c.z};
+  cull_prep_f(rot, tr, c9, o);
 }
 D2_HD float cull_margin(const CullTri& A, const CullTri& B, float thr) {
   const float EPSF = 5.9604645e-08f;

@@ -61,6 +61,13 @@ namespace d2 {
 
 // ------------------------------------------------------------------ records
 constexpr int NB_STRIDE = 12;  // per-node pruning bounds (d2_trees_upload)
+// The FP32 form's copy of the buckets: per triangle (node * 8 + k) 12 floats,
+// its corners rounded to float (a.x a.y a.z b.x ... c.z), one unused, and its
+// eps's double bits in the last two (low word first); three 16-byte loads.
+constexpr int FB_STRIDE = 12;
+// and its planes for the CCD point-triangle culls (soup_tri: normal, offset,
+// edge normals and offsets, acceptance bound) in 20 floats, five 16-byte loads.
+constexpr int ST_STRIDE = 20;
 struct NodeRec {
   int32_t id, bucket, inner, depth, parent, child_first, nchild, ntris;
 };
@@ -211,6 +218,8 @@ struct Dev {
   int *pass_flag_s, *pass_flag_c;
   double* min_time;
   const float* tree_eps_out;   // float-barycentric acceptance bound per tree
+  const float* fbuckets;       // FP32 form: per triangle its corners rounded to float and its eps (FB_STRIDE)
+  const float* stris;          // FP32 form: per triangle its soup_tri planes (ST_STRIDE)
   const float* nbound;         // per node: leaf box, slack, sphere (nodes_apart, spheres_apart)
   int prune;                   // skip node pairs nodes_apart proves contact-free
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
@@ -555,6 +564,46 @@ __device__ __forceinline__ void store_res(double* r, double d, v3<double> P, v3<
   r[0] = d, r[1] = toc, r[2] = P.x, r[3] = P.y, r[4] = P.z, r[5] = Q.x, r[6] = Q.y, r[7] = Q.z;
 }
 
+// The FP32 form's float copy of the buckets (FB_STRIDE floats per triangle)
+// and every triangle's soup_tri planes (ST_STRIDE).
+__global__ void __launch_bounds__(256) k_bucket_f32(const double* buckets, long long n_nodes, float* fb, float* st) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < 8 * n_nodes;
+       t += (long long)gridDim.x * blockDim.x) {
+    const long long node = t >> 3;
+    const int k = (int)(t & 7);
+    const double* bk = buckets + (size_t)BUCKET * node;
+    const tri<double> T = load_tri<double>(bk, k);
+    const double eps = bk[72 + k];
+    float4* o = reinterpret_cast<float4*>(fb + (size_t)FB_STRIDE * t);
+    o[0] = make_float4((float)T.a.x, (float)T.a.y, (float)T.a.z, (float)T.b.x);
+    o[1] = make_float4((float)T.b.y, (float)T.b.z, (float)T.c.x, (float)T.c.y);
+    o[2] = make_float4((float)T.c.z, 0.f, __int_as_float(__double2loint(eps)), __int_as_float(__double2hiint(eps)));
+    const SoupTri S = soup_tri(T);
+    float4* q = reinterpret_cast<float4*>(st + (size_t)ST_STRIDE * t);
+    q[0] = make_float4(S.n[0], S.n[1], S.n[2], S.d);
+    q[1] = make_float4(S.m[0], S.m[1], S.m[2], S.m[3]);
+    q[2] = make_float4(S.m[4], S.m[5], S.m[6], S.m[7]);
+    q[3] = make_float4(S.m[8], S.o[0], S.o[1], S.o[2]);
+    q[4] = make_float4(S.eo, 0.f, 0.f, 0.f);
+  }
+}
+// A triangle's float corners and its soup_tri planes from the FP32 copies
+// (global triangle index = node * 8 + k).
+__device__ __forceinline__ void load_corners_f(const float* fb, size_t t, float* c9) {
+  const float4* f = reinterpret_cast<const float4*>(fb + (size_t)FB_STRIDE * t);
+  const float4 q0 = f[0], q1 = f[1], q2 = f[2];
+  c9[0] = q0.x, c9[1] = q0.y, c9[2] = q0.z, c9[3] = q0.w, c9[4] = q1.x, c9[5] = q1.y, c9[6] = q1.z, c9[7] = q1.w;
+  c9[8] = q2.x;
+}
+__device__ __forceinline__ SoupTri load_soup_tri(const float* st, size_t t) {
+  const float4* f = reinterpret_cast<const float4*>(st + (size_t)ST_STRIDE * t);
+  const float4 q0 = f[0], q1 = f[1], q2 = f[2], q3 = f[3], q4 = f[4];
+  SoupTri S;
+  S.n[0] = q0.x, S.n[1] = q0.y, S.n[2] = q0.z, S.d = q0.w;
+  S.m[0] = q1.x, S.m[1] = q1.y, S.m[2] = q1.z, S.m[3] = q1.w, S.m[4] = q2.x, S.m[5] = q2.y, S.m[6] = q2.z;
+  S.m[7] = q2.w, S.m[8] = q3.x, S.o[0] = q3.y, S.o[1] = q3.z, S.o[2] = q3.w, S.eo = q4.x;
+  return S;
+}
 // ------------------------------------------- FP32 culls of CCD soup sub-tests
 // FP32 form: one thread per soup item (triangle pair) runs the FP32 bounds of
 // its 15 sub-tests (k_soup_keep), keeps a 15-bit mask of the survivors per
@@ -579,7 +628,7 @@ constexpr unsigned SOUP_EE_BITS = 0x739Cu;  // slots 5k+2+l (edge k of a, edge l
 // wavefront, nine per item, with the items' world corners staged in LDS),
 // then the two compact sub-test lists (wavefront prefix sums, one atomic per
 // wavefront and list; the lists' order is free, results go to fixed slots).
-__global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 8))) k_soup_keep(Dev D) {
   __shared__ float s_w[4][64][36];        // per wavefront: world corners of the items left for the per-edge culls
   __shared__ float s_thr[4][64];          // their search distances
   __shared__ float s_ex[4][64];           // and their transforms' input magnitudes
@@ -593,24 +642,23 @@ __global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
     const long long it = b0 + threadIdx.x;
     bool need = false;
     const double* pr = nullptr;
-    const double *bka = nullptr, *bkb = nullptr;
-    int ia = 0, ib = 0;
+    size_t ga = 0, gb = 0;  // global triangle indices (node * 8 + k) of the item's two triangles
     if (it < n) {
       const Item item = D.items[it];
       const NP np = D.soup[item.np];
       const int32_t* pi = D.pair_info + 4 * np.pair;
       pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-      bka = D.buckets + (size_t)BUCKET * (pi[2] + D.nodes[pi[2] + np.na].bucket);
-      bkb = D.buckets + (size_t)BUCKET * (pi[3] + D.nodes[pi[3] + np.nb].bucket);
-      ia = (int)(item.sub >> 3), ib = (int)(item.sub & 7);
+      ga = 8 * (size_t)(pi[2] + D.nodes[pi[2] + np.na].bucket) + (item.sub >> 3);
+      gb = 8 * (size_t)(pi[3] + D.nodes[pi[3] + np.nb].bucket) + (item.sub & 7);
     }
     {  // the triangle-level edge test; the corners of the items it leaves to LDS
       float w[12][3];
       float thr = 0.f, extra = 0.f;
       if (it < n) {
+        float ca[9], cb[9];
+        load_corners_f(D.fbuckets, ga, ca), load_corners_f(D.fbuckets, gb, cb);
         thr = (float)pr[PR_SEARCH];
-        extra = soup_world(pr + PR_TA0, pr + PR_TA1, pr + PR_TB0, pr + PR_TB1, load_tri<double>(bka, ia),
-                           load_tri<double>(bkb, ib), w);
+        extra = soup_world(pr + PR_TA0, pr + PR_TA1, pr + PR_TB0, pr + PR_TB1, ca, cb, w);
         need = !cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra);
       }
       const unsigned long long tm = __ballot(need);
@@ -626,10 +674,13 @@ __global__ void __launch_bounds__(256) k_soup_keep(Dev D) {
       }
     }
     s_em[wv][lane] = 0u;
-    unsigned mask = 0u;  // the point slots (the triangles are read again: fewer live registers)
-    if (it < n)
-      mask = soup_point_mask(pr + PR_MAB0, pr + PR_MAB1, pr + PR_MBA0, pr + PR_MBA1, load_tri<double>(bka, ia),
-                             load_tri<double>(bkb, ib), (float)pr[PR_SEARCH]);
+    unsigned mask = 0u;  // the point slots (the corners are read again: fewer live registers)
+    if (it < n) {
+      float ca[9], cb[9];
+      load_corners_f(D.fbuckets, ga, ca), load_corners_f(D.fbuckets, gb, cb);
+      mask = soup_point_mask(pr + PR_MAB0, pr + PR_MAB1, pr + PR_MBA0, pr + PR_MBA1, ca, cb, load_soup_tri(D.stris, ga),
+                             load_soup_tri(D.stris, gb), (float)pr[PR_SEARCH]);
+    }
     wave_sync_lds();
     const int jobs = 9 * __popcll(__ballot(need));
     for (int q0 = 0; q0 < jobs; q0 += 64) {
@@ -959,6 +1010,7 @@ __device__ __forceinline__ unsigned child_mask(const Dev& D, const NP& np, int d
   }
   return m;
 }
+static_assert(sizeof(CullTri) == 112, "prepared triangle: 28 dwords, a conflict-free LDS stride");
 // Five wavefronts per SIMD (at most 96 VGPRs, no spills) instead of the four
 // its natural 101 VGPRs allow: the kernel waits on dependent loads, and more
 // resident wavefronts hide more of them.
@@ -1028,8 +1080,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
             for (int q = 0; q < 3; ++q) rot[3 * r + q] = (float)pm[4 * r + q];
             tr[r] = side ? 0.f : (float)(pr[PR_TA1 + 4 * r + 3] - pr[PR_TB1 + 4 * r + 3]);
           }
-          cull_prep(rot, tr, load_tri<double>(D.buckets + (size_t)BUCKET * gbk, kk), mine);
-          mine.eps = D.buckets[(size_t)BUCKET * gbk + 72 + kk];
+          const float4* fb = reinterpret_cast<const float4*>(D.fbuckets + (size_t)FB_STRIDE * (8 * (size_t)gbk + kk));
+          const float4 q0 = fb[0], q1 = fb[1], q2 = fb[2];
+          const float c9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
+          cull_prep_f(rot, tr, c9, mine);
+          mine.eps = __hiloint2double(__float_as_int(q2.w), __float_as_int(q2.z));
         }
         s_tri[wid][lane >> 4][slot] = mine;
         wave_sync_lds();
@@ -1266,10 +1321,9 @@ __global__ void __launch_bounds__(256) k_tri_full(Dev D) {
 // keeps every list in the reference's breadth-first order (discrete levels
 // are expanded by k_tri_mask, or by k_disc_expand in FP64 mode).
 // ------------------------------------------- FP32 culls of CCD leaf sub-tests
-// The bounds of d2_cull.hpp (plane side and edge half-planes for
-// point-triangle, separating axes for edge-edge), with the leaf triangles'
-// float-barycentric acceptance bound per tree (tree_eps_out) as the lateral
-// margin.
+// The bounds of d2_cull.hpp (soup_point_keep: plane side and edge
+// half-planes with each triangle's FP64-derived normals and its
+// float-barycentric acceptance bound; cull_edge_edge: separating axes).
 // Per connected node pair, shared across its wavefront: lane v < 24 holds
 // vertex v of a's leaf (triangle v/3, corner v%3), lanes 24-47 those of b:
 // the vertex at both ends in world coordinates (relative to b's start
@@ -1279,11 +1333,11 @@ __global__ void __launch_bounds__(256) k_tri_full(Dev D) {
 // shuffles.
 struct LeafShare {
   float w0[3], w1[3], f0[3], f1[3];  // vertex lanes
-  float n[3], d, m[9], o[3];         // triangle lanes
+  SoupTri t;                         // triangle lanes (soup_tri: FP64 normals, float-barycentric bound)
 };
 struct LeafInfo {
   int n, m;
-  float thr, eo_a, eo_b, margin, mag;  // mag: every magnitude of the node pair's FP32 transforms
+  float thr, margin, mag;  // mag: every magnitude of the node pair's FP32 transforms
 };
 __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
   const int lane = lane_id();
@@ -1291,63 +1345,38 @@ __device__ LeafInfo leaf_share(const Dev& D, const NP& np, LeafShare& sh) {
   const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
   const NodeRec na = D.nodes[pi[2] + np.na];
   const NodeRec nb = D.nodes[pi[3] + np.nb];
-  const double* ba = D.buckets + (size_t)BUCKET * (pi[2] + na.bucket);
-  const double* bb = D.buckets + (size_t)BUCKET * (pi[3] + nb.bucket);
+  const size_t ta0 = 8 * (size_t)(pi[2] + na.bucket), tb0 = 8 * (size_t)(pi[3] + nb.bucket);  // global triangle 0
   LeafInfo I;
   I.n = na.ntris, I.m = nb.ntris;
   I.thr = (float)pr[PR_SEARCH];
-  I.eo_a = D.tree_eps_out[D.body_tree[pi[0]]], I.eo_b = D.tree_eps_out[D.body_tree[pi[1]]];
   float mag = 0.f;
-  for (int k = 0; k < 3; ++k) sh.w0[k] = sh.w1[k] = sh.f0[k] = sh.f1[k] = sh.n[k] = sh.o[k] = 0.f;
-  for (int k = 0; k < 9; ++k) sh.m[k] = 0.f;
-  sh.d = 0.f;
+  for (int k = 0; k < 3; ++k) sh.w0[k] = sh.w1[k] = sh.f0[k] = sh.f1[k] = 0.f;
   if (lane < 48) {
     const bool of_b = lane >= 24;
     const int v = of_b ? lane - 24 : lane;
     if (v / 3 < (of_b ? I.m : I.n)) {
-      const v3<double> p = load_corner(of_b ? bb : ba, v / 3, v % 3);
+      const float* pf = D.fbuckets + (size_t)FB_STRIDE * ((of_b ? tb0 : ta0) + v / 3) + 3 * (v % 3);
+      const float p[3] = {pf[0], pf[1], pf[2]};  // the corner rounded to float
       const M34f W0 = to_m34f(of_b ? pr + PR_TB0 : pr + PR_TA0, pr + PR_TB0);
       const M34f W1 = to_m34f(of_b ? pr + PR_TB1 : pr + PR_TA1, pr + PR_TB0);
       const M34f F0 = to_m34f(of_b ? pr + PR_MBA0 : pr + PR_MAB0, nullptr);
       const M34f F1 = to_m34f(of_b ? pr + PR_MBA1 : pr + PR_MAB1, nullptr);
-      xf(W0, p, sh.w0), xf(W1, p, sh.w1), xf(F0, p, sh.f0), xf(F1, p, sh.f1);
+      xf_f(W0, p, sh.w0), xf_f(W1, p, sh.w1), xf_f(F0, p, sh.f0), xf_f(F1, p, sh.f1);
       mag = fmaxf(fmaxf(amax3(sh.w0), amax3(sh.w1)), fmaxf(amax3(sh.f0), amax3(sh.f1)));
-      mag = fmaxf(mag, fmaxf(fmaxf(fmaxf(m34_mag(W0), m34_mag(W1)), fmaxf(m34_mag(F0), m34_mag(F1))), v3_mag(p)));
+      mag = fmaxf(mag, fmaxf(fmaxf(fmaxf(m34_mag(W0), m34_mag(W1)), fmaxf(m34_mag(F0), m34_mag(F1))), amax3(p)));
     }
+    sh.t = SoupTri{};  // (vertex lanes: unused)
   } else {
     const bool of_b = lane >= 56;
     const int k = lane & 7;
     if (k < (of_b ? I.m : I.n)) {
-      const tri<double> T = load_tri<double>(of_b ? bb : ba, k);
-      const float va[3] = {(float)T.a.x, (float)T.a.y, (float)T.a.z}, vb[3] = {(float)T.b.x, (float)T.b.y, (float)T.b.z},
-                  vc[3] = {(float)T.c.x, (float)T.c.y, (float)T.c.z};
-      mag = fmaxf(amax3(va), fmaxf(amax3(vb), amax3(vc)));
-      const float e0[3] = {vb[0] - va[0], vb[1] - va[1], vb[2] - va[2]};
-      const float e1[3] = {vc[0] - va[0], vc[1] - va[1], vc[2] - va[2]};
-      float n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
-      const float nl = sqrtf(f3dot(n, n));
-      if (nl > 1e-12f) {  // a degenerate triangle keeps n = 0: never culled
-        for (int r = 0; r < 3; ++r) sh.n[r] = n[r] / nl;
-        sh.d = f3dot(sh.n, va);
-        const float* V[3] = {va, vb, vc};
-        for (int q = 0; q < 3; ++q) {
-          const float* x = V[q];
-          const float* y = V[(q + 1) % 3];
-          const float* z = V[(q + 2) % 3];
-          const float ed[3] = {y[0] - x[0], y[1] - x[1], y[2] - x[2]};
-          float mm[3] = {ed[1] * sh.n[2] - ed[2] * sh.n[1], ed[2] * sh.n[0] - ed[0] * sh.n[2],
-                         ed[0] * sh.n[1] - ed[1] * sh.n[0]};
-          const float zx[3] = {z[0] - x[0], z[1] - x[1], z[2] - x[2]};
-          if (f3dot(mm, zx) > 0.f) mm[0] = -mm[0], mm[1] = -mm[1], mm[2] = -mm[2];
-          const float ml = sqrtf(f3dot(mm, mm));
-          if (ml > 1e-12f) {
-            for (int r = 0; r < 3; ++r) sh.m[3 * q + r] = mm[r] / ml;
-            sh.o[q] = f3dot(sh.m + 3 * q, x);
-          } else {
-            sh.o[q] = 3.4e38f;  // never outside
-          }
-        }
-      }
+      const size_t tk = (of_b ? tb0 : ta0) + k;
+      float c9[9];
+      load_corners_f(D.fbuckets, tk, c9);
+      mag = fmaxf(amax3(c9), fmaxf(amax3(c9 + 3), amax3(c9 + 6)));
+      sh.t = load_soup_tri(D.stris, tk);
+    } else {
+      sh.t = SoupTri{};  // n = 0: never culled
     }
   }
   for (int off = 32; off > 0; off >>= 1) mag = fmaxf(mag, __shfl_xor(mag, off));
@@ -1363,28 +1392,21 @@ __device__ __forceinline__ void sh3(const float* v, int src, float* o) {
 __device__ bool leaf_point_keep(const LeafShare& sh, const LeafInfo& I, int s, bool active) {
   const int n = I.n, m = I.m;
   int vl, tl;
-  float eo;
   if (s < 3 * n * m) {
     const int v = s / m, j = s % m;
-    vl = v, tl = 56 + j, eo = I.eo_b;  // vertex of a in b's frame, triangle of b
+    vl = v, tl = 56 + j;  // vertex of a in b's frame, triangle of b
   } else {
     const int s2 = s - 3 * n * m, v = s2 / n, i = s2 % n;
-    vl = 24 + v, tl = 48 + i, eo = I.eo_a;
+    vl = 24 + v, tl = 48 + i;
   }
   if (!active) vl = 0, tl = 48;
-  float p0[3], p1[3], nn[3], mm[9], oo[3];
-  sh3(sh.f0, vl, p0), sh3(sh.f1, vl, p1), sh3(sh.n, tl, nn);
-  const float d = __shfl(sh.d, tl);
-  for (int q = 0; q < 3; ++q) sh3(sh.m + 3 * q, tl, mm + 3 * q), oo[q] = __shfl(sh.o[q], tl);
+  float p0[3], p1[3];
+  SoupTri t;
+  sh3(sh.f0, vl, p0), sh3(sh.f1, vl, p1), sh3(sh.t.n, tl, t.n);
+  t.d = __shfl(sh.t.d, tl), t.eo = __shfl(sh.t.eo, tl);
+  for (int q = 0; q < 3; ++q) sh3(sh.t.m + 3 * q, tl, t.m + 3 * q), t.o[q] = __shfl(sh.t.o[q], tl);
   if (!active) return false;
-  if (f3dot(nn, nn) == 0.f) return true;
-  const float h0 = f3dot(nn, p0) - d, h1 = f3dot(nn, p1) - d;
-  const float lim = I.thr + I.margin;
-  if ((h0 > lim && h1 > lim) || (h0 < -lim && h1 < -lim)) return false;
-  const float lat = I.margin + eo;
-  for (int q = 0; q < 3; ++q)
-    if (f3dot(mm + 3 * q, p0) - oo[q] > lat && f3dot(mm + 3 * q, p1) - oo[q] > lat) return false;
-  return true;
+  return soup_point_keep(p0, p1, t, I.thr, I.margin);
 }
 __device__ bool leaf_edge_keep(const LeafShare& sh, const LeafInfo& I, int s, bool active) {
   const int ea = active ? s / (3 * I.m) : 0, eb = active ? s % (3 * I.m) : 0;
@@ -1398,7 +1420,7 @@ __device__ bool leaf_edge_keep(const LeafShare& sh, const LeafInfo& I, int s, bo
 }
 constexpr int CONN_MASK_WORDS = 16;  // 6 words of point sub-tests (<= 384), 9 of edge sub-tests (<= 576)
 
-__global__ void __launch_bounds__(256) k_level_count(Dev D) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_level_count(Dev D) {
   {  // soup: children per destination list
     const long long n = D.n_soup();
     long long rb, re;
@@ -1440,7 +1462,13 @@ __global__ void __launch_bounds__(256) k_level_count(Dev D) {
       for (long long w = rb + wv; w < re; w += (blockDim.x >> 6)) {  // one wavefront per node pair
         const NP np = D.conn[w];
         LeafShare sh;
-        const LeafInfo L = leaf_share(D, np, sh);
+        LeafInfo L;
+        if (D.fp32) {
+          L = leaf_share(D, np, sh);  // reads the FP32 copies (FP32 contexts only)
+        } else {  // the FP64 form keeps every sub-test: only the sizes
+          const int32_t* pi = D.pair_info + 4 * np.pair;
+          L.n = D.nodes[pi[2] + np.na].ntris, L.m = D.nodes[pi[3] + np.nb].ntris;
+        }
         const int npt = 6 * L.n * L.m, nee = 9 * L.n * L.m;
         int kp = 0, ke = 0;
         unsigned long long* mk = D.conn_mask + (size_t)CONN_MASK_WORDS * w;
@@ -2917,6 +2945,10 @@ struct d2_ctx {
   // trees / bodies
   DBuf<NodeRec> nodes;
   DBuf<double> buckets;
+  DBuf<float> fbuckets;  // FP32 form: float corners + eps per triangle (FB_STRIDE), built on first use
+  DBuf<float> stris;     // FP32 form: soup_tri planes per triangle (ST_STRIDE), built with fbuckets
+  bool fb_ready = false;
+  long long n_nodes_total = 0;  // nodes of the uploaded trees
   DBuf<int32_t> tree_off, body_tree;
   DBuf<double> bodies;
   int n_trees = 0, n_bodies = 0;
@@ -3194,6 +3226,8 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.pass_flag_s = c->pass_flag_s.p, D.pass_flag_c = c->pass_flag_c.p;
   D.min_time = c->min_time.p;
   D.tree_eps_out = c->tree_eps_out.p;
+  D.fbuckets = c->fbuckets.p;
+  D.stris = c->stris.p;
   D.nbound = c->nbound.p;
   D.prune = c->prune_on && !c->touch_on && c->nbound.p ? 1 : 0;  // accounting runs walk the reference's node pairs
   D.conn_mask = c->conn_mask.p;
@@ -3233,6 +3267,15 @@ int prepare(d2_ctx* c) {
   RC(ensure(c, c->lvl, (size_t)2 * (MAX_LEVELS + 1) * LV_STRIDE));
   RC(ensure(c, c->res, (size_t)c->cap_res * RES_SLOTS * 8));
   if (c->precision == D2_FP32) RC(ensure(c, c->res_mask, (size_t)c->cap_res));
+  if (c->precision == D2_FP32 && !c->fb_ready) {  // the float copy of the buckets, once per tree upload
+    const size_t nodes = (size_t)c->n_nodes_total;
+    RC(ensure(c, c->fbuckets, nodes * 8 * FB_STRIDE));
+    RC(ensure(c, c->stris, nodes * 8 * ST_STRIDE));
+    hipLaunchKernelGGL(k_bucket_f32, dim3(grid_for(c, (long long)nodes * 8)), dim3(256), 0, c->stream, c->buckets.p,
+                       (long long)nodes, c->fbuckets.p, c->stris.p);
+    HIPCHK(c, hipGetLastError());
+    c->fb_ready = true;
+  }
   RC(ensure(c, c->pass_min_s, np)); RC(ensure(c, c->pass_min_c, np)); RC(ensure(c, c->pass_flag_s, np));
   RC(ensure(c, c->pass_flag_c, np)); RC(ensure(c, c->min_time, np)); RC(ensure(c, c->scale, np));
   RC(ensure(c, c->pair_cnt, np + 1)); RC(ensure(c, c->pair_off, np + 1)); RC(ensure(c, c->pair_fill, np));
@@ -4005,7 +4048,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   collect_timers(c);
   drop_graphs(c);
-  release(c->nodes), release(c->buckets), release(c->tree_off), release(c->body_tree), release(c->bodies);
+  release(c->nodes), release(c->buckets), release(c->fbuckets), release(c->stris), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
   release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->res_mask), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
@@ -4169,6 +4212,7 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   }
   (void)hipSetDevice(c->device);
   RC(ensure(c, c->nodes, total)); RC(ensure(c, c->buckets, (size_t)total * BUCKET)); RC(ensure(c, c->tree_off, n_trees + 1));
+  c->n_nodes_total = total, c->fb_ready = false;  // the float copy follows on the next FP32 run
   HIPCHK(c, hipMemcpyAsync(c->nodes.p, nodes, sizeof(NodeRec) * total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->buckets.p, buckets, sizeof(double) * BUCKET * (size_t)total, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->tree_off.p, tree_node_offset, sizeof(int32_t) * (n_trees + 1), hipMemcpyHostToDevice, c->stream));

@@ -270,13 +270,19 @@ int main() {
       d2o::Tri<double> oa = rand_tri(sa, U(-0.3, 0.3) + (far == 1 ? off : 0.0));
       d2o::Tri<double> ob = rand_tri(sb, U(-0.3, 0.3) + (far == 2 ? off : 0.0));
       if (k % 8 == 0) ob.C = ob.A + (ob.B - ob.A) * U(0.0, 1.0);  // degenerate (collinear) triangles too
+      if (k % 8 == 4) {  // and thin ones (kappa up to ~1e4), whose float barycentrics stray the most
+        const d2o::V3<double> e = ob.B - ob.A;
+        const d2o::V3<double> q = d2o::cross(e, d2o::V3<double>(U(-1, 1), U(-1, 1), U(-1, 1)));
+        ob.C = ob.A + e * U(0.1, 0.9) + q * (std::pow(10.0, U(-2.5, -1.0)) / std::max(1e-30, d2o::norm(q)) * d2o::norm(e));
+      }
       const d2::tri<double> ta = cvt(oa), tb = cvt(ob);
       const float thr = (float)U(0.002, 0.05);
       const double thr_d = (double)thr;
       const unsigned mask = d2::soup_cull_mask(Ta0, Ta1, Tb0, Tb1, Mab0, Mab1, Mba0, Mba1, ta, tb, thr);
       // the triangle-level edge cull (cull_tri_edges) on the same float world points
-      float w[12][3];
-      const float extra = d2::soup_world(Ta0, Ta1, Tb0, Tb1, ta, tb, w);
+      float w[12][3], ca9[9], cb9[9];
+      d2::corners_f(ta, ca9), d2::corners_f(tb, cb9);
+      const float extra = d2::soup_world(Ta0, Ta1, Tb0, Tb1, ca9, cb9, w);
       const bool tcut = d2::cull_tri_edges(w, w + 3, w + 6, w + 9, thr, extra);
       tri_cut += tcut;
       for (int s = 0; s < 15; ++s) {
@@ -351,6 +357,44 @@ int main() {
       viol += (cls == 0 && hit) || (cls == 2 && !hit);
     }
     printf("disc_culls %lld %lld (miss %lld, hit %lld, fp64 hits %lld)\n", cases, viol, miss, two, hits);
+  }
+  // The point-triangle cull's lateral bound at the edges of thin triangles:
+  // a point just outside an edge, near the plane, standing still in the
+  // triangle's frame; the reference's float barycentrics may accept it as
+  // inside (then it reports the plane distance), so a cull there must hold
+  // the float acceptance bound.
+  {
+    long long cases = 0, culled = 0, hits = 0, viol = 0;
+    double I4[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    for (int k = 0; k < N; ++k) {
+      d2o::Tri<double> t = rand_tri(U(0.05, 1.0), U(-2.0, 2.0));
+      if (k % 2) {  // thin: kappa up to ~1e6
+        const d2o::V3<double> e = t.B - t.A;
+        const d2o::V3<double> q = d2o::cross(e, d2o::V3<double>(U(-1, 1), U(-1, 1), U(-1, 1)));
+        t.C = t.A + e * U(0.1, 0.9) + q * (std::pow(10.0, U(-3.0, -1.0)) / std::max(1e-30, d2o::norm(q)) * d2o::norm(e));
+      }
+      const d2o::V3<double> n = t.normal();
+      const int q = (int)(rng() % 3);
+      const d2o::V3<double> x = t[q], y = t[(q + 1) % 3], z = t[(q + 2) % 3];
+      d2o::V3<double> out = d2o::cross(y - x, n);
+      if (d2o::dot(out, z - x) > 0) out = out * -1.0;
+      out = out / d2o::norm(out);
+      const double L = std::max(d2o::norm(y - x), std::max(d2o::norm(z - x), d2o::norm(z - y)));
+      const float thr = (float)U(0.002, 0.05);
+      const d2o::V3<double> P = x + (y - x) * U(0.05, 0.95) + out * (L * std::pow(10.0, U(-7.0, -2.0))) + n * (U(-0.5, 0.5) * thr);
+      const d2::tri<double> T = cvt(t);
+      d2::v3<double> PP, QQ;
+      double toc = 1.0;
+      const double d = d2::point_tri(d2::v3<double>{P.x, P.y, P.z}, I4, I4, T, I4, I4, PP, QQ, toc);
+      const float p0[3] = {(float)P.x, (float)P.y, (float)P.z};
+      float mag = d2::amax3(p0);
+      for (int v = 0; v < 3; ++v) mag = std::max(mag, d2::v3_mag(d2::corner(T, v)));
+      const float margin = 1e-6f + 64.f * 5.9604645e-08f * mag;
+      const bool keep = d2::soup_point_keep(p0, p0, d2::soup_tri(T), thr, margin);
+      const bool hit = d < (double)thr;
+      ++cases, culled += !keep, hits += hit, viol += hit && !keep;
+    }
+    printf("edge_points %lld %lld (culled %lld, hits %lld)\n", cases, viol, culled, hits);
   }
   return 0;
 }

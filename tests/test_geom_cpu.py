@@ -38,6 +38,11 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
     # the triangle-level edge cull (nine edge-edge sub-tests at once) is sound
     assert int(rows["tri_edges"][2]) == 0, out
     assert int(re.search(r"tri_edges .*culled (\d+)", out).group(1)) > 1000, out
+    # the point-triangle cull's lateral bound holds the reference's float
+    # barycentrics: points just outside an edge of thin triangles
+    assert int(rows["edge_points"][2]) == 0, out
+    culled, hits = (int(x) for x in re.search(r"edge_points .*culled (\d+), hits (\d+)", out).groups())
+    assert culled > 1000 and hits > 100, out
     # FP32 classification of discrete triangle pairs near the hit threshold:
     # no certain miss hits, every certain hit hits; each outcome occurs often
     assert int(rows["disc_culls"][2]) == 0, out
