@@ -482,6 +482,31 @@ def timed_steps(ctx, dt, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def evolving_steps(ctx, scene, steps=20):
+    """A stepping sequence whose inputs change every step: the states advance
+    ballistically, then the device broad phase, clusters and the step run on
+    them (the pair list and clusters differ from step to step).  Returns the
+    step time and how many graph segments were captured on the way."""
+    from delta2_amd import scenes
+    rec = scene.records
+    c0, l0 = ctx.graph_stats()
+    secs, pairs = [], []
+    for _ in range(steps):
+        rec = scenes.advance_gravity(rec)
+        ctx.upload_bodies(scene.tree_of_body, rec)
+        ctx.project_future(np.minimum(2.0 * rec[:, 39], scenes.DT))
+        pairs.append(ctx.broad_phase(mode=1))
+        ctx.build_clusters()
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step_run(scenes.DT, True)
+        ctx.sync()
+        secs.append(time.perf_counter() - t0)
+    c1, l1 = ctx.graph_stats()
+    return {"steps": steps, "ms_per_step": 1e3 * float(np.mean(secs)), "captures": c1 - c0, "graph_launches": l1 - l0,
+            "pairs": [int(min(pairs)), int(max(pairs))]}
+
+
 def run_small_figures(workload, args, weights, steps=None, tune=True):
     """One scene's step replayed: the plain FP64 path and this form (graph
     replay, autotuned shapes), their results compared."""
@@ -502,13 +527,20 @@ def run_small_figures(workload, args, weights, steps=None, tune=True):
     sec = timed_steps(ctx, scenes.DT, steps, 2)
     identical = same_results(step_results(ctx), plain_res)
     gate = ctx.gate_counts() if weights is not None else None
+    # the same form on a sequence whose pair list changes every step, with
+    # graph replay and eagerly: how often it recaptures, and what that costs
+    evolving = {"graph": evolving_steps(ctx, scene)}
+    ctx.set_graph(False)
+    evolving["eager"] = evolving_steps(ctx, scene)
     ctx.close()
+    log(f"[{workload}] evolving 20 steps: graph {evolving['graph']['ms_per_step']:.3f} ms/step with "
+        f"{evolving['graph']['captures']} captures, eager {evolving['eager']['ms_per_step']:.3f} ms/step")
     n = int(scene.pairs.shape[0])
     log(f"[{workload}] plain fp64 {1e3 * plain_sec:.3f} ms/step, this form {1e3 * sec:.3f} ms/step, identical {identical}")
     return {"value": n / sec, "ms_per_step": 1e3 * sec, "steps": steps, "pairs": n,
             "plain_fp64": {"value": n / plain_sec, "ms_per_step": 1e3 * plain_sec},
             "speedup_vs_plain_fp64": plain_sec / sec, "identical_to_plain_fp64": identical, "gate": gate,
-            "graph": args.graph == "on",
+            "graph": args.graph == "on", "evolving": evolving,
             "workload": "4096 bodies: 2048 subdiv-3 icospheres U(0.5,1.5) + 2048 hulls of 64 ellipsoid points; "
                         "Poisson-disk 30% packing; v~N(0,2); dt=1/60; one step replayed"
             if workload == "c2" else "100 subdiv-2 icospheres U(0.5,1.5) in a 10 m box; v~N(0,1); dt=1/60"}

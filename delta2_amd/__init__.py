@@ -111,6 +111,7 @@ def lib():
         "d2_set_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_get_tune": ([_vp, ctypes.POINTER(Tune)], ctypes.c_int),
         "d2_set_graph": ([_vp, ctypes.c_int], ctypes.c_int),
+        "d2_graph_stats": ([_vp, _i64p, _i64p], ctypes.c_int),
         "d2_autotune": ([_vp, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tune), _dp],
                         ctypes.c_int),
         "d2_clusters_upload": ([_vp, _ip, ctypes.c_int], ctypes.c_int),
@@ -359,6 +360,12 @@ class Context:
 
     def set_graph(self, on=True):
         self._check(lib().d2_set_graph(self.h, int(bool(on))))
+
+    def graph_stats(self):
+        """(graph segments captured, graph launches) since the context was created."""
+        c, n = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(lib().d2_graph_stats(self.h, ctypes.byref(c), ctypes.byref(n)))
+        return c.value, n.value
 
     def set_tune(self, chunk_blocks, stream_blocks, post_blocks, gate_phases=3):
         t = Tune(int(chunk_blocks), int(stream_blocks), int(post_blocks), int(gate_phases))

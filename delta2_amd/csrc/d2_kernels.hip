@@ -2949,6 +2949,7 @@ struct d2_ctx {
   DBuf<float> stris;     // FP32 form: soup_tri planes per triangle (ST_STRIDE), built with fbuckets
   bool fb_ready = false;
   long long n_nodes_total = 0;  // nodes of the uploaded trees
+  long long graph_captures = 0, graph_launches = 0;  // graph segments captured / launched (d2_graph_stats)
   DBuf<int32_t> tree_off, body_tree;
   DBuf<double> bodies;
   int n_trees = 0, n_bodies = 0;
@@ -3705,8 +3706,10 @@ int launch_segment(d2_ctx* c, int kind, int seg, const Sig& sig, const std::vect
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(c, D2_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
     g.valid = true, g.sig = sig;
+    ++c->graph_captures;
   }
   HIPCHK(c, hipGraphLaunch(g.exec, c->stream));
+  ++c->graph_launches;
   return D2_OK;
 }
 
@@ -5209,6 +5212,12 @@ int d2_run_info(const d2_ctx* c, int64_t* info, int n) {
   for (int i = 0; i < n && i < total; ++i)
     info[i] = (int64_t)(i < C_COUNT ? c->host_cnt[i] : c->host_lvl[i - C_COUNT]);
   return total;
+}
+int d2_graph_stats(const d2_ctx* c, int64_t* captures, int64_t* launches) {
+  if (!c || !captures || !launches) return D2_ERR_ARG;
+  *captures = c->graph_captures;
+  *launches = c->graph_launches;
+  return D2_OK;
 }
 int d2_sync(d2_ctx* c) {
   if (!c) return D2_ERR_ARG;

@@ -114,6 +114,11 @@ int d2_autotune(d2_ctx* ctx, int kind, float dt, int filter, int reps, d2_tune* 
  * change sizes or launch parameters recapture.  Results are identical to
  * eager runs. */
 int d2_set_graph(d2_ctx* ctx, int on);
+/* Graph segments captured (and instantiated) and graph launches since the
+ * context was created (no reference counterpart): a sequence whose pair
+ * list, clusters or level grids change between steps recaptures; this tells
+ * how often. */
+int d2_graph_stats(const d2_ctx* ctx, int64_t* captures, int64_t* launches);
 
 /* ---- surrogate trees ---------------------------------------------------- */
 /* Builds one surrogate tree on the host from a triangle mesh, as
