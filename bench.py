@@ -582,7 +582,7 @@ def run_small(args):
            "form": {"precision": args.precision, "learned_gate": weights is not None, "graph": args.graph == "on",
                     "weights": wsrc},
            "plain_fp64": fig["plain_fp64"], "speedup_vs_plain_fp64": fig["speedup_vs_plain_fp64"],
-           "identical_to_plain_fp64": fig["identical_to_plain_fp64"], "gate": fig["gate"],
+           "identical_to_plain_fp64": fig["identical_to_plain_fp64"], "gate": fig["gate"], "evolving": fig["evolving"],
            "config": {"workload": fig["workload"], "bodies": scene.n_bodies, "triangles": scene.n_triangles(),
                       "pairs": fig["pairs"]},
            "roofline": roofline, "cpu_baseline": cpu}
