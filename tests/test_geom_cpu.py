@@ -27,8 +27,9 @@ def test_device_geometry_matches_oracle_bitwise(tmp_path):
     # subtree pruning: no violation, and the random placements do prune often
     assert int(rows["nodes_apart"][2]) == 0, out
     line = next(l for l in out.splitlines() if l.startswith("nodes_apart"))
-    prunes = [int(x) for x in re.findall(r"(?:boxes|spheres|f32) (\d+)", line)]  # boxes, spheres, spheres f32
-    assert len(prunes) == 3 and min(prunes) > 1000, out
+    prunes = [int(x) for x in re.findall(r"(?:boxes|spheres|f32|axes) (\d+)", line)]  # boxes, spheres, f32, face axes
+    assert len(prunes) == 4 and min(prunes) > 1000, out
+    assert prunes[3] <= prunes[0], out  # six of nodes_apart's fifteen axes
     # FP32 culls of the CCD soup sub-tests: no culled sub-test reports a hit,
     # on draws with thousands of hits and most sub-tests culled
     assert int(rows["soup_culls"][2]) == 0, out
