@@ -112,8 +112,11 @@ enum Counter : int {
 // atomic and unpacked into LV_ITEMS and the next level's LV_SOUP by k_tri_dist.
 // LV_SPT / LV_SEE (CCD levels, FP32 form): the soup sub-tests that survive
 // the FP32 culls, point and edge (slot 5 is LV_PACK on discrete levels).
+// LV_DEF (discrete levels, FP32 form): the deferred items of inner node pairs
+// (k_tri_def), allocated by k_tri_mask (slot 6 is LV_SEE on CCD levels).
 enum LevelSlot : int {
-  LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_PACK = 5, LV_SPT = 5, LV_SEE = 6, LV_STRIDE = 8
+  LV_SOUP = 0, LV_CONN = 1, LV_ITEMS = 2, LV_PT = 3, LV_EE = 4, LV_PACK = 5, LV_SPT = 5, LV_SEE = 6, LV_DEF = 6,
+  LV_STRIDE = 8
 };
 constexpr int MAX_LEVELS = 16;  // contact keys hold 16 base-8 path digits
 // Key layout: level(5) | list(1) | path(48) | sub(10)
@@ -197,6 +200,7 @@ struct Dev {
   NP* conn_next;
   Item* items;
   DItem* ditems;
+  DItem* ditems_def;           // FP32 form: deferred items of inner node pairs (k_tri_def)
   Item* pt_items;
   Item* ee_items;
   double* res;  // [items][15][8]
@@ -222,6 +226,7 @@ struct Dev {
   const float* stris;          // FP32 form: per triangle its soup_tri planes (ST_STRIDE)
   const float* nbound;         // per node: leaf box, slack, sphere (nodes_apart, spheres_apart)
   int prune;                   // skip node pairs nodes_apart proves contact-free
+  int defer;                   // FP32 form: inner node pairs send one item to k_tri_dist, the rest to k_tri_def
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
   const int32_t* body_tree;
   int fp32;                    // FP32 culls of the CCD leaf sub-tests
@@ -1015,9 +1020,9 @@ static_assert(sizeof(CullTri) == 112, "prepared triangle: 28 dwords, a conflict-
 // its natural 101 VGPRs allow: the kernel waits on dependent loads, and more
 // resident wavefronts hide more of them.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) k_tri_mask(Dev D) {
-  __shared__ int s_tot[2][4], s_ch[2][4];
+  __shared__ int s_tot[2][4], s_ch[2][4], s_df[2][4];
   __shared__ CullTri s_tri[4][4][16];  // per wavefront: 4 node pairs' 8 + 8 prepared triangles
-  __shared__ unsigned long long s_base[2], s_chb[2];
+  __shared__ unsigned long long s_base[2], s_chb[2], s_dfb[2];
   const int lane = lane_id(), nw = blockDim.x >> 6, wid = threadIdx.x >> 6;
   const long long n = D.n_soup();
   long long rb, re;
@@ -1120,19 +1125,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
     // The kept triangle pairs go straight to the level's item list, and the
     // children of node pairs that hit (settled) or may hit (kept pairs) to
     // the next list: one slot range each per block and round, one atomic
-    // each (order does not matter: contacts carry their emission keys).
-    const int mine = lane < batch ? __popcll(my_keep) : 0;
+    // each (order does not matter: contacts carry their emission keys).  An
+    // inner node pair's items only decide whether it expands (any FP64 hit):
+    // its first kept item goes to the item list, the others (deferred) to
+    // k_tri_def, which tests them only while the node pair has no hit.
+    const int kept = lane < batch ? __popcll(my_keep) : 0;
+    const bool defer = D.defer && l_inner != 0;
+    const int mine = defer && kept > 1 ? 1 : kept, mdef = kept - mine;
     const unsigned cmask =
         lane < batch && (my_settled || my_keep != 0ull) ? child_mask<true>(D, l_np, l_desc, l_offa, l_offb) : 0u;
     const int nch = __popc(cmask);
-    const int incl = wave_incl_scan(mine), ich = wave_incl_scan(nch);
+    const int incl = wave_incl_scan(mine), ich = wave_incl_scan(nch), idf = wave_incl_scan(mdef);
     const int par = it & 1;
-    if (lane == 63) s_tot[par][wid] = incl, s_ch[par][wid] = ich;
+    if (lane == 63) s_tot[par][wid] = incl, s_ch[par][wid] = ich, s_df[par][wid] = idf;
     __syncthreads();
     if (threadIdx.x == 0) {  // one atomic for both ranges (same-address atomics from every block are costly)
-      int sum = 0, sch = 0;
-      for (int v = 0; v < nw; ++v) sum += s_tot[par][v], sch += s_ch[par][v];
-      unsigned long long base = 0, chb = 0;
+      int sum = 0, sch = 0, sdf = 0;
+      for (int v = 0; v < nw; ++v) sum += s_tot[par][v], sch += s_ch[par][v], sdf += s_df[par][v];
+      unsigned long long base = 0, chb = 0, dfb = 0;
       if (sum || sch) {
         const unsigned long long old =
             atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_PACK], ((unsigned long long)sch << 32) | (unsigned)sum);
@@ -1140,7 +1150,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
         if ((long long)(base + sum) > D.cap_items || (long long)(chb + sch) > D.cap_np)
           atomicOr(&D.cnt[C_OVERFLOW], 2ull);
       }
-      s_base[par] = base, s_chb[par] = chb;
+      if (sdf) {  // deferred items: their own range (another address)
+        dfb = atomicAdd(&D.lvl[D.level * LV_STRIDE + LV_DEF], (unsigned long long)sdf);
+        if ((long long)(dfb + sdf) > D.cap_items) atomicOr(&D.cnt[C_OVERFLOW], 2ull);
+      }
+      s_base[par] = base, s_chb[par] = chb, s_dfb[par] = dfb;
     }
     __syncthreads();
     if (nch > 0) {
@@ -1148,22 +1162,55 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
       for (int v = 0; v < wid; ++v) slot += s_ch[par][v];
       write_children(D, slot, l_np, l_desc, my_settled ? 0u : (uint32_t)(wl + 1), cmask);
     }
-    long long base = (long long)s_base[par];
-    for (int v = 0; v < wid; ++v) base += s_tot[par][v];
-    const int excl = incl - mine;
+    long long base = (long long)s_base[par], dbase = (long long)s_dfb[par];
+    for (int v = 0; v < wid; ++v) base += s_tot[par][v], dbase += s_df[par][v];
+    const int excl = incl - mine, dexcl = idf - mdef;
+    const int defer_i = defer ? 1 : 0;
     for (int k = 0; k < batch; ++k) {
       const unsigned long long m = readlane_u64(my_keep, k);
       if (m == 0ull) continue;
-      const long long slot = base + rl(excl, k) + __popcll(m & lanes_below());
-      if (((m >> lane) & 1ull) && slot < D.cap_items)
-        D.ditems[slot] = DItem{(uint32_t)(w0 + (long long)nw * k), (uint32_t)rl(l_pair, k),
-                               (uint32_t)(rl(l_offa, k) + rl(l_bka, k)), (uint32_t)(rl(l_offb, k) + rl(l_bkb, k)),
-                               (uint32_t)lane | (rl(l_inner, k) ? 256u : 0u)};
+      // deferring node pair: its lowest kept lane to the item list, the rest
+      // to the deferred list
+      const unsigned long long first = rl(defer_i, k) ? (m & (~m + 1ull)) : m;
+      const unsigned long long rest = m & ~first;
+      const DItem item{(uint32_t)(w0 + (long long)nw * k), (uint32_t)rl(l_pair, k),
+                       (uint32_t)(rl(l_offa, k) + rl(l_bka, k)), (uint32_t)(rl(l_offb, k) + rl(l_bkb, k)),
+                       (uint32_t)lane | (rl(l_inner, k) ? 256u : 0u)};
+      if ((first >> lane) & 1ull) {
+        const long long slot = base + rl(excl, k) + __popcll(first & lanes_below());
+        if (slot < D.cap_items) D.ditems[slot] = item;
+      } else if ((rest >> lane) & 1ull) {
+        const long long slot = dbase + rl(dexcl, k) + __popcll(rest & lanes_below());
+        if (slot < D.cap_items) D.ditems_def[slot] = item;
+      }
     }
   }
 }
 
 
+// The FP64 test of one item (comparison_dist.h:205-240's leaf test): the
+// reference's triangle distance at the phase's poses against search + eps_a
+// + eps_b.
+struct ItemTest {
+  v3<double> P, Q;
+  double ea, eb;
+  int i, j;
+  bool hit;
+};
+__device__ __forceinline__ ItemTest item_test(const Dev& D, const DItem& item) {
+  ItemTest r;
+  const double* pr = D.pair_rec + (size_t)PR_STRIDE * item.pair;
+  const double* ba = D.buckets + (size_t)BUCKET * item.ga;
+  const double* bb = D.buckets + (size_t)BUCKET * item.gb;
+  const int sub = (int)(item.sub & 63u);
+  r.i = sub >> 3, r.j = sub & 7;
+  const tri<double> A = xform(pr + PR_TA1, load_tri<double>(ba, r.i));
+  const tri<double> B = xform(pr + PR_TB1, load_tri<double>(bb, r.j));
+  const double d = tri_tri_dist(A, B, r.P, r.Q);
+  r.ea = ba[72 + r.i], r.eb = bb[72 + r.j];
+  r.hit = d < (pr[PR_SEARCH] + r.ea) + r.eb;
+  return r;
+}
 // FP64 triangle-pair distances of the level's items (grid-stride).
 __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   const unsigned long long pack = D.lv(D.level, LV_PACK);
@@ -1176,17 +1223,8 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
   }
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < count; t += (long long)gridDim.x * blockDim.x) {
     const DItem item = D.ditems[t];
-    const double* pr = D.pair_rec + (size_t)PR_STRIDE * item.pair;
-    const double* ba = D.buckets + (size_t)BUCKET * item.ga;
-    const double* bb = D.buckets + (size_t)BUCKET * item.gb;
-    const int sub = (int)(item.sub & 63u);
-    int i = sub >> 3, j = sub & 7;
-    tri<double> A = xform(pr + PR_TA1, load_tri<double>(ba, i));
-    tri<double> B = xform(pr + PR_TB1, load_tri<double>(bb, j));
-    v3<double> P, Q;
-    double d = tri_tri_dist(A, B, P, Q);
-    double ea = ba[72 + i], eb = bb[72 + j];
-    if (!(d < (pr[PR_SEARCH] + ea) + eb)) continue;
+    const ItemTest r = item_test(D, item);
+    if (!r.hit) continue;
     D.np_hit[item.np] = 1;
     if (item.sub & 256u) continue;  // an inner node: the hit only expands the node pair
     unsigned long long slot = atomicAdd(&D.cnt[C_CONTACTS], 1ull);
@@ -1197,12 +1235,57 @@ __global__ void __launch_bounds__(256) k_tri_dist(Dev D) {
     atomicAdd(&D.pair_cnt[item.pair], 1);
     const NP np = D.soup[item.np];
     ContactRec c;
-    c.key = make_key(D.level, 0, np.key, sub);
+    const double* pr = D.pair_rec + (size_t)PR_STRIDE * item.pair;
+    c.key = make_key(D.level, 0, np.key, (int)(item.sub & 63u));
     c.pair = (int32_t)np.pair, c.na = (int32_t)np.na, c.nb = (int32_t)np.nb, c.kind = 0;
-    c.ia = i, c.ib = j, c.level = D.level, c.pad = 0;
-    c.r[0] = P.x, c.r[1] = P.y, c.r[2] = P.z, c.r[3] = Q.x, c.r[4] = Q.y, c.r[5] = Q.z;
-    c.r[6] = pr[PR_GEOA] + ea, c.r[7] = pr[PR_GEOB] + eb, c.r[8] = ba[80 + i], c.r[9] = bb[80 + j], c.r[10] = 0.0;
+    c.ia = r.i, c.ib = r.j, c.level = D.level, c.pad = 0;
+    c.r[0] = r.P.x, c.r[1] = r.P.y, c.r[2] = r.P.z, c.r[3] = r.Q.x, c.r[4] = r.Q.y, c.r[5] = r.Q.z;
+    c.r[6] = pr[PR_GEOA] + r.ea, c.r[7] = pr[PR_GEOB] + r.eb;
+    c.r[8] = D.buckets[(size_t)BUCKET * item.ga + 80 + r.i], c.r[9] = D.buckets[(size_t)BUCKET * item.gb + 80 + r.j];
+    c.r[10] = 0.0;
     D.contacts[slot] = c;
+  }
+}
+
+// The deferred items of inner node pairs (FP32 form, after k_tri_dist): an
+// inner node pair's FP64 tests only decide whether it expands, so a deferred
+// item is tested only while its node pair has no hit.  The items still
+// needed are queued per wavefront in LDS and tested 64 at a time, so the
+// FP64 test runs on full wavefronts; a hit sets the node pair's flag (inner
+// node pairs emit no contacts).  Every wavefront walks its 64-item chunks to
+// the end of the list, then drains its queue.
+__global__ void __launch_bounds__(256) k_tri_def(Dev D) {
+  __shared__ uint32_t s_q[4][128];
+  const int lane = lane_id(), wid = threadIdx.x >> 6;
+  const long long n_def = D.lv(D.level, LV_DEF);
+  const long long n = n_def < D.cap_items ? n_def : D.cap_items;
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&D.cnt[C_LVL_ITEMS], (unsigned long long)n_def);
+  const long long waves = (long long)gridDim.x * (blockDim.x >> 6);
+  int qn = 0;  // queue length (wavefront-uniform), below 64 between chunks
+  for (long long c0 = ((long long)blockIdx.x * (blockDim.x >> 6) + wid) * 64;; c0 += waves * 64) {
+    const bool more = c0 < n;
+    if (more) {
+      const long long t = c0 + lane;
+      const bool need = t < n && D.np_hit[D.ditems_def[t].np] == 0;
+      const unsigned long long b = __ballot(need);
+      if (need) s_q[wid][qn + __popcll(b & lanes_below())] = (uint32_t)t;
+      qn += __popcll(b);
+      wave_sync_lds();
+    }
+    while (qn >= 64 || (!more && qn > 0)) {
+      const int take = qn < 64 ? qn : 64;
+      if (lane < take) {
+        const DItem item = D.ditems_def[s_q[wid][lane]];
+        if (D.np_hit[item.np] == 0 && item_test(D, item).hit) D.np_hit[item.np] = 1;
+      }
+      const int rest = qn - take;  // below 64: move it to the queue's front
+      const uint32_t v = lane < rest ? s_q[wid][take + lane] : 0u;
+      wave_sync_lds();
+      if (lane < rest) s_q[wid][lane] = v;
+      wave_sync_lds();
+      qn = rest;
+    }
+    if (!more) break;
   }
 }
 
@@ -2892,12 +2975,13 @@ namespace {
 
 const char* kKernelNames[] = {"setup", "soup_cull", "soup_point", "soup_edge", "soup_reduce", "expand",
                               "leaf_point", "leaf_edge", "tri_mask", "tri_enum", "tri_dist", "post", "post_pairs",
-                              "cluster", "gate", "tri_full", "soup_keep"};
+                              "cluster", "gate", "tri_full", "soup_keep", "tri_def"};
 // K_TRI_ENUM launches nothing (the enumeration is part of k_tri_mask); its
 // item count is the discrete node pairs received, kept for instrumentation.
 // FP64 contexts run K_TRI_FULL (k_tri_full) in place of K_TRI_MASK + K_TRI_DIST.
 enum KId { K_SETUP, K_SOUP_CULL, K_SOUP_POINT, K_SOUP_EDGE, K_SOUP_REDUCE, K_EXPAND, K_LEAF_POINT, K_LEAF_EDGE,
-           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_TRI_FULL, K_SOUP_KEEP, K_NUM };
+           K_TRI_MASK, K_TRI_ENUM, K_TRI_DIST, K_POST, K_POST_PAIRS, K_CLUSTER, K_GATE, K_TRI_FULL, K_SOUP_KEEP, K_TRI_DEF,
+           K_NUM };
 
 template <class T> struct DBuf {
   T* p = nullptr;
@@ -2972,6 +3056,7 @@ struct d2_ctx {
   DBuf<NP> soup_list[2], conn_list[2];  // level l reads [l & 1] and writes [(l + 1) & 1]
   DBuf<Item> items, pt_items, ee_items;
   DBuf<DItem> ditems;
+  DBuf<DItem> ditems_def;  // FP32 form: deferred items of inner node pairs
   DBuf<double> res;
   DBuf<unsigned> res_mask;  // FP32 form: per soup item, the slots its FP32 culls keep
   DBuf<int> np_hit, np_hit2, exp_desc;
@@ -2986,6 +3071,7 @@ struct d2_ctx {
   DBuf<int> hscratch;  // 6 x contact capacity: the filter of pairs above every LDS capacity
   DBuf<float> nbound;  // per node: leaf box lo, hi (body frame), largest leaf eps + slack, sphere r, centre, 0
   bool prune_on = true;  // skip node pairs whose leaf triangles are provably apart
+  bool defer_on = true;  // FP32 form: an inner node pair's items after its first go to k_tri_def
   // the solver (d2_solve): per-body mass and friction, per-cluster and per-hit scratch, the warm store
   DBuf<double> sv_mass, sv_fric;
   DBuf<int> sv_ccount, sv_chits, sv_poff, sv_hoff, sv_cfill, sv_cpairs, sv_inpair, sv_loc, sv_status, sv_flag;
@@ -3213,6 +3299,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.conn_next = c->conn_list[(level + 1) & 1].p;
   D.items = c->items.p;
   D.ditems = c->ditems.p;
+  D.ditems_def = c->ditems_def.p;
   D.pt_items = c->pt_items.p;
   D.ee_items = c->ee_items.p;
   D.res = c->res.p;
@@ -3237,6 +3324,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.stris = c->stris.p;
   D.nbound = c->nbound.p;
   D.prune = c->prune_on && !c->touch_on && c->nbound.p ? 1 : 0;  // accounting runs walk the reference's node pairs
+  D.defer = c->defer_on && c->precision == D2_FP32 && c->ditems_def.p ? 1 : 0;
   D.conn_mask = c->conn_mask.p;
   D.body_tree = c->body_tree.p;
   D.fp32 = c->precision == D2_FP32 ? 1 : 0;
@@ -3264,7 +3352,10 @@ int prepare(d2_ctx* c) {
   }
   RC(ensure(c, c->items, c->cap_items)); RC(ensure(c, c->pt_items, c->cap_items));
   RC(ensure(c, c->ee_items, c->cap_items));
-  if (c->precision == D2_FP32) RC(ensure(c, c->ditems, c->cap_items));  // FP64: k_tri_full keeps no item list
+  if (c->precision == D2_FP32) {  // FP64: k_tri_full keeps no item list
+    RC(ensure(c, c->ditems, c->cap_items));
+    RC(ensure(c, c->ditems_def, c->cap_items));
+  }
   RC(ensure(c, c->np_hit, c->cap)); RC(ensure(c, c->np_hit2, c->cap)); RC(ensure(c, c->exp_desc, c->cap)); RC(ensure(c, c->masks, c->cap));
   RC(ensure(c, c->cnt_a, c->cap)); RC(ensure(c, c->cnt_s, c->cap)); RC(ensure(c, c->cnt_c, c->cap));
   RC(ensure(c, c->cnt_pt, c->cap)); RC(ensure(c, c->cnt_ee, c->cap)); RC(ensure(c, c->parts, 5 * g));
@@ -3315,10 +3406,10 @@ void scan_host_n(d2_ctx* c, const int* in, int* out, long long n, unsigned long 
 // connected node pair per wavefront, the stream kernels one unit per thread.
 // Any grid gives the same results; only the time changes.
 struct LevelGrids {
-  int cull, expand, stream;
+  int cull, expand, stream, def;
 };
 LevelGrids level_grids(const d2_ctx* c, int discrete, int level) {
-  LevelGrids g{chunk_grid(c), chunk_grid(c), stream_grid(c)};
+  LevelGrids g{chunk_grid(c), chunk_grid(c), stream_grid(c), stream_grid(c)};
   const int ph = discrete ? 0 : 1;
   if (!c->prev_ok[ph] || level > MAX_LEVELS) return g;
   const long long* v = c->prev_lv[ph][level];
@@ -3332,13 +3423,15 @@ LevelGrids level_grids(const d2_ctx* c, int discrete, int level) {
   g.cull = fit(ns, 4, g.cull);
   g.expand = fit(std::max((ns + 63) / 64, nc), 4, g.expand);
   g.stream = fit(units, 256, g.stream);
+  if (discrete) g.def = fit(v[LV_DEF], 256, g.def);
   return g;
 }
 unsigned long long grids_hash(const d2_ctx* c, int discrete, int levels) {
   unsigned long long h = 1469598103934665603ull;
   for (int l = 0; l < levels; ++l) {
     const LevelGrids g = level_grids(c, discrete, l);
-    h = (h ^ (unsigned long long)((g.cull * 131071ll + g.expand) * 131071ll + g.stream)) * 1099511628211ull;
+    h = (h ^ (unsigned long long)(((g.cull * 131071ll + g.expand) * 131071ll + g.stream) * 131071ll + g.def)) *
+        1099511628211ull;
   }
   return h;
 }
@@ -3405,6 +3498,7 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   if (discrete && c->precision == D2_FP32) {
     timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
+    if (D.defer) timed_launch(c, K_TRI_DEF, k_tri_def, dim3(G.def), dim3(256), D);
   } else if (discrete) {  // one wavefront per 64 node pairs
     timed_launch(c, K_TRI_FULL, k_tri_full, dim3(eg), dim3(256), D);
   } else {
@@ -3614,6 +3708,7 @@ void add_level_items(d2_ctx* c, int level, int discrete) {
     c->k_items[K_TRI_MASK] += (double)s[LV_SOUP];
     c->k_items[K_TRI_ENUM] += (double)s[LV_SOUP];
     c->k_items[K_TRI_DIST] += (double)s[LV_ITEMS];
+    c->k_items[K_TRI_DEF] += (double)s[LV_DEF];
   } else {
     c->k_items[K_SOUP_CULL] += (double)s[LV_SOUP];
     const bool fp32 = c->precision == D2_FP32;  // FP32 form: the sub-tests its culls keep
@@ -3839,7 +3934,7 @@ void grow_capacities(d2_ctx* c, int kind) {
         items = std::max<long long>(items, (long long)std::max(v[LV_PT], v[LV_EE]));
         items = std::max<long long>(items, (long long)std::max(v[LV_SPT], v[LV_SEE]));  // FP32 form's soup survivors
       } else if (c->precision == D2_FP32) {  // FP64 discrete levels keep no item list (k_tri_full)
-        items = std::max<long long>(items, (long long)v[LV_ITEMS]);
+        items = std::max<long long>(items, (long long)std::max(v[LV_ITEMS], v[LV_DEF]));
       }
     }
   };
@@ -4060,7 +4155,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->nodes), release(c->buckets), release(c->fbuckets), release(c->stris), release(c->tree_off), release(c->body_tree), release(c->bodies);
   release(c->pairs), release(c->pair_info), release(c->cluster_of_pair), release(c->max_time), release(c->pair_rec);
   for (int k = 0; k < 2; ++k) release(c->soup_list[k]), release(c->conn_list[k]);
-  release(c->items), release(c->ditems), release(c->pt_items), release(c->ee_items), release(c->res), release(c->res_mask), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
+  release(c->items), release(c->ditems), release(c->ditems_def), release(c->pt_items), release(c->ee_items), release(c->res), release(c->res_mask), release(c->np_hit), release(c->np_hit2), release(c->exp_desc);
   release(c->contacts), release(c->cnt), release(c->lvl), release(c->masks), release(c->conn_mask);
   release(c->cnt_a), release(c->cnt_s), release(c->cnt_c), release(c->cnt_pt), release(c->cnt_ee), release(c->parts);
   release(c->pass_min_s), release(c->pass_min_c), release(c->pass_flag_s), release(c->pass_flag_c);
