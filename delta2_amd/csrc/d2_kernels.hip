@@ -225,6 +225,7 @@ struct Dev {
   const float* fbuckets;       // FP32 form: per triangle its corners rounded to float and its eps (FB_STRIDE)
   const float* stris;          // FP32 form: per triangle its soup_tri planes (ST_STRIDE)
   const float* nbound;         // per node: leaf box, slack, sphere (nodes_apart, spheres_apart)
+  const uint8_t* kid_inner;    // per node: bit c set when child c is an inner node
   int prune;                   // skip node pairs nodes_apart proves contact-free
   int defer;                   // FP32 form: inner node pairs send one item to k_tri_dist, the rest to k_tri_def
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
@@ -1523,14 +1524,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
         const NodeRec na = D.nodes[pi[2] + np.na];
         const NodeRec nb = D.nodes[pi[3] + np.nb];
         if (na.inner || nb.inner) {
-          bool adv_a = expand_side(na, nb);
+          // a child pair goes to the soup list when either node is inner:
+          // all of them when the node that stays is inner, else the inner
+          // children of the one that advances
+          const bool adv_a = expand_side(na, nb);
           const NodeRec& par = adv_a ? na : nb;
-          for (int c = 0; c < par.nchild; ++c) {
-            const NodeRec x = adv_a ? D.nodes[pi[2] + par.child_first + c] : na;
-            const NodeRec y = adv_a ? nb : D.nodes[pi[3] + par.child_first + c];
-            const bool to_soup = x.inner || y.inner;
-            ns += to_soup, nc += !to_soup;
-          }
+          const bool stay_inner = adv_a ? nb.inner : na.inner;
+          ns = stay_inner ? par.nchild : __popc((unsigned)D.kid_inner[(adv_a ? pi[2] : pi[3]) + (adv_a ? np.na : np.nb)]);
+          nc = par.nchild - ns;
         }
       }
       D.cnt_s[w] = ns;
@@ -1614,12 +1615,14 @@ __global__ void __launch_bounds__(256) k_level_write(Dev D) {
         const NodeRec nb = D.nodes[pi[3] + np.nb];
         bool adv_a = expand_side(na, nb);
         const NodeRec& par = adv_a ? na : nb;
+        const bool stay_inner = adv_a ? nb.inner : na.inner;
+        const unsigned kid = D.kid_inner[(adv_a ? pi[2] : pi[3]) + (adv_a ? np.na : np.nb)];
         long long ps = bs + es, pc = bc + ec;
         for (int c = 0; c < par.nchild; ++c) {
           uint32_t ca = adv_a ? (uint32_t)(par.child_first + c) : np.na;
           uint32_t cb = adv_a ? np.nb : (uint32_t)(par.child_first + c);
           NP child{np.pair, ca, cb, 0u, np.key * 8ull + (unsigned long long)c};
-          const bool to_soup = D.nodes[pi[2] + ca].inner || D.nodes[pi[3] + cb].inner;
+          const bool to_soup = stay_inner || ((kid >> c) & 1u);
           long long slot = to_soup ? ps++ : pc++;
           if (slot < D.cap_np) (to_soup ? D.soup_next : D.conn_next)[slot] = child;
         }
@@ -3070,6 +3073,7 @@ struct d2_ctx {
   DBuf<int> pair_cnt, pair_off, pair_fill, perm, keep, kept_cnt, out_off, big, huge, n_big, scan_part;
   DBuf<int> hscratch;  // 6 x contact capacity: the filter of pairs above every LDS capacity
   DBuf<float> nbound;  // per node: leaf box lo, hi (body frame), largest leaf eps + slack, sphere r, centre, 0
+  DBuf<uint8_t> kid_inner;  // per node: which children are inner nodes (bit c: child c)
   bool prune_on = true;  // skip node pairs whose leaf triangles are provably apart
   bool defer_on = true;  // FP32 form: an inner node pair's items after its first go to k_tri_def
   // the solver (d2_solve): per-body mass and friction, per-cluster and per-hit scratch, the warm store
@@ -3323,6 +3327,7 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.fbuckets = c->fbuckets.p;
   D.stris = c->stris.p;
   D.nbound = c->nbound.p;
+  D.kid_inner = c->kid_inner.p;
   D.prune = c->prune_on && !c->touch_on && c->nbound.p ? 1 : 0;  // accounting runs walk the reference's node pairs
   D.defer = c->defer_on && c->precision == D2_FP32 && c->ditems_def.p ? 1 : 0;
   D.conn_mask = c->conn_mask.p;
@@ -4163,7 +4168,7 @@ int d2_ctx_destroy(d2_ctx* c) {
   release(c->pair_cnt), release(c->pair_off), release(c->pair_fill), release(c->perm), release(c->keep),
       release(c->kept_cnt), release(c->out_off), release(c->out_i), release(c->out_r), release(c->scan_part);
   release(c->big), release(c->huge), release(c->n_big), release(c->hscratch);
-  release(c->nbound);
+  release(c->nbound), release(c->kid_inner);
   release(c->sv_mass), release(c->sv_fric), release(c->sv_ccount), release(c->sv_chits), release(c->sv_poff),
       release(c->sv_hoff), release(c->sv_cfill), release(c->sv_cpairs), release(c->sv_inpair), release(c->sv_loc),
       release(c->sv_status), release(c->sv_flag), release(c->sv_hl), release(c->sv_ord), release(c->sv_hrow),
@@ -4315,6 +4320,7 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
     }
   }
   (void)hipSetDevice(c->device);
+  c->n_trees = 0;  // no trees until every per-node table of this upload is in place
   RC(ensure(c, c->nodes, total)); RC(ensure(c, c->buckets, (size_t)total * BUCKET)); RC(ensure(c, c->tree_off, n_trees + 1));
   c->n_nodes_total = total, c->fb_ready = false;  // the float copy follows on the next FP32 run
   HIPCHK(c, hipMemcpyAsync(c->nodes.p, nodes, sizeof(NodeRec) * total, hipMemcpyHostToDevice, c->stream));
@@ -4454,6 +4460,23 @@ int d2_trees_upload(d2_ctx* c, int n_trees, const int32_t* tree_node_offset, con
   }
   RC(ensure(c, c->nbound, nb.size()));
   HIPCHK(c, hipMemcpyAsync(c->nbound.p, nb.data(), sizeof(float) * nb.size(), hipMemcpyHostToDevice, c->stream));
+  // Per node, which of its (at most 8) children are inner nodes: the CCD
+  // expansion routes a node pair's children to the soup or the connected
+  // list by it instead of reading every child's record.
+  std::vector<uint8_t> kid((size_t)total, 0);
+  for (int t = 0; t < n_trees; ++t) {
+    const int b0 = tree_node_offset[t];
+    for (int i = b0; i < tree_node_offset[t + 1]; ++i) {
+      const int32_t* r = nodes + 8 * (size_t)i;
+      if (!r[2]) continue;
+      unsigned m = 0;
+      for (int ch = 0; ch < r[6]; ++ch)
+        if (nodes[8 * ((size_t)b0 + r[5] + ch) + 2]) m |= 1u << ch;
+      kid[i] = (uint8_t)m;
+    }
+  }
+  RC(ensure(c, c->kid_inner, kid.size()));
+  HIPCHK(c, hipMemcpyAsync(c->kid_inner.p, kid.data(), kid.size(), hipMemcpyHostToDevice, c->stream));
   // Per tree for the broad phase, as MeshData::initBoudingData (model/mesh.cpp:44-51)
   // computes it over the mesh vertices (here: the leaf triangles' vertices,
   // which are every vertex of a mesh whose vertices all belong to a face):
