@@ -228,6 +228,7 @@ struct Dev {
   const uint8_t* kid_inner;    // per node: bit c set when child c is an inner node
   int prune;                   // skip node pairs nodes_apart proves contact-free
   int defer;                   // FP32 form: inner node pairs send one item to k_tri_dist, the rest to k_tri_def
+  int box_test;                // FP32 form: k_tri_mask tests arriving node pairs with boxes_apart_faces
   unsigned long long* conn_mask;  // per connected node pair: surviving leaf sub-tests
   const int32_t* body_tree;
   int fp32;                    // FP32 culls of the CCD leaf sub-tests
@@ -1054,7 +1055,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))
         const float* bA = D.nbound + NB_STRIDE * ((size_t)l_offa + np.na);
         const float* bB = D.nbound + NB_STRIDE * ((size_t)l_offb + np.nb);
         live = !((D.level == 0 && spheres_apart_f32(pr + PR_TA1, pr + PR_TB1, bA, bB, pr[PR_SEARCH])) ||
-                 boxes_apart_faces(pr + PR_TA1, pr + PR_TB1, bA, bB, pr[PR_SEARCH]));
+                 (D.box_test && boxes_apart_faces(pr + PR_TA1, pr + PR_TB1, bA, bB, pr[PR_SEARCH])));
       }
       if (live) {
         const NodeRec na = D.nodes[l_offa + np.na];
@@ -3329,7 +3330,8 @@ Dev make_dev(d2_ctx* c, int level, int discrete) {
   D.nbound = c->nbound.p;
   D.kid_inner = c->kid_inner.p;
   D.prune = c->prune_on && !c->touch_on && c->nbound.p ? 1 : 0;  // accounting runs walk the reference's node pairs
-  D.defer = c->defer_on && c->precision == D2_FP32 && c->ditems_def.p ? 1 : 0;
+  D.defer = c->defer_on && c->precision == D2_FP32 && c->ditems_def.p ? 1 : 0;  // per level: enqueue_level
+  D.box_test = 1;
   D.conn_mask = c->conn_mask.p;
   D.body_tree = c->body_tree.p;
   D.fp32 = c->precision == D2_FP32 ? 1 : 0;
@@ -3410,11 +3412,16 @@ void scan_host_n(d2_ctx* c, const int* in, int* out, long long n, unsigned long 
 // there is work), the count/write pair one soup node pair per thread or one
 // connected node pair per wavefront, the stream kernels one unit per thread.
 // Any grid gives the same results; only the time changes.
+// def: k_tri_def's grid, 0 where the level defers nothing; box: whether
+// k_tri_mask runs the face-axis box test.  Both pay on large levels only (on
+// small ones the extra launch and the extra dependent load cost more than
+// they save), judged by the last run's sizes of the level: at least 64 K
+// items, at least 256 K node pairs.
 struct LevelGrids {
-  int cull, expand, stream, def;
+  int cull, expand, stream, def, box;
 };
 LevelGrids level_grids(const d2_ctx* c, int discrete, int level) {
-  LevelGrids g{chunk_grid(c), chunk_grid(c), stream_grid(c), stream_grid(c)};
+  LevelGrids g{chunk_grid(c), chunk_grid(c), stream_grid(c), stream_grid(c), 1};
   const int ph = discrete ? 0 : 1;
   if (!c->prev_ok[ph] || level > MAX_LEVELS) return g;
   const long long* v = c->prev_lv[ph][level];
@@ -3428,14 +3435,18 @@ LevelGrids level_grids(const d2_ctx* c, int discrete, int level) {
   g.cull = fit(ns, 4, g.cull);
   g.expand = fit(std::max((ns + 63) / 64, nc), 4, g.expand);
   g.stream = fit(units, 256, g.stream);
-  if (discrete) g.def = fit(v[LV_DEF], 256, g.def);
+  if (discrete) {
+    g.def = v[LV_ITEMS] + v[LV_DEF] >= (1ll << 16) ? fit(v[LV_DEF], 256, g.def) : 0;
+    g.box = ns >= (1ll << 18) ? 1 : 0;
+  }
   return g;
 }
 unsigned long long grids_hash(const d2_ctx* c, int discrete, int levels) {
   unsigned long long h = 1469598103934665603ull;
   for (int l = 0; l < levels; ++l) {
     const LevelGrids g = level_grids(c, discrete, l);
-    h = (h ^ (unsigned long long)(((g.cull * 131071ll + g.expand) * 131071ll + g.stream) * 131071ll + g.def)) *
+    h = (h ^ (unsigned long long)((((g.cull * 131071ll + g.expand) * 131071ll + g.stream) * 131071ll + g.def) * 2 +
+                                  g.box)) *
         1099511628211ull;
   }
   return h;
@@ -3500,6 +3511,8 @@ void enqueue_level(d2_ctx* c, int level, int discrete) {
   Dev D = make_dev(c, level, discrete);
   const LevelGrids G = level_grids(c, discrete, level);
   const int cg = G.cull, eg = G.expand, sg = G.stream;
+  D.defer = D.defer && G.def > 0 ? 1 : 0;
+  D.box_test = G.box;
   if (discrete && c->precision == D2_FP32) {
     timed_launch(c, K_TRI_MASK, k_tri_mask, dim3(cg), dim3(256), D);
     timed_launch(c, K_TRI_DIST, k_tri_dist, dim3(sg), dim3(256), D);
